@@ -1,0 +1,151 @@
+"""Synthetic workloads of the structured-block explicit update, with their sizes.
+
+Each builder returns plain numpy arrays (vertex coordinates, per-cell initial p, T,
+velocity, boundary-condition kinds, inflow states) that both the HIP block API
+(gdtk_amd.block) and the CPU oracle accept.  Vertex ordering is the structured-grid
+order of the reference, index (k*njv + j)*niv + i (src/eilmer/sfluidblock.d:315-328).
+
+Gas: ideal air from src/gas/sample-data/ideal-air-gas-model.lua:5-7 (mMass 0.02896,
+gamma 1.4, so R = 8.31451/0.02896 = 287.103 J/kg/K).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Boundary-condition kinds shared by the C ABI and the oracle.
+BC_SLIP_WALL = 0
+BC_INFLOW_SUPERSONIC = 1
+BC_OUTFLOW_SIMPLE_FLUX = 2
+BC_OUTFLOW_SIMPLE_EXTRAPOLATE = 3
+BC_EXCHANGE = 4
+
+# Face order of the reference (src/geom/elements/nomenclature.d:12-20).
+WEST, EAST, SOUTH, NORTH, BOTTOM, TOP = range(6)
+
+IDEAL_AIR_MMASS = 0.02896
+IDEAL_AIR_GAMMA = 1.4
+R_UNIVERSAL = 8.31451
+
+
+@dataclass
+class BlockCase:
+    dims: int
+    nic: int
+    njc: int
+    nkc: int
+    vertices: np.ndarray            # (niv*njv*nkv, 3) float64
+    p: np.ndarray                   # (ncells,) float64
+    T: np.ndarray
+    vel: np.ndarray                 # (ncells, 3)
+    bc: list                        # 6 BC kinds
+    inflow: np.ndarray = field(default_factory=lambda: np.zeros((6, 5)))  # [p,T,vx,vy,vz] per face
+    other_block: list = field(default_factory=lambda: [-1] * 6)
+    other_face: list = field(default_factory=lambda: [-1] * 6)
+
+    @property
+    def ncells(self) -> int:
+        return self.nic * self.njc * self.nkc
+
+
+def box_vertices(nic, njc, nkc, lx, ly, lz, dims, x0=0.0, y0=0.0, z0=0.0):
+    """Uniform Cartesian vertices, evaluated as x0 + lx*(i/nic) like a linear volume map."""
+    niv, njv = nic + 1, njc + 1
+    nkv = nkc + 1 if dims == 3 else 1
+    r = np.arange(niv, dtype=np.float64) / nic
+    s = np.arange(njv, dtype=np.float64) / njc
+    t = np.arange(nkv, dtype=np.float64) / nkc if dims == 3 else np.zeros(1)
+    K, J, I = np.meshgrid(t, s, r, indexing="ij")
+    v = np.empty((nkv, njv, niv, 3))
+    v[..., 0] = x0 + lx * I
+    v[..., 1] = y0 + ly * J
+    v[..., 2] = (z0 + lz * K) if dims == 3 else 0.0
+    return v.reshape(-1, 3)
+
+
+def cell_centres(case: BlockCase) -> np.ndarray:
+    """Vertex-average cell centres (for initial-condition placement only)."""
+    niv, njv = case.nic + 1, case.njc + 1
+    nkv = case.nkc + 1 if case.dims == 3 else 1
+    v = case.vertices.reshape(nkv, njv, niv, 3)
+    if case.dims == 2:
+        c = 0.25 * (v[0, :-1, :-1] + v[0, :-1, 1:] + v[0, 1:, :-1] + v[0, 1:, 1:])
+        return c.reshape(-1, 3)
+    c = 0.125 * (v[:-1, :-1, :-1] + v[:-1, :-1, 1:] + v[:-1, 1:, :-1] + v[:-1, 1:, 1:]
+                 + v[1:, :-1, :-1] + v[1:, :-1, 1:] + v[1:, 1:, :-1] + v[1:, 1:, 1:])
+    return c.reshape(-1, 3)
+
+
+def sod_2d(nic=200, njc=100, T_right=278.7) -> BlockCase:
+    """Plain method, small: 2-D Sod split at x = 0.5 on [0,1]x[0,0.5], slip walls all round.
+
+    Left p=1e5 Pa, T=348.4 K; right p=1e4 Pa, T=278.7 K; u=0.
+    """
+    v = box_vertices(nic, njc, 1, 1.0, 0.5, 0.0, dims=2)
+    case = BlockCase(2, nic, njc, 1, v, None, None, None, [BC_SLIP_WALL] * 6)
+    xc = cell_centres(case)[:, 0]
+    left = xc < 0.5
+    case.p = np.where(left, 1.0e5, 1.0e4)
+    case.T = np.where(left, 348.4, T_right)
+    case.vel = np.zeros((case.ncells, 3))
+    return case
+
+
+def sod_3d_tube(nic=100) -> BlockCase:
+    """The reference's 3-D Sod tube: 100x2x2 cells on 1.0 x 0.1 x 0.1 m, slip walls.
+
+    examples/eilmer/3D/sod-shock-tube/sg/sod.lua:13-45 (p=1e5/1e4 Pa, T=348.4/278.8 K).
+    """
+    v = box_vertices(nic, 2, 2, 1.0, 0.1, 0.1, dims=3)
+    case = BlockCase(3, nic, 2, 2, v, None, None, None, [BC_SLIP_WALL] * 6)
+    xc = cell_centres(case)[:, 0]
+    left = xc < 0.5
+    case.p = np.where(left, 1.0e5, 1.0e4)
+    case.T = np.where(left, 348.4, 278.8)
+    case.vel = np.zeros((case.ncells, 3))
+    return case
+
+
+def mach6_box(nic=256, njc=128, nkc=128, seed=1234, dims=3) -> BlockCase:
+    """Flux-path bandwidth stress: Mach-6 supersonic box, unit cell spacing.
+
+    Free stream p = 2e3 Pa, T = 220 K, u = 6a along +x.  Density and temperature are
+    perturbed by +/-5 % with a sum of 8 seeded Fourier modes; pressure follows from the
+    EOS (p = rho R T with the perturbed rho and T).  West face supersonic inflow, all
+    other faces simple outflow (flux).
+    """
+    rng = np.random.default_rng(seed)
+    nk = nkc if dims == 3 else 1
+    v = box_vertices(nic, njc, nk, float(nic), float(njc), float(nk), dims=dims)
+    bc = [BC_INFLOW_SUPERSONIC] + [BC_OUTFLOW_SIMPLE_FLUX] * 5
+    case = BlockCase(dims, nic, njc, nk, v, None, None, None, bc)
+    R = R_UNIVERSAL / IDEAL_AIR_MMASS
+    p0, T0 = 2.0e3, 220.0
+    a0 = np.sqrt(IDEAL_AIR_GAMMA * R * T0)
+    u0 = 6.0 * a0
+    xc = cell_centres(case)
+    L = np.array([nic, njc, max(nk, 1)], dtype=np.float64)
+
+    def field_():
+        s = np.zeros(case.ncells)
+        amps = rng.uniform(0.2, 1.0, 8)
+        for m in range(8):
+            kvec = 2.0 * np.pi * rng.integers(1, 6, 3) / L
+            if dims == 2:
+                kvec[2] = 0.0
+            phi = rng.uniform(0.0, 2.0 * np.pi)
+            s += amps[m] * np.sin(xc @ kvec + phi)
+        return 0.05 * s / np.abs(amps).sum()
+
+    drho = field_()
+    dT = field_()
+    rho0 = p0 / (R * T0)
+    rho = rho0 * (1.0 + drho)
+    case.T = T0 * (1.0 + dT)
+    case.p = rho * R * case.T
+    case.vel = np.zeros((case.ncells, 3))
+    case.vel[:, 0] = u0
+    case.inflow = np.zeros((6, 5))
+    case.inflow[WEST] = [p0, T0, u0, 0.0, 0.0]
+    return case

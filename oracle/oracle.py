@@ -1,0 +1,226 @@
+"""ctypes wrapper over oracle/_build/liboracle.so (test infrastructure only).
+
+The oracle restates the D reference in C++ (oracle/eilmer_oracle.cpp).  This module
+is loaded by tests, smoke() and the CPU-baseline leg of bench.py only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_gas_create_ideal.restype = C.c_void_p
+        L.orc_gas_create_ideal.argtypes = [C.c_double, C.c_double]
+        L.orc_gas_destroy.argtypes = [C.c_void_p]
+        L.orc_gas_update.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_sim_create.restype = C.c_void_p
+        L.orc_sim_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_int]
+        L.orc_sim_destroy.argtypes = [C.c_void_p]
+        L.orc_sim_add_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, ip, dp, ip, ip]
+        L.orc_block_set_initial.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp]
+        L.orc_block_get.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.orc_block_faces.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_block_face_flux.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_sim_run.argtypes = [C.c_void_p, C.c_int, C.c_double, dp]
+        L.orc_sim_step_fixed_dt.argtypes = [C.c_void_p, C.c_double, dp]
+        L.orc_sim_stage.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_block_dt.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, dp]
+        L.orc_sim_set_dt.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.orc_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        L.orc_exxef.argtypes = [C.c_double, dp, dp]
+        L.orc_l2r2_scalar.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double, dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ip)
+
+
+def err():
+    return lib().orc_last_error().decode()
+
+
+class IdealGas:
+    def __init__(self, mMass=0.02896, gamma=1.4):
+        self.h = lib().orc_gas_create_ideal(mMass, gamma)
+        self.nsp = 1
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_gas_destroy(self.h)
+            self.h = None
+
+    def update(self, mode, rho=0.0, p=0.0, T=0.0, u=0.0, massf=None):
+        st, stp = _d([rho, p, T, u, 0.0])
+        mf = None
+        if massf is not None:
+            mfa, mf = _d(massf)
+        rc = lib().orc_gas_update(self.h, mode, stp, mf)
+        if rc != 0:
+            raise RuntimeError(err())
+        return dict(rho=st[0], p=st[1], T=st[2], u=st[3], a=st[4])
+
+
+FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11)
+
+
+class Sim:
+    """Whole-simulation oracle: blocks, BCs, the pc explicit update and the dt controller."""
+
+    def __init__(self, gas, dims, flux="ausmdv", cfl=0.5, dt_init=1.0e-3, dt_max=1.0e-3, cfl_count=10,
+                 mutate_cell_velocities=True):
+        self.gas = gas
+        self.dims = dims
+        fc = {"ausmdv": 0, "efm": 1}[flux]
+        self.h = lib().orc_sim_create(gas.h, dims, fc, cfl, dt_init, dt_max, cfl_count,
+                                      1 if mutate_cell_velocities else 0)
+        self.blocks = []
+
+    def close(self):
+        if self.h:
+            lib().orc_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_block(self, case):
+        v, vp = _d(case.vertices.reshape(-1))
+        bc, bcp = _i(case.bc)
+        nsp = self.gas.nsp
+        inflow = np.zeros((6, 5 + nsp))
+        inflow[:, :5] = case.inflow
+        if nsp == 1:
+            inflow[:, 5] = 1.0
+        inf, infp = _d(inflow)
+        ob, obp = _i(case.other_block)
+        of, ofp = _i(case.other_face)
+        b = lib().orc_sim_add_block(self.h, case.nic, case.njc, case.nkc, vp, bcp, infp, obp, ofp)
+        if b < 0:
+            raise RuntimeError(err())
+        arrs = [_d(case.p), _d(case.T), _d(case.vel[:, 0]), _d(case.vel[:, 1]), _d(case.vel[:, 2])]
+        rc = lib().orc_block_set_initial(self.h, b, *[a[1] for a in arrs], None)
+        if rc != 0:
+            raise RuntimeError(err())
+        self.blocks.append(case)
+        return b
+
+    def get(self, b, name):
+        case = self.blocks[b]
+        out = np.empty(case.ncells)
+        if isinstance(name, str):
+            f = FIELDS[name]
+        else:
+            f = name
+        rc = lib().orc_block_get(self.h, b, f, out.ctypes.data_as(dp))
+        if rc != 0:
+            raise RuntimeError(err())
+        return out
+
+    def get_U(self, b, ncq):
+        return np.stack([self.get(b, 100 + j) for j in range(ncq)], axis=1)
+
+    def faces(self, b):
+        case = self.blocks[b]
+        niv, njv = case.nic + 1, case.njc + 1
+        nkv = case.nkc + 1
+        n = niv * case.njc * case.nkc + case.nic * njv * case.nkc
+        if self.dims == 3:
+            n += case.nic * case.njc * nkv
+        out = np.empty((n, 10))
+        lib().orc_block_faces(self.h, b, out.ctypes.data_as(dp))
+        return out
+
+    def face_flux(self, b, ncq):
+        n = self.faces(b).shape[0]
+        out = np.empty((n, ncq))
+        lib().orc_block_face_flux(self.h, b, out.ctypes.data_as(dp))
+        return out
+
+    def run(self, max_steps, max_time=1.0e30):
+        st = np.zeros(6)
+        n = lib().orc_sim_run(self.h, max_steps, max_time, st.ctypes.data_as(dp))
+        if n < 0:
+            raise RuntimeError(err())
+        return n, dict(time=st[0], dt=st[1], step=int(st[2]), cfl_max=st[3], dt_allow=st[4], attempts=int(st[5]))
+
+    def step_fixed_dt(self, dt):
+        st = np.zeros(6)
+        rc = lib().orc_sim_step_fixed_dt(self.h, dt, st.ctypes.data_as(dp))
+        if rc != 0:
+            raise RuntimeError(err())
+        return dict(time=st[0], dt=st[1], step=int(st[2]))
+
+    def stage(self, stage, dt):
+        rc = lib().orc_sim_stage(self.h, stage, dt)
+        if rc != 0:
+            raise RuntimeError(err())
+
+    def block_dt(self, b, dt_current, cfl, check_cfl):
+        out = np.zeros(3)
+        lib().orc_block_dt(self.h, b, dt_current, cfl, 1 if check_cfl else 0, out.ctypes.data_as(dp))
+        return out
+
+    def set_dt(self, dt_global, step):
+        lib().orc_sim_set_dt(self.h, dt_global, step)
+
+
+def flux(gas, kind, dims, L, R, massfL=None, massfR=None):
+    """Unit flux: L, R = [rho, p, T, u, a, vx, vy, vz] in the face-local frame."""
+    a, ap = _d(L)
+    b, bp = _d(R)
+    F = np.zeros(5 + max(gas.nsp, 1) + 1)
+    mL = mR = None
+    if massfL is not None:
+        ml, mL = _d(massfL)
+    if massfR is not None:
+        mr, mR = _d(massfR)
+    n = lib().orc_flux(gas.h, {"ausmdv": 0, "efm": 1}[kind], dims, ap, bp, mL, mR, F.ctypes.data_as(dp))
+    return F[:n].copy()
+
+
+def exxef(sn):
+    e1, e2 = C.c_double(), C.c_double()
+    lib().orc_exxef(sn, C.byref(e1), C.byref(e2))
+    return e1.value, e2.value
+
+
+def l2r2_scalar(lens, q, apply_limiter=True, extrema_clipping=True, eps=1.0e-12):
+    l, lp = _d(lens)
+    qq, qp = _d(q)
+    out = np.zeros(2)
+    lib().orc_l2r2_scalar(lp, qp, 1 if apply_limiter else 0, 1 if extrema_clipping else 0, eps,
+                          out.ctypes.data_as(dp))
+    return out[0], out[1]

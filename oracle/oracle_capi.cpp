@@ -1,0 +1,297 @@
+// oracle_capi.cpp -- C entry points over the oracle, for the Python test harness (ctypes).
+//
+// TEST INFRASTRUCTURE ONLY.  Loaded by tests/, smoke() and the CPU-baseline leg of
+// bench.py as the checker; never by the product path.
+#include <cstring>
+#include <string>
+
+#include "eilmer_oracle.hpp"
+
+using namespace orc;
+
+static thread_local std::string g_err;
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_gas_create_ideal(double mMass, double gamma) { return new IdealGas(mMass, gamma); }
+void orc_gas_destroy(void* g) { delete static_cast<GasModel*>(g); }
+int orc_gas_nsp(void* g) { return static_cast<GasModel*>(g)->n_species; }
+
+// Simple gas-state functions for unit tests.  st = [rho, p, T, u, a], massf[nsp].
+static void to_gs(const GasModel* gm, const double* st, const double* massf, GasState& Q) {
+    Q.rho = st[0]; Q.p = st[1]; Q.T = st[2]; Q.u = st[3]; Q.a = st[4];
+    Q.massf.assign(gm->n_species, 1.0);
+    Q.rho_s.assign(gm->n_species, 0.0);
+    if (massf) for (int i = 0; i < gm->n_species; ++i) Q.massf[i] = massf[i];
+}
+static void from_gs(const GasState& Q, double* st) {
+    st[0] = Q.rho; st[1] = Q.p; st[2] = Q.T; st[3] = Q.u; st[4] = Q.a;
+}
+int orc_gas_update(void* g, int mode, double* st, const double* massf) {
+    GasModel* gm = static_cast<GasModel*>(g);
+    GasState Q;
+    to_gs(gm, st, massf, Q);
+    try {
+        switch (mode) {
+        case 0: gm->update_thermo_from_pT(Q); break;
+        case 1: gm->update_thermo_from_rhou(Q); break;
+        case 2: gm->update_thermo_from_rhoT(Q); break;
+        default: g_err = "bad mode"; return -2;
+        }
+        gm->update_sound_speed(Q);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    from_gs(Q, st);
+    return 0;
+}
+
+struct Sim {
+    Simulation* sim;
+    const GasModel* gm;
+};
+
+void* orc_sim_create(void* gas, int dims, int flux_calc, double cfl, double dt_init, double dt_max, int cfl_count,
+                     int mutate_cell_velocities) {
+    Config c;
+    c.dimensions = dims;
+    c.flux_calculator = (flux_calc == 1) ? FluxCalc::efm : FluxCalc::ausmdv;
+    c.cfl_value = cfl;
+    c.dt_init = dt_init;
+    c.dt_max = dt_max;
+    c.cfl_count = cfl_count;
+    c.mutate_cell_velocities = mutate_cell_velocities != 0;
+    Sim* s = new Sim{new Simulation(c), static_cast<GasModel*>(gas)};
+    return s;
+}
+void orc_sim_destroy(void* h) {
+    Sim* s = static_cast<Sim*>(h);
+    delete s->sim;
+    delete s;
+}
+
+// Inflow state given as [p, T, vx, vy, vz] + massf[nsp]; set through pT + sound speed.
+static FlowState make_fs(const GasModel* gm, const double* pTv, const double* massf) {
+    FlowState fs;
+    fs.gas.p = pTv[0];
+    fs.gas.T = pTv[1];
+    fs.vel = Vec3(pTv[2], pTv[3], pTv[4]);
+    fs.gas.massf.assign(gm->n_species, 1.0);
+    if (massf) for (int i = 0; i < gm->n_species; ++i) fs.gas.massf[i] = massf[i];
+    gm->update_thermo_from_pT(fs.gas);
+    gm->update_sound_speed(fs.gas);
+    fs.gas.rho_s.assign(gm->n_species, 0.0);
+    for (int i = 0; i < gm->n_species; ++i) fs.gas.rho_s[i] = fs.gas.rho * fs.gas.massf[i];
+    return fs;
+}
+
+// bc_kinds[6]: BCKind values; other_block/other_face used by exchange faces.
+// inflow: 6 x (5 + nsp) doubles, one row per face (only read for inflow faces).
+int orc_sim_add_block(void* h, int nic, int njc, int nkc, const double* vtx, const int* bc_kinds,
+                      const double* inflow, const int* other_block, const int* other_face) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        const Config& c = s->sim->cfg;
+        int niv = nic + 1, njv = njc + 1, nkv = (c.dimensions == 3) ? nkc + 1 : 1;
+        std::vector<Vec3> v((size_t)niv * njv * nkv);
+        for (size_t n = 0; n < v.size(); ++n) v[n] = Vec3(vtx[3 * n], vtx[3 * n + 1], vtx[3 * n + 2]);
+        BoundaryCondition bcs[6];
+        int nsp = s->gm->n_species;
+        for (int b = 0; b < 6; ++b) {
+            bcs[b].kind = static_cast<BCKind>(bc_kinds[b]);
+            bcs[b].ghost_cell_data_available = true;
+            bcs[b].convective_flux_computed_in_bc = (bcs[b].kind == BCKind::outflow_simple_flux);
+            if (bcs[b].kind == BCKind::inflow_supersonic)
+                bcs[b].inflow = make_fs(s->gm, inflow + b * (5 + nsp), inflow + b * (5 + nsp) + 5);
+            if (bcs[b].kind == BCKind::exchange) {
+                bcs[b].other_block = other_block[b];
+                bcs[b].other_face = other_face[b];
+            }
+        }
+        int id = (int)s->sim->blocks.size();
+        s->sim->blocks.push_back(new SFluidBlock(id, nic, njc, nkc, c, s->gm, v, bcs));
+        return id;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Per interior cell: p, T, vx, vy, vz, and massf (ncell x nsp, may be null for nsp == 1).
+int orc_block_set_initial(void* h, int b, const double* p, const double* T, const double* vx, const double* vy,
+                          const double* vz, const double* massf) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        SFluidBlock* B = s->sim->blocks.at(b);
+        int nsp = s->gm->n_species;
+        for (size_t ci = 0; ci < B->cells.size(); ++ci) {
+            double row[5] = {p[ci], T[ci], vx[ci], vy[ci], vz[ci]};
+            FlowState fs = make_fs(s->gm, row, massf ? massf + ci * nsp : nullptr);
+            B->set_flowstate((int)ci, fs);
+        }
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// field: 0 rho, 1 vx, 2 vy, 3 vz, 4 u, 5 p, 6 T, 7 a, 8 volume, 9 iLength, 10 jLength,
+// 11 kLength, 100+j: U[0][j], 200+j: dUdt[0][j], 300+isp: massf[isp]
+int orc_block_get(void* h, int b, int field, double* out) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        SFluidBlock* B = s->sim->blocks.at(b);
+        for (size_t ci = 0; ci < B->cells.size(); ++ci) {
+            const FVCell* c = B->cells[ci];
+            double v;
+            if (field >= 300) v = c->fs.gas.massf.at(field - 300);
+            else if (field >= 200) v = c->dUdt[0].at(field - 200);
+            else if (field >= 100) v = c->U[0].at(field - 100);
+            else switch (field) {
+                case 0: v = c->fs.gas.rho; break;
+                case 1: v = c->fs.vel.x; break;
+                case 2: v = c->fs.vel.y; break;
+                case 3: v = c->fs.vel.z; break;
+                case 4: v = c->fs.gas.u; break;
+                case 5: v = c->fs.gas.p; break;
+                case 6: v = c->fs.gas.T; break;
+                case 7: v = c->fs.gas.a; break;
+                case 8: v = c->volume; break;
+                case 9: v = c->iLength; break;
+                case 10: v = c->jLength; break;
+                case 11: v = c->kLength; break;
+                default: g_err = "bad field"; return -2;
+            }
+            out[ci] = v;
+        }
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Face geometry of face index range: out rows of [nx,ny,nz,t1x,t1y,t1z,t2x,t2y,t2z,area]
+int orc_block_faces(void* h, int b, double* out) {
+    Sim* s = static_cast<Sim*>(h);
+    SFluidBlock* B = s->sim->blocks.at(b);
+    for (size_t n = 0; n < B->faces.size(); ++n) {
+        const FVInterface* f = B->faces[n];
+        double* r = out + 10 * n;
+        r[0] = f->n.x; r[1] = f->n.y; r[2] = f->n.z;
+        r[3] = f->t1.x; r[4] = f->t1.y; r[5] = f->t1.z;
+        r[6] = f->t2.x; r[7] = f->t2.y; r[8] = f->t2.z;
+        r[9] = f->area;
+    }
+    return (int)B->faces.size();
+}
+
+// Face fluxes after the last stage, rows of F[cqi.n] in face order.
+int orc_block_face_flux(void* h, int b, double* out) {
+    Sim* s = static_cast<Sim*>(h);
+    SFluidBlock* B = s->sim->blocks.at(b);
+    int n = B->cqi.n;
+    for (size_t k = 0; k < B->faces.size(); ++k)
+        for (int j = 0; j < n; ++j) out[k * n + j] = B->faces[k]->F[j];
+    return (int)B->faces.size();
+}
+
+// st_out = [time, dt_global, step, cfl_max, dt_allow, attempts_last_step]
+static void sim_state(Sim* s, double* st_out) {
+    if (!st_out) return;
+    st_out[0] = s->sim->st.time;
+    st_out[1] = s->sim->st.dt_global;
+    st_out[2] = s->sim->st.step;
+    st_out[3] = s->sim->st.cfl_max;
+    st_out[4] = s->sim->st.dt_allow;
+    st_out[5] = s->sim->st.attempts_last_step;
+}
+
+int orc_sim_run(void* h, int max_steps, double max_time, double* st_out) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        int n = s->sim->run(max_steps, max_time);
+        sim_state(s, st_out);
+        return n;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        sim_state(s, st_out);
+        return -1;
+    }
+}
+
+// One full gasdynamic step (with retries) at the given dt (no dt control).
+int orc_sim_step_fixed_dt(void* h, double dt, double* st_out) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        s->sim->st.dt_global = dt;
+        s->sim->gasdynamic_step();
+        s->sim->st.step += 1;
+        sim_state(s, st_out);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// One stage at dt; stage 1 leaves fs = decode(U1), stage 2 fs = decode(U2) (U not swapped).
+int orc_sim_stage(void* h, int stage, double dt) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        s->sim->stage(stage, dt);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Block-level dt estimate: out = [dt_allow, cfl_max, 0]
+int orc_block_dt(void* h, int b, double dt_current, double cfl, int check_cfl, double* out) {
+    Sim* s = static_cast<Sim*>(h);
+    s->sim->blocks.at(b)->determine_time_step_size(dt_current, cfl, check_cfl != 0, out);
+    return 0;
+}
+
+void orc_sim_set_dt(void* h, double dt_global, int step) {
+    Sim* s = static_cast<Sim*>(h);
+    s->sim->st.dt_global = dt_global;
+    s->sim->st.step = step;
+}
+
+// Unit-level flux calculators: L, R = [rho, p, T, u, a, velx, vely, velz] in the local frame.
+int orc_flux(void* gas, int flux_calc, int dims, const double* L, const double* R, const double* massfL,
+             const double* massfR, double* F) {
+    GasModel* gm = static_cast<GasModel*>(gas);
+    CQI cqi;
+    cqi.init(dims, gm->n_species);
+    FlowState l, r;
+    l.gas.rho = L[0]; l.gas.p = L[1]; l.gas.T = L[2]; l.gas.u = L[3]; l.gas.a = L[4]; l.vel = Vec3(L[5], L[6], L[7]);
+    r.gas.rho = R[0]; r.gas.p = R[1]; r.gas.T = R[2]; r.gas.u = R[3]; r.gas.a = R[4]; r.vel = Vec3(R[5], R[6], R[7]);
+    l.gas.massf.assign(gm->n_species, 1.0);
+    r.gas.massf.assign(gm->n_species, 1.0);
+    if (massfL) for (int i = 0; i < gm->n_species; ++i) l.gas.massf[i] = massfL[i];
+    if (massfR) for (int i = 0; i < gm->n_species; ++i) r.gas.massf[i] = massfR[i];
+    std::vector<double> Fv(cqi.n, 0.0);
+    if (flux_calc == 1) efmflx(l, r, Fv, cqi, *gm);
+    else ausmdv(l, r, Fv, cqi, *gm, true);
+    for (int j = 0; j < cqi.n; ++j) F[j] = Fv[j];
+    return cqi.n;
+}
+
+void orc_exxef(double sn, double* exx, double* ef) { exxef(sn, *exx, *ef); }
+
+// l2r2 scalar reconstruction through the oracle's own L2R2 (same code the block uses).
+void orc_l2r2_scalar(const double* len4, const double* q4, int apply_limiter, int extrema_clipping, double eps,
+                     double* qLqR) {
+    L2R2 c;
+    c.prepare(len4[0], len4[1], len4[2], len4[3]);
+    c.scalar(q4[0], q4[1], q4[2], q4[3], qLqR[0], qLqR[1], apply_limiter != 0, extrema_clipping != 0, eps);
+}
+
+}  // extern "C"
