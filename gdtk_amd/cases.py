@@ -149,3 +149,32 @@ def mach6_box(nic=256, njc=128, nkc=128, seed=1234, dims=3) -> BlockCase:
     case.inflow = np.zeros((6, 5))
     case.inflow[WEST] = [p0, T0, u0, 0.0, 0.0]
     return case
+
+
+def perturb_vertices(case: BlockCase, amp=0.15, seed=7) -> BlockCase:
+    """Smoothly distort the interior vertices (a curvilinear grid, boundary faces kept planar).
+
+    The displacement is a seeded sum of sines scaled by amp times the local spacing, zero on
+    the block boundary, so the grid stays valid but faces are no longer axis-aligned.
+    """
+    rng = np.random.default_rng(seed)
+    niv, njv = case.nic + 1, case.njc + 1
+    nkv = case.nkc + 1 if case.dims == 3 else 1
+    v = case.vertices.reshape(nkv, njv, niv, 3).copy()
+    r = np.linspace(0.0, 1.0, niv)[None, None, :]
+    s = np.linspace(0.0, 1.0, njv)[None, :, None]
+    t = np.linspace(0.0, 1.0, nkv)[:, None, None] if case.dims == 3 else np.zeros((1, 1, 1))
+    bump = np.sin(np.pi * r) * np.sin(np.pi * s)
+    if case.dims == 3:
+        bump = bump * np.sin(np.pi * t)
+    dx = (v[0, 0, -1, 0] - v[0, 0, 0, 0]) / case.nic
+    dy = (v[0, -1, 0, 1] - v[0, 0, 0, 1]) / case.njc
+    ph = rng.uniform(0.0, 2.0 * np.pi, 6)
+    v[..., 0] += amp * dx * bump * np.sin(2.0 * np.pi * s + ph[0]) * np.cos(2.0 * np.pi * r + ph[1])
+    v[..., 1] += amp * dy * bump * np.sin(2.0 * np.pi * r + ph[2]) * np.cos(2.0 * np.pi * s + ph[3])
+    if case.dims == 3:
+        dz = (v[-1, 0, 0, 2] - v[0, 0, 0, 2]) / case.nkc
+        v[..., 2] += amp * dz * bump * np.sin(2.0 * np.pi * (r + s) + ph[4]) * np.cos(2.0 * np.pi * t + ph[5])
+    out = BlockCase(case.dims, case.nic, case.njc, case.nkc, v.reshape(-1, 3), case.p, case.T, case.vel,
+                    list(case.bc), case.inflow.copy(), list(case.other_block), list(case.other_face))
+    return out

@@ -1,0 +1,438 @@
+// flux_kernels.hip -- HIP kernels of the structured-block explicit update (gfx950).
+//
+// Face-per-thread path: ghost fill, per-direction face flux into face arrays, then a
+// cell kernel that integrates the face fluxes, applies the predictor-corrector stage
+// update and decodes the new flow state.  The fused LDS-tiled stage kernel lives in
+// fused_stage.hip and shares the device physics in gdtk_device.hpp.
+#include <hip/hip_runtime.h>
+
+#include "block.hpp"
+#include "gdtk_device.hpp"
+#include "launchers.hpp"
+
+namespace gdtk {
+
+__device__ __forceinline__ FaceGeom load_face(const double* g, int64_t nface, int64_t f) {
+    FaceGeom r;
+    r.n.x = g[f];
+    r.n.y = g[f + nface];
+    r.n.z = g[f + 2 * nface];
+    r.t1.x = g[f + 3 * nface];
+    r.t1.y = g[f + 4 * nface];
+    r.t1.z = g[f + 5 * nface];
+    r.t2.x = g[f + 6 * nface];
+    r.t2.y = g[f + 7 * nface];
+    r.t2.z = g[f + 8 * nface];
+    r.area = g[f + 9 * nface];
+    return r;
+}
+
+__device__ __forceinline__ int64_t face_index(const BlockDev& b, int d, int fi, int fj, int fk) {
+    return (int64_t)fi + (int64_t)b.nf[d][0] * ((int64_t)fj + (int64_t)b.nf[d][1] * (int64_t)fk);
+}
+
+// Ghost-cell effects, one thread per (boundary face, layer):
+// internal_copy_then_reflect.d:111-134, flow_state_copy.d:92-107,
+// extrapolate_copy.d:124-140 (xOrder 0).
+__global__ void k_ghost_fill(BlockDev b, double* fs) {
+    const int nsides = (b.dims == 3) ? 6 : 4;
+    int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // Per side: number of faces.
+    int64_t nW = (int64_t)b.njc * b.nkc, nS = (int64_t)b.nic * b.nkc, nB = (int64_t)b.nic * b.njc;
+    int64_t counts[6] = {nW, nW, nS, nS, nB, nB};
+    int side = -1;
+    int64_t r = tid;
+    for (int s = 0; s < nsides; ++s) {
+        int64_t c = counts[s] * NGHOST;
+        if (r < c) { side = s; break; }
+        r -= c;
+    }
+    if (side < 0) return;
+    int layer = (int)(r % NGHOST);
+    int64_t face = r / NGHOST;
+    int i0, j0, k0;  // face location in the side
+    int d;           // index direction normal to the side
+    int fi, fj, fk;  // face index
+    if (side <= 1) {
+        d = 0;
+        j0 = (int)(face % b.njc);
+        k0 = (int)(face / b.njc);
+        fi = (side == 0) ? 0 : b.nic; fj = j0; fk = k0;
+        i0 = 0;
+    } else if (side <= 3) {
+        d = 1;
+        i0 = (int)(face % b.nic);
+        k0 = (int)(face / b.nic);
+        fi = i0; fj = (side == 2) ? 0 : b.njc; fk = k0;
+        j0 = 0;
+    } else {
+        d = 2;
+        i0 = (int)(face % b.nic);
+        j0 = (int)(face / b.nic);
+        fi = i0; fj = j0; fk = (side == 4) ? 0 : b.nkc;
+        k0 = 0;
+    }
+    const bool lo = (side % 2) == 0;
+    const int nd = (d == 0) ? b.nic : (d == 1) ? b.njc : b.nkc;
+    // Ghost position and interior source position along d.
+    int g_at = lo ? (-1 - layer) : (nd + layer);
+    int kind = b.bc_kind[side];
+    int src_at;
+    if (kind == BC_SLIP_WALL) src_at = lo ? layer : (nd - 1 - layer);
+    else src_at = lo ? 0 : (nd - 1);
+    auto at = [&](int s) -> int64_t {
+        if (d == 0) return pidx(b, s, j0, k0);
+        if (d == 1) return pidx(b, i0, s, k0);
+        return pidx(b, i0, j0, s);
+    };
+    int64_t g = at(g_at);
+    const int64_t np = b.npad;
+    if (kind == BC_INFLOW_SUPERSONIC) {
+        for (int f = 0; f < FS_NBASE; ++f) fs[g + f * np] = b.inflow[side][f];
+        return;
+    }
+    if (kind == BC_EXCHANGE) return;
+    int64_t s = at(src_at);
+    for (int f = 0; f < FS_NBASE; ++f) fs[g + f * np] = fs[s + f * np];
+    if (kind == BC_SLIP_WALL) {
+        // ghost_cell.d:33-43
+        FaceGeom fg = load_face(b.fgeom[d], b.nface[d], face_index(b, d, fi, fj, fk));
+        Vec3d v{fs[s + FS_VX * np], fs[s + FS_VY * np], fs[s + FS_VZ * np]};
+        Vec3d l = to_local(v, fg);
+        l.x = -(l.x);
+        Vec3d w = to_global(l, fg);
+        fs[g + FS_VX * np] = w.x;
+        fs[g + FS_VY * np] = w.y;
+        fs[g + FS_VZ * np] = w.z;
+    }
+}
+
+__device__ __forceinline__ StencilCell load_sc(const double* fs, int64_t np, int64_t c) {
+    StencilCell s;
+    s.rho = fs[c + FS_RHO * np];
+    s.u = fs[c + FS_U * np];
+    s.vel.x = fs[c + FS_VX * np];
+    s.vel.y = fs[c + FS_VY * np];
+    s.vel.z = fs[c + FS_VZ * np];
+    return s;
+}
+
+// One thread per face of index direction d: reconstruction + flux calculator, or the
+// simple-outflow boundary flux effect on faces whose BC computes the flux itself.
+template <int DIM, int FC>
+__global__ void k_face_flux(BlockDev b, IdealGasDev gas, int d, double eps) {
+    using C = CQ<DIM>;
+    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= b.nface[d]) return;
+    int fi = (int)(f % b.nf[d][0]);
+    int64_t rest = f / b.nf[d][0];
+    int fj = (int)(rest % b.nf[d][1]);
+    int fk = (int)(rest / b.nf[d][1]);
+    const double* fs = b.fs_in;
+    const int64_t np = b.npad;
+    int pos = (d == 0) ? fi : (d == 1) ? fj : fk;
+    int nd = (d == 0) ? b.nic : (d == 1) ? b.njc : b.nkc;
+    auto at = [&](int s) -> int64_t {
+        if (d == 0) return pidx(b, s, fj, fk);
+        if (d == 1) return pidx(b, fi, s, fk);
+        return pidx(b, fi, fj, s);
+    };
+    FaceGeom g = load_face(b.fgeom[d], b.nface[d], f);
+    double F[C::n];
+    int side = -1;
+    if (pos == 0) side = 2 * d;
+    else if (pos == nd) side = 2 * d + 1;
+    double* out = b.flux[d];
+    const int64_t nfc = b.nface[d];
+    if (side >= 0 && b.bc_kind[side] == BC_OUTFLOW_SIMPLE_FLUX) {
+        int outsign = (side % 2 == 0) ? -1 : 1;
+        int64_t c = at(side % 2 == 0 ? 0 : nd - 1);
+        Vec3d v{fs[c + FS_VX * np], fs[c + FS_VY * np], fs[c + FS_VZ * np]};
+        outflow_flux<DIM>(g, outsign, fs[c + FS_RHO * np], fs[c + FS_U * np], fs[c + FS_P * np], v, F);
+    } else {
+        int64_t cL1 = at(pos - 2), cL0 = at(pos - 1), cR0 = at(pos), cR1 = at(pos + 1);
+        const double* len = b.len + d * np;
+        StencilCell L1 = load_sc(fs, np, cL1), L0 = load_sc(fs, np, cL0), R0 = load_sc(fs, np, cR0),
+                    R1 = load_sc(fs, np, cR1);
+        face_flux_l2r2<DIM, FC>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cL0 + FS_P * np],
+                                fs[cL0 + FS_T * np], fs[cR0 + FS_A * np], fs[cR0 + FS_P * np], fs[cR0 + FS_T * np],
+                                len[cL1], len[cL0], len[cR0], len[cR1], eps, F);
+    }
+    for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
+}
+
+// Per interior cell: time derivatives (fvcell.d:824-854), the pc stage update
+// (simcore_gasdynamic_step.d:1225-1321), decode (fvcell.d:586-821) and the
+// invalid-cell check (fluidblock.d:607-676).
+template <int DIM, int STAGE>
+__global__ void k_cell_update(BlockDev b, IdealGasDev gas, double dt, double low_T, double max_vel, double min_T,
+                              double max_T) {
+    using C = CQ<DIM>;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ncell = (int64_t)b.nic * b.njc * b.nkc;
+    if (t >= ncell) return;
+    int i = (int)(t % b.nic);
+    int64_t rest = t / b.nic;
+    int j = (int)(rest % b.njc);
+    int k = (int)(rest / b.njc);
+    const int64_t np = b.npad;
+    int64_t c = pidx(b, i, j, k);
+    // Faces in the order W, E, S, N, B, T with outsigns -1, +1.
+    const int nfaces = (DIM == 3) ? 6 : 4;
+    int64_t fidx[6];
+    int fdir[6];
+    fidx[0] = face_index(b, 0, i, j, k); fdir[0] = 0;
+    fidx[1] = face_index(b, 0, i + 1, j, k); fdir[1] = 0;
+    fidx[2] = face_index(b, 1, i, j, k); fdir[2] = 1;
+    fidx[3] = face_index(b, 1, i, j + 1, k); fdir[3] = 1;
+    if (DIM == 3) {
+        fidx[4] = face_index(b, 2, i, j, k); fdir[4] = 2;
+        fidx[5] = face_index(b, 2, i, j, k + 1); fdir[5] = 2;
+    }
+    double area[6];
+    for (int q = 0; q < nfaces; ++q) {
+        double A = b.fgeom[fdir[q]][fidx[q] + 9 * b.nface[fdir[q]]];
+        area[q] = ((q % 2 == 0) ? -1 : 1) * A;
+    }
+    double vol_inv = 1.0 / b.vol[c];
+    double U[C::n];
+    for (int jq = 0; jq < C::n; ++jq) {
+        double si = 0.0;
+        for (int q = 0; q < nfaces; ++q) si -= b.flux[fdir[q]][fidx[q] + jq * b.nface[fdir[q]]] * area[q];
+        double dUdt = vol_inv * si + 0.0;
+        double u0 = b.U0[c + jq * np];
+        if (STAGE == 1) {
+            const double gamma_1 = 1.0;
+            b.dUdt0[c + jq * np] = dUdt;
+            U[jq] = u0 + dt * gamma_1 * dUdt;
+        } else {
+            const double gamma_1 = 0.5, gamma_2 = 0.5;
+            U[jq] = u0 + dt * (gamma_1 * b.dUdt0[c + jq * np] + gamma_2 * dUdt);
+        }
+    }
+    double rho, u, p, T, a;
+    Vec3d vel;
+    bool reenc;
+    int bad = decode_ideal<DIM>(gas, U, low_T, rho, vel, u, p, T, a, reenc);
+    if (bad) {
+        atomicAdd(&b.counters[0], 1);
+        return;
+    }
+    if (STAGE == 2)
+        for (int jq = 0; jq < C::n; ++jq) b.Unew[c + jq * np] = U[jq];
+    double* fo = b.fs_out;
+    fo[c + FS_RHO * np] = rho;
+    fo[c + FS_VX * np] = vel.x;
+    fo[c + FS_VY * np] = vel.y;
+    fo[c + FS_VZ * np] = vel.z;
+    fo[c + FS_U * np] = u;
+    fo[c + FS_P * np] = p;
+    fo[c + FS_T * np] = T;
+    fo[c + FS_A * np] = a;
+    if (!check_data(rho, p, T, a, vel, max_vel, min_T, max_T)) atomicAdd(&b.counters[1], 1);
+}
+
+// Cell signal frequency (fvcell.d:975-1058) and per-workgroup reductions for the
+// block time-step estimate (fluidblock.d:987-1084).  part[3*wg] = {min dt_local,
+// max cfl_local, -}; the last interior cell writes its signal to last_signal.
+template <int DIM>
+__global__ void k_dt_partial(BlockDev b, double dt_current, double cfl_value, double* part, double* last_signal) {
+    __shared__ double sdt[256], scfl[256];
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ncell = (int64_t)b.nic * b.njc * b.nkc;
+    double dtl = __builtin_huge_val(), cfl = -__builtin_huge_val();
+    bool have = t < ncell;
+    if (have) {
+        int i = (int)(t % b.nic);
+        int64_t rest = t / b.nic;
+        int j = (int)(rest % b.njc);
+        int k = (int)(rest / b.njc);
+        const int64_t np = b.npad;
+        int64_t c = pidx(b, i, j, k);
+        const double* fs = b.fs_in;
+        Vec3d v{fs[c + FS_VX * np], fs[c + FS_VY * np], fs[c + FS_VZ * np]};
+        double a = fs[c + FS_A * np];
+        FaceGeom gN = load_face(b.fgeom[1], b.nface[1], face_index(b, 1, i, j + 1, k));
+        FaceGeom gE = load_face(b.fgeom[0], b.nface[0], face_index(b, 0, i + 1, j, k));
+        double un_N = fabs(v.x * gN.n.x + v.y * gN.n.y + v.z * gN.n.z);
+        double un_E = fabs(v.x * gE.n.x + v.y * gE.n.y + v.z * gE.n.z);
+        double signal = 0.0;
+        double signalN = (un_N + a) / b.len[c + 1 * np];
+        signal = fmax(signal, signalN);
+        double signalE = (un_E + a) / b.len[c + 0 * np];
+        signal = fmax(signal, signalE);
+        if (DIM == 3) {
+            FaceGeom gT = load_face(b.fgeom[2], b.nface[2], face_index(b, 2, i, j, k + 1));
+            double un_T = fabs(v.x * gT.n.x + v.y * gT.n.y + v.z * gT.n.z);
+            double signalT = (un_T + a) / b.len[c + 2 * np];
+            signal = fmax(signal, signalT);
+        }
+        cfl = dt_current * signal;
+        dtl = cfl_value / signal;
+        if (t == ncell - 1) *last_signal = signal;
+    }
+    sdt[threadIdx.x] = dtl;
+    scfl[threadIdx.x] = cfl;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sdt[threadIdx.x] = fmin(sdt[threadIdx.x], sdt[threadIdx.x + s]);
+            scfl[threadIdx.x] = fmax(scfl[threadIdx.x], scfl[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sdt[0];
+        part[2 * blockIdx.x + 1] = scfl[0];
+    }
+}
+
+__global__ void k_dt_final(const double* part, int nparts, double* out) {
+    __shared__ double sdt[256], scfl[256];
+    double dtl = __builtin_huge_val(), cfl = -__builtin_huge_val();
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+        dtl = fmin(dtl, part[2 * p]);
+        cfl = fmax(cfl, part[2 * p + 1]);
+    }
+    sdt[threadIdx.x] = dtl;
+    scfl[threadIdx.x] = cfl;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sdt[threadIdx.x] = fmin(sdt[threadIdx.x], sdt[threadIdx.x + s]);
+            scfl[threadIdx.x] = fmax(scfl[threadIdx.x], scfl[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sdt[0];
+        out[1] = scfl[0];
+    }
+}
+
+// Encode U from the flow state (fvcell.d:511-583), single species: used at start-up.
+template <int DIM>
+__global__ void k_encode(BlockDev b, double* U) {
+    using C = CQ<DIM>;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ncell = (int64_t)b.nic * b.njc * b.nkc;
+    if (t >= ncell) return;
+    int i = (int)(t % b.nic);
+    int64_t rest = t / b.nic;
+    int j = (int)(rest % b.njc);
+    int k = (int)(rest / b.njc);
+    const int64_t np = b.npad;
+    int64_t c = pidx(b, i, j, k);
+    const double* fs = b.fs_in;
+    double rho = fs[c + FS_RHO * np];
+    double vx = fs[c + FS_VX * np], vy = fs[c + FS_VY * np], vz = fs[c + FS_VZ * np];
+    U[c + C::mass * np] = rho;
+    U[c + C::xMom * np] = rho * vx;
+    U[c + C::yMom * np] = rho * vy;
+    if (DIM == 3) U[c + C::zMom * np] = rho * vz;
+    double u = fs[c + FS_U * np];
+    double ke = 0.5 * (vx * vx + vy * vy + vz * vz);
+    U[c + C::totEnergy * np] = rho * (u + ke);
+}
+
+// Decode U into the flow state without a stage update (start-up and step retries).
+template <int DIM>
+__global__ void k_decode(BlockDev b, IdealGasDev gas, double* U, double* fs, double low_T) {
+    using C = CQ<DIM>;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t ncell = (int64_t)b.nic * b.njc * b.nkc;
+    if (t >= ncell) return;
+    int i = (int)(t % b.nic);
+    int64_t rest = t / b.nic;
+    int j = (int)(rest % b.njc);
+    int k = (int)(rest / b.njc);
+    const int64_t np = b.npad;
+    int64_t c = pidx(b, i, j, k);
+    double Uc[C::n];
+    for (int q = 0; q < C::n; ++q) Uc[q] = U[c + q * np];
+    double rho, u, p, T, a;
+    Vec3d vel;
+    bool reenc;
+    if (decode_ideal<DIM>(gas, Uc, low_T, rho, vel, u, p, T, a, reenc)) {
+        atomicAdd(&b.counters[0], 1);
+        return;
+    }
+    if (reenc)
+        for (int q = 0; q < C::n; ++q) U[c + q * np] = Uc[q];
+    fs[c + FS_RHO * np] = rho;
+    fs[c + FS_VX * np] = vel.x;
+    fs[c + FS_VY * np] = vel.y;
+    fs[c + FS_VZ * np] = vel.z;
+    fs[c + FS_U * np] = u;
+    fs[c + FS_P * np] = p;
+    fs[c + FS_T * np] = T;
+    fs[c + FS_A * np] = a;
+}
+
+//---------------------------------------------------------------------------------
+// Launchers
+
+static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_ghost_fill(const BlockDev& b, double* fs, hipStream_t s) {
+    int64_t nW = (int64_t)b.njc * b.nkc, nS = (int64_t)b.nic * b.nkc, nB = (int64_t)b.nic * b.njc;
+    int64_t n = 2 * (nW + nS) * NGHOST + ((b.dims == 3) ? 2 * nB * NGHOST : 0);
+    hipLaunchKernelGGL(k_ghost_fill, dim3(nblk(n, 256)), dim3(256), 0, s, b, fs);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_flux(const BlockDev& b, const IdealGasDev& gas, int flux_calc, double eps, hipStream_t s) {
+    int nd = (b.dims == 3) ? 3 : 2;
+    for (int d = 0; d < nd; ++d) {
+        dim3 grid(nblk(b.nface[d], 256)), blk(256);
+        if (b.dims == 3) {
+            if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<3, 1>), grid, blk, 0, s, b, gas, d, eps);
+            else hipLaunchKernelGGL((k_face_flux<3, 0>), grid, blk, 0, s, b, gas, d, eps);
+        } else {
+            if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<2, 1>), grid, blk, 0, s, b, gas, d, eps);
+            else hipLaunchKernelGGL((k_face_flux<2, 0>), grid, blk, 0, s, b, gas, d, eps);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_update(const BlockDev& b, const IdealGasDev& gas, int stage, double dt, const CellLimits& lim,
+                              hipStream_t s) {
+    int64_t n = (int64_t)b.nic * b.njc * b.nkc;
+    dim3 grid(nblk(n, 256)), blk(256);
+    if (b.dims == 3) {
+        if (stage == 1) hipLaunchKernelGGL((k_cell_update<3, 1>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
+        else hipLaunchKernelGGL((k_cell_update<3, 2>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
+    } else {
+        if (stage == 1) hipLaunchKernelGGL((k_cell_update<2, 1>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
+        else hipLaunchKernelGGL((k_cell_update<2, 2>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
+    }
+    return hipGetLastError();
+}
+
+int dt_partial_count(const BlockDev& b) { return (int)nblk((int64_t)b.nic * b.njc * b.nkc, 256); }
+
+hipError_t launch_dt(const BlockDev& b, double dt_current, double cfl, double* part, double* out3, hipStream_t s) {
+    int64_t n = (int64_t)b.nic * b.njc * b.nkc;
+    int np = (int)nblk(n, 256);
+    if (b.dims == 3) hipLaunchKernelGGL((k_dt_partial<3>), dim3(np), dim3(256), 0, s, b, dt_current, cfl, part, out3 + 2);
+    else hipLaunchKernelGGL((k_dt_partial<2>), dim3(np), dim3(256), 0, s, b, dt_current, cfl, part, out3 + 2);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, part, np, out3);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode(const BlockDev& b, double* U, hipStream_t s) {
+    int64_t n = (int64_t)b.nic * b.njc * b.nkc;
+    if (b.dims == 3) hipLaunchKernelGGL((k_encode<3>), dim3(nblk(n, 256)), dim3(256), 0, s, b, U);
+    else hipLaunchKernelGGL((k_encode<2>), dim3(nblk(n, 256)), dim3(256), 0, s, b, U);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode(const BlockDev& b, const IdealGasDev& gas, double* U, double* fs, double low_T, hipStream_t s) {
+    int64_t n = (int64_t)b.nic * b.njc * b.nkc;
+    if (b.dims == 3) hipLaunchKernelGGL((k_decode<3>), dim3(nblk(n, 256)), dim3(256), 0, s, b, gas, U, fs, low_T);
+    else hipLaunchKernelGGL((k_decode<2>), dim3(nblk(n, 256)), dim3(256), 0, s, b, gas, U, fs, low_T);
+    return hipGetLastError();
+}
+
+}  // namespace gdtk

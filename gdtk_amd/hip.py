@@ -1,0 +1,179 @@
+"""ctypes host over libgdtkhip.so, mirroring the reference's simulation/block API.
+
+The class layout follows the reference's control flow: a Simulation owns the
+GlobalConfig subset and SimState (src/eilmer/simcore.d), blocks are added from a grid
+and an initial flow state (src/eilmer/sfluidblock.d, simcore.d:326-330), and run()
+is the time loop (simcore.d:1014-1377).  Every call goes to the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(HERE, "libgdtkhip.so")
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+GDTK_OK = 0
+ERR_NAMES = {-1: "GDTK_ERR_ARG", -2: "GDTK_ERR_HIP", -3: "GDTK_ERR_BAD_CELL", -4: "GDTK_ERR_TOO_MANY_BAD_CELLS",
+             -5: "GDTK_ERR_STEP_FAILED", -6: "GDTK_ERR_UNSUPPORTED"}
+
+FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11)
+
+# Every symbol the C ABI exports (include/gdtk_hip.h).
+EXPORTS = [
+    "gdtk_hip_sim_create", "gdtk_hip_sim_destroy", "gdtk_hip_sim_add_block", "gdtk_hip_block_set_initial",
+    "gdtk_hip_block_get_field", "gdtk_hip_stage", "gdtk_hip_block_dt", "gdtk_hip_sim_run",
+    "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
+    "gdtk_hip_last_error",
+]
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SimConfig(C.Structure):
+    _fields_ = [("dimensions", C.c_int), ("flux_calculator", C.c_int), ("interpolation_order", C.c_int),
+                ("cfl_value", C.c_double), ("dt_init", C.c_double), ("dt_max", C.c_double),
+                ("cfl_count", C.c_int), ("max_attempts_for_step", C.c_int), ("max_invalid_cells", C.c_int),
+                ("mMass", C.c_double), ("gamma", C.c_double), ("use_fused", C.c_int)]
+
+
+class BlockDesc(C.Structure):
+    _fields_ = [("nic", C.c_int), ("njc", C.c_int), ("nkc", C.c_int), ("vertices", dp),
+                ("bc_kind", C.c_int * 6), ("inflow", (C.c_double * 5) * 6),
+                ("other_block", C.c_int * 6), ("other_face", C.c_int * 6)]
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise HipError(-2, f"{_LIB_PATH} is not built; run gdtk_amd.build (hipcc, gfx950)")
+        L = C.CDLL(_LIB_PATH)
+        L.gdtk_hip_last_error.restype = C.c_char_p
+        L.gdtk_hip_sim_create.argtypes = [C.POINTER(SimConfig), ip]
+        L.gdtk_hip_sim_destroy.argtypes = [C.c_int]
+        L.gdtk_hip_sim_add_block.argtypes = [C.c_int, C.POINTER(BlockDesc), ip]
+        L.gdtk_hip_block_set_initial.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        L.gdtk_hip_block_get_field.argtypes = [C.c_int, C.c_int, C.c_int, dp]
+        L.gdtk_hip_stage.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.gdtk_hip_block_dt.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp]
+        L.gdtk_hip_sim_run.argtypes = [C.c_int, C.c_int, C.c_double, dp]
+        L.gdtk_hip_sim_step_fixed_dt.argtypes = [C.c_int, C.c_double, dp]
+        L.gdtk_hip_sim_set_dt.argtypes = [C.c_int, C.c_double, C.c_int]
+        L.gdtk_hip_sim_profile.argtypes = [C.c_int, C.c_int]
+        L.gdtk_hip_sim_kernel_stats.argtypes = [C.c_int, dp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != GDTK_OK:
+        raise HipError(rc, lib().gdtk_hip_last_error().decode())
+    return rc
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(dp)
+
+
+class Simulation:
+    """A set of structured blocks advanced by the HIP explicit update."""
+
+    def __init__(self, dims, flux="ausmdv", cfl=0.5, dt_init=1.0e-3, dt_max=1.0e-3, cfl_count=10,
+                 max_attempts_for_step=3, max_invalid_cells=0, mMass=0.02896, gamma=1.4, fused=True):
+        self.dims = dims
+        self.ncq = 5 if dims == 3 else 4
+        cfg = SimConfig(dims, {"ausmdv": 0, "efm": 1}[flux], 2, cfl, dt_init, dt_max, cfl_count,
+                        max_attempts_for_step, max_invalid_cells, mMass, gamma, 1 if fused else 0)
+        h = C.c_int(-1)
+        _check(lib().gdtk_hip_sim_create(C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        self.cases = []
+
+    def close(self):
+        if getattr(self, "h", -1) >= 0 and _lib is not None:
+            _lib.gdtk_hip_sim_destroy(self.h)
+            self.h = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_block(self, case) -> int:
+        v, vp = _d(case.vertices.reshape(-1))
+        d = BlockDesc()
+        d.nic, d.njc, d.nkc = case.nic, case.njc, case.nkc
+        d.vertices = vp
+        for s in range(6):
+            d.bc_kind[s] = int(case.bc[s])
+            d.other_block[s] = int(case.other_block[s])
+            d.other_face[s] = int(case.other_face[s])
+            for q in range(5):
+                d.inflow[s][q] = float(case.inflow[s][q])
+        bid = C.c_int(-1)
+        _check(lib().gdtk_hip_sim_add_block(self.h, C.byref(d), C.byref(bid)))
+        arrs = [_d(case.p), _d(case.T), _d(case.vel[:, 0]), _d(case.vel[:, 1]), _d(case.vel[:, 2])]
+        _check(lib().gdtk_hip_block_set_initial(self.h, bid.value, *[a[1] for a in arrs]))
+        self.cases.append(case)
+        return bid.value
+
+    def get(self, b, name):
+        f = FIELDS[name] if isinstance(name, str) else int(name)
+        out = np.empty(self.cases[b].ncells)
+        _check(lib().gdtk_hip_block_get_field(self.h, b, f, out.ctypes.data_as(dp)))
+        return out
+
+    def get_U(self, b):
+        return np.stack([self.get(b, 100 + j) for j in range(self.ncq)], axis=1)
+
+    def stage(self, stage, dt):
+        return lib().gdtk_hip_stage(self.h, stage, dt)
+
+    def block_dt(self, b, dt_current, cfl, check_cfl):
+        out = np.zeros(3)
+        _check(lib().gdtk_hip_block_dt(self.h, b, dt_current, cfl, 1 if check_cfl else 0, out.ctypes.data_as(dp)))
+        return out
+
+    @staticmethod
+    def _state(st):
+        return dict(time=st[0], dt=st[1], step=int(st[2]), cfl_max=st[3], dt_allow=st[4], attempts=int(st[5]))
+
+    def run(self, max_steps, max_time=1.0e30):
+        st = np.zeros(6)
+        rc = lib().gdtk_hip_sim_run(self.h, max_steps, max_time, st.ctypes.data_as(dp))
+        if rc < 0:
+            _check(rc)
+        return rc, self._state(st)
+
+    def step_fixed_dt(self, dt):
+        st = np.zeros(6)
+        _check(lib().gdtk_hip_sim_step_fixed_dt(self.h, dt, st.ctypes.data_as(dp)))
+        return self._state(st)
+
+    def set_dt(self, dt_global, step):
+        _check(lib().gdtk_hip_sim_set_dt(self.h, dt_global, step))
+
+    def profile(self, enable=True):
+        _check(lib().gdtk_hip_sim_profile(self.h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        out = np.zeros(3)
+        _check(lib().gdtk_hip_sim_kernel_stats(self.h, out.ctypes.data_as(dp), 3))
+        return dict(launches=int(out[0]), ms=out[1], cells_per_launch=out[2])

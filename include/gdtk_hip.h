@@ -1,0 +1,133 @@
+/* gdtk_hip.h -- C ABI of the MI355X (gfx950) structured-block explicit update.
+ *
+ * This is the drop-in boundary for Eilmer 4's fixed-grid explicit transient update.
+ * The entry points are the per-block and per-step operations that the reference's
+ * time-marching core performs on its blocks, so that a D host (or any FFI) can keep
+ * its own prep, I/O and control flow and hand the hot loop to the GPU.
+ *
+ * Conventions follow the reference's own C ABI, src/gas/gas_cwrap.d:33-80, 892-921:
+ * integer handles into library-owned tables, int status returns (0 success, negative
+ * failure), no exceptions across the ABI, and a retrievable message for the last error.
+ * Arrays are plain host pointers in the reference's cell order
+ * (k*njc + j)*nic + i (src/eilmer/sfluidblock.d:299-312) and vertex order
+ * (k*njv + j)*niv + i (sfluidblock.d:315-328).  All arithmetic is IEEE FP64.
+ */
+#ifndef GDTK_HIP_H
+#define GDTK_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define GDTK_OK 0
+#define GDTK_ERR_ARG (-1)             /* bad handle or argument */
+#define GDTK_ERR_HIP (-2)             /* HIP runtime error */
+#define GDTK_ERR_BAD_CELL (-3)        /* decode failed in some cell: the reference's step_failed */
+#define GDTK_ERR_TOO_MANY_BAD_CELLS (-4) /* count_invalid_cells > max_invalid_cells (fatal) */
+#define GDTK_ERR_STEP_FAILED (-5)     /* step failed after max_attempts_for_step */
+#define GDTK_ERR_UNSUPPORTED (-6)
+
+/* Flux calculators, a subset of src/eilmer/globalconfig.d:270-290. */
+#define GDTK_FLUX_AUSMDV 0
+#define GDTK_FLUX_EFM 1
+
+/* Boundary conditions (src/eilmer/bc.lua:757, 1320, 1580, 1601, 1650). */
+#define GDTK_BC_SLIP_WALL 0
+#define GDTK_BC_INFLOW_SUPERSONIC 1
+#define GDTK_BC_OUTFLOW_SIMPLE_FLUX 2
+#define GDTK_BC_OUTFLOW_SIMPLE_EXTRAPOLATE 3
+#define GDTK_BC_EXCHANGE 4
+
+/* Flow-state fields for gdtk_hip_block_get_field: the FlowState/GasState members. */
+#define GDTK_FIELD_RHO 0
+#define GDTK_FIELD_VX 1
+#define GDTK_FIELD_VY 2
+#define GDTK_FIELD_VZ 3
+#define GDTK_FIELD_U 4
+#define GDTK_FIELD_P 5
+#define GDTK_FIELD_T 6
+#define GDTK_FIELD_A 7
+#define GDTK_FIELD_VOLUME 8
+#define GDTK_FIELD_ILENGTH 9
+#define GDTK_FIELD_JLENGTH 10
+#define GDTK_FIELD_KLENGTH 11
+#define GDTK_FIELD_U0 100   /* + conserved-quantity index: U[0][j] */
+#define GDTK_FIELD_DUDT0 200 /* + conserved-quantity index: dUdt[0][j] */
+
+/* Simulation-wide options: the subset of GlobalConfig (src/eilmer/globalconfig.d:936-1294)
+ * read on this path.  Gas: ideal gas from mMass and gamma (src/gas/ideal_gas.d:30-72). */
+typedef struct gdtk_sim_config {
+    int dimensions;            /* 2 or 3 */
+    int flux_calculator;       /* GDTK_FLUX_* */
+    int interpolation_order;   /* 2 (l2r2); the only order implemented */
+    double cfl_value;          /* 0.5 */
+    double dt_init;            /* 1e-3 */
+    double dt_max;             /* 1e-3 */
+    int cfl_count;             /* 10 */
+    int max_attempts_for_step; /* 3 */
+    int max_invalid_cells;     /* 0 */
+    double mMass;              /* kg/mol */
+    double gamma;
+    int use_fused;             /* 1: fused LDS-tiled stage kernel where the block shape allows */
+} gdtk_sim_config;
+
+/* One structured block (src/eilmer/sfluidblock.d:68-110). */
+typedef struct gdtk_block_desc {
+    int nic, njc, nkc;         /* nkc = 1 in 2-D */
+    const double* vertices;    /* niv*njv*nkv x 3 coordinates */
+    int bc_kind[6];            /* per face west, east, south, north, bottom, top */
+    double inflow[6][5];       /* p, T, vx, vy, vz of an inflow face (FlowState(p,T,vel)) */
+    int other_block[6];        /* exchange faces: neighbour block id and face */
+    int other_face[6];
+} gdtk_block_desc;
+
+/* Replaces the GlobalConfig/SimState set-up of init_simulation (src/eilmer/simcore.d:83-720). */
+int gdtk_hip_sim_create(const gdtk_sim_config* cfg, int* sim);
+int gdtk_hip_sim_destroy(int sim);
+
+/* Replaces SFluidBlock construction and compute_primary_cell_geometric_data
+ * (sfluidblock.d:443-1000): builds the block on the device from its vertices. */
+int gdtk_hip_sim_add_block(int sim, const gdtk_block_desc* desc, int* block_id);
+
+/* Replaces the initial FlowState fill plus encode/decode of init_simulation
+ * (simcore.d:326-330): FlowState(p, T, vel) per cell, U[0] = encode(fs), fs = decode(U[0]). */
+int gdtk_hip_block_set_initial(int sim, int block, const double* p, const double* T, const double* vx,
+                               const double* vy, const double* vz);
+
+/* Copies one field of all active cells to host memory (GDTK_FIELD_*). */
+int gdtk_hip_block_get_field(int sim, int block, int field, double* out);
+
+/* One stage of the explicit update over all blocks (simcore_gasdynamic_step.d:946-1426):
+ * exchange, ghost-cell effects, convective_flux_phase0 (sfluidblock.d:2188), boundary flux
+ * effects, time_derivatives, the pc stage update and decode_conserved.  Returns
+ * GDTK_ERR_BAD_CELL where the reference would set step_failed. */
+int gdtk_hip_stage(int sim, int stage, double dt);
+
+/* FluidBlock.determine_time_step_size (fluidblock.d:987-1084): out = {dt_allow, cfl_max, 0}. */
+int gdtk_hip_block_dt(int sim, int block, double dt_current, double cfl_value, int check_cfl, double* out);
+
+/* determine_time_step_size (simcore_gasdynamic_step.d:60-159) followed by
+ * gasdynamic_explicit_increment_with_fixed_grid (simcore_gasdynamic_step.d:906-1575),
+ * steps until max_steps or max_time (simcore.d:1014-1377).
+ * state = {time, dt_global, step, cfl_max, dt_allow, attempts_of_last_step}. */
+int gdtk_hip_sim_run(int sim, int max_steps, double max_time, double* state);
+
+/* One gasdynamic step at a fixed dt (no dt control), with the reference's retries. */
+int gdtk_hip_sim_step_fixed_dt(int sim, double dt, double* state);
+
+/* Sets SimState.dt_global and SimState.step (for restarts and tests). */
+int gdtk_hip_sim_set_dt(int sim, double dt_global, int step);
+
+/* Kernel timing on the library's stream: enable collects HIP events around every stage
+ * kernel; stats = {n_stage_launches, total_stage_ms, n_cells_per_launch, ...}. */
+int gdtk_hip_sim_profile(int sim, int enable);
+int gdtk_hip_sim_kernel_stats(int sim, double* stats, int nstats);
+
+/* Message of the last failing call on this thread. */
+const char* gdtk_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDTK_HIP_H */

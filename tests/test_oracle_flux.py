@@ -1,0 +1,220 @@
+"""Pins of the CPU oracle's flux path against the reference's own tests, closed forms
+and invariants (no GPU).  See the header of oracle/eilmer_oracle.cpp for the map."""
+import math
+
+import numpy as np
+import pytest
+
+from gdtk_amd import cases
+
+R_AIR = 8.31451 / 0.02896
+GAMMA = 1.4
+
+
+def test_ideal_gas_closed_forms(oracle_lib):
+    O = oracle_lib
+    g = O.IdealGas()
+    st = g.update(0, p=1.0e5, T=300.0)
+    assert st["rho"] == pytest.approx(1.0e5 / (R_AIR * 300.0), rel=1e-15)
+    assert st["u"] == pytest.approx(R_AIR / (GAMMA - 1.0) * 300.0, rel=1e-15)
+    assert st["a"] == pytest.approx(math.sqrt(GAMMA * R_AIR * 300.0), rel=1e-15)
+    back = g.update(1, rho=st["rho"], u=st["u"])
+    assert back["T"] == pytest.approx(300.0, rel=1e-14)
+    assert back["p"] == pytest.approx(1.0e5, rel=1e-14)
+    with pytest.raises(RuntimeError):
+        g.update(1, rho=1.0, u=-1.0)
+
+
+def test_l2r2_linear_data_uniform_cells(oracle_lib):
+    """l2r2 is exact for linear data on uniform cells: q_L = q_R = face value."""
+    O = oracle_lib
+    for slope in (0.3, -2.0, 1.0e3):
+        q = [1.0 + slope * x for x in (-1.5, -0.5, 0.5, 1.5)]
+        qL, qR = O.l2r2_scalar([1.0] * 4, q)
+        assert qL == pytest.approx(1.0, rel=1e-12, abs=1e-12)
+        assert qR == pytest.approx(1.0, rel=1e-12, abs=1e-12)
+
+
+def test_l2r2_extremum_and_clipping(oracle_lib):
+    O = oracle_lib
+    # Local maximum at L0: the left slope product is negative, the limiter drops to ~0,
+    # so the reconstruction is first order: q_L = q_L0.
+    qL, qR = O.l2r2_scalar([1.0] * 4, [0.0, 1.0, 0.5, 0.4])
+    assert qL == pytest.approx(1.0, abs=1e-9)
+    # Results always lie within [min(qL0,qR0), max(qL0,qR0)] (extrema clipping).
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        lens = rng.uniform(0.2, 3.0, 4)
+        q = rng.normal(size=4)
+        a, b = O.l2r2_scalar(lens, q)
+        lo, hi = min(q[1], q[2]), max(q[1], q[2])
+        assert lo <= a <= hi and lo <= b <= hi
+    # Without limiting or clipping, on uniform cells the reconstruction is the parabola
+    # through the cell values: samples of x^2 at x = 0,1,2,3 give 1.5^2 at the face.
+    q = [0.0, 1.0, 4.0, 9.0]
+    a, b = O.l2r2_scalar([1.0] * 4, q, apply_limiter=False, extrema_clipping=False)
+    assert a == pytest.approx(2.25, rel=1e-15)
+    assert b == pytest.approx(2.25, rel=1e-15)
+    # Non-uniform cells: the one-sided parabolas through three cell-centre values.
+    lens = [1.0, 2.0, 0.5, 1.5]
+    xc = np.array([-2.0 - 0.5, -1.0, 0.25, 0.5 + 0.75])  # centres relative to the face
+    q = list(3.0 - 2.0 * xc + 0.7 * xc ** 2)
+    a, b = O.l2r2_scalar(lens, q, apply_limiter=False, extrema_clipping=False)
+    assert a == pytest.approx(3.0, rel=1e-13) and b == pytest.approx(3.0, rel=1e-13)
+
+
+def _state(rho, p, vel):
+    T = p / (rho * R_AIR)
+    u = R_AIR / (GAMMA - 1.0) * T
+    a = math.sqrt(GAMMA * R_AIR * T)
+    return [rho, p, T, u, a] + list(vel)
+
+
+def _euler_flux(s):
+    rho, p, T, u, a, vx, vy, vz = s
+    H = u + p / rho + 0.5 * (vx * vx + vy * vy + vz * vz)
+    return np.array([rho * vx, rho * vx * vx + p, rho * vx * vy, rho * vx * vz, rho * vx * H])
+
+
+@pytest.mark.parametrize("vx", [-800.0, -120.0, 0.0, 35.0, 250.0, 900.0])
+def test_ausmdv_consistency(oracle_lib, vx):
+    O = oracle_lib
+    g = O.IdealGas()
+    s = _state(0.9, 8.0e4, (vx, 12.0, -7.0))
+    F = O.flux(g, "ausmdv", 3, s, s)
+    assert np.allclose(F, _euler_flux(s), rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("vx", [-800.0, -120.0, 35.0, 250.0, 900.0])
+def test_efm_consistency(oracle_lib, vx):
+    """EFM(U,U) = Euler flux up to the A&S 7.1.26 erf approximation error (< 1.5e-7)."""
+    O = oracle_lib
+    g = O.IdealGas()
+    s = _state(0.9, 8.0e4, (vx, 12.0, -7.0))
+    F = O.flux(g, "efm", 3, s, s)
+    E = _euler_flux(s)
+    scale = np.array([abs(E[0]) + s[0] * s[4], abs(E[1]) + s[1], abs(E[2]) + s[0] * s[4] * 12.0,
+                      abs(E[3]) + s[0] * s[4] * 7.0, abs(E[4]) + s[0] * s[4] * s[3]])
+    assert np.all(np.abs(F - E) <= 1e-6 * scale)
+
+
+@pytest.mark.parametrize("kind", ["ausmdv", "efm"])
+def test_supersonic_upwinding(oracle_lib, kind):
+    O = oracle_lib
+    g = O.IdealGas()
+    L = _state(0.5, 2.0e4, (3000.0, 10.0, 0.0))
+    R = _state(1.3, 9.0e4, (2500.0, -4.0, 3.0))
+    F = O.flux(g, kind, 3, L, R)
+    E = _euler_flux(L)
+    if kind == "ausmdv":
+        assert np.allclose(F, E, rtol=1e-12, atol=1e-9)
+    else:
+        # EFM keeps an exp(-25)-sized contribution of the far side (EXLIM, fluxcalc.d:1296).
+        assert np.allclose(F, E, rtol=1e-6, atol=1e-9 * np.max(np.abs(E)))
+
+
+@pytest.mark.parametrize("kind", ["ausmdv", "efm"])
+def test_mirror_symmetry(oracle_lib, kind):
+    """Swapping sides and reversing normal velocities reverses the mass/energy flux."""
+    O = oracle_lib
+    g = O.IdealGas()
+    L = _state(0.7, 5.0e4, (120.0, 30.0, 0.0))
+    R = _state(1.1, 9.0e4, (-40.0, -5.0, 0.0))
+    F = O.flux(g, kind, 3, L, R)
+    Lm = R[:5] + [-R[5], R[6], R[7]]
+    Rm = L[:5] + [-L[5], L[6], L[7]]
+    Fm = O.flux(g, kind, 3, Lm, Rm)
+    assert Fm[0] == pytest.approx(-F[0], rel=1e-12)
+    assert Fm[1] == pytest.approx(F[1], rel=1e-12)
+    assert Fm[4] == pytest.approx(-F[4], rel=1e-12)
+
+
+def test_exxef_error_bound(oracle_lib):
+    O = oracle_lib
+    for x in np.linspace(-6.0, 6.0, 241):
+        ex, ef = O.exxef(x)
+        if abs(x) <= 5.0:
+            assert ex == pytest.approx(math.exp(-x * x), rel=1e-15)
+            assert abs(ef - math.erf(x)) < 1.5e-7
+        else:
+            assert ef == math.copysign(1.0, x)
+
+
+def test_sod_tube_reference_probe(oracle_lib):
+    """examples/eilmer/3D/sod-shock-tube/sg/sod-test.rb: 75 +/- 3 steps, probes within 1 %."""
+    O = oracle_lib
+    g = O.IdealGas()
+    case = cases.sod_3d_tube()
+    s = O.Sim(g, 3, "ausmdv")
+    s.add_block(case)
+    n, st = s.run(1000, 0.6e-3)
+    assert abs(st["step"] - 75) < 3
+    xc = cases.cell_centres(case)[:, 0]
+    vals = {k: s.get(0, k) for k in ("rho", "p", "T", "vx")}
+    i78 = np.argsort(np.abs(xc - 0.78))[0]
+    i60 = np.argsort(np.abs(xc - 0.6))[0]
+    ref78 = dict(rho=0.2647, p=30.2e3, T=398.0, vx=293.0)
+    ref60 = dict(rho=0.4271, p=30.2e3, T=247.0, vx=293.0)
+    for k in ref78:
+        assert abs(vals[k][i78] - ref78[k]) / ref78[k] < 1e-2, (k, vals[k][i78])
+        assert abs(vals[k][i60] - ref60[k]) / ref60[k] < 1e-2, (k, vals[k][i60])
+
+
+def _analytic_sod():
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    return np.loadtxt(os.path.join(here, "golden", "sod_analytic.data"))
+
+
+def test_sod_against_analytic(oracle_lib):
+    """Density profile of the 3-D tube at 0.6 ms against the reference's analytic data."""
+    O = oracle_lib
+    ref = _analytic_sod()
+    g = O.IdealGas()
+    case = cases.sod_3d_tube()
+    s = O.Sim(g, 3, "ausmdv")
+    s.add_block(case)
+    s.run(1000, 0.6e-3)
+    xc = cases.cell_centres(case)[:, 0]
+    rho = s.get(0, "rho")
+    col_x, col_rho = 0, 1
+    rho_a = np.interp(xc, ref[:, col_x], ref[:, col_rho])
+    # A captured shock smears over a few cells: the L1 error is a few percent at most.
+    l1 = np.mean(np.abs(rho - rho_a)) / np.mean(rho_a)
+    assert l1 < 0.03, l1
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_closed_box_conservation(oracle_lib, dims):
+    """Slip walls everywhere on a curvilinear grid: total mass and energy are conserved."""
+    O = oracle_lib
+    g = O.IdealGas()
+    base = cases.sod_2d(30, 16) if dims == 2 else cases.sod_3d_tube(24)
+    case = cases.perturb_vertices(base, amp=0.2, seed=3)
+    s = O.Sim(g, dims, "ausmdv")
+    s.add_block(case)
+    vol = s.get(0, "volume")
+    ncq = 5 if dims == 3 else 4
+    U0 = s.get_U(0, ncq)
+    s.run(15)
+    U1 = s.get_U(0, ncq)
+    m0, m1 = np.sum(U0[:, 0] * vol), np.sum(U1[:, 0] * vol)
+    e0, e1 = np.sum(U0[:, -1] * vol), np.sum(U1[:, -1] * vol)
+    assert abs(m1 - m0) / m0 < 1e-13
+    assert abs(e1 - e0) / e0 < 1e-13
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_quiescent_gas_stays_at_rest(oracle_lib, dims):
+    """A uniform gas at rest on a distorted grid stays at rest (geometric closure)."""
+    O = oracle_lib
+    g = O.IdealGas()
+    base = cases.sod_2d(20, 12) if dims == 2 else cases.sod_3d_tube(16)
+    base.p[:] = 1.0e5
+    base.T[:] = 300.0
+    case = cases.perturb_vertices(base, amp=0.3, seed=1)
+    s = O.Sim(g, dims, "ausmdv")
+    s.add_block(case)
+    s.run(5)
+    assert np.max(np.abs(s.get(0, "vx"))) < 1e-9
+    assert np.max(np.abs(s.get(0, "p") - 1.0e5)) < 1e-6
