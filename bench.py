@@ -42,6 +42,18 @@ def stage_bytes_per_cell(dims=3):
     return 8.0 * (s1 + s2) / 2.0
 
 
+def face_bytes_per_face(dims=3):
+    """Algorithmic HBM bytes per face of the face-flux kernel (FP64).
+
+    Each face reads its frame and area (10 doubles) and writes F (ncq doubles).  Its four
+    stencil cells are shared with the neighbouring faces of the same direction, so per
+    face one cell's worth of reconstruction inputs is new: rho, vx, vy, vz, u, a and the
+    length along the direction (7 doubles).
+    """
+    ncq = 5 if dims == 3 else 4
+    return 8.0 * (10 + ncq + 7)
+
+
 def cpu_baseline(seconds_target=15.0):
     """The oracle (C++ restatement of the D path, one thread as in the reference's one-block
     mode) timed on a bounded sample: a 256x128x8 slab of the same Mach-6 box."""
@@ -64,6 +76,32 @@ def cpu_baseline(seconds_target=15.0):
                 sample=f"oracle, 1 thread, 256x128x8 slab of the Mach-6 box, {steps} steps in {el:.1f} s")
 
 
+def measured_traffic(kernel, items):
+    """HBM bytes per launch of the kernel from the committed counter summary, if one exists
+    for this kernel and workload (profiles/*_traffic.json, written from rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "flux_mach6_256x128x128_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("kernel") == kernel and abs(d.get("items_per_launch", 0) - items) < 0.5:
+            return d["hbm_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
+def copy_probe(nbytes=1 << 30):
+    """A plain device copy of known size for HBM counter calibration (FETCH_SIZE/WRITE_SIZE)."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,20 +109,27 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--flux", default="ausmdv", choices=["ausmdv", "efm"])
     ap.add_argument("--size", default="256,128,128")
-    ap.add_argument("--unfused", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a device copy of known size (HBM counter calibration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     nic, njc, nkc = (int(x) for x in args.size.split(","))
     from gdtk_amd import Simulation, cases
     case = cases.mach6_box(nic, njc, nkc, seed=1234)
-    sim = Simulation(3, flux=args.flux, fused=not args.unfused)
+    sim = Simulation(3, flux=args.flux)
     sim.add_block(case)
     print(f"bench: {nic}x{njc}x{nkc} = {case.ncells} cells, flux={args.flux}, warmup {args.warmup}", file=sys.stderr,
           flush=True)
+    # The autotuner (part of set-up, not timed) picks the stage kernels for this block.
+    tune = sim.autotune()
+    print("bench: autotune", tune, file=sys.stderr, flush=True)
     if args.warmup:
         sim.run(args.warmup)
     # Timed region: K whole steps; run() synchronises on the device before returning.
@@ -93,15 +138,21 @@ def main():
     el = time.perf_counter() - t0
     ms_per_step = 1e3 * el / args.steps
     value = case.ncells * args.steps / el
-    # Dominant kernel: the stage kernel, timed with HIP events on the library's stream.
+    # Dominant kernel: timed with HIP events around each of its launches on the library's
+    # stream (the stream every kernel of the library runs on).
     sim.profile(True)
     sim.run(min(10, args.steps))
     ks = sim.kernel_stats()
     sim.profile(False)
-    avg_ms = ks["ms"] / max(ks["launches"], 1)
-    bpc = stage_bytes_per_cell(3)
-    traffic = None
-    achieved = bpc * ks["cells_per_launch"] / (avg_ms * 1e-3) / 1e9
+    dom = max(("face_flux", "fused_stage"), key=lambda k: ks[k]["ms"])
+    kd = ks[dom]
+    avg_ms = kd["ms"] / max(kd["launches"], 1)
+    if dom == "face_flux":
+        bpu, unit_name, kname = face_bytes_per_face(3), "face", "k_face_flux<3,%d>" % (1 if args.flux == "efm" else 0)
+    else:
+        bpu, unit_name, kname = stage_bytes_per_cell(3), "cell", "k_fused_stage<3,%d,*>" % (1 if args.flux == "efm" else 0)
+    achieved = bpu * kd["items_per_launch"] / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(kname, kd["items_per_launch"])
     out = {
         "metric": "cell-updates/s (flux+chem) FP64",
         "value": value,
@@ -116,11 +167,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"Mach-6 3-D box {nic}x{njc}x{nkc}, ideal air, l2r2+van Albada, {args.flux}, pc RK2",
-                   "cells": case.ncells, "flux": args.flux, "fused": not args.unfused, "final_dt": st["dt"]},
+                   "cells": case.ncells, "flux": args.flux, "stage_kernels": tune["best"],
+                   "kchunk": tune["kchunk"], "final_dt": st["dt"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage (fused flux+update)" if not args.unfused else "stage (face flux + cell update)",
-                     "avg_stage_ms": avg_ms, "bytes_per_cell_stage": bpc},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
+                     "avg_launch_ms": avg_ms, "bytes_per_" + unit_name: bpu,
+                     "items_per_launch": kd["items_per_launch"]},
+        "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1)} for k, v in ks.items()
+                    if v["launches"]},
     }
     if not args.no_cpu_baseline:
         try:

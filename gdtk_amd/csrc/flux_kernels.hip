@@ -8,28 +8,10 @@
 
 #include "block.hpp"
 #include "gdtk_device.hpp"
+#include "kernel_common.hpp"
 #include "launchers.hpp"
 
 namespace gdtk {
-
-__device__ __forceinline__ FaceGeom load_face(const double* g, int64_t nface, int64_t f) {
-    FaceGeom r;
-    r.n.x = g[f];
-    r.n.y = g[f + nface];
-    r.n.z = g[f + 2 * nface];
-    r.t1.x = g[f + 3 * nface];
-    r.t1.y = g[f + 4 * nface];
-    r.t1.z = g[f + 5 * nface];
-    r.t2.x = g[f + 6 * nface];
-    r.t2.y = g[f + 7 * nface];
-    r.t2.z = g[f + 8 * nface];
-    r.area = g[f + 9 * nface];
-    return r;
-}
-
-__device__ __forceinline__ int64_t face_index(const BlockDev& b, int d, int fi, int fj, int fk) {
-    return (int64_t)fi + (int64_t)b.nf[d][0] * ((int64_t)fj + (int64_t)b.nf[d][1] * (int64_t)fk);
-}
 
 // Ghost-cell effects, one thread per (boundary face, layer):
 // internal_copy_then_reflect.d:111-134, flow_state_copy.d:92-107,
@@ -107,16 +89,6 @@ __global__ void k_ghost_fill(BlockDev b, double* fs) {
     }
 }
 
-__device__ __forceinline__ StencilCell load_sc(const double* fs, int64_t np, int64_t c) {
-    StencilCell s;
-    s.rho = fs[c + FS_RHO * np];
-    s.u = fs[c + FS_U * np];
-    s.vel.x = fs[c + FS_VX * np];
-    s.vel.y = fs[c + FS_VY * np];
-    s.vel.z = fs[c + FS_VZ * np];
-    return s;
-}
-
 // One thread per face of index direction d: reconstruction + flux calculator, or the
 // simple-outflow boundary flux effect on faces whose BC computes the flux itself.
 template <int DIM, int FC>
@@ -154,9 +126,9 @@ __global__ void k_face_flux(BlockDev b, IdealGasDev gas, int d, double eps) {
         const double* len = b.len + d * np;
         StencilCell L1 = load_sc(fs, np, cL1), L0 = load_sc(fs, np, cL0), R0 = load_sc(fs, np, cR0),
                     R1 = load_sc(fs, np, cR1);
-        face_flux_l2r2<DIM, FC>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cL0 + FS_P * np],
-                                fs[cL0 + FS_T * np], fs[cR0 + FS_A * np], fs[cR0 + FS_P * np], fs[cR0 + FS_T * np],
-                                len[cL1], len[cL0], len[cR0], len[cR1], eps, F);
+        FallbackFS fb{fs, np, cL0, cR0};
+        face_flux_l2r2<DIM, FC>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cR0 + FS_A * np], fb, len[cL1],
+                                len[cL0], len[cR0], len[cR1], eps, F);
     }
     for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
 }
@@ -175,7 +147,6 @@ __global__ void k_cell_update(BlockDev b, IdealGasDev gas, double dt, double low
     int64_t rest = t / b.nic;
     int j = (int)(rest % b.njc);
     int k = (int)(rest / b.njc);
-    const int64_t np = b.npad;
     int64_t c = pidx(b, i, j, k);
     // Faces in the order W, E, S, N, B, T with outsigns -1, +1.
     const int nfaces = (DIM == 3) ? 6 : 4;
@@ -194,42 +165,13 @@ __global__ void k_cell_update(BlockDev b, IdealGasDev gas, double dt, double low
         double A = b.fgeom[fdir[q]][fidx[q] + 9 * b.nface[fdir[q]]];
         area[q] = ((q % 2 == 0) ? -1 : 1) * A;
     }
-    double vol_inv = 1.0 / b.vol[c];
-    double U[C::n];
+    double si[C::n];
     for (int jq = 0; jq < C::n; ++jq) {
-        double si = 0.0;
-        for (int q = 0; q < nfaces; ++q) si -= b.flux[fdir[q]][fidx[q] + jq * b.nface[fdir[q]]] * area[q];
-        double dUdt = vol_inv * si + 0.0;
-        double u0 = b.U0[c + jq * np];
-        if (STAGE == 1) {
-            const double gamma_1 = 1.0;
-            b.dUdt0[c + jq * np] = dUdt;
-            U[jq] = u0 + dt * gamma_1 * dUdt;
-        } else {
-            const double gamma_1 = 0.5, gamma_2 = 0.5;
-            U[jq] = u0 + dt * (gamma_1 * b.dUdt0[c + jq * np] + gamma_2 * dUdt);
-        }
+        double s = 0.0;
+        for (int q = 0; q < nfaces; ++q) s -= b.flux[fdir[q]][fidx[q] + jq * b.nface[fdir[q]]] * area[q];
+        si[jq] = s;
     }
-    double rho, u, p, T, a;
-    Vec3d vel;
-    bool reenc;
-    int bad = decode_ideal<DIM>(gas, U, low_T, rho, vel, u, p, T, a, reenc);
-    if (bad) {
-        atomicAdd(&b.counters[0], 1);
-        return;
-    }
-    if (STAGE == 2)
-        for (int jq = 0; jq < C::n; ++jq) b.Unew[c + jq * np] = U[jq];
-    double* fo = b.fs_out;
-    fo[c + FS_RHO * np] = rho;
-    fo[c + FS_VX * np] = vel.x;
-    fo[c + FS_VY * np] = vel.y;
-    fo[c + FS_VZ * np] = vel.z;
-    fo[c + FS_U * np] = u;
-    fo[c + FS_P * np] = p;
-    fo[c + FS_T * np] = T;
-    fo[c + FS_A * np] = a;
-    if (!check_data(rho, p, T, a, vel, max_vel, min_T, max_T)) atomicAdd(&b.counters[1], 1);
+    update_decode_store<DIM, STAGE>(b, gas, c, si, dt, low_T, max_vel, min_T, max_T);
 }
 
 // Cell signal frequency (fvcell.d:975-1058) and per-workgroup reductions for the
@@ -392,6 +334,19 @@ hipError_t launch_face_flux(const BlockDev& b, const IdealGasDev& gas, int flux_
             if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<2, 1>), grid, blk, 0, s, b, gas, d, eps);
             else hipLaunchKernelGGL((k_face_flux<2, 0>), grid, blk, 0, s, b, gas, d, eps);
         }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_face_flux_dir(const BlockDev& b, const IdealGasDev& gas, int flux_calc, double eps, int d,
+                                hipStream_t s) {
+    dim3 grid(nblk(b.nface[d], 256)), blk(256);
+    if (b.dims == 3) {
+        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<3, 1>), grid, blk, 0, s, b, gas, d, eps);
+        else hipLaunchKernelGGL((k_face_flux<3, 0>), grid, blk, 0, s, b, gas, d, eps);
+    } else {
+        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<2, 1>), grid, blk, 0, s, b, gas, d, eps);
+        else hipLaunchKernelGGL((k_face_flux<2, 0>), grid, blk, 0, s, b, gas, d, eps);
     }
     return hipGetLastError();
 }

@@ -277,12 +277,18 @@ struct StencilCell {
 // cL0/cR0 carry the extra fields the reference copies into Lft/Rght before
 // reconstruction: a (never updated by the rhou EOS call), and p, T for the
 // EOS-failure fallback.
+// Where the face EOS call fails, the reference copies the whole cell state (onedinterp.d:45-68);
+// p and T of the cells L0/R0 are then read from the flow-state planes.
+struct FallbackFS {
+    const double* fs;
+    int64_t np, cL0, cR0;
+};
+
 template <int DIM, int FC>
 __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const FaceGeom& g, const StencilCell& L1,
                                                const StencilCell& L0, const StencilCell& R0, const StencilCell& R1,
-                                               double aL0, double pL0, double TL0, double aR0, double pR0, double TR0,
-                                               double lenL1, double lenL0, double lenR0, double lenR1, double eps,
-                                               double* F) {
+                                               double aL0, double aR0, const FallbackFS& fb, double lenL1,
+                                               double lenL0, double lenR0, double lenR1, double eps, double* F) {
     using C = CQ<DIM>;
     Vec3d vL1 = to_local(L1.vel, g), vL0 = to_local(L0.vel, g), vR0 = to_local(R0.vel, g), vR1 = to_local(R1.vel, g);
     L2R2 c;
@@ -297,10 +303,14 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
     Rs.a = aR0;
     if (!gas.update_rhou(Ls.rho, Ls.u, Ls.T, Ls.p)) {
         // onedinterp.d:45-55: Lft.copy_values_from(cL0.fs), velocity already local.
-        Ls.rho = L0.rho; Ls.u = L0.u; Ls.p = pL0; Ls.T = TL0; Ls.vel = vL0;
+        Ls.rho = L0.rho; Ls.u = L0.u; Ls.vel = vL0;
+        Ls.p = fb.fs[fb.cL0 + FS_P * fb.np];
+        Ls.T = fb.fs[fb.cL0 + FS_T * fb.np];
     }
     if (!gas.update_rhou(Rs.rho, Rs.u, Rs.T, Rs.p)) {
-        Rs.rho = R0.rho; Rs.u = R0.u; Rs.p = pR0; Rs.T = TR0; Rs.vel = vR0;
+        Rs.rho = R0.rho; Rs.u = R0.u; Rs.vel = vR0;
+        Rs.p = fb.fs[fb.cR0 + FS_P * fb.np];
+        Rs.T = fb.fs[fb.cR0 + FS_T * fb.np];
     }
     // Back to the global frame (onedinterp.d:979-987), then into the face frame
     // again for the flux calculator (fluxcalc.d:65-67).
@@ -309,18 +319,12 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
     for (int j = 0; j < C::n; ++j) F[j] = 0.0;
     if (FC == 1) efmflx<DIM>(Ls, Rs, gas.Cv, gas.Cv, F);
     else ausmdv<DIM>(Ls, Rs, F);
-    // fluxcalc.d:159-177 with a zero grid velocity.
-    const double gx = 0.0, gy = 0.0, gz = 0.0;
-    double v_sqr = gx * gx + gy * gy + gz * gz;
-    F[C::totEnergy] += 0.5 * F[C::mass] * v_sqr +
-                       (F[C::xMom] * gx + F[C::yMom] * gy + ((DIM == 3) ? F[C::zMom] * gz : 0.0));
-    F[C::xMom] += gx * F[C::mass];
-    F[C::yMom] += gy * F[C::mass];
-    double fz = 0.0;
-    if (DIM == 3) {
-        F[C::zMom] += gz * F[C::mass];
-        fz = F[C::zMom];
-    }
+    // fluxcalc.d:159-171 add grid-velocity terms, here with gvel = 0: each adds a signed
+    // zero to a flux component.  Every component starts at +0 and is only ever changed by
+    // x + y or x - y, so it is never -0 (a sum is -0 only if both terms are -0), and
+    // adding a signed zero to a value that is not -0 returns that value bit for bit.  The
+    // terms are therefore omitted without changing any result.
+    double fz = (DIM == 3) ? F[C::zMom] : 0.0;
     Vec3d m = to_global(Vec3d{F[C::xMom], F[C::yMom], fz}, g);
     F[C::xMom] = m.x;
     F[C::yMom] = m.y;

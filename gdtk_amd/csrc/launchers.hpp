@@ -18,6 +18,8 @@ struct CellLimits {
 
 hipError_t launch_ghost_fill(const BlockDev& b, double* fs, hipStream_t s);
 hipError_t launch_face_flux(const BlockDev& b, const IdealGasDev& gas, int flux_calc, double eps, hipStream_t s);
+hipError_t launch_face_flux_dir(const BlockDev& b, const IdealGasDev& gas, int flux_calc, double eps, int d,
+                                hipStream_t s);
 hipError_t launch_cell_update(const BlockDev& b, const IdealGasDev& gas, int stage, double dt, const CellLimits& lim,
                               hipStream_t s);
 int dt_partial_count(const BlockDev& b);
@@ -27,6 +29,7 @@ hipError_t launch_decode(const BlockDev& b, const IdealGasDev& gas, double* U, d
                          hipStream_t s);
 // Fused LDS-tiled stage kernel (fused_stage.hip); returns hipErrorNotSupported for
 // shapes it does not handle.
+void set_fused_kchunk(int kc);
 hipError_t launch_fused_stage(const BlockDev& b, const IdealGasDev& gas, int stage, int flux_calc, double dt,
                               double eps, const CellLimits& lim, hipStream_t s);
 

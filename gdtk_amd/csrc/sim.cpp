@@ -77,12 +77,34 @@ struct Sim {
     // SimState (simcore.d)
     double time = 0.0, dt_global = 0.0, dt_allow = 0.0, cfl_max = 0.0;
     int step = 0, attempts_last = 0;
-    // Profiling
+    // Profiling: HIP events around each kernel launch of the stage, per kernel kind
+    // (0 face flux, 1 cell update, 2 fused stage, 3 ghost fill), on the library's stream.
     bool profile = false;
-    double prof_ms = 0.0;
-    long long prof_launches = 0;
-    double prof_cells = 0.0;
+    static constexpr int NKIND = 4;
+    double prof_ms[NKIND] = {0, 0, 0, 0};
+    long long prof_launches[NKIND] = {0, 0, 0, 0};
+    double prof_items[NKIND] = {0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Stage variant chosen by the autotuner: -1 not tuned (use cfg.use_fused), 0 face-per-
+    // thread, 1 fused.
+    int variant = -1;
+
+    template <class L>
+    void timed(int kind, double items, L&& launch) {
+        if (!profile) {
+            HIPCHK(launch());
+            return;
+        }
+        HIPCHK(hipEventRecord(ev0, stream));
+        HIPCHK(launch());
+        HIPCHK(hipEventRecord(ev1, stream));
+        HIPCHK(hipEventSynchronize(ev1));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        prof_ms[kind] += ms;
+        prof_launches[kind] += 1;
+        prof_items[kind] += items;
+    }
 
     ~Sim() {
         blocks.clear();
@@ -276,32 +298,75 @@ struct Sim {
                 }
     }
 
+    bool use_fused() const { return variant >= 0 ? variant == 1 : cfg.use_fused != 0; }
+
     // Launch one stage on every block; returns without synchronising.
     void launch_stage(int stage, double dt) {
         const double eps = 1.0e-12;  // epsilon_van_albada, globalconfig.d:1079
         for (auto& bp : blocks) {
             Block& b = *bp;
             refresh_dev(b);
-            HIPCHK(launch_ghost_fill(b.dev, b.fs[b.cur].p, stream));
-            if (profile) HIPCHK(hipEventRecord(ev0, stream));
-            hipError_t fe = hipErrorNotSupported;
-            if (cfg.use_fused) fe = launch_fused_stage(b.dev, gas, stage, cfg.flux_calculator, dt, eps, lim, stream);
-            if (fe == hipErrorNotSupported) {
-                HIPCHK(launch_face_flux(b.dev, gas, cfg.flux_calculator, eps, stream));
-                HIPCHK(launch_cell_update(b.dev, gas, stage, dt, lim, stream));
+            const double nc = (double)b.ncell();
+            timed(3, nc, [&] { return launch_ghost_fill(b.dev, b.fs[b.cur].p, stream); });
+            if (use_fused()) {
+                timed(2, nc, [&] { return launch_fused_stage(b.dev, gas, stage, cfg.flux_calculator, dt, eps, lim, stream); });
             } else {
-                HIPCHK(fe);
-            }
-            if (profile) {
-                HIPCHK(hipEventRecord(ev1, stream));
-                HIPCHK(hipEventSynchronize(ev1));
-                float ms = 0.0f;
-                HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-                prof_ms += ms;
-                prof_launches += 1;
-                prof_cells += (double)b.ncell();
+                const int nd = (b.dims == 3) ? 3 : 2;
+                for (int d = 0; d < nd; ++d)
+                    timed(0, (double)b.dev.nface[d],
+                          [&] { return launch_face_flux_dir(b.dev, gas, cfg.flux_calculator, eps, d, stream); });
+                timed(1, nc, [&] { return launch_cell_update(b.dev, gas, stage, dt, lim, stream); });
             }
         }
+    }
+
+    // Autotuner: times both stages of every candidate launch shape on the current flow
+    // state (stage 1 reads fs and U0 and writes the other fs copy and dUdt0; stage 2 then
+    // reads the same fs and writes fs_out and U[1]: the state the next step starts from,
+    // fs[cur] and U[0], is untouched) and keeps the fastest.  Candidates: face-per-thread
+    // kernels, and the fused kernel with k chunks of 4..64 planes.
+    std::vector<double> autotune(double dt, int reps) {
+        std::vector<double> result;
+        const int kcs[] = {4, 8, 16, 32, 64};
+        bool saved_profile = profile;
+        profile = false;
+        // Stage 2 on the same input writes only the scratch copies (fs_out, U[1]), so
+        // a stage-1 + stage-2 pair can be timed without advancing the solution.
+        auto time_variant = [&](int var, int kc) {
+            variant = var;
+            if (kc > 0) set_fused_kchunk(kc);
+            launch_stage(1, dt);  // warm-up
+            HIPCHK(hipEventRecord(ev0, stream));
+            for (int r = 0; r < reps; ++r) {
+                launch_stage(1, dt);
+                launch_stage(2, dt);
+            }
+            HIPCHK(hipEventRecord(ev1, stream));
+            HIPCHK(hipEventSynchronize(ev1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            return (double)ms / (2 * reps);
+        };
+        double best = time_variant(0, 0);
+        int best_var = 0, best_kc = 0;
+        result.push_back(0); result.push_back(0); result.push_back(best);
+        if (cfg.dimensions == 3) {
+            for (int kc : kcs) {
+                double t = time_variant(1, kc);
+                result.push_back(1); result.push_back(kc); result.push_back(t);
+                if (t < best) { best = t; best_var = 1; best_kc = kc; }
+            }
+        } else {
+            double t = time_variant(1, 1);
+            result.push_back(1); result.push_back(1); result.push_back(t);
+            if (t < best) { best = t; best_var = 1; best_kc = 1; }
+        }
+        variant = best_var;
+        if (best_var == 1) set_fused_kchunk(best_kc);
+        HIPCHK(hipStreamSynchronize(stream));
+        profile = saved_profile;
+        result.insert(result.begin(), {(double)best_var, (double)best_kc, best});
+        return result;
     }
 
     // Runs one stage on all blocks and reports (decode failures, invalid cells).
@@ -578,9 +643,11 @@ int gdtk_hip_sim_profile(int sim, int enable) {
     return guarded([&] {
         Sim& s = get_sim(sim);
         s.profile = enable != 0;
-        s.prof_ms = 0.0;
-        s.prof_launches = 0;
-        s.prof_cells = 0.0;
+        for (int q = 0; q < Sim::NKIND; ++q) {
+            s.prof_ms[q] = 0.0;
+            s.prof_launches[q] = 0;
+            s.prof_items[q] = 0.0;
+        }
         return GDTK_OK;
     });
 }
@@ -588,9 +655,34 @@ int gdtk_hip_sim_profile(int sim, int enable) {
 int gdtk_hip_sim_kernel_stats(int sim, double* stats, int nstats) {
     return guarded([&] {
         Sim& s = get_sim(sim);
-        double v[3] = {(double)s.prof_launches, s.prof_ms, s.prof_launches ? s.prof_cells / s.prof_launches : 0.0};
-        for (int q = 0; q < nstats && q < 3; ++q) stats[q] = v[q];
+        for (int k = 0; k < Sim::NKIND; ++k) {
+            double v[3] = {(double)s.prof_launches[k], s.prof_ms[k],
+                           s.prof_launches[k] ? s.prof_items[k] / s.prof_launches[k] : 0.0};
+            for (int q = 0; q < 3; ++q)
+                if (3 * k + q < nstats) stats[3 * k + q] = v[q];
+        }
         return GDTK_OK;
+    });
+}
+
+int gdtk_hip_sim_autotune(int sim, double dt, int reps, double* out, int nout) {
+    return guarded([&] {
+        if (reps < 1) throw Error(GDTK_ERR_ARG, "reps must be >= 1");
+        std::vector<double> r = get_sim(sim).autotune(dt, reps);
+        for (int q = 0; q < nout && q < (int)r.size(); ++q) out[q] = r[q];
+        return (int)r.size();
+    });
+}
+
+int gdtk_hip_sim_set_tuning(int sim, int key, int value) {
+    return guarded([&] {
+        (void)get_sim(sim);
+        if (key == GDTK_TUNE_KCHUNK) {
+            if (value < 1 || value > 4096) throw Error(GDTK_ERR_ARG, "k chunk out of range");
+            set_fused_kchunk(value);
+            return GDTK_OK;
+        }
+        throw Error(GDTK_ERR_ARG, "unknown tuning key");
     });
 }
 

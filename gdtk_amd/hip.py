@@ -30,7 +30,7 @@ EXPORTS = [
     "gdtk_hip_sim_create", "gdtk_hip_sim_destroy", "gdtk_hip_sim_add_block", "gdtk_hip_block_set_initial",
     "gdtk_hip_block_get_field", "gdtk_hip_stage", "gdtk_hip_block_dt", "gdtk_hip_sim_run",
     "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
-    "gdtk_hip_last_error",
+    "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error",
 ]
 
 
@@ -76,6 +76,8 @@ def lib():
         L.gdtk_hip_sim_set_dt.argtypes = [C.c_int, C.c_double, C.c_int]
         L.gdtk_hip_sim_profile.argtypes = [C.c_int, C.c_int]
         L.gdtk_hip_sim_kernel_stats.argtypes = [C.c_int, dp, C.c_int]
+        L.gdtk_hip_sim_set_tuning.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.gdtk_hip_sim_autotune.argtypes = [C.c_int, C.c_double, C.c_int, dp, C.c_int]
         _lib = L
     return _lib
 
@@ -173,7 +175,25 @@ class Simulation:
     def profile(self, enable=True):
         _check(lib().gdtk_hip_sim_profile(self.h, 1 if enable else 0))
 
+    def set_kchunk(self, kc):
+        _check(lib().gdtk_hip_sim_set_tuning(self.h, 1, int(kc)))
+
+    KERNEL_KINDS = ("face_flux", "cell_update", "fused_stage", "ghost_fill")
+
     def kernel_stats(self):
-        out = np.zeros(3)
-        _check(lib().gdtk_hip_sim_kernel_stats(self.h, out.ctypes.data_as(dp), 3))
-        return dict(launches=int(out[0]), ms=out[1], cells_per_launch=out[2])
+        out = np.zeros(12)
+        _check(lib().gdtk_hip_sim_kernel_stats(self.h, out.ctypes.data_as(dp), 12))
+        return {k: dict(launches=int(out[3 * q]), ms=out[3 * q + 1], items_per_launch=out[3 * q + 2])
+                for q, k in enumerate(self.KERNEL_KINDS)}
+
+    def autotune(self, dt=None, reps=5):
+        """Time the stage-kernel candidates on the current state and keep the fastest."""
+        if dt is None:
+            dt = 0.25 * self.block_dt(0, 1.0, 0.5, False)[0]
+        out = np.zeros(64)
+        n = lib().gdtk_hip_sim_autotune(self.h, dt, reps, out.ctypes.data_as(dp), 64)
+        if n < 0:
+            _check(n)
+        cands = [dict(variant="fused" if out[q] == 1 else "face", kchunk=int(out[q + 1]), ms=out[q + 2])
+                 for q in range(3, n, 3)]
+        return dict(best="fused" if out[0] == 1 else "face", kchunk=int(out[1]), ms=out[2], candidates=cands)

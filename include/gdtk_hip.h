@@ -119,10 +119,24 @@ int gdtk_hip_sim_step_fixed_dt(int sim, double dt, double* state);
 /* Sets SimState.dt_global and SimState.step (for restarts and tests). */
 int gdtk_hip_sim_set_dt(int sim, double dt_global, int step);
 
-/* Kernel timing on the library's stream: enable collects HIP events around every stage
- * kernel; stats = {n_stage_launches, total_stage_ms, n_cells_per_launch, ...}. */
+/* Kernel timing on the library's stream: enable collects HIP events around every kernel
+ * launch of a stage.  stats holds 3 values {launches, total_ms, items_per_launch} for
+ * each kernel kind in the order face flux (items = faces), cell update (cells), fused
+ * stage (cells), ghost fill (cells): nstats up to 12. */
 int gdtk_hip_sim_profile(int sim, int enable);
 int gdtk_hip_sim_kernel_stats(int sim, double* stats, int nstats);
+
+/* Project autotuner (no reference counterpart): times both stages of each launch-shape
+ * candidate (face-per-thread kernels; fused kernel with k chunks 4..64) on the current
+ * state, keeps the fastest for the rest of the run and returns the number of values
+ * written to out: {best_variant, best_kchunk, best_ms, then (variant, kchunk, ms) per
+ * candidate, ms per stage}.  The flow state and U[0] are unchanged. */
+int gdtk_hip_sim_autotune(int sim, double dt, int reps, double* out, int nout);
+
+/* Launch-shape knobs of the fused stage kernel (chosen by the autotuner):
+ * GDTK_TUNE_KCHUNK = planes in k marched by one workgroup. */
+#define GDTK_TUNE_KCHUNK 1
+int gdtk_hip_sim_set_tuning(int sim, int key, int value);
 
 /* Message of the last failing call on this thread. */
 const char* gdtk_hip_last_error(void);
