@@ -1,0 +1,118 @@
+"""Thermochemistry records of the C ABI (gdtk_tpg_gas, gdtk_mechanism) built from the JSON
+gas-model and mechanism data, and the ctypes host of the chemistry entry points.
+
+The record layout is the one declared in include/gdtk_hip.h.  The data format follows the
+reference's machine-readable gas and reaction files (src/gas/therm_perf_gas.d:40-120,
+src/kinetics/reaction.d:460-540).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+MAX_SPECIES = 8
+MAX_REACTIONS = 32
+ODE_ALPHA_QSS = 0
+ODE_RKF = 1
+
+
+class TPGGas(C.Structure):
+    _fields_ = [("nsp", C.c_int), ("M", C.c_double * MAX_SPECIES), ("nseg", C.c_int * MAX_SPECIES),
+                ("T_breaks", (C.c_double * 4) * MAX_SPECIES), ("T_blends", (C.c_double * 2) * MAX_SPECIES),
+                ("coeffs", ((C.c_double * 9) * 3) * MAX_SPECIES)]
+
+
+class Reaction(C.Structure):
+    _fields_ = [("collider", C.c_int), ("nreac", C.c_int), ("reac_sp", C.c_int * 3), ("reac_nu", C.c_int * 3),
+                ("nprod", C.c_int), ("prod_sp", C.c_int * 3), ("prod_nu", C.c_int * 3),
+                ("neff", C.c_int), ("eff_sp", C.c_int * MAX_SPECIES), ("eff", C.c_double * MAX_SPECIES),
+                ("f_from_eq", C.c_int), ("fA", C.c_double), ("fn", C.c_double), ("fC", C.c_double),
+                ("b_from_eq", C.c_int), ("bA", C.c_double), ("bn", C.c_double), ("bC", C.c_double)]
+
+
+class Mechanism(C.Structure):
+    _fields_ = [("nr", C.c_int), ("r", Reaction * MAX_REACTIONS), ("T_lower", C.c_double), ("T_upper", C.c_double),
+                ("method", C.c_int), ("eps1", C.c_double), ("eps2", C.c_double), ("delta", C.c_double),
+                ("maxIters", C.c_int), ("errTol", C.c_double), ("tightTempCoupling", C.c_int),
+                ("maxSubcycles", C.c_int), ("maxAttempts", C.c_int), ("T_frozen", C.c_double)]
+
+
+def gas_record(d: dict) -> TPGGas:
+    """From a gas JSON: {species, M, thermo: [{T_breaks, T_blends, coeffs}]} (2 or 3 CEA segments)."""
+    g = TPGGas()
+    nsp = len(d["M"])
+    if nsp > MAX_SPECIES:
+        raise ValueError("too many species")
+    g.nsp = nsp
+    for i in range(nsp):
+        g.M[i] = d["M"][i]
+        th = d["thermo"][i]
+        nseg = len(th["coeffs"])
+        if not 2 <= nseg <= 3 or len(th["T_breaks"]) != nseg + 1 or len(th["T_blends"]) != nseg - 1:
+            raise ValueError("CEA curves need 2 or 3 segments with matching breaks and blends")
+        g.nseg[i] = nseg
+        for k in range(nseg + 1):
+            g.T_breaks[i][k] = th["T_breaks"][k]
+        for k in range(nseg - 1):
+            g.T_blends[i][k] = th["T_blends"][k]
+        for s in range(nseg):
+            for j in range(9):
+                g.coeffs[i][s][j] = th["coeffs"][s][j]
+    return g
+
+
+def mech_record(d: dict, method=None, T_frozen=300.0, **overrides) -> Mechanism:
+    """From a mechanism JSON {species, config, reactions}; method/overrides replace config."""
+    m = Mechanism()
+    rs = d["reactions"]
+    if len(rs) > MAX_REACTIONS:
+        raise ValueError("too many reactions")
+    m.nr = len(rs)
+    for k, r in enumerate(rs):
+        rr = m.r[k]
+        rr.collider = 1 if r["type"] == "anonymous_collider" else 0
+        rr.nreac = len(r["reactants"])
+        for q, (i, c) in enumerate(r["reactants"]):
+            rr.reac_sp[q], rr.reac_nu[q] = int(i), int(c)
+        rr.nprod = len(r["products"])
+        for q, (i, c) in enumerate(r["products"]):
+            rr.prod_sp[q], rr.prod_nu[q] = int(i), int(c)
+        eff = r.get("efficiencies", [])
+        rr.neff = len(eff)
+        for q, (i, v) in enumerate(eff):
+            rr.eff_sp[q], rr.eff[q] = int(i), float(v)
+        for pre, key in (("f", "frc"), ("b", "brc")):
+            rc = r[key]
+            if rc["model"] == "fromEqConst":
+                setattr(rr, pre + "_from_eq", 1)
+            else:
+                setattr(rr, pre + "_from_eq", 0)
+                setattr(rr, pre + "A", float(rc["A"]))
+                setattr(rr, pre + "n", float(rc["n"]))
+                setattr(rr, pre + "C", float(rc["C"]))
+    cfg = d["config"]
+    m.T_lower = float(cfg["tempLimits"]["lower"])
+    m.T_upper = float(cfg["tempLimits"]["upper"])
+    ode = cfg["odeStep"]
+    meth = method if method is not None else ode["method"]
+    m.method = ODE_RKF if meth == "rkf" else ODE_ALPHA_QSS
+    eps1 = ode.get("eps1", 1.0e-3)
+    m.eps1 = eps1
+    m.eps2 = ode.get("eps2", eps1 / 2.0)
+    m.delta = ode.get("delta", 1.0e-10)
+    m.maxIters = int(ode.get("maxIters", 10))
+    m.errTol = ode.get("errTol", 1.0e-3)
+    m.tightTempCoupling = 1 if cfg.get("tightTempCoupling", False) else 0
+    m.maxSubcycles = int(cfg.get("maxSubcycles", 10000))
+    m.maxAttempts = int(cfg.get("maxAttempts", 4))
+    m.T_frozen = T_frozen
+    for k, v in overrides.items():
+        setattr(m, k, v)
+    return m
+
+
+def element_matrix(species):
+    """Atoms of N and O per species (rows N, O) for the 5-species air species names."""
+    atoms = {"N2": (2, 0), "O2": (0, 2), "NO": (1, 1), "N": (1, 0), "O": (0, 1)}
+    return np.array([[atoms[s][0] for s in species], [atoms[s][1] for s in species]], dtype=np.float64)

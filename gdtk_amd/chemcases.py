@@ -1,0 +1,67 @@
+"""Synthetic chemistry workloads: cells of 5-species air with uneven stiffness.
+
+Plain chemistry, uneven stiffness: T log-uniform in [1500, 12000] K, p log-uniform in
+[1e3, 1e6] Pa, mass fractions Dirichlet-perturbed about free-stream air (0.767 N2, 0.233 O2)
+with a seeded partial dissociation (O2 -> 2 O, N2 -> 2 N, N2 + O2 -> 2 NO moved at fixed
+element totals), rho and u from the thermally perfect gas EOS, dt = 1e-7 s.
+Species order N2, O2, NO, N, O (src/gas/sample-data/therm-perf-5-species-air.lua).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+M = np.array([0.0280134, 0.0319988, 0.0300061, 0.0140067, 0.0159994])
+
+
+def air_massf(n, rng):
+    base = np.array([0.767, 0.233])
+    alpha = 400.0
+    y = rng.dirichlet(alpha * base, size=n)          # N2, O2 about free stream
+    # moles per kg of N2 and O2
+    nN2 = y[:, 0] / M[0]
+    nO2 = y[:, 1] / M[1]
+    fO = rng.uniform(0.0, 0.6, n) * rng.uniform(0.0, 1.0, n)    # fraction of O2 dissociated
+    fN = rng.uniform(0.0, 0.3, n) * rng.uniform(0.0, 1.0, n)    # fraction of N2 dissociated
+    fNO = rng.uniform(0.0, 0.05, n)                             # fraction of O2 into NO
+    dO2 = fO * nO2
+    dN2 = fN * nN2
+    dNO = fNO * nO2
+    moles = np.stack([nN2 - dN2 - dNO, nO2 - dO2 - dNO, 2.0 * dNO, 2.0 * dN2, 2.0 * dO2], axis=1)
+    Y = moles * M[None, :]
+    Y /= Y.sum(axis=1, keepdims=True)
+    return Y
+
+
+def air_cells(n, seed=0, T_lo=1500.0, T_hi=12000.0, p_lo=1.0e3, p_hi=1.0e6):
+    """Returns dict(rho, T, p, u, massf (species-major, 5*n)) with rho, u from the oracle-free
+    closed forms of the thermally perfect gas evaluated by the caller's gas model."""
+    from gdtk_amd import data as gdata
+    rng = np.random.default_rng(seed)
+    T = np.exp(rng.uniform(np.log(T_lo), np.log(T_hi), n))
+    p = np.exp(rng.uniform(np.log(p_lo), np.log(p_hi), n))
+    Y = air_massf(n, rng)
+    gd = gdata.load("tpg-air-5sp")
+    R = 8.31451 / np.array(gd["M"])
+    Rmix = Y @ R
+    rho = p / (Rmix * T)
+    u = Y @ np.stack([_cea_h(gd["thermo"][i], T) - R[i] * T for i in range(5)], axis=0) \
+        if False else np.einsum("ns,sn->n", Y, np.stack([_cea_h(gd["thermo"][i], T, R[i]) - R[i] * T
+                                                         for i in range(5)], axis=0))
+    return dict(rho=rho, T=T, p=p, u=u, massf=np.ascontiguousarray(Y.T).reshape(-1))
+
+
+def _cea_h(th, T, R):
+    """Specific enthalpy from a 3-segment NASA-9 curve (blends ignored: used only to set
+    a consistent starting u; the EOS under test recomputes T from it)."""
+    T = np.asarray(T)
+    h = np.empty_like(T)
+    br = th["T_breaks"]
+    for s in range(3):
+        a = th["coeffs"][s]
+        m = (T >= br[s]) & (T < br[s + 1]) if s < 2 else (T >= br[s])
+        if s == 0:
+            m |= T < br[0]
+        t = T[m]
+        h[m] = R * t * (-a[0] / t ** 2 + a[1] * np.log(t) / t + a[2] + a[3] * t / 2 + a[4] * t ** 2 / 3
+                        + a[5] * t ** 3 / 4 + a[6] * t ** 4 / 5 + a[7] / t)
+    return h

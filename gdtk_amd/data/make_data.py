@@ -1,0 +1,108 @@
+"""Writes the JSON gas-model and mechanism data of the chemistry path from the reference's
+Lua data files (run once where the reference tree is mounted; the JSON is committed).
+
+  tpg-air-5sp.json      thermally perfect 5-species air, species N2, O2, NO, N, O
+                        (src/gas/sample-data/therm-perf-5-species-air.lua)
+  park-air-5sp-1T.json  one-temperature Park 5-reaction mechanism authored from
+                        examples/lmr/3D/cylinder-larc/rm-park-5sp-2T.inp: the 'Park'
+                        rate constants (T^s Tv^(1-s), identical to T when Tv = T) become
+                        'Arrhenius' with the same A, n, C; A is converted from CGS to SI
+                        with prep-chem's factor 1e-6^(sum of reactant coefficients + 1 for
+                        an anonymous collider - 1) (src/kinetics/reaction.lua:26-45);
+                        backward rates come from the equilibrium constant; the ODE
+                        settings are prep-chem's defaults for alpha-QSS
+                        (src/kinetics/prep_chem.lua:46-90).
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from gdtk_amd.data.lua_tables import parse_file  # noqa: E402
+
+REF = os.environ.get("GDTK_REFERENCE", "/root/reference")
+
+
+def thermo_table(tc):
+    nseg = tc["nsegments"]
+    return {"T_breaks": [float(x) for x in tc["T_break_points"]],
+            "T_blends": [float(x) for x in tc["T_blend_ranges"]],
+            "coeffs": [[float(x) for x in tc[f"segment{i}"]] for i in range(nseg)]}
+
+
+def gas_from_db(path, thermo_key="thermoCoeffs"):
+    env, _ = parse_file(path)
+    species = env["species"]
+    if isinstance(species, dict):
+        species = [species[k] for k in sorted(species)]
+    db = env["db"] if "db" in env else env
+    return {"model": "ThermallyPerfectGas", "source": os.path.relpath(path, REF), "species": species,
+            "M": [float(db[s]["M"]) for s in species],
+            "thermo": [thermo_table(db[s][thermo_key]) for s in species]}
+
+
+def parse_side(side, species):
+    """'N + N + M' -> ({isp: coeff}, has_collider)"""
+    coeffs, collider = {}, False
+    for term in side.split("+"):
+        term = term.strip()
+        m = re.fullmatch(r"(\d+)?\s*([A-Za-z0-9_]+)", term)
+        n, name = int(m.group(1) or 1), m.group(2)
+        if name == "M":
+            collider = True
+            continue
+        isp = species.index(name)
+        coeffs[isp] = coeffs.get(isp, 0) + n
+    return coeffs, collider
+
+
+def park_1T(gas):
+    species = gas["species"]
+    _, calls = parse_file(os.path.join(REF, "examples/lmr/3D/cylinder-larc/rm-park-5sp-2T.inp"))
+    reactions = []
+    for name, t in calls:
+        assert name == "Reaction"
+        eq = t[1]
+        lhs, rhs = eq.split("<=>")
+        reac, colL = parse_side(lhs, species)
+        prod, colR = parse_side(rhs, species)
+        assert colL == colR
+        fr = t["fr"]
+        nc = sum(reac.values()) + (1 if colL else 0)
+        conv = 1.0e-6 ** (nc - 1)
+        r = {"equation": eq, "label": t.get("label", ""),
+             "type": "anonymous_collider" if colL else "elementary",
+             "reactants": [[i, c] for i, c in sorted(reac.items())],
+             "products": [[i, c] for i, c in sorted(prod.items())],
+             # prep-chem writes the converted A with "%.12e" (reaction.lua, transformRateConstant).
+             "frc": {"model": "Arrhenius", "A": float("%.12e" % (fr["A"] * conv)), "n": float(fr["n"]),
+                     "C": float(fr["C"]),
+                     "A_cgs": fr["A"], "source_model": fr[1]},
+             "brc": {"model": "fromEqConst"}}
+        if colL:
+            eff = t.get("efficiencies", {})
+            r["efficiencies"] = [[species.index(s), float(v)] for s, v in
+                                 sorted(eff.items(), key=lambda kv: species.index(kv[0]))]
+        reactions.append(r)
+    return {"species": species, "source": "examples/lmr/3D/cylinder-larc/rm-park-5sp-2T.inp (1-T Arrhenius form)",
+            "config": {"tempLimits": {"lower": 300.0, "upper": 50000.0},
+                       "odeStep": {"method": "alpha-qss", "eps1": 1.0e-3, "eps2": 5.0e-4, "delta": 1.0e-10,
+                                   "maxIters": 10},
+                       "tightTempCoupling": False, "maxSubcycles": 10000, "maxAttempts": 4},
+            "reactions": reactions}
+
+
+def main():
+    gas = gas_from_db(os.path.join(REF, "src/gas/sample-data/therm-perf-5-species-air.lua"))
+    with open(os.path.join(HERE, "tpg-air-5sp.json"), "w") as f:
+        json.dump(gas, f, indent=1)
+    with open(os.path.join(HERE, "park-air-5sp-1T.json"), "w") as f:
+        json.dump(park_1T(gas), f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -138,6 +138,72 @@ int gdtk_hip_sim_autotune(int sim, double dt, int reps, double* out, int nout);
 #define GDTK_TUNE_KCHUNK 1
 int gdtk_hip_sim_set_tuning(int sim, int key, int value);
 
+/* ---------------------------------------------------------------------------------
+ * Finite-rate chemistry behind the ThermochemicalReactor interface
+ * (src/kinetics/thermochemical_reactor.d:45-47), batched over cells like the
+ * reference's GPU plug-in boundary GPUChem.thermochemical_increment
+ * (src/eilmer/cuda_gpu_chem.d:155-231).  Gas: thermally perfect mixture with CEA
+ * NASA-9 curves (src/gas/therm_perf_gas.d); mechanism: elementary and
+ * anonymous-collider reactions with Arrhenius or equilibrium-constant rates
+ * (src/kinetics/reaction.d, rate_constant.d); ODE step alpha-QSS or RKF
+ * (src/kinetics/chemistry_update.d). */
+#define GDTK_MAX_SPECIES 8
+#define GDTK_MAX_REACTIONS 32
+#define GDTK_ODE_ALPHA_QSS 0
+#define GDTK_ODE_RKF 1
+
+typedef struct gdtk_tpg_gas {
+    int nsp;
+    double M[GDTK_MAX_SPECIES];                 /* kg/mol */
+    int nseg[GDTK_MAX_SPECIES];                 /* CEA segments per species, 2 or 3 */
+    double T_breaks[GDTK_MAX_SPECIES][4];       /* nseg+1 break points */
+    double T_blends[GDTK_MAX_SPECIES][2];
+    double coeffs[GDTK_MAX_SPECIES][3][9];
+} gdtk_tpg_gas;
+
+typedef struct gdtk_reaction {
+    int collider;                               /* anonymous collider "M" */
+    int nreac, reac_sp[3], reac_nu[3];
+    int nprod, prod_sp[3], prod_nu[3];
+    int neff, eff_sp[GDTK_MAX_SPECIES];
+    double eff[GDTK_MAX_SPECIES];
+    int f_from_eq; double fA, fn, fC;           /* forward: Arrhenius or from K_c */
+    int b_from_eq; double bA, bn, bC;           /* backward: Arrhenius or from K_c */
+} gdtk_reaction;
+
+typedef struct gdtk_mechanism {
+    int nr;
+    gdtk_reaction r[GDTK_MAX_REACTIONS];
+    double T_lower, T_upper;                    /* config.tempLimits */
+    int method;                                 /* GDTK_ODE_* */
+    double eps1, eps2, delta;                   /* alpha-QSS */
+    int maxIters;
+    double errTol;                              /* RKF */
+    int tightTempCoupling, maxSubcycles, maxAttempts;
+    double T_frozen;                            /* globalconfig T_frozen (300 K) */
+} gdtk_mechanism;
+
+/* Replaces init_thermochemical_reactor (src/kinetics/init_thermochemical_reactor.d:54-138)
+ * and the GPUChem constructor (cuda_gpu_chem.d:71-150). */
+int gdtk_hip_chem_create(const gdtk_tpg_gas* gas, const gdtk_mechanism* mech, int* chem);
+int gdtk_hip_chem_destroy(int chem);
+
+/* FVCell.thermochemical_increment (src/eilmer/fvcell.d:858-972) for n cells at once:
+ * ChemistryUpdate.opCall over dt with the cell's dt_chem (retried once with dt_chem = -1
+ * on failure), then update_thermo_from_rhou.  Host arrays: rho[n]; T, u, p, dt_chem [n]
+ * in/out; massf[nsp*n] in/out, species-major (massf[isp*n + i]); status[n] out
+ * (0 ok, -1 failed cell).  stats (may be NULL, 4 values) = {total substeps, total
+ * corrector iterations, max substeps of a cell, kernel ms}.  Returns the number of
+ * failed cells (>= 0) or a negative GDTK_ERR_* code. */
+int gdtk_hip_chem_step(int chem, int n, const double* rho, double* T, double* u, double* p, double* massf,
+                       double* dt_chem, double dt, int* status, double* stats);
+
+/* Thermodynamic updates of n gas states on the device, for checking the gas model:
+ * mode 0 pT (p, T -> rho, u), 1 rhou (rho, u -> T, p; T in is the Newton start),
+ * 2 rhoT (rho, T -> p, u); then the sound speed a.  massf species-major as above. */
+int gdtk_hip_tpg_update(int chem, int mode, int n, double* rho, double* p, double* T, double* u, double* a,
+                        const double* massf);
+
 /* Message of the last failing call on this thread. */
 const char* gdtk_hip_last_error(void);
 

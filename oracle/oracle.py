@@ -224,3 +224,120 @@ def l2r2_scalar(lens, q, apply_limiter=True, extrema_clipping=True, eps=1.0e-12)
     lib().orc_l2r2_scalar(lp, qp, 1 if apply_limiter else 0, 1 if extrema_clipping else 0, eps,
                           out.ctypes.data_as(dp))
     return out[0], out[1]
+
+
+# ---------------------------------------------------------------------------------
+# Thermochemistry oracle (chem_oracle.cpp)
+vp = C.c_void_p
+
+
+def _chem_lib():
+    L = lib()
+    if not getattr(L, "_chem_ready", False):
+        L.orc_tpg_sizes.argtypes = [ip]
+        L.orc_tpg_create.restype = vp
+        L.orc_tpg_create.argtypes = [vp]
+        L.orc_cea_eval.argtypes = [vp, C.c_int, C.c_double, dp]
+        L.orc_tpg_update.argtypes = [vp, C.c_int, dp, dp, ip]
+        L.orc_chem_create.restype = vp
+        L.orc_chem_create.argtypes = [vp, vp]
+        L.orc_chem_destroy.argtypes = [vp]
+        L.orc_chem_rate_constants.argtypes = [vp, C.c_double, dp, dp]
+        L.orc_chem_split_rates.argtypes = [vp, dp, dp, dp]
+        L.orc_chem_ode_step.argtypes = [vp, C.c_int, dp, C.c_double, dp, dp]
+        L.orc_chem_update.argtypes = [vp, dp, dp, C.c_double, dp, dp]
+        L.orc_chem_step_batch.argtypes = [vp, vp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double, ip, ip, C.c_int]
+        L.orc_gas_destroy_tpg.argtypes = [vp]
+        from gdtk_amd.chem import Mechanism, TPGGas
+        sizes = (C.c_int * 3)()
+        L.orc_tpg_sizes(sizes)
+        assert sizes[0] == C.sizeof(TPGGas) and sizes[1] == C.sizeof(Mechanism), "record layout mismatch"
+        L._chem_ready = True
+    return L
+
+
+class TPG:
+    """Thermally perfect gas of the oracle, from a gdtk_amd.chem.TPGGas record."""
+
+    def __init__(self, rec):
+        self.rec = rec
+        self.nsp = rec.nsp
+        self.h = _chem_lib().orc_tpg_create(C.byref(rec))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_gas_destroy_tpg(self.h)
+            self.h = None
+
+    def cea(self, isp, T):
+        out = np.zeros(3)
+        _chem_lib().orc_cea_eval(self.h, isp, T, out.ctypes.data_as(dp))
+        return dict(Cp=out[0], h=out[1], s=out[2])
+
+    def update(self, mode, massf, rho=0.0, p=0.0, T=0.0, u=0.0):
+        st, stp = _d([rho, p, T, u, 0.0])
+        mf, mfp = _d(massf)
+        ne = C.c_int(0)
+        rc = _chem_lib().orc_tpg_update(self.h, {"pT": 0, "rhou": 1, "rhoT": 2, "rhop": 3}[mode], stp, mfp,
+                                        C.byref(ne))
+        if rc != 0:
+            raise RuntimeError(err())
+        return dict(rho=st[0], p=st[1], T=st[2], u=st[3], a=st[4], newton_evals=ne.value)
+
+
+class Chem:
+    """ChemistryUpdate of the oracle from gas and mechanism records."""
+
+    def __init__(self, gas_rec, mech_rec):
+        self.gas_rec, self.mech_rec = gas_rec, mech_rec
+        self.nsp = gas_rec.nsp
+        self.nr = mech_rec.nr
+        self.h = _chem_lib().orc_chem_create(C.byref(gas_rec), C.byref(mech_rec))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_chem_destroy(self.h)
+            self.h = None
+
+    def rate_constants(self, T):
+        kf, kb = np.zeros(self.nr), np.zeros(self.nr)
+        _chem_lib().orc_chem_rate_constants(self.h, T, kf.ctypes.data_as(dp), kb.ctypes.data_as(dp))
+        return kf, kb
+
+    def split_rates(self, conc):
+        c, cp = _d(conc)
+        q, L = np.zeros(self.nsp), np.zeros(self.nsp)
+        _chem_lib().orc_chem_split_rates(self.h, cp, q.ctypes.data_as(dp), L.ctypes.data_as(dp))
+        return q, L
+
+    def ode_step(self, method, y0, h):
+        y, ypp = _d(y0)
+        out = np.zeros(self.nsp)
+        hs = C.c_double(0.0)
+        ok = _chem_lib().orc_chem_ode_step(self.h, {"alpha-qss": 0, "rkf": 1}[method], ypp, h,
+                                           out.ctypes.data_as(dp), C.byref(hs))
+        return bool(ok), out, hs.value
+
+    def update(self, rho, p, T, u, massf, tInterval, dtSuggest):
+        st, stp = _d([rho, p, T, u])
+        mf, mfp = _d(massf)
+        mf = mf.copy()
+        mfp = mf.ctypes.data_as(dp)
+        dts = C.c_double(dtSuggest)
+        stats = np.zeros(4)
+        rc = _chem_lib().orc_chem_update(self.h, stp, mfp, tInterval, C.byref(dts), stats.ctypes.data_as(dp))
+        return rc, mf, dts.value, dict(subcycles=int(stats[0]), corrector_iters=int(stats[1]))
+
+
+def chem_step_batch(gas_rec, mech_rec, rho, T, u, p, massf, dt_chem, dt, nthreads=1):
+    """Batched thermochemical_increment; arrays are copied, results returned."""
+    n = len(rho)
+    T, u, p, massf, dt_chem = (np.array(a, dtype=np.float64, copy=True) for a in (T, u, p, massf, dt_chem))
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    status = np.zeros(n, dtype=np.int32)
+    sub = np.zeros(n, dtype=np.int32)
+    failed = _chem_lib().orc_chem_step_batch(C.byref(gas_rec), C.byref(mech_rec), n, rho.ctypes.data_as(dp),
+                                             T.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp),
+                                             massf.ctypes.data_as(dp), dt_chem.ctypes.data_as(dp), dt,
+                                             status.ctypes.data_as(ip), sub.ctypes.data_as(ip), nthreads)
+    return dict(T=T, u=u, p=p, massf=massf, dt_chem=dt_chem, status=status, substeps=sub, failed=failed)

@@ -295,3 +295,212 @@ void orc_l2r2_scalar(const double* len4, const double* q4, int apply_limiter, in
 }
 
 }  // extern "C"
+
+//---------------------------------------------------------------------------------
+// Thermochemistry (chem_oracle.hpp).  The structures below have the same layout as the
+// gdtk_tpg_gas / gdtk_mechanism records of the C ABI so that one ctypes record can be
+// handed to both the oracle and the library.
+#include "chem_oracle.hpp"
+
+namespace {
+constexpr int MS = 8, MR = 32;
+struct OGas {
+    int nsp;
+    double M[MS];
+    int nseg[MS];
+    double T_breaks[MS][4];
+    double T_blends[MS][2];
+    double coeffs[MS][3][9];
+};
+struct OReaction {
+    int collider;
+    int nreac, reac_sp[3], reac_nu[3];
+    int nprod, prod_sp[3], prod_nu[3];
+    int neff, eff_sp[MS];
+    double eff[MS];
+    int f_from_eq; double fA, fn, fC;
+    int b_from_eq; double bA, bn, bC;
+};
+struct OMech {
+    int nr;
+    OReaction r[MR];
+    double T_lower, T_upper;
+    int method;
+    double eps1, eps2, delta;
+    int maxIters;
+    double errTol;
+    int tightTempCoupling, maxSubcycles, maxAttempts;
+    double T_frozen;
+};
+struct OChem {
+    ThermallyPerfectGas* gas;
+    ChemistryUpdate* chem;
+    double T_frozen;
+};
+ThermallyPerfectGas* make_tpg(const OGas* g) {
+    std::vector<double> M(g->M, g->M + g->nsp);
+    std::vector<CEAThermoCurve> curves;
+    for (int i = 0; i < g->nsp; ++i) {
+        double R = R_universal / g->M[i];
+        const int ns = g->nseg[i];
+        std::vector<double> tb(g->T_breaks[i], g->T_breaks[i] + ns + 1), bl(g->T_blends[i], g->T_blends[i] + ns - 1);
+        std::vector<std::array<double, 9>> co(ns);
+        for (int s = 0; s < ns; ++s) for (int j = 0; j < 9; ++j) co[s][j] = g->coeffs[i][s][j];
+        curves.emplace_back(R, tb, bl, co);
+    }
+    return new ThermallyPerfectGas(M, curves);
+}
+}  // namespace
+
+extern "C" {
+
+int orc_tpg_sizes(int* sizes) {
+    sizes[0] = (int)sizeof(OGas);
+    sizes[1] = (int)sizeof(OMech);
+    sizes[2] = (int)sizeof(OReaction);
+    return 0;
+}
+
+void* orc_tpg_create(const void* gas) { return make_tpg(static_cast<const OGas*>(gas)); }
+
+// A single CEA curve (species isp of gas): out = {Cp, h, s} at T.
+void orc_cea_eval(void* gas, int isp, double T, double* out) {
+    auto* g = static_cast<ThermallyPerfectGas*>(gas);
+    out[0] = g->curves[isp].eval_Cp(T);
+    out[1] = g->curves[isp].eval_h(T);
+    out[2] = g->curves[isp].eval_s(T);
+}
+
+// st = [rho, p, T, u, a]; mode 0 pT, 1 rhou, 2 rhoT, 3 rhop; then sound speed.
+int orc_tpg_update(void* gas, int mode, double* st, const double* massf, int* newton_evals) {
+    auto* g = static_cast<ThermallyPerfectGas*>(gas);
+    GasState Q;
+    Q.rho = st[0]; Q.p = st[1]; Q.T = st[2]; Q.u = st[3]; Q.a = st[4];
+    Q.massf.assign(massf, massf + g->n_species);
+    try {
+        switch (mode) {
+        case 0: g->update_thermo_from_pT(Q); break;
+        case 1: g->update_thermo_from_rhou(Q); break;
+        case 2: g->update_thermo_from_rhoT(Q); break;
+        case 3: g->update_thermo_from_rhop(Q); break;
+        default: g_err = "bad mode"; return -2;
+        }
+        g->update_sound_speed(Q);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        st[0] = Q.rho; st[1] = Q.p; st[2] = Q.T; st[3] = Q.u;
+        return -1;
+    }
+    if (newton_evals) *newton_evals = g->newton_evals;
+    st[0] = Q.rho; st[1] = Q.p; st[2] = Q.T; st[3] = Q.u; st[4] = Q.a;
+    return 0;
+}
+
+void* orc_chem_create(const void* gasv, const void* mechv) {
+    const OGas* gd = static_cast<const OGas*>(gasv);
+    const OMech* md = static_cast<const OMech*>(mechv);
+    ThermallyPerfectGas* gas = make_tpg(gd);
+    std::vector<ReactionDef> defs;
+    for (int ir = 0; ir < md->nr; ++ir) {
+        const OReaction& r = md->r[ir];
+        ReactionDef d;
+        d.collider = r.collider != 0;
+        for (int k = 0; k < r.nreac; ++k) d.reactants.push_back({r.reac_sp[k], r.reac_nu[k]});
+        for (int k = 0; k < r.nprod; ++k) d.products.push_back({r.prod_sp[k], r.prod_nu[k]});
+        for (int k = 0; k < r.neff; ++k) d.efficiencies.push_back({r.eff_sp[k], r.eff[k]});
+        d.frc.fromEqConst = r.f_from_eq != 0; d.frc.A = r.fA; d.frc.n = r.fn; d.frc.C = r.fC;
+        d.brc.fromEqConst = r.b_from_eq != 0; d.brc.A = r.bA; d.brc.n = r.bn; d.brc.C = r.bC;
+        defs.push_back(d);
+    }
+    ChemConfig cfg;
+    cfg.method = md->method == 1 ? OdeMethod::rkf : OdeMethod::alpha_qss;
+    cfg.eps1 = md->eps1; cfg.eps2 = md->eps2; cfg.delta = md->delta; cfg.maxIters = md->maxIters;
+    cfg.errTol = md->errTol; cfg.tightTempCoupling = md->tightTempCoupling != 0;
+    cfg.maxSubcycles = md->maxSubcycles; cfg.maxAttempts = md->maxAttempts;
+    cfg.T_lower = md->T_lower; cfg.T_upper = md->T_upper;
+    return new OChem{gas, new ChemistryUpdate(gas, defs, cfg), md->T_frozen};
+}
+
+void orc_chem_destroy(void* h) {
+    auto* c = static_cast<OChem*>(h);
+    delete c->chem;
+    delete c->gas;
+    delete c;
+}
+
+// Rate constants at the state (rho, T, massf): kf[nr], kb[nr].
+int orc_chem_rate_constants(void* h, double T, double* kf, double* kb) {
+    auto* c = static_cast<OChem*>(h);
+    GasState Q;
+    Q.T = T;
+    Q.massf.assign(c->gas->n_species, 0.0);
+    c->chem->rmech.eval_rate_constants(Q);
+    for (int i = 0; i < c->chem->rmech.nr; ++i) { kf[i] = c->chem->rmech.k_f[i]; kb[i] = c->chem->rmech.k_b[i]; }
+    return 0;
+}
+
+// Split rates q, L at concentrations conc (rate constants from the last evaluation).
+int orc_chem_split_rates(void* h, const double* conc, double* q, double* L) {
+    auto* c = static_cast<OChem*>(h);
+    c->chem->rmech.eval_split_rates(conc, q, L);
+    return 0;
+}
+
+// One ODE step from y0 over h: returns 1 success, 0 failure; hSuggest out.
+int orc_chem_ode_step(void* h, int method, const double* y0, double hh, double* yOut, double* hSuggest) {
+    auto* c = static_cast<OChem*>(h);
+    if (method == 1) return c->chem->rkf_step(y0, hh, yOut, *hSuggest) ? 1 : 0;
+    return c->chem->alpha_qss_step(y0, hh, yOut, *hSuggest) ? 1 : 0;
+}
+
+// ChemistryUpdate.opCall on one state: st = [rho, p, T, u], massf in/out; returns 0 ok, -1 failed.
+int orc_chem_update(void* h, double* st, double* massf, double tInterval, double* dtSuggest, double* stats) {
+    auto* c = static_cast<OChem*>(h);
+    GasState Q;
+    Q.rho = st[0]; Q.p = st[1]; Q.T = st[2]; Q.u = st[3];
+    Q.massf.assign(massf, massf + c->gas->n_species);
+    ChemStats cs;
+    bool ok = false;
+    try {
+        ok = c->chem->update(Q, tInterval, *dtSuggest, &cs);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        ok = false;
+    }
+    for (int i = 0; i < c->gas->n_species; ++i) massf[i] = Q.massf[i];
+    st[0] = Q.rho; st[1] = Q.p; st[2] = Q.T; st[3] = Q.u;
+    if (stats) { stats[0] = cs.subcycles; stats[1] = cs.corrector_iters; stats[2] = cs.attempts; stats[3] = cs.failures; }
+    return ok ? 0 : -1;
+}
+
+// Batched thermochemical_increment (fvcell.d:858-972) over n cells, species-major massf,
+// with nthreads OpenMP threads (one oracle object per thread).  sub[n] = substeps per cell.
+int orc_chem_step_batch(const void* gasv, const void* mechv, int n, const double* rho, double* T, double* u,
+                        double* p, double* massf, double* dt_chem, double dt, int* status, int* sub, int nthreads) {
+    int failed = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : failed)
+    {
+        OChem* c = static_cast<OChem*>(orc_chem_create(gasv, mechv));
+        const int nsp = c->gas->n_species;
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i) {
+            GasState Q;
+            Q.rho = rho[i]; Q.T = T[i]; Q.u = u[i]; Q.p = p[i];
+            Q.massf.resize(nsp);
+            for (int s = 0; s < nsp; ++s) Q.massf[s] = massf[(size_t)s * n + i];
+            ChemStats cs;
+            int rc = thermochemical_increment(*c->chem, *c->gas, Q, dt, dt_chem[i], c->T_frozen, &cs);
+            status[i] = rc;
+            if (sub) sub[i] = cs.subcycles;
+            if (rc != 0) { ++failed; continue; }
+            T[i] = Q.T; u[i] = Q.u; p[i] = Q.p;
+            for (int s = 0; s < nsp; ++s) massf[(size_t)s * n + i] = Q.massf[s];
+        }
+        orc_chem_destroy(c);
+    }
+    return failed;
+}
+
+void orc_gas_destroy_tpg(void* g) { delete static_cast<ThermallyPerfectGas*>(g); }
+
+}  // extern "C"

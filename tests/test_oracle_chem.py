@@ -1,0 +1,211 @@
+"""Pins of the CPU oracle's thermochemistry against the reference's own unit tests
+(known answers cited per test) and closed forms.  No GPU."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from gdtk_amd import data as gdata
+from gdtk_amd.chem import gas_record, mech_record
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+R_U = 8.31451
+
+
+def _json(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def _isclose(a, b, rel):
+    """D's std.math.isClose(a, b, maxRelDiff): |a-b| <= maxRelDiff * max(|a|, |b|)."""
+    return abs(a - b) <= rel * max(abs(a), abs(b))
+
+
+def test_cea_curve_oxygen(oracle_lib):
+    """src/gas/thermo/cea_thermo_curves.d:186-217 (monatomic O)."""
+    O = oracle_lib
+    d = _json("O-thermo.json")
+    gas = gas_record({"M": [0.0159994], "thermo": [d["thermo"]]})
+    g = O.TPG(gas)
+    assert _isclose(1328.627, g.cea(0, 500.0)["Cp"], 1.0e-6)
+    assert _isclose(20_030_794.683, g.cea(0, 3700.0)["h"], 1.0e-6)
+    assert _isclose(14_772.717, g.cea(0, 10_000.0)["s"], 1.0e-3)
+
+
+def test_cea_curve_consistency(oracle_lib):
+    """dh/dT = Cp and ds/dT = Cp/T across segments, blends and the constant-Cp extensions."""
+    O = oracle_lib
+    gd = gdata.load("tpg-air-5sp")
+    g = O.TPG(gas_record(gd))
+    for isp in range(5):
+        for T in (150.0, 450.0, 800.0, 1000.0, 1234.5, 3000.0, 5600.0, 6000.0, 6400.0, 15000.0, 25000.0):
+            dT = 1e-3 * T
+            hp, hm = g.cea(isp, T + dT)["h"], g.cea(isp, T - dT)["h"]
+            sp, sm = g.cea(isp, T + dT)["s"], g.cea(isp, T - dT)["s"]
+            cp = g.cea(isp, T)["Cp"]
+            assert abs((hp - hm) / (2 * dT) - cp) < 2e-3 * cp, (isp, T)
+            assert abs((sp - sm) / (2 * dT) - cp / T) < 2e-3 * cp / T, (isp, T)
+
+
+def test_tpg_mix_energy_and_temperature(oracle_lib):
+    """src/gas/thermo/therm_perf_gas_mix_eos.d:177-200 (O2/N2/H2)."""
+    O = oracle_lib
+    d = _json("O2-N2-H2.json")
+    g = O.TPG(gas_record(d))
+    mf = [0.2, 0.7, 0.1]
+    st = g.update("rhoT", mf, rho=1.0, T=1000.0)
+    assert _isclose(1_031_849.875, st["u"], 1.0e-6)
+    back = g.update("rhou", mf, rho=1.0, u=st["u"], T=1500.0)
+    assert _isclose(1000.0, back["T"], 1.0e-6)
+
+
+def test_tpg_5species_air(oracle_lib):
+    """src/gas/therm_perf_gas.d:565-609 (5-species air, Y = 0.2 each)."""
+    O = oracle_lib
+    g = O.TPG(gas_record(gdata.load("tpg-air-5sp")))
+    mf = [0.2] * 5
+    st = g.update("pT", mf, p=1.0e6, T=2000.0)
+    assert _isclose(11_801_825.6, st["u"], 1.0e-6)
+    assert _isclose(1.2840117, st["rho"], 1.0e-6)
+    st = g.update("rhou", mf, rho=2.0, u=14.0e6, T=st["T"])
+    assert _isclose(3_373_757.4, st["p"], 1.0e-6)
+    assert _isclose(4331.944, st["T"], 1.0e-6)
+    st = g.update("rhoT", mf, rho=1.5, T=10_000.0)
+    assert _isclose(5_841_068.3, st["p"], 1.0e-6)
+    assert _isclose(20_340_105.9, st["u"], 1.0e-6)
+    st = g.update("rhop", mf, rho=10.0, p=5.0e6, T=st["T"])
+    assert _isclose(11_164_648.5, st["u"], 1.0e-6)
+    assert _isclose(1284.012, st["T"], 1.0e-6)
+
+
+def test_tpg_sound_speed_closed_form(oracle_lib):
+    """a^2 = (Cp/Cv) R T; for T below the curves' range Cp is constant (frozen value)."""
+    O = oracle_lib
+    gd = gdata.load("tpg-air-5sp")
+    g = O.TPG(gas_record(gd))
+    mf = np.array([0.7, 0.2, 0.05, 0.03, 0.02])
+    st = g.update("pT", mf, p=1.0e5, T=2500.0)
+    R = np.sum(mf * R_U / np.array(gd["M"]))
+    cps = np.array([g.cea(i, 2500.0)["Cp"] for i in range(5)])
+    cp = np.sum(mf * cps)
+    cv = np.sum(mf * (cps - R_U / np.array(gd["M"])))
+    assert st["a"] == pytest.approx(math.sqrt(cp / cv * R * 2500.0), rel=1e-13)
+    assert st["rho"] == pytest.approx(1.0e5 / (R * 2500.0), rel=1e-14)
+
+
+def _mech(reactions, species, lower=100.0, upper=10_000.0, method="alpha-qss", **cfg):
+    d = {"species": species, "reactions": reactions,
+         "config": {"tempLimits": {"lower": lower, "upper": upper}, "odeStep": {"method": method},
+                    "tightTempCoupling": False, "maxSubcycles": cfg.get("maxSubcycles", 10000),
+                    "maxAttempts": cfg.get("maxAttempts", 4)}}
+    return d
+
+
+def test_arrhenius_rate_constants(oracle_lib):
+    """src/kinetics/rate_constant.d:599-618."""
+    O = oracle_lib
+    gas = gas_record(_json("H2-I2-HI.json"))
+    n2 = _json("N2-diss.json")
+    rx = [{"type": "elementary", "reactants": [[0, 1], [1, 1]], "products": [[2, 2]],
+           "frc": {"model": "Arrhenius", "A": 1.94e14 * 1e-6, "n": 0.0, "C": 20_620.0},
+           "brc": {"model": "Arrhenius", "A": n2["A"], "n": n2["n"], "C": n2["C"]}}]
+    c = O.Chem(gas, mech_record(_mech(rx, ["H2", "I2", "HI"], upper=50_000.0)))
+    kf, kb = c.rate_constants(700.0)
+    assert _isclose(3.10850956e-5, kf[0], 1.0e-2)
+    kf, kb = c.rate_constants(4000.0)
+    assert _isclose(0.00159439, kb[0], 1.0e-5)
+
+
+def test_split_rates_h2_i2(oracle_lib):
+    """src/kinetics/reaction_mechanism.d:264-283 and reaction.d:538-556."""
+    O = oracle_lib
+    gas = gas_record(_json("H2-I2-HI.json"))
+    rc = {"model": "Arrhenius", "A": 1.94e14, "n": 0.0, "C": 20_620.0}
+    rx = [{"type": "elementary", "reactants": [[0, 1], [1, 1]], "products": [[2, 2]], "frc": rc, "brc": rc}]
+    c = O.Chem(gas, mech_record(_mech(rx, ["H2", "I2", "HI"])))
+    c.rate_constants(700.0)
+    q, L = c.split_rates([4.54, 4.54, 0.0])
+    rates = q - L
+    for a, b in zip(rates, [-643.9303, -643.9303, 1287.8606]):
+        assert _isclose(b, a, 1.0e-6)
+    assert _isclose(1287.8606, q[2], 1e-6) and _isclose(643.9303, L[0], 1e-6) and L[2] == 0.0
+
+
+def _h2i2():
+    gas = gas_record(_json("H2-I2-HI.json"))
+    return gas, _json("H2-I2-inp.json")
+
+
+def test_alpha_qss_order_h2_i2(oracle_lib):
+    """src/kinetics/chemistry_update.d:725-737: [HI] after 60 000 s against the analytic
+    7.1420197868416215, and the error ratio 4.001 when h is halved (2nd order)."""
+    O = oracle_lib
+    gas, md = _h2i2()
+    c = O.Chem(gas, mech_record(md, method="alpha-qss", eps1=0.001, eps2=0.0005, delta=1.0e-10, maxIters=10))
+    c.rate_constants(700.0)
+    analytic = 7.1420197868416215
+
+    def estimate(dt):
+        conc = np.array([4.54, 4.54, 0.0])
+        t, tmax = 0.0, 60_000.0
+        while (tmax - t) > 1.0e-9:
+            dt = min(dt, tmax - t)
+            ok, conc, _ = c.ode_step("alpha-qss", conc, dt)
+            t += dt
+        return conc[2]
+
+    v0 = estimate(2.0)
+    v1 = estimate(1.0)
+    assert _isclose(analytic, v1, 1.0e-4)
+    assert _isclose(4.001, (analytic - v0) / (analytic - v1), 1.0e-4)
+
+
+def test_chemistry_update_h2_i2_rkf(oracle_lib):
+    """src/kinetics/chemistry_update.d:739-750: the full ChemistryUpdate on H2-I2-HI with
+    the file's RKF step gives [HI] = 7.14201983840 (1e-9)."""
+    O = oracle_lib
+    gas, md = _h2i2()
+    c = O.Chem(gas, mech_record(md))
+    g = O.TPG(gas)
+    T = 700.0
+    c0 = 4.54
+    p = 2.0 * c0 * R_U * T
+    M = np.array(_json("H2-I2-HI.json")["M"])
+    molef = np.array([0.5, 0.5, 0.0])
+    mix = np.sum(molef * M)
+    massf = molef * M / mix
+    st = g.update("pT", massf, p=p, T=T)
+    rc, mf, dts, stats = c.update(st["rho"], p, T, st["u"], massf, 60_000.0, 200.0)
+    assert rc == 0
+    conc_hi = mf[2] * st["rho"] / M[2]
+    assert _isclose(7.14201983840, conc_hi, 1.0e-9)
+
+
+def _air_cells(n, seed=0, T_lo=1500.0, T_hi=12000.0):
+    from gdtk_amd.chemcases import air_cells
+    return air_cells(n, seed, T_lo, T_hi)
+
+
+def test_alpha_qss_air_conserves_mass_and_relaxes(oracle_lib):
+    """Park 1-T air: sum Y = 1, elements change only by the final renormalisation, and a
+    dissociating hot cell loses N2/O2 while T drops (endothermic) at constant rho, u."""
+    O = oracle_lib
+    from gdtk_amd.chem import element_matrix
+    gd, md = gdata.load("tpg-air-5sp"), gdata.load("park-air-5sp-1T")
+    gas, mech = gas_record(gd), mech_record(md)
+    cells = _air_cells(64, seed=3)
+    out = O.chem_step_batch(gas, mech, cells["rho"], cells["T"], cells["u"], cells["p"], cells["massf"],
+                            np.full(64, -1.0), 1.0e-7, nthreads=4)
+    assert out["failed"] == 0
+    Y = out["massf"].reshape(5, -1)
+    assert np.allclose(Y.sum(axis=0), 1.0, atol=1e-14)
+    E = element_matrix(gd["species"]) / np.array(gd["M"])[None, :]
+    e0 = E @ cells["massf"].reshape(5, -1)
+    e1 = E @ Y
+    assert np.max(np.abs(e1 - e0) / e0) < 2e-3
+    hot = cells["T"] > 8000.0
+    assert np.all(out["T"][hot] < cells["T"][hot])
+    assert np.all(out["substeps"] >= 1)
