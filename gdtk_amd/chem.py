@@ -116,3 +116,130 @@ def element_matrix(species):
     """Atoms of N and O per species (rows N, O) for the 5-species air species names."""
     atoms = {"N2": (2, 0), "O2": (0, 2), "NO": (1, 1), "N": (1, 0), "O": (0, 1)}
     return np.array([[atoms[s][0] for s in species], [atoms[s][1] for s in species]], dtype=np.float64)
+
+
+# --------------------------------------------------------------------------------------
+# Host of the chemistry entry points (include/gdtk_hip.h), the batched counterpart of
+# ThermochemicalReactor.opCall (src/kinetics/thermochemical_reactor.d:45-47) applied per
+# cell as FVCell.thermochemical_increment does (src/eilmer/fvcell.d:858-972).
+
+TUNE_PERSISTENT, TUNE_CHUNK, TUNE_REFILL, TUNE_BLOCKS_PER_CU = 1, 2, 3, 4
+TPG_MODES = {"pT": 0, "rhou": 1, "rhoT": 2}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_bound = False
+
+
+def _lib():
+    global _bound
+    from . import hip
+    L = hip.lib()
+    if not _bound:
+        L.gdtk_hip_chem_create.argtypes = [C.POINTER(TPGGas), C.POINTER(Mechanism), _ip]
+        L.gdtk_hip_chem_destroy.argtypes = [C.c_int]
+        L.gdtk_hip_chem_step.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _ip, _dp]
+        L.gdtk_hip_chem_step_device.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_double] + \
+            [C.c_void_p] * 6 + [C.c_void_p]
+        L.gdtk_hip_chem_stats.argtypes = [C.c_int, _dp, C.c_int]
+        L.gdtk_hip_chem_substeps.argtypes = [C.c_int, C.c_int, _ip]
+        L.gdtk_hip_chem_set_tuning.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.gdtk_hip_chem_autotune.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_double, _dp, C.c_int]
+        L.gdtk_hip_tpg_update.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        _bound = True
+    return L
+
+
+def _check(rc):
+    from . import hip
+    if rc < 0:
+        raise hip.HipError(rc, hip.lib().gdtk_hip_last_error().decode())
+    return rc
+
+
+def _arr(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class Chemistry:
+    """Thermally perfect gas + reaction mechanism on the GPU (gdtk_hip_chem_create)."""
+
+    def __init__(self, gas: TPGGas, mech: Mechanism):
+        self.gas, self.mech = gas, mech
+        self.nsp, self.nr = gas.nsp, mech.nr
+        h = C.c_int(-1)
+        _check(_lib().gdtk_hip_chem_create(C.byref(gas), C.byref(mech), C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", -1) >= 0:
+            _lib().gdtk_hip_chem_destroy(self.h)
+            self.h = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, rho, T, u, p, massf, dt_chem, dt):
+        """One thermochemical increment of n cells on host arrays (copied, returned)."""
+        n = len(rho)
+        rho = _arr(rho, n)
+        T, u, p, dtc = (_arr(a, n).copy() for a in (T, u, p, dt_chem))
+        mf = _arr(massf, self.nsp * n).copy()
+        status = np.zeros(n, dtype=np.int32)
+        stats = np.zeros(4)
+        failed = _check(_lib().gdtk_hip_chem_step(
+            self.h, n, rho.ctypes.data_as(_dp), T.ctypes.data_as(_dp), u.ctypes.data_as(_dp),
+            p.ctypes.data_as(_dp), mf.ctypes.data_as(_dp), dtc.ctypes.data_as(_dp), dt,
+            status.ctypes.data_as(_ip), stats.ctypes.data_as(_dp)))
+        return dict(T=T, u=u, p=p, massf=mf, dt_chem=dtc, status=status, failed=failed,
+                    substeps_total=int(stats[0]), iters_total=int(stats[1]), substeps_max=int(stats[2]),
+                    ms=float(stats[3]))
+
+    def step_device(self, n, rho, T, u, p, massf, dt_chem, dt, T_out, u_out, p_out, massf_out, dt_chem_out,
+                    status, stream=None):
+        """Asynchronous step on device pointers (ints, e.g. torch tensor.data_ptr())."""
+        _check(_lib().gdtk_hip_chem_step_device(self.h, n, rho, T, u, p, massf, dt_chem, dt, T_out, u_out, p_out,
+                                                massf_out, dt_chem_out, status, stream))
+
+    def stats(self):
+        v = np.zeros(8)
+        _check(_lib().gdtk_hip_chem_stats(self.h, v.ctypes.data_as(_dp), 8))
+        return dict(substeps_total=int(v[0]), iters_total=int(v[1]), substeps_max=int(v[2]),
+                    attempts_total=int(v[3]), ms_setup=v[4], ms_integrate=v[5], ms_finish=v[6], cells=int(v[7]))
+
+    def substeps(self, n):
+        out = np.zeros(n, dtype=np.int32)
+        _check(_lib().gdtk_hip_chem_substeps(self.h, n, out.ctypes.data_as(_ip)))
+        return out
+
+    def set_tuning(self, persistent=None, chunk=None, refill=None, blocks_per_cu=None):
+        for key, val in ((TUNE_PERSISTENT, persistent), (TUNE_CHUNK, chunk), (TUNE_REFILL, refill),
+                         (TUNE_BLOCKS_PER_CU, blocks_per_cu)):
+            if val is not None:
+                _check(_lib().gdtk_hip_chem_set_tuning(self.h, key, int(val)))
+
+    def autotune(self, n, rho, T, u, p, massf, dt_chem, dt):
+        out = np.zeros(5 * 64)
+        k = _check(_lib().gdtk_hip_chem_autotune(self.h, n, rho, T, u, p, massf, dt_chem, dt,
+                                                 out.ctypes.data_as(_dp), out.size))
+        return [dict(persistent=int(r[0]), chunk=int(r[1]), refill=int(r[2]), blocks_per_cu=int(r[3]), ms=r[4])
+                for r in out[:5 * k].reshape(k, 5)]
+
+    def tpg_update(self, mode, massf, rho=None, p=None, T=None, u=None):
+        """Gas-model update of n states: mode 'pT', 'rhou' or 'rhoT'; returns dict + failed count."""
+        mf = _arr(massf)
+        n = mf.size // self.nsp
+        z = np.zeros(n)
+        rho, p, T, u = (z.copy() if a is None else _arr(a, n).copy() for a in (rho, p, T, u))
+        a = np.zeros(n)
+        failed = _check(_lib().gdtk_hip_tpg_update(self.h, TPG_MODES[mode], n, rho.ctypes.data_as(_dp),
+                                                   p.ctypes.data_as(_dp), T.ctypes.data_as(_dp),
+                                                   u.ctypes.data_as(_dp), a.ctypes.data_as(_dp),
+                                                   mf.ctypes.data_as(_dp)))
+        return dict(rho=rho, p=p, T=T, u=u, a=a, failed=failed)

@@ -1,0 +1,432 @@
+// chem.cpp -- host side of the finite-rate chemistry C ABI (include/gdtk_hip.h).
+//
+// gdtk_hip_chem_create restates the set-up the reference does once per run:
+//   ThermallyPerfectGas + CEAThermoCurve construction   src/gas/therm_perf_gas.d:40-120,
+//                                                        gas/thermo/cea_thermo_curves.d:24-55
+//   ReactionMechanism (participants, net nu)            src/kinetics/reaction.d:58-143
+//   ChemistryUpdate configuration                       src/kinetics/chemistry_update.d:51-136
+// and gdtk_hip_chem_step is FVCell.thermochemical_increment (src/eilmer/fvcell.d:858-972)
+// over a batch of cells, the granularity of the reference's GPU plug-in boundary
+// GPUChem.thermochemical_increment (src/eilmer/cuda_gpu_chem.d:155-191).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/gdtk_hip.h"
+#include "chem_device.hpp"
+#include "chem_launch.hpp"
+#include "host_common.hpp"
+
+namespace gdtk {
+
+static_assert(GDTK_MAX_SPECIES == CH_MAX_SP && GDTK_MAX_REACTIONS == CH_MAX_R, "table sizes");
+
+struct Chem {
+    ChemDev host{};
+    ChemDev* dev = nullptr;
+    int nsp = 0, nr = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ChemTuning tuning;
+    DevArr<double> kfkb, conc0, h0, conc;
+    DevArr<int> flag, st, nsub;
+    DevArr<unsigned> queue;
+    DevArr<unsigned long long> stats;
+    // Staging of the host-array entry points.
+    DevArr<double> s_rho, s_T, s_u, s_p, s_massf, s_dtc, s_a;
+    DevArr<int> s_status;
+    // Last call: cells, kernel milliseconds (setup, integrate, finish), event stream.
+    int last_n = 0;
+    hipStream_t last_stream = nullptr;
+    bool have_events = false;
+
+    ~Chem() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (dev) (void)hipFree(dev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    ChemWork work(int n) {
+        const size_t N = (size_t)std::max(n, 1);
+        kfkb.reserve(2 * (size_t)std::max(nr, 1) * N);
+        conc0.reserve((size_t)nsp * N);
+        conc.reserve((size_t)nsp * N);
+        h0.reserve(N);
+        flag.reserve(N);
+        st.reserve(N);
+        nsub.reserve(N);
+        queue.reserve(8 * 32);
+        stats.reserve(4);
+        return ChemWork{kfkb.p, conc0.p, h0.p, conc.p, flag.p, st.p, nsub.p, queue.p, stats.p};
+    }
+
+    // The three kernels on device arrays; outputs may alias inputs.
+    void step_device(int n, double dt, const double* rho, const double* T, const double* u, const double* p,
+                     const double* massf, const double* dtc, double* T_out, double* u_out, double* p_out,
+                     double* massf_out, double* dtc_out, int* status, hipStream_t s) {
+        if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
+        if (!(dt > 0.0)) throw Error(GDTK_ERR_ARG, "dt must be positive");
+        ChemWork w = work(n);
+        HIPCHK(hipEventRecord(ev[0], s));
+        HIPCHK(launch_chem_clear(w, s));
+        HIPCHK(launch_chem_setup(dev, nsp, nr, n, dt, rho, T, massf, dtc, w, s));
+        HIPCHK(hipEventRecord(ev[1], s));
+        HIPCHK(launch_chem_integrate(dev, nsp, nr, n, dt, rho, dtc, dtc_out, w, tuning, num_cu, s));
+        HIPCHK(hipEventRecord(ev[2], s));
+        HIPCHK(launch_chem_finish(dev, nsp, n, rho, T, u, p, massf, dtc, T_out, u_out, p_out, massf_out, dtc_out,
+                                  status, w, s));
+        HIPCHK(hipEventRecord(ev[3], s));
+        last_n = n;
+        last_stream = s;
+        have_events = true;
+    }
+
+    // stats: {total substeps, total corrector iterations, max substeps, step attempts,
+    //         ms setup, ms integrate, ms finish, cells}
+    int read_stats(double* out, int nout) {
+        if (!have_events) throw Error(GDTK_ERR_ARG, "no chemistry step has run");
+        HIPCHK(hipEventSynchronize(ev[3]));
+        unsigned long long h[4] = {0, 0, 0, 0};
+        HIPCHK(hipMemcpy(h, stats.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        float ms[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+        const double v[8] = {(double)h[0], (double)h[1], (double)h[2], (double)h[3],
+                             ms[0], ms[1], ms[2], (double)last_n};
+        int m = std::min(nout, 8);
+        for (int k = 0; k < m; ++k) out[k] = v[k];
+        return m;
+    }
+};
+
+static std::mutex g_chem_mu;
+static std::vector<std::unique_ptr<Chem>> g_chems;
+
+static Chem& get_chem(int h) {
+    std::lock_guard<std::mutex> lk(g_chem_mu);
+    if (h < 0 || h >= (int)g_chems.size() || !g_chems[h]) throw Error(GDTK_ERR_ARG, "bad chemistry handle");
+    return *g_chems[h];
+}
+
+// Gas and mechanism records -> the device table, with the checks the reference's input
+// readers make (species indices, segment counts) and the options the kernels implement.
+static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDev& cd) {
+    std::memset(&cd, 0, sizeof(cd));
+    if (g.nsp < 1 || g.nsp > CH_MAX_SP) throw Error(GDTK_ERR_ARG, "nsp must be 1..8");
+    if (m.nr < 0 || m.nr > CH_MAX_R) throw Error(GDTK_ERR_ARG, "nr must be 0..32");
+    cd.nsp = g.nsp;
+    cd.nr = m.nr;
+    for (int i = 0; i < g.nsp; ++i) {
+        if (!(g.M[i] > 0.0)) throw Error(GDTK_ERR_ARG, "molecular mass must be positive");
+        cd.M[i] = g.M[i];
+        cd.R[i] = CH_R_UNIVERSAL / g.M[i];  // therm_perf_gas.d:62-64
+        CEADev& c = cd.cea[i];
+        const int ns = g.nseg[i];
+        if (ns < 2 || ns > 3) throw Error(GDTK_ERR_ARG, "CEA curves need 2 or 3 segments");
+        c.nseg = ns;
+        for (int k = 0; k <= ns; ++k) c.T_breaks[k] = g.T_breaks[i][k];
+        for (int k = 0; k < ns - 1; ++k) c.T_blends[k] = g.T_blends[i][k];
+        for (int s = 0; s < ns; ++s)
+            for (int j = 0; j < 9; ++j) c.coeffs[s][j] = g.coeffs[i][s][j];
+        for (int k = 0; k < ns; ++k)
+            if (!(c.T_breaks[k] < c.T_breaks[k + 1])) throw Error(GDTK_ERR_ARG, "CEA breaks must increase");
+        // cea_thermo_curves.d:45-54
+        c.R = cd.R[i];
+        c.T_low = c.T_breaks[0];
+        c.T_high = c.T_breaks[ns];
+        c.Cp_low = cea_Cp(c, c.T_low);
+        c.Cp_high = cea_Cp(c, c.T_high);
+        c.h_low = cea_h(c, c.T_low);
+        c.h_high = cea_h(c, c.T_high);
+        c.s_low = cea_s(c, c.T_low);
+        c.s_high = cea_s(c, c.T_high);
+    }
+    for (int ir = 0; ir < m.nr; ++ir) {
+        const gdtk_reaction& r = m.r[ir];
+        ReacDev& d = cd.r[ir];
+        if (r.nreac < 1 || r.nreac > 3 || r.nprod < 1 || r.nprod > 3 || r.neff < 0 || r.neff > CH_MAX_SP)
+            throw Error(GDTK_ERR_ARG, "reaction participant counts out of range");
+        d.collider = r.collider ? 1 : 0;
+        d.nreac = r.nreac;
+        d.nprod = r.nprod;
+        for (int k = 0; k < r.nreac; ++k) {
+            if (r.reac_sp[k] < 0 || r.reac_sp[k] >= g.nsp || r.reac_nu[k] < 0)
+                throw Error(GDTK_ERR_ARG, "bad reactant");
+            d.reac_mask[k] = 1u << r.reac_sp[k];
+            d.reac_nu[k] = r.reac_nu[k];
+        }
+        for (int k = 0; k < r.nprod; ++k) {
+            if (r.prod_sp[k] < 0 || r.prod_sp[k] >= g.nsp || r.prod_nu[k] < 0) throw Error(GDTK_ERR_ARG, "bad product");
+            d.prod_mask[k] = 1u << r.prod_sp[k];
+            d.prod_nu[k] = r.prod_nu[k];
+        }
+        d.neff = d.collider ? r.neff : 0;
+        for (int k = 0; k < d.neff; ++k) {
+            if (r.eff_sp[k] < 0 || r.eff_sp[k] >= g.nsp) throw Error(GDTK_ERR_ARG, "bad efficiency species");
+            d.eff_mask[k] = 1u << r.eff_sp[k];
+            d.eff[k] = r.eff[k];
+        }
+        d.f_from_eq = r.f_from_eq ? 1 : 0;
+        d.b_from_eq = r.b_from_eq ? 1 : 0;
+        if (d.f_from_eq && d.b_from_eq) throw Error(GDTK_ERR_ARG, "both rate constants from K_c");
+        d.fA = r.fA; d.fn = r.fn; d.fC = r.fC;
+        d.bA = r.bA; d.bn = r.bn; d.bC = r.bC;
+        // reaction.d:100-143: participants and net coefficients (first listing of a species)
+        for (int s = 0; s < g.nsp; ++s) {
+            int nu1 = 0, nu2 = 0, in = 0;
+            for (int k = 0; k < r.nreac; ++k)
+                if (r.reac_sp[k] == s) { nu1 = r.reac_nu[k]; in = 1; break; }
+            for (int k = 0; k < r.nprod; ++k)
+                if (r.prod_sp[k] == s) { nu2 = r.prod_nu[k]; in = 1; break; }
+            d.nu[s] = nu2 - nu1;
+            d.part[s] = in;
+        }
+    }
+    if (!(m.T_lower < m.T_upper)) throw Error(GDTK_ERR_ARG, "tempLimits lower must be below upper");
+    cd.T_lower = m.T_lower;
+    cd.T_upper = m.T_upper;
+    if (m.method != GDTK_ODE_ALPHA_QSS && m.method != GDTK_ODE_RKF) throw Error(GDTK_ERR_ARG, "bad ODE method");
+    if (m.tightTempCoupling) throw Error(GDTK_ERR_UNSUPPORTED, "tightTempCoupling is not implemented");
+    if (m.maxIters < 1 || m.maxSubcycles < 1 || m.maxAttempts < 1) throw Error(GDTK_ERR_ARG, "bad step control");
+    cd.method = m.method;
+    cd.eps1 = m.eps1;
+    cd.eps2 = m.eps2;
+    cd.delta = m.delta;
+    cd.maxIters = m.maxIters;
+    cd.errTol = m.errTol;
+    cd.maxSubcycles = m.maxSubcycles;
+    cd.maxAttempts = m.maxAttempts;
+    cd.T_frozen = m.T_frozen;
+}
+
+template <class T>
+static void h2d(DevArr<T>& d, const T* h, size_t count, hipStream_t s) {
+    d.reserve(count);
+    if (count) HIPCHK(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+static void d2h(T* h, const DevArr<T>& d, size_t count, hipStream_t s) {
+    if (count) HIPCHK(hipMemcpyAsync(h, d.p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+}  // namespace gdtk
+
+using namespace gdtk;
+
+extern "C" {
+
+int gdtk_hip_chem_create(const gdtk_tpg_gas* gas, const gdtk_mechanism* mech, int* chem) {
+    return guarded([&] {
+        if (!gas || !mech || !chem) throw Error(GDTK_ERR_ARG, "null argument");
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw Error(GDTK_ERR_HIP, "no HIP device");
+        auto c = std::make_unique<Chem>();
+        build_chemdev(*gas, *mech, c->host);
+        c->nsp = c->host.nsp;
+        c->nr = c->host.nr;
+        if (!chem_supported(c->nsp, c->nr)) throw Error(GDTK_ERR_UNSUPPORTED, "mechanism size not supported");
+        int devid = 0;
+        HIPCHK(hipGetDevice(&devid));
+        HIPCHK(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, devid));
+        HIPCHK(hipMalloc(&c->dev, sizeof(ChemDev)));
+        HIPCHK(hipMemcpy(c->dev, &c->host, sizeof(ChemDev), hipMemcpyHostToDevice));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+        std::lock_guard<std::mutex> lk(g_chem_mu);
+        g_chems.push_back(std::move(c));
+        *chem = (int)g_chems.size() - 1;
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_chem_destroy(int chem) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_chem_mu);
+        if (chem < 0 || chem >= (int)g_chems.size() || !g_chems[chem]) throw Error(GDTK_ERR_ARG, "bad chemistry handle");
+        HIPCHK(hipStreamSynchronize(g_chems[chem]->stream));
+        g_chems[chem].reset();
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_chem_step(int chem, int n, const double* rho, double* T, double* u, double* p, double* massf,
+                       double* dt_chem, double dt, int* status, double* stats) {
+    return guarded([&] {
+        Chem& c = get_chem(chem);
+        if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
+        if (n > 0 && (!rho || !T || !u || !p || !massf || !dt_chem || !status)) throw Error(GDTK_ERR_ARG, "null array");
+        hipStream_t s = c.stream;
+        const size_t N = (size_t)n, NS = (size_t)c.nsp * N;
+        h2d(c.s_rho, rho, N, s);
+        h2d(c.s_T, (const double*)T, N, s);
+        h2d(c.s_u, (const double*)u, N, s);
+        h2d(c.s_p, (const double*)p, N, s);
+        h2d(c.s_massf, (const double*)massf, NS, s);
+        h2d(c.s_dtc, (const double*)dt_chem, N, s);
+        c.s_status.reserve(N);
+        c.step_device(n, dt, c.s_rho.p, c.s_T.p, c.s_u.p, c.s_p.p, c.s_massf.p, c.s_dtc.p, c.s_T.p, c.s_u.p, c.s_p.p,
+                      c.s_massf.p, c.s_dtc.p, c.s_status.p, s);
+        d2h(T, c.s_T, N, s);
+        d2h(u, c.s_u, N, s);
+        d2h(p, c.s_p, N, s);
+        d2h(massf, c.s_massf, NS, s);
+        d2h(dt_chem, c.s_dtc, N, s);
+        d2h(status, c.s_status, N, s);
+        HIPCHK(hipStreamSynchronize(s));
+        if (stats) {
+            double v[8];
+            c.read_stats(v, 8);
+            stats[0] = v[0];
+            stats[1] = v[1];
+            stats[2] = v[2];
+            stats[3] = v[4] + v[5] + v[6];
+        }
+        int failed = 0;
+        for (int i = 0; i < n; ++i) failed += (status[i] != 0);
+        return failed;
+    });
+}
+
+int gdtk_hip_chem_step_device(int chem, int n, const double* rho, const double* T, const double* u, const double* p,
+                              const double* massf, const double* dt_chem, double dt, double* T_out, double* u_out,
+                              double* p_out, double* massf_out, double* dt_chem_out, int* status, void* stream) {
+    return guarded([&] {
+        Chem& c = get_chem(chem);
+        if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
+        if (n > 0 && (!rho || !T || !u || !p || !massf || !dt_chem || !T_out || !u_out || !p_out || !massf_out ||
+                      !dt_chem_out || !status))
+            throw Error(GDTK_ERR_ARG, "null array");
+        hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the default stream
+        c.step_device(n, dt, rho, T, u, p, massf, dt_chem, T_out, u_out, p_out, massf_out, dt_chem_out, status, s);
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_chem_stats(int chem, double* stats, int nstats) {
+    return guarded([&] {
+        Chem& c = get_chem(chem);
+        if (!stats || nstats < 1) throw Error(GDTK_ERR_ARG, "null stats");
+        return c.read_stats(stats, nstats);
+    });
+}
+
+int gdtk_hip_chem_substeps(int chem, int n, int* substeps) {
+    return guarded([&] {
+        Chem& c = get_chem(chem);
+        if (n != c.last_n || !substeps) throw Error(GDTK_ERR_ARG, "n must be the cell count of the last step");
+        HIPCHK(hipEventSynchronize(c.ev[3]));
+        // Cells that were not integrated (frozen, invalid) report 0 substeps.
+        std::vector<int> f((size_t)n);
+        HIPCHK(hipMemcpy(f.data(), c.flag.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(substeps, c.nsub.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i)
+            if (f[i] != CF_INTEGRATE) substeps[i] = 0;
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_chem_set_tuning(int chem, int key, int value) {
+    return guarded([&] {
+        Chem& c = get_chem(chem);
+        switch (key) {
+        case GDTK_CHEM_TUNE_PERSISTENT: c.tuning.persistent = value ? 1 : 0; break;
+        case GDTK_CHEM_TUNE_CHUNK:
+            if (value < 1 || value > 4096) throw Error(GDTK_ERR_ARG, "chunk must be 1..4096");
+            c.tuning.chunk = value;
+            break;
+        case GDTK_CHEM_TUNE_REFILL:
+            if (value < 1 || value > 4096) throw Error(GDTK_ERR_ARG, "refill period must be 1..4096");
+            c.tuning.refill_period = value;
+            break;
+        case GDTK_CHEM_TUNE_BLOCKS_PER_CU:
+            if (value < 0 || value > 64) throw Error(GDTK_ERR_ARG, "blocks per CU must be 0..64");
+            c.tuning.blocks_per_cu = value;
+            break;
+        default: throw Error(GDTK_ERR_ARG, "unknown tuning key");
+        }
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_chem_autotune(int chem, int n, const double* rho, const double* T, const double* u, const double* p,
+                           const double* massf, const double* dt_chem, double dt, double* out, int nout) {
+    return guarded([&] {
+        Chem& c = get_chem(chem);
+        if (n <= 0 || !rho || !T || !u || !p || !massf || !dt_chem) throw Error(GDTK_ERR_ARG, "bad arguments");
+        const size_t N = (size_t)n, NS = (size_t)c.nsp * N;
+        // Scratch outputs: the inputs are left untouched.
+        DevArr<double> To, uo, po, mo, dto;
+        DevArr<int> so;
+        To.alloc(N); uo.alloc(N); po.alloc(N); mo.alloc(NS); dto.alloc(N); so.alloc(N);
+        struct Cand { int persistent, chunk, refill, bpc; };
+        std::vector<Cand> cands = {{0, 64, 1, 0}};
+        for (int chunk : {32, 64, 256})
+            for (int refill : {1, 4, 16}) cands.push_back({1, chunk, refill, 0});
+        ChemTuning saved = c.tuning;
+        double best = 1e300;
+        ChemTuning best_t = saved;
+        int k = 0;
+        hipStream_t s = c.stream;
+        for (const Cand& cd : cands) {
+            c.tuning = ChemTuning{cd.persistent, cd.chunk, cd.refill, cd.bpc};
+            c.step_device(n, dt, rho, T, u, p, massf, dt_chem, To.p, uo.p, po.p, mo.p, dto.p, so.p, s);
+            double v[8];
+            c.read_stats(v, 8);
+            const double ms = v[4] + v[5] + v[6];
+            if (out && 5 * (k + 1) <= nout) {
+                out[5 * k + 0] = cd.persistent;
+                out[5 * k + 1] = cd.chunk;
+                out[5 * k + 2] = cd.refill;
+                out[5 * k + 3] = cd.bpc;
+                out[5 * k + 4] = ms;
+            }
+            ++k;
+            if (ms < best) {
+                best = ms;
+                best_t = c.tuning;
+            }
+        }
+        c.tuning = best_t;
+        HIPCHK(hipStreamSynchronize(s));
+        return k;
+    });
+}
+
+int gdtk_hip_tpg_update(int chem, int mode, int n, double* rho, double* p, double* T, double* u, double* a,
+                        const double* massf) {
+    return guarded([&] {
+        Chem& c = get_chem(chem);
+        if (mode < 0 || mode > 2) throw Error(GDTK_ERR_ARG, "mode must be 0 (pT), 1 (rhou) or 2 (rhoT)");
+        if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
+        if (n > 0 && (!rho || !p || !T || !u || !a || !massf)) throw Error(GDTK_ERR_ARG, "null array");
+        hipStream_t s = c.stream;
+        const size_t N = (size_t)n, NS = (size_t)c.nsp * N;
+        h2d(c.s_rho, (const double*)rho, N, s);
+        h2d(c.s_p, (const double*)p, N, s);
+        h2d(c.s_T, (const double*)T, N, s);
+        h2d(c.s_u, (const double*)u, N, s);
+        h2d(c.s_a, (const double*)a, N, s);
+        h2d(c.s_massf, massf, NS, s);
+        c.s_status.reserve(N);
+        HIPCHK(launch_tpg_update(c.dev, c.nsp, mode, n, c.s_rho.p, c.s_p.p, c.s_T.p, c.s_u.p, c.s_a.p, c.s_massf.p,
+                                 c.s_status.p, s));
+        d2h(rho, c.s_rho, N, s);
+        d2h(p, c.s_p, N, s);
+        d2h(T, c.s_T, N, s);
+        d2h(u, c.s_u, N, s);
+        d2h(a, c.s_a, N, s);
+        std::vector<int> st(N);
+        if (N) HIPCHK(hipMemcpyAsync(st.data(), c.s_status.p, N * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        int failed = 0;
+        for (int v : st) failed += (v != 0);
+        return failed;
+    });
+}
+
+}  // extern "C"

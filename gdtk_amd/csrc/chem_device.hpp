@@ -1,0 +1,531 @@
+// chem_device.hpp -- device-side thermally perfect gas and finite-rate chemistry (gfx950).
+//
+// One lane advances one cell.  The gas and mechanism tables (ChemDev) live in a read-only
+// device buffer; every lane of a wave reads the same entries, so those loads are uniform
+// (scalar).  Per-cell state is held in registers: arrays are sized by the template
+// parameters NSP (species, exact) and NRM (an upper bound on the reactions, the actual
+// count cd->nr is uniform at run time).  FP64 throughout, built without FMA contraction;
+// the operation order of every expression follows the reference function cited beside it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace gdtk {
+
+constexpr int CH_MAX_SP = 8;
+constexpr int CH_MAX_R = 32;
+
+// src/gas/thermo/cea_thermo_curves.d:24-176; T_low..s_high are the constructor's
+// end-point values (:45-54).
+struct CEADev {
+    int nseg;
+    double T_breaks[4], T_blends[2], coeffs[3][9];
+    double R, T_low, T_high, Cp_low, Cp_high, h_low, h_high, s_low, s_high;
+};
+
+// src/kinetics/reaction.d:58-457.  The reactant, product and efficiency lists keep the
+// reference's order; each entry names its species by a one-hot mask (bit s = species s), so
+// that device code selects concentrations without indexing a register array.  nu[s] is the
+// net stoichiometric coefficient of species s and part[s] marks the participants
+// (reaction.d:100-143).
+struct ReacDev {
+    int collider;
+    int nreac, reac_nu[3];
+    unsigned reac_mask[3];
+    int nprod, prod_nu[3];
+    unsigned prod_mask[3];
+    int neff;
+    unsigned eff_mask[CH_MAX_SP];
+    double eff[CH_MAX_SP];
+    int f_from_eq, b_from_eq;
+    double fA, fn, fC, bA, bn, bC;
+    int nu[CH_MAX_SP];
+    int part[CH_MAX_SP];
+};
+
+struct ChemDev {
+    int nsp, nr;
+    double M[CH_MAX_SP], R[CH_MAX_SP];
+    CEADev cea[CH_MAX_SP];
+    ReacDev r[CH_MAX_R];
+    double T_lower, T_upper;
+    int method;  // 0 alpha-QSS, 1 RKF
+    double eps1, eps2, delta;
+    int maxIters;
+    double errTol;
+    int maxSubcycles, maxAttempts;
+    double T_frozen;
+};
+
+constexpr double CH_R_UNIVERSAL = 8.31451;               // gas/physical_constants.d
+constexpr double CH_P_ATM = 101.325e3;
+constexpr double CH_T_MIN = 10.0, CH_T_MAX = 100000.0;   // gas/gas_model.d:45-46
+constexpr double CH_MIN_MOLES = 1.0e-30;                 // gas/gas_model.d:44
+constexpr double CH_MIN_MASSF = 1.0e-30;                 // gas/gas_model.d:43
+
+// The element of a register array named by a one-hot mask (wave-uniform): a chain of
+// selects over compile-time indices, so the array stays in registers.
+template <int N>
+__device__ __forceinline__ double pick(const double* a, unsigned mask) {
+    double r = 0.0;
+#pragma unroll
+    for (int s = 0; s < N; ++s) r = ((mask >> s) & 1u) ? a[s] : r;
+    return r;
+}
+
+// cea_thermo_curves.d:137-176.  The reference throws where no segment matches (only a NaN
+// temperature gets there with contiguous breaks); NaN coefficients stand in for that here.
+__host__ __device__ inline void cea_coeffs(const CEADev& c, double T, double a[9]) {
+    const int nb = c.nseg + 1;
+    if (T < (c.T_breaks[1] - 0.5 * c.T_blends[0])) {
+        for (int j = 0; j < 9; ++j) a[j] = c.coeffs[0][j];
+        return;
+    }
+    if (T > (c.T_breaks[nb - 2] + 0.5 * c.T_blends[c.nseg - 2])) {
+        for (int j = 0; j < 9; ++j) a[j] = c.coeffs[c.nseg - 1][j];
+        return;
+    }
+    for (int i = 1; i < nb - 1; ++i) {
+        double T_blend_low = c.T_breaks[i] - 0.5 * c.T_blends[i - 1];
+        double T_blend_high = c.T_breaks[i] + 0.5 * c.T_blends[i - 1];
+        if (T >= T_blend_low && T <= T_blend_high) {
+            double wB = (1. / c.T_blends[i - 1]) * (T - T_blend_low);
+            double wA = 1.0 - wB;
+            for (int j = 0; j < 9; ++j) a[j] = wA * c.coeffs[i - 1][j] + wB * c.coeffs[i][j];
+            return;
+        }
+        if (i < c.nseg - 1 && T > T_blend_high && T < (c.T_breaks[i + 1] - 0.5 * c.T_blends[i])) {
+            for (int j = 0; j < 9; ++j) a[j] = c.coeffs[i][j];
+            return;
+        }
+    }
+    for (int j = 0; j < 9; ++j) a[j] = __builtin_nan("");
+}
+
+// cea_thermo_curves.d:56-70
+__host__ __device__ inline double cea_Cp(const CEADev& c, double T) {
+    if (T < c.T_low) return c.Cp_low;
+    if (T > c.T_high) return c.Cp_high;
+    double a[9];
+    cea_coeffs(c, T, a);
+    double Cp_on_R = a[0] / (T * T) + a[1] / T + a[2] + a[3] * T;
+    Cp_on_R += a[4] * T * T + a[5] * T * T * T + a[6] * T * T * T * T;
+    return c.R * Cp_on_R;
+}
+
+// cea_thermo_curves.d:71-102
+__host__ __device__ inline double cea_h(const CEADev& c, double T) {
+    if (T < c.T_low) return c.h_low - c.Cp_low * (c.T_low - T);
+    if (T > c.T_high) return c.h_high + c.Cp_high * (T - c.T_high);
+    double a[9];
+    cea_coeffs(c, T, a);
+    double logT = log(T);
+    double h_on_RT = -a[0] / T + a[1] * logT + a[2] * T + a[3] * T * T / 2.0;
+    h_on_RT += a[4] * T * T * T / 3.0 + a[5] * T * T * T * T / 4.0 + a[6] * T * T * T * T * T / 5.0 + a[7];
+    return c.R * h_on_RT;
+}
+
+// cea_thermo_curves.d:103-130
+__host__ __device__ inline double cea_s(const CEADev& c, double T) {
+    if (T < c.T_low) return c.s_low - c.Cp_low * log(c.T_low / T);
+    if (T > c.T_high) return c.s_high + c.Cp_high * log(T / c.T_high);
+    double a[9];
+    cea_coeffs(c, T, a);
+    double logT = log(T);
+    double s_on_R = -a[0] / (2.0 * T * T) - a[1] / T + a[2] * logT + a[3] * T;
+    s_on_R += a[4] * T * T / 2.0 + a[5] * T * T * T / 3.0 + a[6] * T * T * T * T / 4.0 + a[8];
+    return c.R * s_on_R;
+}
+
+// therm_perf_gas_mix_eos.d:61-67 with gas_model.d:410-419 (mass_average)
+template <int NSP>
+__device__ __forceinline__ double tpg_energy(const ChemDev& cd, const double* massf, double T) {
+    double u = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) u += massf[i] * (cea_h(cd.cea[i], T) - cd.R[i] * T);
+    return u;
+}
+template <int NSP>
+__device__ __forceinline__ double tpg_Cv(const ChemDev& cd, const double* massf, double T) {
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) v += massf[i] * (cea_Cp(cd.cea[i], T) - cd.R[i]);
+    return v;
+}
+template <int NSP>
+__device__ __forceinline__ double tpg_Cp(const ChemDev& cd, const double* massf, double T) {
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) v += massf[i] * cea_Cp(cd.cea[i], T);
+    return v;
+}
+// perf_gas_mix_eos.d:43-76
+template <int NSP>
+__device__ __forceinline__ double tpg_Rmix(const ChemDev& cd, const double* massf) {
+    double R = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) R += massf[i] * cd.R[i];
+    return R;
+}
+
+// therm_perf_gas_mix_eos.d:68-151 with the safeguarded Newton of nm/newton.d:71-140.
+// T and u are in/out as in the reference: every evaluation of the zero function writes T
+// and u, and a converged root then overwrites T, so u ends as the energy of the last
+// evaluated temperature.  Returns false where the reference throws; T is then restored
+// and u = energy(T), which is the state the reference leaves behind.
+template <int NSP>
+__device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const double* massf, double& T, double& u) {
+    const double Tsave = T;
+    const double TOL = 1.0e-6;
+    const double e_tgt = u;
+    const double delT = 1000.0;
+    const double T1 = fmax(T - 0.5 * delT, CH_T_MIN);
+    const double T2 = T1 + delT;
+    for (int pass = 0; pass < 2; ++pass) {
+        const double xMin = (pass == 0) ? T1 : CH_T_MIN;
+        const double xMax = (pass == 0) ? T2 : CH_T_MAX;
+        double xL = xMin, xH = xMax;
+        T = xL;
+        u = tpg_energy<NSP>(cd, massf, T);
+        const double fL = e_tgt - u;
+        T = xH;
+        u = tpg_energy<NSP>(cd, massf, T);
+        const double fH = e_tgt - u;
+        if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) continue;
+        if (fL == 0.0) { T = xMin; return true; }
+        if (fH == 0.0) { T = xMax; return true; }
+        if (fL < 0.0) { xL = xMin; xH = xMax; }
+        else { xH = xMin; xL = xMax; }
+        double rts = Tsave;
+        double dxold = (xMax - xMin);
+        double dx = dxold;
+        T = rts;
+        u = tpg_energy<NSP>(cd, massf, T);
+        double f0 = e_tgt - u;
+        double df0 = -1.0 * tpg_Cv<NSP>(cd, massf, rts);
+        for (int j = 0; j < 30; ++j) {
+            if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
+                dxold = dx;
+                dx = 0.5 * (xH - xL);
+                rts = xL + dx;
+                if (xL == rts) { T = rts; return true; }
+            } else {
+                dxold = dx;
+                dx = f0 / df0;
+                double tmp = rts;
+                rts -= dx;
+                if (tmp == rts) { T = rts; return true; }
+            }
+            if (fabs(dx) < TOL) { T = rts; return true; }
+            T = rts;
+            u = tpg_energy<NSP>(cd, massf, T);
+            f0 = e_tgt - u;
+            df0 = -1.0 * tpg_Cv<NSP>(cd, massf, rts);
+            if (f0 < 0.0) xL = rts;
+            else xH = rts;
+        }
+    }
+    T = Tsave;
+    u = tpg_energy<NSP>(cd, massf, T);
+    return false;
+}
+
+// therm_perf_gas.d:394-404: frozen sound speed sqrt(gamma * (R T)), gamma = Cp/Cv.
+template <int NSP>
+__device__ __forceinline__ double tpg_sound_speed(const ChemDev& cd, const double* massf, double T) {
+    double gamma = tpg_Cp<NSP>(cd, massf, T) / tpg_Cv<NSP>(cd, massf, T);
+    double R = tpg_Rmix<NSP>(cd, massf);
+    return sqrt(gamma * (R * T));
+}
+
+// c^n for the small whole stoichiometric coefficients of reaction.d's rate products.
+__device__ __forceinline__ double ch_ipow(double c, int n) {
+    return (n == 1) ? c : (n == 2) ? c * c : (n == 3) ? c * (c * c) : (n == 0) ? 1.0 : pow(c, (double)n);
+}
+// pow(P_atm/(R_u T), nuSum) of reaction.d:43-44 for whole nuSum.
+__device__ __forceinline__ double ch_pow_nusum(double x, double nuSum) {
+    return (nuSum == 0.0) ? 1.0 : (nuSum == 1.0) ? x : (nuSum == -1.0) ? 1.0 / x : (nuSum == 2.0) ? x * x : pow(x, nuSum);
+}
+
+// reaction_mechanism.d:50-63 with reaction.d:27-52 (equilibrium constant), 94-143 and
+// rate_constant.d (Arrhenius A T^n exp(-C/T)); T clamped to the mechanism's tempLimits.
+// Participants are summed in ascending species order, the order of the reference's
+// sorted participant list.
+template <int NSP, int NRM>
+__device__ __forceinline__ void eval_rate_constants(const ChemDev& cd, double Tin, double* kf, double* kb) {
+    double T = Tin;
+    if (T < cd.T_lower) T = cd.T_lower;
+    if (T > cd.T_upper) T = cd.T_upper;
+    double g[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        double h = cea_h(cd.cea[s], T);
+        double sv = cea_s(cd.cea[s], T);
+        g[s] = h - T * sv;
+    }
+#pragma unroll
+    for (int ir = 0; ir < NRM; ++ir) {
+        if (ir >= cd.nr) continue;
+        const ReacDev& r = cd.r[ir];
+        double Keq = 0.0;
+        if (r.f_from_eq || r.b_from_eq) {
+            double dG = 0.0, nuSum = 0.0;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) {
+                if (!r.part[s]) continue;
+                dG += r.nu[s] * g[s] * cd.M[s];
+                nuSum += r.nu[s];
+            }
+            double K_p = exp(-dG / (CH_R_UNIVERSAL * T));
+            Keq = K_p * ch_pow_nusum(CH_P_ATM / (CH_R_UNIVERSAL * T), nuSum);
+        }
+        double kfv, kbv;
+        if (!r.f_from_eq) {
+            kfv = r.fA * pow(T, r.fn) * exp(-r.fC / T);
+            kbv = r.b_from_eq ? kfv / Keq : r.bA * pow(T, r.bn) * exp(-r.bC / T);
+        } else {
+            kbv = r.bA * pow(T, r.bn) * exp(-r.bC / T);
+            kfv = kbv * Keq;
+        }
+        kf[ir] = kfv;
+        kb[ir] = kbv;
+    }
+}
+
+// reaction_mechanism.d:83-96 (eval_split_rates) with reaction.d's forward/backward rates
+// (collider sum over the efficiency list, then the concentration products in list order)
+// and production/loss per participant in reaction order.
+template <int NSP, int NRM>
+__device__ __forceinline__ void eval_split_rates(const ChemDev& cd, const double* kf, const double* kb,
+                                                 const double* conc, double* q, double* L) {
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) { q[s] = 0.0; L[s] = 0.0; }
+#pragma unroll
+    for (int ir = 0; ir < NRM; ++ir) {
+        if (ir >= cd.nr) continue;
+        const ReacDev& r = cd.r[ir];
+        double collider = 0.0;
+        if (r.collider) {
+#pragma unroll
+            for (int e = 0; e < NSP; ++e)
+                if (e < r.neff) collider += r.eff[e] * pick<NSP>(conc, r.eff_mask[e]);
+        }
+        double wf = r.collider ? kf[ir] * collider : kf[ir];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (k < r.nreac) wf *= ch_ipow(pick<NSP>(conc, r.reac_mask[k]), r.reac_nu[k]);
+        double wb = r.collider ? kb[ir] * collider : kb[ir];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (k < r.nprod) wb *= ch_ipow(pick<NSP>(conc, r.prod_mask[k]), r.prod_nu[k]);
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+            const int nu = r.nu[s];
+            if (nu > 0) {
+                q[s] += nu * wf;
+                L[s] += nu * wb;
+            } else if (nu < 0) {
+                q[s] += -nu * wb;
+                L[s] += -nu * wf;
+            }
+        }
+    }
+}
+
+// reaction_mechanism.d:162-203
+template <int NSP, int NRM>
+__device__ __forceinline__ double estimate_step_size(const ChemDev& cd, const double* kf, const double* kb, const double* conc) {
+    const double MUCH_LARGER_FACTOR = 10000.0, ZERO_EPS = 1.0e-50, ZERO_TOL = 1.0e-30, EPS1 = 1.0e-3;
+    double q[NSP], L[NSP];
+    eval_split_rates<NSP, NRM>(cd, kf, kb, conc, q, L);
+    double min_dt = 1.0e6, dt;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        double c = conc[s];
+        if (c > 0.0 && (fabs(q[s] - L[s]) > ZERO_TOL)) {
+            if (q[s] > MUCH_LARGER_FACTOR * L[s]) dt = c / (L[s] + ZERO_EPS);
+            else dt = fabs(c / (q[s] - L[s]));
+            min_dt = (dt < min_dt) ? dt : min_dt;
+        }
+    }
+    double dt_chem = EPS1 * min_dt;
+    dt_chem = (1.0e-10 < dt_chem) ? 1.0e-10 : dt_chem;
+    dt_chem = (dt_chem < 1.0e-15) ? 1.0e-15 : dt_chem;
+    return dt_chem;
+}
+
+// alpha(r) of chemistry_update.d:599-604 for r = 1/(p h + eps)
+__device__ __forceinline__ double qss_alpha(double p, double h) {
+    const double ZERO_EPS = 1.0e-50;
+    double r = 1.0 / (p * h + ZERO_EPS);
+    return (180.0 * r * r * r + 60.0 * r * r + 11.0 * r + 1.0) / (360.0 * r * r * r + 60.0 * r * r + 12.0 * r + 1.0);
+}
+
+// AlphaQssStep.opCall, chemistry_update.d:533-655: predictor, up to maxIters correctors
+// with alpha-bar from p_p, convergence against the first predictor, and the step
+// suggestion from three Newton iterations for sqrt(sigma).  yOut is written on success.
+template <int NSP, int NRM>
+__device__ __forceinline__ bool alpha_qss_step(const ChemDev& cd, const double* kf, const double* kb, const double* y0, double h,
+                               double* yOut, double& hSuggest, int& iters) {
+    const double ZERO_EPS = 1.0e-50;
+    double q0[NSP], L0[NSP], p0[NSP], yp[NSP], yp0[NSP], yc[NSP];
+    eval_split_rates<NSP, NRM>(cd, kf, kb, y0, q0, L0);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        p0[s] = L0[s] / (y0[s] + ZERO_EPS);
+        double alpha = qss_alpha(p0[s], h);
+        yp[s] = y0[s] + (h * (q0[s] - p0[s] * y0[s])) / (1.0 + alpha * h * p0[s]);
+        yp0[s] = yp[s];
+        yc[s] = yp[s];
+    }
+    bool converged = false;
+    for (int corr = 0; corr < cd.maxIters; ++corr) {
+        ++iters;
+        double qp[NSP], Lp[NSP];
+        eval_split_rates<NSP, NRM>(cd, kf, kb, yp, qp, Lp);
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+            double pp = Lp[s] / (yp[s] + ZERO_EPS);
+            double ab = qss_alpha(pp, h);
+            double qt = ab * qp[s] + (1 - ab) * q0[s];
+            double pb = 0.5 * (p0[s] + pp);
+            yc[s] = y0[s] + (h * (qt - pb * y0[s])) / (1.0 + ab * h * pb);
+        }
+        bool passes = true;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+            if (yc[s] < ZERO_EPS) continue;
+            double test = fabs(yc[s] - yp0[s]);
+            if (test >= (cd.eps1 * (yc[s] + cd.delta))) passes = false;
+        }
+        if (passes) {
+            converged = true;
+            break;
+        }
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) yp[s] = yc[s];
+    }
+    double sigma = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        if (yc[s] < ZERO_EPS) continue;
+        double test = fabs(yc[s] - yp0[s]) / (cd.eps2 * (yc[s] + cd.delta));
+        if (test > sigma) sigma = test;
+    }
+    if (sigma <= 0.0) {
+        hSuggest = 10.0 * h;
+    } else {
+        double x = sigma;
+        x = x - 0.5 * (x * x - sigma) / x;
+        x = x - 0.5 * (x * x - sigma) / x;
+        x = x - 0.5 * (x * x - sigma) / x;
+        hSuggest = h * ((1.0 / x) + 0.005);
+    }
+    if (converged) {
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) yOut[s] = yc[s];
+    }
+    return converged;
+}
+
+// RKFStep.opCall, chemistry_update.d:385-484 (Cash-Karp tableau); yOut is always written.
+template <int NSP, int NRM>
+__device__ __forceinline__ bool rkf_step(const ChemDev& cd, const double* kf, const double* kb, const double* y0, double h,
+                         double* yOut, double& hSuggest) {
+    const double a21 = 1. / 5., a31 = 3. / 40., a32 = 9. / 40., a41 = 3. / 10., a42 = -9. / 10., a43 = 6. / 5.,
+                 a51 = -11. / 54., a52 = 5. / 2., a53 = -70. / 27., a54 = 35. / 27., a61 = 1631. / 55296.,
+                 a62 = 175. / 512., a63 = 575. / 13824., a64 = 44275. / 110592., a65 = 253. / 4096.;
+    const double b51 = 37. / 378., b53 = 250. / 621., b54 = 125. / 594., b56 = 512. / 1771., b41 = 2825. / 27648.,
+                 b43 = 18575. / 48384., b44 = 13525. / 55296., b45 = 277. / 14336., b46 = 1.0 / 4.0;
+    double k1[NSP], k2[NSP], k3[NSP], k4[NSP], k5[NSP], k6[NSP], yT[NSP], q[NSP], L[NSP];
+    auto rates = [&](const double* y, double* k) {
+        eval_split_rates<NSP, NRM>(cd, kf, kb, y, q, L);
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) k[s] = q[s] - L[s];
+    };
+    rates(y0, k1);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) yT[s] = y0[s] + h * (a21 * k1[s]);
+    rates(yT, k2);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) yT[s] = y0[s] + h * (a31 * k1[s] + a32 * k2[s]);
+    rates(yT, k3);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) yT[s] = y0[s] + h * (a41 * k1[s] + a42 * k2[s] + a43 * k3[s]);
+    rates(yT, k4);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) yT[s] = y0[s] + h * (a51 * k1[s] + a52 * k2[s] + a53 * k3[s] + a54 * k4[s]);
+    rates(yT, k5);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s)
+        yT[s] = y0[s] + h * (a61 * k1[s] + a62 * k2[s] + a63 * k3[s] + a64 * k4[s] + a65 * k5[s]);
+    rates(yT, k6);
+    double err = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        yOut[s] = y0[s] + h * (b51 * k1[s] + b53 * k3[s] + b54 * k4[s] + b56 * k6[s]);
+        double yE = yOut[s] - (y0[s] + h * (b41 * k1[s] + b43 * k3[s] + b44 * k4[s] + b45 * k5[s] + b46 * k6[s]));
+        double ay0 = fabs(y0[s]), ay1 = fabs(yOut[s]);
+        double sk = cd.errTol + cd.errTol * ((ay0 < ay1) ? ay1 : ay0);
+        err += (yE / sk) * (yE / sk);
+    }
+    err = sqrt(err / NSP);
+    const double maxscale = 10.0, minscale = 0.2, safe = 0.9, alpha = 0.2;
+    double scale;
+    if (err <= 1.0) {
+        if (err == 0.0) {
+            scale = maxscale;
+        } else {
+            scale = safe * pow(err, -alpha);
+            if (scale < minscale) scale = minscale;
+            if (scale > maxscale) scale = maxscale;
+        }
+        hSuggest = scale * h;
+        return true;
+    }
+    scale = safe * pow(err, -alpha);
+    scale = (scale < minscale) ? minscale : scale;
+    hSuggest = scale * h;
+    return false;
+}
+
+// Phobos std.algorithm.sum over NSP doubles: 0.0 + the pairwise sum of the D runtime
+// (pairs summed into a store collapsed at powers of two, the store then summed from its
+// last entry down).
+template <int NSP>
+__device__ __forceinline__ double phobos_sum(const double* x) {
+    double store[NSP];
+    int idx = 0;
+#pragma unroll
+    for (int k = 0; k < NSP / 2; ++k) {
+        store[idx] = x[2 * k] + x[2 * k + 1];
+        int lastToKeep = idx - __builtin_ctz((unsigned)(k + 1));
+        while (idx > lastToKeep) {
+            store[idx - 1] += store[idx];
+            --idx;
+        }
+        ++idx;
+    }
+    if (NSP % 2 == 1) {
+        store[idx] = x[NSP - 1];
+        ++idx;
+    }
+    double s = store[idx - 1];
+    for (int j = idx - 2; j >= 0; --j) s += store[j];
+    return 0.0 + s;
+}
+
+// gas_model.d:255-271 (conc2massf) followed by the mass-fraction sum of
+// chemistry_update.d:180-185 and 292-294.
+template <int NSP>
+__device__ __forceinline__ double conc_to_massf(const ChemDev& cd, const double* conc, double rho, double* massf) {
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        massf[s] = conc[s] * cd.M[s] / rho;
+        if (massf[s] < CH_MIN_MASSF) massf[s] = 0.0;
+    }
+    return phobos_sum<NSP>(massf);
+}
+
+}  // namespace gdtk

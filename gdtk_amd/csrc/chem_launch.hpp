@@ -1,0 +1,82 @@
+// chem_launch.hpp -- host-side launchers of the chemistry kernels (chem_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "chem_device.hpp"
+
+namespace gdtk {
+
+constexpr int CHEM_BLOCK = 256;   // threads per workgroup of every chemistry kernel
+constexpr int QUEUE_STRIDE = 32;  // queue counters 128 bytes apart
+
+// Per-cell flag written by the setup kernel.
+enum ChemFlag : int { CF_INTEGRATE = 0, CF_FROZEN = 1, CF_FAILED = 2 };
+
+// Device work buffers of one chemistry call over n cells (species-major arrays).
+struct ChemWork {
+    double* kfkb;       // [2*nr][n]  forward then backward rate constants
+    double* conc0;      // [nsp][n]   initial concentrations
+    double* h0;         // [n]        first substep size
+    double* conc;       // [nsp][n]   final concentrations of integrated cells
+    int* flag;          // [n]        ChemFlag
+    int* st;            // [n]        0 integrated, -1 failed (integrated cells)
+    int* nsub;          // [n]        substeps per cell
+    unsigned* queue;    // [8*32]     per-XCD next-cell counters, 128 B apart
+    unsigned long long* stats;  // [4] total substeps, corrector iterations, max substeps, step attempts
+};
+
+// Launch shape of the integration kernel.
+struct ChemTuning {
+    int persistent = 1;     // 1: persistent lanes refilled from the work queue; 0: one lane per cell
+    int chunk = 64;         // cells a wave takes from the queue per grab
+    int refill_period = 1;  // substep iterations between refills of finished lanes
+    int blocks_per_cu = 0;  // persistent grid: workgroups per CU (0 = occupancy limit)
+};
+
+enum ChemPhase : int { CHEM_PHASE_SETUP = 0, CHEM_PHASE_INTEGRATE = 1, CHEM_PHASE_FINISH = 2, CHEM_PHASE_TPG = 3 };
+
+// Arguments of one kernel phase (the fields a phase does not use stay zero).
+struct ChemCall {
+    int phase;
+    const ChemDev* cd;
+    int nr, n, mode;
+    double dt;
+    const double *rho, *T, *u, *p, *massf, *dtc_in;
+    double *T_out, *u_out, *p_out, *massf_out, *dtc_out;
+    double *rho_io, *p_io, *T_io, *u_io, *a_io;
+    int* status;
+    ChemWork w;
+    ChemTuning tu;
+    int num_cu;
+    hipStream_t s;
+};
+
+// Instantiated per species count in chem_nsp<N>.hip.
+template <int NSP>
+hipError_t chem_run(const ChemCall& c);
+extern template hipError_t chem_run<1>(const ChemCall&);
+extern template hipError_t chem_run<2>(const ChemCall&);
+extern template hipError_t chem_run<3>(const ChemCall&);
+extern template hipError_t chem_run<4>(const ChemCall&);
+extern template hipError_t chem_run<5>(const ChemCall&);
+extern template hipError_t chem_run<6>(const ChemCall&);
+extern template hipError_t chem_run<7>(const ChemCall&);
+extern template hipError_t chem_run<8>(const ChemCall&);
+
+bool chem_supported(int nsp, int nr);
+// Zero the queue counters and statistics (a fill kernel, so the call may be captured).
+hipError_t launch_chem_clear(const ChemWork& w, hipStream_t s);
+hipError_t launch_chem_setup(const ChemDev* cd, int nsp, int nr, int n, double dt, const double* rho,
+                             const double* T, const double* massf, const double* dtc, const ChemWork& w,
+                             hipStream_t s);
+hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int n, double dt, const double* rho,
+                                 const double* dtc_in, double* dtc_out, const ChemWork& w, const ChemTuning& tu,
+                                 int num_cu, hipStream_t s);
+hipError_t launch_chem_finish(const ChemDev* cd, int nsp, int n, const double* rho, const double* T,
+                              const double* u, const double* p, const double* massf, const double* dtc_in,
+                              double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
+                              int* status, const ChemWork& w, hipStream_t s);
+hipError_t launch_tpg_update(const ChemDev* cd, int nsp, int mode, int n, double* rho, double* p, double* T,
+                             double* u, double* a, const double* massf, int* status, hipStream_t s);
+
+}  // namespace gdtk

@@ -22,24 +22,12 @@
 #include "block.hpp"
 #include "gdtk_device.hpp"
 #include "geometry.hpp"
+#include "host_common.hpp"
 #include "launchers.hpp"
 
 namespace gdtk {
 
-static thread_local std::string g_last_error;
-
-struct Error : std::exception {
-    int code;
-    std::string msg;
-    Error(int c, std::string m) : code(c), msg(std::move(m)) {}
-    const char* what() const noexcept override { return msg.c_str(); }
-};
-
-#define HIPCHK(expr)                                                                                 \
-    do {                                                                                             \
-        hipError_t e_ = (expr);                                                                      \
-        if (e_ != hipSuccess) throw Error(GDTK_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
+thread_local std::string g_last_error;
 
 struct DevBuf {
     double* p = nullptr;
@@ -48,6 +36,7 @@ struct DevBuf {
         n = count;
         HIPCHK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)));
         HIPCHK(hipMemset(p, 0, std::max<size_t>(count, 1) * sizeof(double)));
+        HIPCHK(hipDeviceSynchronize());  // the fill is on the null stream
     }
     ~DevBuf() {
         if (p) (void)hipFree(p);
@@ -505,19 +494,6 @@ static Sim& get_sim(int h) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (h < 0 || h >= (int)g_sims.size() || !g_sims[h]) throw Error(GDTK_ERR_ARG, "bad simulation handle");
     return *g_sims[h];
-}
-
-template <class F>
-static int guarded(F&& f) {
-    try {
-        return f();
-    } catch (Error& e) {
-        g_last_error = e.msg;
-        return e.code;
-    } catch (std::exception& e) {
-        g_last_error = e.what();
-        return GDTK_ERR_ARG;
-    }
 }
 
 }  // namespace gdtk

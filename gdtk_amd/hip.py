@@ -31,6 +31,9 @@ EXPORTS = [
     "gdtk_hip_block_get_field", "gdtk_hip_stage", "gdtk_hip_block_dt", "gdtk_hip_sim_run",
     "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
     "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error",
+    "gdtk_hip_chem_create", "gdtk_hip_chem_destroy", "gdtk_hip_chem_step", "gdtk_hip_chem_step_device",
+    "gdtk_hip_chem_stats", "gdtk_hip_chem_substeps", "gdtk_hip_chem_set_tuning", "gdtk_hip_chem_autotune",
+    "gdtk_hip_tpg_update",
 ]
 
 
@@ -57,11 +60,24 @@ def lib_path() -> str:
     return _LIB_PATH
 
 
+def _bind_torch_runtime():
+    """A process can hold one HIP runtime per device.  PyTorch ships its own libamdhip64;
+    importing torch before the library makes the library's libamdhip64 dependency resolve
+    (by SONAME) to that already-loaded runtime, so torch tensors and this library share
+    one runtime, one device context and one address space.  Without torch the library uses
+    the system ROCm runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             raise HipError(-2, f"{_LIB_PATH} is not built; run gdtk_amd.build (hipcc, gfx950)")
+        _bind_torch_runtime()
         L = C.CDLL(_LIB_PATH)
         L.gdtk_hip_last_error.restype = C.c_char_p
         L.gdtk_hip_sim_create.argtypes = [C.POINTER(SimConfig), ip]

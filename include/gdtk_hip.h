@@ -198,6 +198,44 @@ int gdtk_hip_chem_destroy(int chem);
 int gdtk_hip_chem_step(int chem, int n, const double* rho, double* T, double* u, double* p, double* massf,
                        double* dt_chem, double dt, int* status, double* stats);
 
+/* gdtk_hip_chem_step on device-resident arrays, out of place and asynchronous: the
+ * work is queued on `stream` (a hipStream_t; NULL = the default stream) and nothing
+ * is copied to or from the host.  Inputs rho, T, u, p, dt_chem [n] and massf [nsp*n];
+ * outputs T_out, u_out, p_out, dt_chem_out, status [n] and massf_out [nsp*n], every entry
+ * written (cells that fail or are frozen get their input values, as the reference leaves
+ * them).  Outputs may be the input arrays.  Same per-cell semantics as gdtk_hip_chem_step;
+ * the same chemistry handle must not run two steps at once. */
+int gdtk_hip_chem_step_device(int chem, int n, const double* rho, const double* T, const double* u, const double* p,
+                              const double* massf, const double* dt_chem, double dt, double* T_out, double* u_out,
+                              double* p_out, double* massf_out, double* dt_chem_out, int* status, void* stream);
+
+/* Statistics of the last step (waits for it): {total substeps, total corrector
+ * iterations, max substeps of a cell, total ODE step attempts, ms setup kernel, ms
+ * integration kernel, ms finish kernel, cells}; returns the number of values written
+ * (<= nstats, at most 8). */
+int gdtk_hip_chem_stats(int chem, double* stats, int nstats);
+
+/* Substeps taken by each cell of the last step (both attempts of a retried cell; 0 for
+ * cells not integrated).  n must be that step's cell count. */
+int gdtk_hip_chem_substeps(int chem, int n, int* substeps);
+
+/* Launch shape of the integration kernel (set by gdtk_hip_chem_autotune):
+ * GDTK_CHEM_TUNE_PERSISTENT  1 persistent lanes fed by a work queue, 0 one lane per cell
+ * GDTK_CHEM_TUNE_CHUNK       cells taken from the queue per wave grab
+ * GDTK_CHEM_TUNE_REFILL      substep trips between refills of finished lanes
+ * GDTK_CHEM_TUNE_BLOCKS_PER_CU  persistent workgroups per CU (0 = occupancy limit) */
+#define GDTK_CHEM_TUNE_PERSISTENT 1
+#define GDTK_CHEM_TUNE_CHUNK 2
+#define GDTK_CHEM_TUNE_REFILL 3
+#define GDTK_CHEM_TUNE_BLOCKS_PER_CU 4
+int gdtk_hip_chem_set_tuning(int chem, int key, int value);
+
+/* Times the integration variants on a sample batch of device arrays (left unchanged) and
+ * keeps the fastest.  out (may be NULL) receives 5 values per candidate {persistent,
+ * chunk, refill, blocks_per_cu, ms}; returns the number of candidates. */
+int gdtk_hip_chem_autotune(int chem, int n, const double* rho, const double* T, const double* u, const double* p,
+                           const double* massf, const double* dt_chem, double dt, double* out, int nout);
+
 /* Thermodynamic updates of n gas states on the device, for checking the gas model:
  * mode 0 pT (p, T -> rho, u), 1 rhou (rho, u -> T, p; T in is the Newton start),
  * 2 rhoT (rho, T -> p, u); then the sound speed a.  massf species-major as above. */

@@ -246,7 +246,7 @@ def _chem_lib():
         L.orc_chem_split_rates.argtypes = [vp, dp, dp, dp]
         L.orc_chem_ode_step.argtypes = [vp, C.c_int, dp, C.c_double, dp, dp]
         L.orc_chem_update.argtypes = [vp, dp, dp, C.c_double, dp, dp]
-        L.orc_chem_step_batch.argtypes = [vp, vp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double, ip, ip, C.c_int]
+        L.orc_chem_step_batch.argtypes = [vp, vp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double, ip, ip, ip, C.c_int]
         L.orc_gas_destroy_tpg.argtypes = [vp]
         from gdtk_amd.chem import Mechanism, TPGGas
         sizes = (C.c_int * 3)()
@@ -336,8 +336,11 @@ def chem_step_batch(gas_rec, mech_rec, rho, T, u, p, massf, dt_chem, dt, nthread
     rho = np.ascontiguousarray(rho, dtype=np.float64)
     status = np.zeros(n, dtype=np.int32)
     sub = np.zeros(n, dtype=np.int32)
+    att = np.zeros(n, dtype=np.int32)
     failed = _chem_lib().orc_chem_step_batch(C.byref(gas_rec), C.byref(mech_rec), n, rho.ctypes.data_as(dp),
                                              T.ctypes.data_as(dp), u.ctypes.data_as(dp), p.ctypes.data_as(dp),
                                              massf.ctypes.data_as(dp), dt_chem.ctypes.data_as(dp), dt,
-                                             status.ctypes.data_as(ip), sub.ctypes.data_as(ip), nthreads)
-    return dict(T=T, u=u, p=p, massf=massf, dt_chem=dt_chem, status=status, substeps=sub, failed=failed)
+                                             status.ctypes.data_as(ip), sub.ctypes.data_as(ip),
+                                             att.ctypes.data_as(ip), nthreads)
+    return dict(T=T, u=u, p=p, massf=massf, dt_chem=dt_chem, status=status, substeps=sub, attempts=att,
+                failed=failed)

@@ -474,9 +474,11 @@ int orc_chem_update(void* h, double* st, double* massf, double tInterval, double
 }
 
 // Batched thermochemical_increment (fvcell.d:858-972) over n cells, species-major massf,
-// with nthreads OpenMP threads (one oracle object per thread).  sub[n] = substeps per cell.
+// with nthreads OpenMP threads (one oracle object per thread).  sub[n] = substeps per cell,
+// att[n] = ODE step attempts per cell (either may be NULL).
 int orc_chem_step_batch(const void* gasv, const void* mechv, int n, const double* rho, double* T, double* u,
-                        double* p, double* massf, double* dt_chem, double dt, int* status, int* sub, int nthreads) {
+                        double* p, double* massf, double* dt_chem, double dt, int* status, int* sub, int* att,
+                        int nthreads) {
     int failed = 0;
 #pragma omp parallel num_threads(nthreads) reduction(+ : failed)
     {
@@ -492,6 +494,7 @@ int orc_chem_step_batch(const void* gasv, const void* mechv, int n, const double
             int rc = thermochemical_increment(*c->chem, *c->gas, Q, dt, dt_chem[i], c->T_frozen, &cs);
             status[i] = rc;
             if (sub) sub[i] = cs.subcycles;
+            if (att) att[i] = cs.attempts;
             if (rc != 0) { ++failed; continue; }
             T[i] = Q.T; u[i] = Q.u; p[i] = Q.p;
             for (int s = 0; s < nsp; ++s) massf[(size_t)s * n + i] = Q.massf[s];

@@ -1,0 +1,252 @@
+"""GPU parity of the chemistry path: the HIP kernels (through the C ABI) against the CPU
+oracle's restatement of FVCell.thermochemical_increment (src/eilmer/fvcell.d:858-972),
+ChemistryUpdate (src/kinetics/chemistry_update.d) and the thermally perfect gas.
+
+The state (T, u, p, mass fractions) is compared with the relative metric of
+tests/parity.py at 1e-12 (FP64).  Per-cell status and substep counts must be identical: a
+flipped step-size or convergence branch shows up there first.  The suggested dt_chem has
+its own bar, 1e-10: it is h (1/sqrt(sigma) + 0.005) with sigma built from |yc - yp0|,
+the difference of corrector and predictor, which is ~eps2 * y (chemistry_update.d:630-655),
+so a relative change d of the concentrations moves dt_chem by up to ~d / eps2 = 2000 d.
+The RKF step's suggestion is scale * h with scale = 0.9 err^-0.2 and err the norm of
+the embedded error estimate, itself a difference of the 4th- and 5th-order solutions of
+size ~errTol (chemistry_update.d:456-483): its conditioning is ~1/errTol, so with
+errTol = 1e-6 the RKF test holds dt_chem to 1e-5 (the state still to 1e-12).
+The H2-I2-HI case also meets the reference's own known answer (chemistry_update.d:739-750).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gdtk_amd import data as gdata
+from gdtk_amd.chem import Chemistry, gas_record, mech_record
+from gdtk_amd.chemcases import air_cells
+from tests.parity import TOL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+R_U = 8.31451
+DT = 1.0e-7
+DTCHEM_TOL = 1.0e-10
+DTCHEM_TOL_RKF = 1.0e-5
+
+
+def _json(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def _air(method=None, **kw):
+    gas = gas_record(gdata.load("tpg-air-5sp"))
+    mech = mech_record(gdata.load("park-air-5sp-1T"), method=method, **kw)
+    return gas, mech
+
+
+def _oracle_step(gas, mech, c, dtc, dt=DT):
+    from oracle import oracle as O
+    return O.chem_step_batch(gas, mech, c["rho"], c["T"], c["u"], c["p"], c["massf"], dtc, dt, nthreads=8)
+
+
+def _compare(g, o, label, sub_g=None, tol=TOL, dtc_tol=DTCHEM_TOL):
+    st_g, st_o = g["status"], o["status"]
+    assert np.array_equal(st_g, st_o), f"{label}: status differs in {np.count_nonzero(st_g != st_o)} cells"
+    if sub_g is not None:
+        diff = np.nonzero(sub_g != o["substeps"])[0]
+        assert diff.size == 0, f"{label}: substep counts differ in cells {diff[:10]}"
+    errs = {k: rel_err(g[k], o[k]) for k in ("T", "u", "p", "massf", "dt_chem")}
+    print(label, {k: f"{v:.2e}" for k, v in errs.items()}, "failed", int(np.count_nonzero(st_o)), flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= (dtc_tol if k == "dt_chem" else tol)}
+    assert not bad, f"{label}: parity above {tol} (dt_chem {dtc_tol}): {bad}"
+
+
+def test_tpg_update_modes():
+    """pT, rhou (Newton from a perturbed start) and rhoT updates plus the sound speed,
+    state by state against the oracle's ThermallyPerfectGas."""
+    from oracle import oracle as O
+    gas, mech = _air()
+    ch = Chemistry(gas, mech)
+    og = O.TPG(gas)
+    c = air_cells(400, seed=11, T_lo=200.0, T_hi=20000.0)
+    n = 400
+    Y = c["massf"].reshape(5, n)
+    g_pT = ch.tpg_update("pT", c["massf"], p=c["p"], T=c["T"])
+    T0 = c["T"] * (1.0 + 0.2 * np.sin(np.arange(n)))
+    g_ru = ch.tpg_update("rhou", c["massf"], rho=c["rho"], T=T0, u=c["u"])
+    g_rT = ch.tpg_update("rhoT", c["massf"], rho=c["rho"], T=c["T"])
+    ref = {m: {k: np.zeros(n) for k in ("rho", "p", "T", "u", "a")} for m in ("pT", "rhou", "rhoT")}
+    for i in range(n):
+        for mode, kw in (("pT", dict(p=c["p"][i], T=c["T"][i])),
+                         ("rhou", dict(rho=c["rho"][i], T=T0[i], u=c["u"][i])),
+                         ("rhoT", dict(rho=c["rho"][i], T=c["T"][i]))):
+            st = og.update(mode, Y[:, i], **kw)
+            for k in ref[mode]:
+                ref[mode][k][i] = st[k]
+    for mode, g in (("pT", g_pT), ("rhou", g_ru), ("rhoT", g_rT)):
+        assert g["failed"] == 0, mode
+        errs = {k: rel_err(g[k], ref[mode][k]) for k in ("rho", "p", "T", "u", "a")}
+        print(mode, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+        assert max(errs.values()) <= TOL, (mode, errs)
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_air_alpha_qss_cold_then_warm(persistent):
+    """Two chemistry steps of 4096 uneven-stiffness air cells: the first from dt_chem = -1
+    (step estimate), the second from the suggested dt_chem of the first."""
+    gas, mech = _air()
+    ch = Chemistry(gas, mech)
+    ch.set_tuning(persistent=persistent)
+    n = 4096
+    c = air_cells(n, seed=5)
+    dtc = np.full(n, -1.0)
+    g1 = ch.step(c["rho"], c["T"], c["u"], c["p"], c["massf"], dtc, DT)
+    s1 = ch.substeps(n)
+    o1 = _oracle_step(gas, mech, c, dtc)
+    _compare(g1, o1, f"cold persistent={persistent}", s1)
+    assert g1["failed"] == o1["failed"]
+    assert g1["substeps_total"] == int(o1["substeps"].sum())
+    assert g1["substeps_max"] == int(o1["substeps"].max())
+    assert ch.stats()["attempts_total"] == int(o1["attempts"].sum())
+    c2 = dict(rho=c["rho"], T=o1["T"], u=o1["u"], p=o1["p"], massf=o1["massf"])
+    g2 = ch.step(c2["rho"], c2["T"], c2["u"], c2["p"], c2["massf"], o1["dt_chem"], DT)
+    o2 = _oracle_step(gas, mech, c2, o1["dt_chem"])
+    _compare(g2, o2, f"warm persistent={persistent}", ch.substeps(n))
+
+
+def test_air_rkf_matches_oracle():
+    gas, mech = _air(method="rkf", errTol=1.0e-6)
+    ch = Chemistry(gas, mech)
+    n = 1024
+    c = air_cells(n, seed=8, T_lo=2000.0, T_hi=9000.0)
+    dtc = np.full(n, -1.0)
+    g = ch.step(c["rho"], c["T"], c["u"], c["p"], c["massf"], dtc, DT)
+    o = _oracle_step(gas, mech, c, dtc)
+    _compare(g, o, "rkf", ch.substeps(n), dtc_tol=DTCHEM_TOL_RKF)
+
+
+def test_h2_i2_known_answer_and_alpha_qss():
+    """chemistry_update.d:739-750: H2 + I2 <=> 2 HI at 700 K, 60 000 s with the RKF step
+    gives [HI] = 7.14201983840 (1e-9); the alpha-QSS update of the same cell matches the
+    oracle."""
+    from oracle import oracle as O
+    gas = gas_record(_json("H2-I2-HI.json"))
+    md = _json("H2-I2-inp.json")
+    M = np.array(_json("H2-I2-HI.json")["M"])
+    T, c0 = 700.0, 4.54
+    p = 2.0 * c0 * R_U * T
+    molef = np.array([0.5, 0.5, 0.0])
+    massf = molef * M / np.sum(molef * M)
+    og = O.TPG(gas)
+    st = og.update("pT", massf, p=p, T=T)
+    cell = dict(rho=np.array([st["rho"]]), T=np.array([T]), u=np.array([st["u"]]), p=np.array([p]),
+                massf=massf.copy())
+    for method in ("rkf", "alpha-qss"):
+        mech = mech_record(md, method=method)
+        ch = Chemistry(gas, mech)
+        g = ch.step(cell["rho"], cell["T"], cell["u"], cell["p"], cell["massf"], np.array([200.0]), 60_000.0)
+        o = O.chem_step_batch(gas, mech, cell["rho"], cell["T"], cell["u"], cell["p"], cell["massf"],
+                              np.array([200.0]), 60_000.0)
+        _compare(g, o, f"H2-I2 {method}", ch.substeps(1), dtc_tol=DTCHEM_TOL_RKF if method == "rkf" else DTCHEM_TOL)
+        if method == "rkf":
+            conc_hi = g["massf"][2] * st["rho"] / M[2]
+            assert abs(conc_hi - 7.14201983840) <= 1.0e-9 * 7.14201983840
+
+
+def test_frozen_invalid_and_empty_cells():
+    """Cells at or below T_frozen are left unchanged with status 0; a non-positive or NaN
+    density fails both attempts (status -1, dt_chem = -1, state unchanged); n = 0 is a no-op."""
+    gas, mech = _air()
+    ch = Chemistry(gas, mech)
+    n = 256
+    c = air_cells(n, seed=2)
+    c["T"][:32] = np.linspace(200.0, 300.0, 32)
+    c["rho"][40:48] = -1.0
+    c["rho"][48] = 0.0
+    c["rho"][49] = np.nan
+    dtc = np.full(n, 3.0e-9)
+    g = ch.step(c["rho"], c["T"], c["u"], c["p"], c["massf"], dtc, DT)
+    o = _oracle_step(gas, mech, c, dtc)
+    assert np.all(g["status"][:32] == 0) and np.all(g["T"][:32] == c["T"][:32])
+    assert np.all(g["dt_chem"][:32] == 3.0e-9)
+    assert np.all(g["status"][40:50] == -1) and np.all(g["dt_chem"][40:50] == -1.0)
+    assert np.array_equal(g["massf"].reshape(5, n)[:, 40:50], c["massf"].reshape(5, n)[:, 40:50])
+    # the oracle leaves failed cells' T/u/p/massf untouched as well
+    for k in ("T", "u", "p"):
+        o[k][40:50] = c[k][40:50]
+    keep = np.ones(n, bool)
+    keep[40:50] = False
+    _compare({k: (v[keep] if k != "massf" else v.reshape(5, n)[:, keep]) for k, v in g.items()
+              if k in ("T", "u", "p", "massf", "dt_chem", "status")},
+             {k: (v[keep] if k != "massf" else v.reshape(5, n)[:, keep]) for k, v in o.items()
+              if k in ("T", "u", "p", "massf", "dt_chem", "status")}, "frozen+invalid")
+    e = ch.step(np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), DT)
+    assert e["failed"] == 0 and e["T"].size == 0
+
+
+def test_queue_variants_identical_and_autotune():
+    """Every launch shape of the integration kernel gives bit-identical cells; the
+    autotuner times them on device arrays without changing its inputs."""
+    import torch
+    gas, mech = _air()
+    ch = Chemistry(gas, mech)
+    n = 20000
+    c = air_cells(n, seed=9)
+    dtc = np.full(n, -1.0)
+    ref = None
+    for tu in (dict(persistent=0), dict(persistent=1, chunk=1, refill=1), dict(persistent=1, chunk=64, refill=8),
+               dict(persistent=1, chunk=256, refill=1, blocks_per_cu=1)):
+        ch.set_tuning(**{"blocks_per_cu": 0, **tu})
+        g = ch.step(c["rho"], c["T"], c["u"], c["p"], c["massf"], dtc, DT)
+        if ref is None:
+            ref = g
+        else:
+            for k in ("T", "u", "p", "massf", "dt_chem", "status"):
+                assert np.array_equal(g[k], ref[k]), (tu, k)
+    dev = torch.device("cuda:0")
+    t = {k: torch.tensor(v, dtype=torch.float64, device=dev) for k, v in c.items()}
+    td = torch.tensor(dtc, device=dev)
+    snap = {k: v.clone() for k, v in t.items()}
+    res = ch.autotune(n, t["rho"].data_ptr(), t["T"].data_ptr(), t["u"].data_ptr(), t["p"].data_ptr(),
+                      t["massf"].data_ptr(), td.data_ptr(), DT)
+    torch.cuda.synchronize()
+    assert len(res) >= 2 and all(r["ms"] > 0 for r in res)
+    for k in t:
+        assert torch.equal(t[k], snap[k])
+    g = ch.step(c["rho"], c["T"], c["u"], c["p"], c["massf"], dtc, DT)
+    for k in ("T", "u", "p", "massf", "dt_chem", "status"):
+        assert np.array_equal(g[k], ref[k]), ("tuned", k)
+
+
+def test_device_entry_point_large_batch_sampled():
+    """262 144 cells through gdtk_hip_chem_step_device on torch-held device arrays: each
+    cell's result is independent of the batch, so a sample of cells checked against the
+    oracle pins the whole launch at a size where the queue wraps many times."""
+    import torch
+    gas, mech = _air()
+    ch = Chemistry(gas, mech)
+    n = 1 << 18
+    c = air_cells(n, seed=21)
+    dev = torch.device("cuda:0")
+    t = {k: torch.tensor(v, dtype=torch.float64, device=dev) for k, v in c.items()}
+    dtc = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
+    out = {k: torch.empty_like(t[k]) for k in ("T", "u", "p", "massf")}
+    dto = torch.empty_like(dtc)
+    st = torch.empty(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    ch.step_device(n, t["rho"].data_ptr(), t["T"].data_ptr(), t["u"].data_ptr(), t["p"].data_ptr(),
+                   t["massf"].data_ptr(), dtc.data_ptr(), DT, out["T"].data_ptr(), out["u"].data_ptr(),
+                   out["p"].data_ptr(), out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), stream)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(4)
+    idx = np.sort(rng.choice(n, 3000, replace=False))
+    Y = c["massf"].reshape(5, n)
+    sub = {k: c[k][idx] for k in ("rho", "T", "u", "p")}
+    sub["massf"] = np.ascontiguousarray(Y[:, idx]).reshape(-1)
+    o = _oracle_step(gas, mech, sub, np.full(idx.size, -1.0))
+    g = {k: out[k].cpu().numpy() for k in out}
+    g = dict(T=g["T"][idx], u=g["u"][idx], p=g["p"][idx],
+             massf=np.ascontiguousarray(g["massf"].reshape(5, n)[:, idx]).reshape(-1),
+             dt_chem=dto.cpu().numpy()[idx], status=st.cpu().numpy()[idx])
+    _compare(g, o, "device 2^18 sampled", ch.substeps(n)[idx])
