@@ -91,6 +91,153 @@ def measured_traffic(kernel, items):
     return None
 
 
+def split_rate_flops(mech):
+    """Algorithmic FP64 flops of one eval_split_rates (reaction_mechanism.d:83-96) for the
+    mechanism record: per reaction, the collider sum (a multiply and an add per efficiency,
+    then one multiply each for the forward and backward rates), nu multiplies per listed
+    reactant or product (the concentration power and the product), and per species with a
+    nonzero net coefficient two multiply-adds into q and L."""
+    f = 0
+    for ir in range(mech.nr):
+        r = mech.r[ir]
+        if r.collider:
+            f += 2 * r.neff + 2
+        f += sum(r.reac_nu[k] for k in range(r.nreac)) + sum(r.prod_nu[k] for k in range(r.nprod))
+        nu = {}
+        for k in range(r.nreac):
+            nu[r.reac_sp[k]] = nu.get(r.reac_sp[k], 0) - r.reac_nu[k]
+        for k in range(r.nprod):
+            nu[r.prod_sp[k]] = nu.get(r.prod_sp[k], 0) + r.prod_nu[k]
+        f += 4 * sum(1 for v in nu.values() if v != 0)
+    return f
+
+
+def alpha_qss_flops(mech, nsp):
+    """Algorithmic FP64 flops (add, multiply, divide each 1) of the alpha-QSS integration,
+    as (per step attempt, per corrector iteration), counted from chemistry_update.d:533-655
+    and the controller :162-277.  Per attempt: the rates at y0, per species p0 (2), alpha
+    (3 + 19 for the two cubic polynomials and their quotient) and the predictor (8); the
+    step suggestion (4 per species, 15 for three Newton square-root iterations, 3); the
+    mass-fraction test (2 per species, nsp - 1 + 2 for the pairwise sum and its check) and 5
+    for the controller.  Per corrector: the rates at yp, per species pp (2), alpha-bar (22),
+    q-tilde (4), p-bar (2), the corrector (8) and the convergence test (3)."""
+    fs = split_rate_flops(mech)
+    per_attempt = fs + nsp * (2 + 22 + 8) + (4 * nsp + 15 + 3) + (2 * nsp + nsp - 1 + 2) + 5
+    per_iter = fs + nsp * (2 + 22 + 4 + 2 + 8 + 3)
+    return per_attempt, per_iter
+
+
+def chem_cpu_baseline(gas, mech, cells, dt, seconds_target=15.0):
+    """The oracle's batched thermochemical increment (OpenMP over cells, as the reference
+    runs blocks in parallel) on a bounded sample of the same cells, chunk by chunk."""
+    import numpy as np
+
+    from oracle import oracle as O
+    n = cells["rho"].size
+    threads = max(1, min(16, os.cpu_count() or 1))
+    chunk = 32768
+    Y = cells["massf"].reshape(-1, n)
+    done = 0
+    t0 = time.perf_counter()
+    while done < n:
+        sl = slice(done, min(n, done + chunk))
+        m = sl.stop - sl.start
+        O.chem_step_batch(gas, mech, cells["rho"][sl], cells["T"][sl], cells["u"][sl], cells["p"][sl],
+                          np.ascontiguousarray(Y[:, sl]).reshape(-1), np.full(m, -1.0), dt, nthreads=threads)
+        done += m
+        if time.perf_counter() - t0 > seconds_target:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=done / el, unit="cells/s", cores=threads, kind="port",
+                sample=f"oracle, {threads} OpenMP threads, first {done} of the {n} cells, dt_chem = -1, "
+                       f"{el:.1f} s")
+
+
+def bench_chem(ncells, steps, warmup, with_cpu=True, seed=2024, dt=1.0e-7):
+    """Plain chemistry, uneven stiffness: alpha-QSS over dt on independent 5-species air
+    cells (gdtk_amd.chemcases.air_cells), each timed step from the same resident inputs with
+    dt_chem = -1, out of place."""
+    import numpy as np
+    import torch
+
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, gas_record, mech_record
+    from gdtk_amd.chemcases import air_cells
+    gas = gas_record(gdata.load("tpg-air-5sp"))
+    mech = mech_record(gdata.load("park-air-5sp-1T"))
+    ch = Chemistry(gas, mech)
+    cells = air_cells(ncells, seed=seed)
+    dev = torch.device("cuda:0")
+    t = {k: torch.tensor(v, dtype=torch.float64, device=dev) for k, v in cells.items()}
+    dtc = torch.full((ncells,), -1.0, dtype=torch.float64, device=dev)
+    out = {k: torch.empty_like(t[k]) for k in ("T", "u", "p", "massf")}
+    dto = torch.empty_like(dtc)
+    st = torch.empty(ncells, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    ptrs = (t["rho"].data_ptr(), t["T"].data_ptr(), t["u"].data_ptr(), t["p"].data_ptr(), t["massf"].data_ptr(),
+            dtc.data_ptr())
+
+    def step():
+        ch.step_device(ncells, *ptrs, dt, out["T"].data_ptr(), out["u"].data_ptr(), out["p"].data_ptr(),
+                       out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), stream.cuda_stream)
+
+    tune = ch.autotune(ncells, *ptrs, dt)
+    torch.cuda.synchronize()
+    best = min(tune, key=lambda r: r["ms"])
+    print("bench: chem autotune", best, file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kern = {"ms_setup": 0.0, "ms_integrate": 0.0, "ms_finish": 0.0}
+    e0.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = e0.elapsed_time(e1)
+    # per-kernel durations (HIP events the library records around each kernel on `stream`)
+    for _ in range(3):
+        step()
+        s = ch.stats()
+        for k in kern:
+            kern[k] += s[k] / 3.0
+    failed = int((st != 0).sum().item())
+    per_att, per_it = alpha_qss_flops(mech, gas.nsp)
+    flops = per_att * s["attempts_total"] + per_it * s["iters_total"]
+    achieved = flops / (kern["ms_integrate"] * 1e-3) / 1e12
+    res = {
+        "metric": "chem cells/s (alpha-QSS, FP64)",
+        "value": ncells * steps / (ms_total * 1e-3),
+        "unit": "cells/s",
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms_total / steps,
+        "wall_ms_per_step": 1e3 * wall / steps,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain chemistry, uneven stiffness: {ncells} cells of 5-species air (Park 1-T), "
+                               "T log-uniform 1500-12000 K, p log-uniform 1e3-1e6 Pa, dt 1e-7 s, dt_chem -1",
+                   "cells": ncells, "dt": dt, "failed_cells": failed,
+                   "substeps_total": s["substeps_total"], "substeps_max": s["substeps_max"],
+                   "attempts_total": s["attempts_total"], "corrector_iters_total": s["iters_total"],
+                   "tuning": {k: best[k] for k in ("persistent", "chunk", "refill", "blocks_per_cu")}},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_chem_integrate<5,5>", "avg_launch_ms": kern["ms_integrate"],
+                     "flops_per_attempt": per_att, "flops_per_corrector": per_it, "flops_per_launch": flops},
+        "kernels": kern,
+    }
+    if with_cpu:
+        try:
+            res["cpu_baseline"] = chem_cpu_baseline(gas, mech, cells, dt)
+        except Exception as e:  # the baseline is reported, never required
+            res["cpu_baseline"] = {"error": str(e)}
+    return res
+
+
 def copy_probe(nbytes=1 << 30):
     """A plain device copy of known size for HBM counter calibration (FETCH_SIZE/WRITE_SIZE)."""
     import torch
@@ -112,6 +259,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a device copy of known size (HBM counter calibration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--chem-cells", type=int, default=1 << 22,
+                    help="cells of the chemistry workload (0 skips the chemistry measurement)")
+    ap.add_argument("--chem-steps", type=int, default=5)
+    ap.add_argument("--chem-only", action="store_true", help="measure only the chemistry workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -119,6 +270,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.chem_only:
+        print(json.dumps(bench_chem(args.chem_cells, args.chem_steps, 1, not args.no_cpu_baseline)), flush=True)
         return
     nic, njc, nkc = (int(x) for x in args.size.split(","))
     from gdtk_amd import Simulation, cases
@@ -181,6 +335,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": str(e)}
+    if args.chem_cells > 0:
+        out["chem"] = bench_chem(args.chem_cells, args.chem_steps, 1, not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

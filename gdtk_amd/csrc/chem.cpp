@@ -32,7 +32,7 @@ struct Chem {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     ChemTuning tuning;
-    DevArr<double> kfkb, conc0, h0, conc;
+    DevArr<double> kfkb, conc0, h0, hest, conc;
     DevArr<int> flag, st, nsub;
     DevArr<unsigned> queue;
     DevArr<unsigned long long> stats;
@@ -57,12 +57,13 @@ struct Chem {
         conc0.reserve((size_t)nsp * N);
         conc.reserve((size_t)nsp * N);
         h0.reserve(N);
+        hest.reserve(N);
         flag.reserve(N);
         st.reserve(N);
         nsub.reserve(N);
         queue.reserve(8 * 32);
         stats.reserve(4);
-        return ChemWork{kfkb.p, conc0.p, h0.p, conc.p, flag.p, st.p, nsub.p, queue.p, stats.p};
+        return ChemWork{kfkb.p, conc0.p, h0.p, hest.p, conc.p, flag.p, st.p, nsub.p, queue.p, stats.p};
     }
 
     // The three kernels on device arrays; outputs may alias inputs.
@@ -76,7 +77,7 @@ struct Chem {
         HIPCHK(launch_chem_clear(w, s));
         HIPCHK(launch_chem_setup(dev, nsp, nr, n, dt, rho, T, massf, dtc, w, s));
         HIPCHK(hipEventRecord(ev[1], s));
-        HIPCHK(launch_chem_integrate(dev, nsp, nr, n, dt, rho, dtc, dtc_out, w, tuning, num_cu, s));
+        HIPCHK(launch_chem_integrate(dev, nsp, nr, host.method, n, dt, rho, dtc, dtc_out, w, tuning, num_cu, s));
         HIPCHK(hipEventRecord(ev[2], s));
         HIPCHK(launch_chem_finish(dev, nsp, n, rho, T, u, p, massf, dtc, T_out, u_out, p_out, massf_out, dtc_out,
                                   status, w, s));
@@ -154,20 +155,21 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
         d.nreac = r.nreac;
         d.nprod = r.nprod;
         for (int k = 0; k < r.nreac; ++k) {
-            if (r.reac_sp[k] < 0 || r.reac_sp[k] >= g.nsp || r.reac_nu[k] < 0)
-                throw Error(GDTK_ERR_ARG, "bad reactant");
-            d.reac_mask[k] = 1u << r.reac_sp[k];
+            if (r.reac_sp[k] < 0 || r.reac_sp[k] >= g.nsp || r.reac_nu[k] < 1 || r.reac_nu[k] > 3)
+                throw Error(GDTK_ERR_ARG, "bad reactant (species index, or coefficient outside 1..3)");
+            d.reac_sp[k] = r.reac_sp[k];
             d.reac_nu[k] = r.reac_nu[k];
         }
         for (int k = 0; k < r.nprod; ++k) {
-            if (r.prod_sp[k] < 0 || r.prod_sp[k] >= g.nsp || r.prod_nu[k] < 0) throw Error(GDTK_ERR_ARG, "bad product");
-            d.prod_mask[k] = 1u << r.prod_sp[k];
+            if (r.prod_sp[k] < 0 || r.prod_sp[k] >= g.nsp || r.prod_nu[k] < 1 || r.prod_nu[k] > 3)
+                throw Error(GDTK_ERR_ARG, "bad product (species index, or coefficient outside 1..3)");
+            d.prod_sp[k] = r.prod_sp[k];
             d.prod_nu[k] = r.prod_nu[k];
         }
         d.neff = d.collider ? r.neff : 0;
         for (int k = 0; k < d.neff; ++k) {
             if (r.eff_sp[k] < 0 || r.eff_sp[k] >= g.nsp) throw Error(GDTK_ERR_ARG, "bad efficiency species");
-            d.eff_mask[k] = 1u << r.eff_sp[k];
+            d.eff_sp[k] = r.eff_sp[k];
             d.eff[k] = r.eff[k];
         }
         d.f_from_eq = r.f_from_eq ? 1 : 0;
@@ -175,6 +177,22 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
         if (d.f_from_eq && d.b_from_eq) throw Error(GDTK_ERR_ARG, "both rate constants from K_c");
         d.fA = r.fA; d.fn = r.fn; d.fC = r.fC;
         d.bA = r.bA; d.bn = r.bn; d.bC = r.bC;
+        // Dense form when every list is in strictly ascending species order.
+        auto ascending = [](const int* sp, int k) {
+            for (int q = 1; q < k; ++q)
+                if (!(sp[q - 1] < sp[q])) return false;
+            return true;
+        };
+        d.sorted = ascending(r.reac_sp, r.nreac) && ascending(r.prod_sp, r.nprod) &&
+                   (!d.collider || ascending(r.eff_sp, r.neff));
+        if (d.sorted) {
+            for (int k = 0; k < r.nreac; ++k) d.reac_d[r.reac_sp[k]] = r.reac_nu[k];
+            for (int k = 0; k < r.nprod; ++k) d.prod_d[r.prod_sp[k]] = r.prod_nu[k];
+            for (int k = 0; k < d.neff; ++k) {
+                d.eff_set |= 1u << r.eff_sp[k];
+                d.eff_d[r.eff_sp[k]] = r.eff[k];
+            }
+        }
         // reaction.d:100-143: participants and net coefficients (first listing of a species)
         for (int s = 0; s < g.nsp; ++s) {
             int nu1 = 0, nu2 = 0, in = 0;

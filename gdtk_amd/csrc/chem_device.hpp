@@ -25,19 +25,24 @@ struct CEADev {
 };
 
 // src/kinetics/reaction.d:58-457.  The reactant, product and efficiency lists keep the
-// reference's order; each entry names its species by a one-hot mask (bit s = species s), so
-// that device code selects concentrations without indexing a register array.  nu[s] is the
-// net stoichiometric coefficient of species s and part[s] marks the participants
-// (reaction.d:100-143).
+// reference's order (reaction.d:213, 338, 483-491); nu[s] is the net stoichiometric
+// coefficient of species s and part[s] marks the participants (reaction.d:100-143).
+// Device code never indexes a register array with a run-time species index (that would
+// put the per-cell arrays in scratch memory): it walks the species in compile-time order,
+// or picks an entry through a select chain for lists that are not in ascending order.
 struct ReacDev {
     int collider;
-    int nreac, reac_nu[3];
-    unsigned reac_mask[3];
-    int nprod, prod_nu[3];
-    unsigned prod_mask[3];
-    int neff;
-    unsigned eff_mask[CH_MAX_SP];
+    int nreac, reac_sp[3], reac_nu[3];
+    int nprod, prod_sp[3], prod_nu[3];
+    int neff, eff_sp[CH_MAX_SP];
     double eff[CH_MAX_SP];
+    // Dense form, used when every list is in ascending species order (then a walk over the
+    // species in index order visits the entries in list order): coefficient per species
+    // (0 = absent) and efficiency per species with a presence mask.
+    int sorted;
+    int reac_d[CH_MAX_SP], prod_d[CH_MAX_SP];
+    unsigned eff_set;
+    double eff_d[CH_MAX_SP];
     int f_from_eq, b_from_eq;
     double fA, fn, fC, bA, bn, bC;
     int nu[CH_MAX_SP];
@@ -64,13 +69,13 @@ constexpr double CH_T_MIN = 10.0, CH_T_MAX = 100000.0;   // gas/gas_model.d:45-4
 constexpr double CH_MIN_MOLES = 1.0e-30;                 // gas/gas_model.d:44
 constexpr double CH_MIN_MASSF = 1.0e-30;                 // gas/gas_model.d:43
 
-// The element of a register array named by a one-hot mask (wave-uniform): a chain of
-// selects over compile-time indices, so the array stays in registers.
+// Element k of a register array for a wave-uniform index k: a select chain over
+// compile-time indices, so the array stays in registers.
 template <int N>
-__device__ __forceinline__ double pick(const double* a, unsigned mask) {
-    double r = 0.0;
+__device__ __forceinline__ double pick(const double* a, int k) {
+    double r = a[0];
 #pragma unroll
-    for (int s = 0; s < N; ++s) r = ((mask >> s) & 1u) ? a[s] : r;
+    for (int s = 1; s < N; ++s) r = (k == s) ? a[s] : r;
     return r;
 }
 
@@ -239,9 +244,10 @@ __device__ __forceinline__ double tpg_sound_speed(const ChemDev& cd, const doubl
     return sqrt(gamma * (R * T));
 }
 
-// c^n for the small whole stoichiometric coefficients of reaction.d's rate products.
+// c^n for the stoichiometric coefficients 1..3 of reaction.d's rate products (the host
+// rejects other coefficients).
 __device__ __forceinline__ double ch_ipow(double c, int n) {
-    return (n == 1) ? c : (n == 2) ? c * c : (n == 3) ? c * (c * c) : (n == 0) ? 1.0 : pow(c, (double)n);
+    return (n == 1) ? c : (n == 2) ? c * c : c * (c * c);
 }
 // pow(P_atm/(R_u T), nuSum) of reaction.d:43-44 for whole nuSum.
 __device__ __forceinline__ double ch_pow_nusum(double x, double nuSum) {
@@ -305,20 +311,36 @@ __device__ __forceinline__ void eval_split_rates(const ChemDev& cd, const double
     for (int ir = 0; ir < NRM; ++ir) {
         if (ir >= cd.nr) continue;
         const ReacDev& r = cd.r[ir];
-        double collider = 0.0;
-        if (r.collider) {
+        double collider = 0.0, wf, wb;
+        if (r.sorted) {
+            if (r.collider) {
 #pragma unroll
-            for (int e = 0; e < NSP; ++e)
-                if (e < r.neff) collider += r.eff[e] * pick<NSP>(conc, r.eff_mask[e]);
+                for (int s = 0; s < NSP; ++s)
+                    if ((r.eff_set >> s) & 1u) collider += r.eff_d[s] * conc[s];
+            }
+            wf = r.collider ? kf[ir] * collider : kf[ir];
+#pragma unroll
+            for (int s = 0; s < NSP; ++s)
+                if (r.reac_d[s]) wf *= ch_ipow(conc[s], r.reac_d[s]);
+            wb = r.collider ? kb[ir] * collider : kb[ir];
+#pragma unroll
+            for (int s = 0; s < NSP; ++s)
+                if (r.prod_d[s]) wb *= ch_ipow(conc[s], r.prod_d[s]);
+        } else {
+            if (r.collider) {
+#pragma unroll
+                for (int e = 0; e < NSP; ++e)
+                    if (e < r.neff) collider += r.eff[e] * pick<NSP>(conc, r.eff_sp[e]);
+            }
+            wf = r.collider ? kf[ir] * collider : kf[ir];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (k < r.nreac) wf *= ch_ipow(pick<NSP>(conc, r.reac_sp[k]), r.reac_nu[k]);
+            wb = r.collider ? kb[ir] * collider : kb[ir];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (k < r.nprod) wb *= ch_ipow(pick<NSP>(conc, r.prod_sp[k]), r.prod_nu[k]);
         }
-        double wf = r.collider ? kf[ir] * collider : kf[ir];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            if (k < r.nreac) wf *= ch_ipow(pick<NSP>(conc, r.reac_mask[k]), r.reac_nu[k]);
-        double wb = r.collider ? kb[ir] * collider : kb[ir];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            if (k < r.nprod) wb *= ch_ipow(pick<NSP>(conc, r.prod_mask[k]), r.prod_nu[k]);
 #pragma unroll
         for (int s = 0; s < NSP; ++s) {
             const int nu = r.nu[s];
@@ -365,43 +387,51 @@ __device__ __forceinline__ double qss_alpha(double p, double h) {
 // AlphaQssStep.opCall, chemistry_update.d:533-655: predictor, up to maxIters correctors
 // with alpha-bar from p_p, convergence against the first predictor, and the step
 // suggestion from three Newton iterations for sqrt(sigma).  yOut is written on success.
+// Trip 0 of the loop is the predictor (rates at y0), trips 1..maxIters the correctors
+// (rates at yp), so the rate code is inlined once; each trip evaluates exactly the
+// reference's expressions for its kind:
+//   predictor  yp = y0 + h (q0 - p0 y0) / (1 + alpha(p0) h p0)
+//   corrector  yc = y0 + h (q~ - p- y0) / (1 + alpha(pp) h p-),
+//              q~ = alpha(pp) qp + (1 - alpha(pp)) q0,  p- = (p0 + pp) / 2
 template <int NSP, int NRM>
-__device__ __forceinline__ bool alpha_qss_step(const ChemDev& cd, const double* kf, const double* kb, const double* y0, double h,
-                               double* yOut, double& hSuggest, int& iters) {
+__device__ __forceinline__ bool alpha_qss_step(const ChemDev& cd, const double* kf, const double* kb, const double* y0,
+                                               double h, double* yOut, double& hSuggest, int& iters) {
     const double ZERO_EPS = 1.0e-50;
-    double q0[NSP], L0[NSP], p0[NSP], yp[NSP], yp0[NSP], yc[NSP];
-    eval_split_rates<NSP, NRM>(cd, kf, kb, y0, q0, L0);
+    double q0[NSP], p0[NSP], yp[NSP], yp0[NSP], yc[NSP];
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
-        p0[s] = L0[s] / (y0[s] + ZERO_EPS);
-        double alpha = qss_alpha(p0[s], h);
-        yp[s] = y0[s] + (h * (q0[s] - p0[s] * y0[s])) / (1.0 + alpha * h * p0[s]);
-        yp0[s] = yp[s];
-        yc[s] = yp[s];
+        q0[s] = 0.0;
+        p0[s] = 0.0;
+        yp[s] = y0[s];
     }
     bool converged = false;
-    for (int corr = 0; corr < cd.maxIters; ++corr) {
-        ++iters;
-        double qp[NSP], Lp[NSP];
-        eval_split_rates<NSP, NRM>(cd, kf, kb, yp, qp, Lp);
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) {
-            double pp = Lp[s] / (yp[s] + ZERO_EPS);
-            double ab = qss_alpha(pp, h);
-            double qt = ab * qp[s] + (1 - ab) * q0[s];
-            double pb = 0.5 * (p0[s] + pp);
-            yc[s] = y0[s] + (h * (qt - pb * y0[s])) / (1.0 + ab * h * pb);
-        }
+    for (int trip = 0; trip <= cd.maxIters; ++trip) {
+        const bool pred = (trip == 0);
+        double q[NSP], L[NSP];
+        eval_split_rates<NSP, NRM>(cd, kf, kb, yp, q, L);  // yp = y0 on the predictor trip
         bool passes = true;
 #pragma unroll
         for (int s = 0; s < NSP; ++s) {
-            if (yc[s] < ZERO_EPS) continue;
-            double test = fabs(yc[s] - yp0[s]);
-            if (test >= (cd.eps1 * (yc[s] + cd.delta))) passes = false;
+            const double p = L[s] / (yp[s] + ZERO_EPS);
+            const double a = qss_alpha(p, h);
+            const double qx = pred ? q[s] : a * q[s] + (1 - a) * q0[s];
+            const double px = pred ? p : 0.5 * (p0[s] + p);
+            const double ynew = y0[s] + (h * (qx - px * y0[s])) / (1.0 + a * h * px);
+            if (pred) {
+                q0[s] = q[s];
+                p0[s] = p;
+                yp0[s] = ynew;
+            } else if (ynew >= ZERO_EPS) {
+                if (fabs(ynew - yp0[s]) >= (cd.eps1 * (ynew + cd.delta))) passes = false;
+            }
+            yc[s] = ynew;
         }
-        if (passes) {
-            converged = true;
-            break;
+        if (!pred) {
+            ++iters;
+            if (passes) {
+                converged = true;
+                break;
+            }
         }
 #pragma unroll
         for (int s = 0; s < NSP; ++s) yp[s] = yc[s];

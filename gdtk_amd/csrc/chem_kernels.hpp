@@ -62,13 +62,16 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_setup(const ChemDev* __rest
         w.kfkb[(size_t)ir * n + i] = kf[ir];
         w.kfkb[(size_t)(nr + ir) * n + i] = kb[ir];
     }
-    // chemistry_update.d:152-158
+    // chemistry_update.d:152-158.  The estimate is also the first step of the retry with
+    // dt_chem = -1 (fvcell.d:899-905), which starts from the same state.
     const double d = dtc[i];
+    const double est = estimate_step_size<NSP, NRM>(cd, kf, kb, conc);
     double h;
     if (d > dt) h = dt;
-    else if (d <= 0.0) h = estimate_step_size<NSP, NRM>(cd, kf, kb, conc);
+    else if (d <= 0.0) h = est;
     else h = d;
     w.h0[i] = h;
+    w.hest[i] = est;
     w.flag[i] = CF_INTEGRATE;
 }
 
@@ -107,7 +110,7 @@ __device__ __forceinline__ void load_cell(const ChemDev& cd, int n, int i, const
     c.iters = 0;
 }
 
-template <int NSP, int NRM>
+template <int NSP, int NRM, int METHOD>
 __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __restrict__ cdp, int n, double dt,
                                                                const double* __restrict__ rho,
                                                                const double* __restrict__ dtc_in,
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
         }
         double y[NSP];
         bool ok;
-        if (cd.method == 1) {
+        if (METHOD == 1) {
             ok = rkf_step<NSP, NRM>(cd, c.kf, c.kb, c.c0, c.h, y, c.dtSug);
         } else {
             ok = alpha_qss_step<NSP, NRM>(cd, c.kf, c.kb, c.c0, c.h, y, c.dtSug, c.iters);
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
             for (int s = 0; s < NSP; ++s) c.c0[s] = w.conc0[(size_t)s * n + i];
             c.pass = 1;
             c.dtSug = -1.0;
-            c.h = estimate_step_size<NSP, NRM>(cd, c.kf, c.kb, c.c0);
+            c.h = w.hest[i];
             c.cycle = 0;
             c.attempt = 0;
             c.t = 0.0;
@@ -375,15 +378,15 @@ struct SetupL {
 
 struct IntegrateL {
     const ChemDev* cd; int n; double dt; const double* rho; const double* dtc_in; double* dtc_out; ChemWork w;
-    ChemTuning tu; int num_cu; hipStream_t s;
-    template <int NSP, int NRM>
-    hipError_t run() {
+    ChemTuning tu; int num_cu; int method; hipStream_t s;
+    template <int NSP, int NRM, int METHOD>
+    hipError_t launch() {
         unsigned grid = grid_for(n);
         if (tu.persistent) {
             int per_cu = tu.blocks_per_cu;
             if (per_cu <= 0) {
                 hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void*>(&k_chem_integrate<NSP, NRM>), CHEM_BLOCK, 0);
+                    &per_cu, reinterpret_cast<const void*>(&k_chem_integrate<NSP, NRM, METHOD>), CHEM_BLOCK, 0);
                 if (e != hipSuccess) return e;
                 if (per_cu < 1) per_cu = 1;
             }
@@ -392,9 +395,13 @@ struct IntegrateL {
             if (grid < 8) grid = 8;  // one workgroup per XCD range at least
         }
         const int chunk = tu.chunk > 0 ? tu.chunk : 64;
-        hipLaunchKernelGGL((k_chem_integrate<NSP, NRM>), dim3(grid), dim3(CHEM_BLOCK), 0, s, cd, n, dt, rho, dtc_in,
-                           dtc_out, w, tu.persistent, chunk, tu.refill_period);
+        hipLaunchKernelGGL((k_chem_integrate<NSP, NRM, METHOD>), dim3(grid), dim3(CHEM_BLOCK), 0, s, cd, n, dt, rho,
+                           dtc_in, dtc_out, w, tu.persistent, chunk, tu.refill_period);
         return hipGetLastError();
+    }
+    template <int NSP, int NRM>
+    hipError_t run() {
+        return method == 1 ? launch<NSP, NRM, 1>() : launch<NSP, NRM, 0>();
     }
 };
 
@@ -431,7 +438,7 @@ hipError_t chem_run(const ChemCall& c) {
         return l.template run<NSP, 32>();
     }
     case CHEM_PHASE_INTEGRATE: {
-        IntegrateL l{c.cd, c.n, c.dt, c.rho, c.dtc_in, c.dtc_out, c.w, c.tu, c.num_cu, c.s};
+        IntegrateL l{c.cd, c.n, c.dt, c.rho, c.dtc_in, c.dtc_out, c.w, c.tu, c.num_cu, c.method, c.s};
         if (NSP == 5 && c.nr == 5) return l.template run<NSP, 5>();
         if (c.nr <= 8) return l.template run<NSP, 8>();
         return l.template run<NSP, 32>();

@@ -17,6 +17,7 @@ struct ChemWork {
     double* kfkb;       // [2*nr][n]  forward then backward rate constants
     double* conc0;      // [nsp][n]   initial concentrations
     double* h0;         // [n]        first substep size
+    double* hest;       // [n]        step estimate from the initial state (retry's first step)
     double* conc;       // [nsp][n]   final concentrations of integrated cells
     int* flag;          // [n]        ChemFlag
     int* st;            // [n]        0 integrated, -1 failed (integrated cells)
@@ -39,7 +40,7 @@ enum ChemPhase : int { CHEM_PHASE_SETUP = 0, CHEM_PHASE_INTEGRATE = 1, CHEM_PHAS
 struct ChemCall {
     int phase;
     const ChemDev* cd;
-    int nr, n, mode;
+    int nr, n, mode, method;
     double dt;
     const double *rho, *T, *u, *p, *massf, *dtc_in;
     double *T_out, *u_out, *p_out, *massf_out, *dtc_out;
@@ -69,7 +70,7 @@ hipError_t launch_chem_clear(const ChemWork& w, hipStream_t s);
 hipError_t launch_chem_setup(const ChemDev* cd, int nsp, int nr, int n, double dt, const double* rho,
                              const double* T, const double* massf, const double* dtc, const ChemWork& w,
                              hipStream_t s);
-hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int n, double dt, const double* rho,
+hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int method, int n, double dt, const double* rho,
                                  const double* dtc_in, double* dtc_out, const ChemWork& w, const ChemTuning& tu,
                                  int num_cu, hipStream_t s);
 hipError_t launch_chem_finish(const ChemDev* cd, int nsp, int n, const double* rho, const double* T,

@@ -209,3 +209,69 @@ def test_alpha_qss_air_conserves_mass_and_relaxes(oracle_lib):
     hot = cells["T"] > 8000.0
     assert np.all(out["T"][hot] < cells["T"][hot])
     assert np.all(out["substeps"] >= 1)
+
+
+def _nasa9_g_molar(th, T):
+    """Molar Gibbs energy g = h - T s (J/mol, standard state 1 atm) of the NASA-9 segment
+    that holds T (T away from the blend ranges), from the published polynomial forms
+    (McBride, Zehe & Gordon, NASA TP-2002-211556, eqs. 4.8-4.10)."""
+    br = th["T_breaks"]
+    seg = max(i for i in range(len(th["coeffs"])) if T >= br[i])
+    a = th["coeffs"][seg]
+    lt = math.log(T)
+    h_rt = -a[0] / T ** 2 + a[1] * lt / T + a[2] + a[3] * T / 2 + a[4] * T ** 2 / 3 + a[5] * T ** 3 / 4 \
+        + a[6] * T ** 4 / 5 + a[7] / T
+    s_r = -a[0] / (2 * T ** 2) - a[1] / T + a[2] * lt + a[3] * T + a[4] * T ** 2 / 2 + a[5] * T ** 3 / 3 \
+        + a[6] * T ** 4 / 4 + a[8]
+    return R_U * T * (h_rt - s_r)
+
+
+def test_equilibrium_constant_against_mass_action(oracle_lib):
+    """The backward rate from K_c (reaction.d:27-52) drives O2 + M <=> O + O + M at fixed
+    T = 3500 K to the composition the law of mass action gives in mole fractions and
+    pressure, x_O^2 / x_O2 * p / P_atm = exp(-(2 g_O - g_O2) / (R T)), with the Gibbs
+    energies from the NASA-9 polynomials evaluated here.  A wrong sign, a missing molar
+    mass or a wrong Delta-nu exponent of K_c = K_p (P_atm / (R T))^Delta-nu moves the
+    kinetic equilibrium by a factor of 3 or more."""
+    O = oracle_lib
+    gd, md = gdata.load("tpg-air-5sp"), gdata.load("park-air-5sp-1T")
+    md = dict(md)
+    md["reactions"] = [r for r in md["reactions"] if r["equation"].startswith("O2 + M")]
+    c = O.Chem(gas_record(gd), mech_record(md))
+    T = 3500.0
+    M = np.array(gd["M"])
+    Y = np.array([0.0, 1.0, 0.0, 0.0, 0.0])
+    rho = 1.0e4 / (R_U / M[1] * T)
+    st = O.TPG(gas_record(gd)).update("rhoT", Y, rho=rho, T=T)
+    rc, mf, dts, stats = c.update(rho, st["p"], T, st["u"], Y, 1.0, -1.0)
+    assert rc == 0
+    conc = mf * rho / M
+    p_eq = (conc[1] + conc[4]) * R_U * T
+    x_o, x_o2 = conc[4] / (conc[1] + conc[4]), conc[1] / (conc[1] + conc[4])
+    lhs = x_o ** 2 / x_o2 * p_eq / 101.325e3
+    g_o = _nasa9_g_molar(gd["thermo"][4], T)
+    g_o2 = _nasa9_g_molar(gd["thermo"][1], T)
+    kp = math.exp(-(2.0 * g_o - g_o2) / (R_U * T))
+    assert 0.05 < x_o < 0.95, x_o  # a real equilibrium, not a pure end state
+    assert _isclose(kp, lhs, 2.0e-3), (kp, lhs)
+
+
+def test_hi_equilibrium_constant_textbook(oracle_lib):
+    """K = kf/kb of H2 + I2 <=> 2 HI with kb from K_c at 700 K against the classic measured
+    value (Bodenstein; K = 54.3 at 698.6 K, e.g. Atkins, Physical Chemistry): the gas data
+    of the reference's H2-I2-HI test put it within 15 %, and exp(-dG/RT) from the NASA-9
+    polynomials here within 1e-10."""
+    O = oracle_lib
+    gj = _json("H2-I2-HI.json")
+    gas = gas_record(gj)
+    rc_f = {"model": "Arrhenius", "A": 1.94e14 * 1e-6, "n": 0.0, "C": 20_620.0}
+    rx = [{"type": "elementary", "reactants": [[0, 1], [1, 1]], "products": [[2, 2]], "frc": rc_f,
+           "brc": {"model": "fromEqConst"}}]
+    c = O.Chem(gas, mech_record(_mech(rx, ["H2", "I2", "HI"], upper=50_000.0)))
+    T = 698.6
+    kf, kb = c.rate_constants(T)
+    K = kf[0] / kb[0]
+    assert abs(K - 54.3) < 0.15 * 54.3, K
+    dg = 2 * _nasa9_g_molar(gj["thermo"][2], T) - _nasa9_g_molar(gj["thermo"][0], T) \
+        - _nasa9_g_molar(gj["thermo"][1], T)
+    assert _isclose(math.exp(-dg / (R_U * T)), K, 1.0e-10), (K, math.exp(-dg / (R_U * T)))
