@@ -238,6 +238,74 @@ def bench_chem(ncells, steps, warmup, with_cpu=True, seed=2024, dt=1.0e-7):
     return res
 
 
+MLP_FLOPS_PER_CELL = 2 * (8 * 256 + 256 * 256 + 256 * 256 + 256 * 5)  # 268 800 per network
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, log=None):
+    """The learned step on the same cells as the alpha-QSS measurement: two networks trained
+    (set-up, untimed) on n_train integrator samples of independently seeded cells, then K
+    timed steps from the same resident inputs, compared cell by cell with the integrator."""
+    import numpy as np
+    import torch
+
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, gas_record, mech_record
+    from gdtk_amd.chemcases import air_cells
+    from gdtk_amd.surrogate import integrator_samples, train_air_surrogate
+    gd = gdata.load("tpg-air-5sp")
+    ch = Chemistry(gas_record(gd), mech_record(gdata.load("park-air-5sp-1T")))
+    sur, info = train_air_surrogate(ch, gd, n_train=n_train, dt=dt, seed=77, log=log)
+    dev = torch.device("cuda:0")
+    cells = air_cells(ncells, seed=seed)
+    t_in, ref, ref_st = integrator_samples(ch, cells, dt, dev)
+    out = {k: torch.empty_like(t_in[k]) for k in ("T", "u", "p", "massf")}
+    dtc = torch.full((ncells,), -1.0, dtype=torch.float64, device=dev)
+    dto = torch.empty_like(dtc)
+    st = torch.empty(ncells, dtype=torch.int32, device=dev)
+    path = torch.empty(ncells, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    args = (t_in["rho"].data_ptr(), t_in["T"].data_ptr(), t_in["u"].data_ptr(), t_in["p"].data_ptr(),
+            t_in["massf"].data_ptr(), dtc.data_ptr(), dt, out["T"].data_ptr(), out["u"].data_ptr(),
+            out["p"].data_ptr(), out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), path.data_ptr(),
+            stream.cuda_stream)
+    sur.step_device(ncells, *args)  # warm-up (packs the BF16 weights)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        sur.step_device(ncells, *args)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    s = sur.stats()
+    P = path.cpu().numpy()
+    dY = (out["massf"] - ref["massf"]).abs().reshape(5, ncells).max(dim=0).values.cpu().numpy()
+    Tref = ref["T"].cpu().numpy()
+    Terr = np.abs(out["T"].cpu().numpy() - Tref) / Tref
+    ok = ref_st.cpu().numpy() == 0
+    achieved = 2 * MLP_FLOPS_PER_CELL * ncells / (s["ms_network"] * 1e-3) / 1e12
+    return {
+        "metric": "chem cells/s (surrogate + alpha-QSS fallback)",
+        "value": ncells / (ms * 1e-3),
+        "unit": "cells/s",
+        "ms_per_step": ms,
+        "steps": steps,
+        "dtype": "bf16 network, f64 projection/T(u)/fallback",
+        "accuracy": {"max_abs_dY": float(dY[ok].max()), "max_rel_T_err": float(Terr[ok].max()),
+                     "p99_abs_dY": float(np.percentile(dY[ok], 99)),
+                     "fallback_fraction": float(np.count_nonzero(P == 1)) / ncells,
+                     "network_cells": int(np.count_nonzero(P == 0)), "frozen_cells": int(np.count_nonzero(P == 2))},
+        "training": {"samples": info["samples"], "sample_s": info["sample_s"], "train_s": info["train_s"],
+                     "final_loss": [float(h[-1]) for h in info["loss"]]},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_mlp_infer<256,3> x2 + features", "avg_launch_ms": s["ms_network"],
+                     "flops_per_cell": 2 * MLP_FLOPS_PER_CELL},
+        "kernels": {k: s[k] for k in ("ms_network", "ms_project", "ms_integrator")},
+    }
+
+
 def copy_probe(nbytes=1 << 30):
     """A plain device copy of known size for HBM counter calibration (FETCH_SIZE/WRITE_SIZE)."""
     import torch
@@ -263,6 +331,8 @@ def main():
                     help="cells of the chemistry workload (0 skips the chemistry measurement)")
     ap.add_argument("--chem-steps", type=int, default=5)
     ap.add_argument("--chem-only", action="store_true", help="measure only the chemistry workload")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
+    ap.add_argument("--surrogate-train", type=int, default=1 << 22, help="integrator samples to train on")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -271,8 +341,19 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
+    def log(msg):
+        print("bench:", msg, file=sys.stderr, flush=True)
+
+    def chem_legs():
+        res = bench_chem(args.chem_cells, args.chem_steps, 1, not args.no_cpu_baseline)
+        if not args.no_surrogate:
+            sres = bench_surrogate(args.chem_cells, args.chem_steps, n_train=args.surrogate_train, log=log)
+            sres["speedup_vs_alpha_qss"] = sres["value"] / res["value"]
+            res["surrogate"] = sres
+        return res
+
     if args.chem_only:
-        print(json.dumps(bench_chem(args.chem_cells, args.chem_steps, 1, not args.no_cpu_baseline)), flush=True)
+        print(json.dumps(chem_legs(), default=float), flush=True)
         return
     nic, njc, nkc = (int(x) for x in args.size.split(","))
     from gdtk_amd import Simulation, cases
@@ -336,8 +417,8 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": str(e)}
     if args.chem_cells > 0:
-        out["chem"] = bench_chem(args.chem_cells, args.chem_steps, 1, not args.no_cpu_baseline)
-    print(json.dumps(out), flush=True)
+        out["chem"] = chem_legs()
+    print(json.dumps(out, default=float), flush=True)
 
 
 if __name__ == "__main__":

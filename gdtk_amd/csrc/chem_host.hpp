@@ -1,0 +1,95 @@
+// chem_host.hpp -- the chemistry handle of the C ABI (defined in chem.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "chem_device.hpp"
+#include "chem_launch.hpp"
+#include "host_common.hpp"
+
+namespace gdtk {
+
+struct Chem {
+    ChemDev host{};
+    ChemDev* dev = nullptr;
+    int nsp = 0, nr = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ChemTuning tuning;
+    DevArr<double> kfkb, conc0, h0, hest, conc;
+    DevArr<int> flag, st, nsub;
+    DevArr<unsigned> queue;
+    DevArr<unsigned long long> stats;
+    // Staging of the host-array entry points.
+    DevArr<double> s_rho, s_T, s_u, s_p, s_massf, s_dtc, s_a;
+    DevArr<int> s_status;
+    // Last call: cells, kernel milliseconds (setup, integrate, finish), event stream.
+    int last_n = 0;
+    hipStream_t last_stream = nullptr;
+    bool have_events = false;
+
+    ~Chem() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (dev) (void)hipFree(dev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    ChemWork work(int n) {
+        const size_t N = (size_t)std::max(n, 1);
+        kfkb.reserve(2 * (size_t)std::max(nr, 1) * N);
+        conc0.reserve((size_t)nsp * N);
+        conc.reserve((size_t)nsp * N);
+        h0.reserve(N);
+        hest.reserve(N);
+        flag.reserve(N);
+        st.reserve(N);
+        nsub.reserve(N);
+        queue.reserve(8 * 32);
+        stats.reserve(4);
+        return ChemWork{kfkb.p, conc0.p, h0.p, hest.p, conc.p, flag.p, st.p, nsub.p, queue.p, stats.p};
+    }
+
+    // The three kernels on device arrays; outputs may alias inputs.
+    void step_device(int n, double dt, const double* rho, const double* T, const double* u, const double* p,
+                     const double* massf, const double* dtc, double* T_out, double* u_out, double* p_out,
+                     double* massf_out, double* dtc_out, int* status, hipStream_t s) {
+        if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
+        if (!(dt > 0.0)) throw Error(GDTK_ERR_ARG, "dt must be positive");
+        ChemWork w = work(n);
+        HIPCHK(hipEventRecord(ev[0], s));
+        HIPCHK(launch_chem_clear(w, s));
+        HIPCHK(launch_chem_setup(dev, nsp, nr, n, dt, rho, T, massf, dtc, w, s));
+        HIPCHK(hipEventRecord(ev[1], s));
+        HIPCHK(launch_chem_integrate(dev, nsp, nr, host.method, n, dt, rho, dtc, dtc_out, w, tuning, num_cu, s));
+        HIPCHK(hipEventRecord(ev[2], s));
+        HIPCHK(launch_chem_finish(dev, nsp, n, rho, T, u, p, massf, dtc, T_out, u_out, p_out, massf_out, dtc_out,
+                                  status, w, s));
+        HIPCHK(hipEventRecord(ev[3], s));
+        last_n = n;
+        last_stream = s;
+        have_events = true;
+    }
+
+    // stats: {total substeps, total corrector iterations, max substeps, step attempts,
+    //         ms setup, ms integrate, ms finish, cells}
+    int read_stats(double* out, int nout) {
+        if (!have_events) throw Error(GDTK_ERR_ARG, "no chemistry step has run");
+        HIPCHK(hipEventSynchronize(ev[3]));
+        unsigned long long h[4] = {0, 0, 0, 0};
+        HIPCHK(hipMemcpy(h, stats.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        float ms[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+        const double v[8] = {(double)h[0], (double)h[1], (double)h[2], (double)h[3],
+                             ms[0], ms[1], ms[2], (double)last_n};
+        int m = std::min(nout, 8);
+        for (int k = 0; k < m; ++k) out[k] = v[k];
+        return m;
+    }
+};
+
+Chem& get_chem(int h);
+
+}  // namespace gdtk

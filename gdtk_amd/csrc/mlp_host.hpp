@@ -1,0 +1,150 @@
+// mlp_host.hpp -- the network handle of the C ABI (defined in mlp.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "host_common.hpp"
+#include "mlp_launch.hpp"
+
+namespace gdtk {
+
+struct Mlp {
+    int n_in = 0, H = 0, NH = 0, n_out = 0;
+    int nlayer = 0;  // NH + 1
+    std::vector<int> in_w, out_w;
+    std::vector<size_t> woff, boff;
+    size_t nparam = 0;
+    DevArr<float> params, grads, am, av;
+    int adam_t = 0;
+    // BF16 inference fragments and the concatenated biases
+    DevArr<unsigned short> frag;
+    DevArr<float> bias_cat;
+    bool packed = false;
+    // training batch buffers
+    int bcap = 0;
+    std::vector<DevArr<float>> Z, A;
+    DevArr<float> xb, yb, dA, dZ, part, xh, yh;
+    DevArr<int> idx;
+    hipStream_t stream = nullptr;
+
+    ~Mlp() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void layout() {
+        nlayer = NH + 1;
+        in_w.assign(nlayer, H);
+        out_w.assign(nlayer, H);
+        in_w[0] = n_in;
+        out_w[nlayer - 1] = n_out;
+        woff.resize(nlayer);
+        boff.resize(nlayer);
+        size_t o = 0;
+        for (int l = 0; l < nlayer; ++l) {
+            woff[l] = o;
+            o += (size_t)out_w[l] * in_w[l];
+            boff[l] = o;
+            o += out_w[l];
+        }
+        nparam = o;
+    }
+
+    void ensure_batch(int B) {
+        if (B <= bcap) return;
+        bcap = B;
+        Z.clear();
+        A.clear();
+        Z.resize(nlayer);
+        A.resize(nlayer);
+        for (int l = 0; l < nlayer; ++l) {
+            Z[l].alloc((size_t)B * out_w[l]);
+            A[l].alloc((size_t)B * out_w[l]);
+        }
+        const size_t wmax = (size_t)std::max(std::max(H, n_in), n_out);
+        xb.alloc((size_t)B * n_in);
+        yb.alloc((size_t)B * n_out);
+        dA.alloc((size_t)B * wmax);
+        dZ.alloc((size_t)B * wmax);
+        part.alloc((size_t)mse_parts(B, n_out));
+    }
+
+    // Forward pass of a cell-major batch X [B][n_in]; output A[last] = Z[last].
+    void forward(int B, const float* X, hipStream_t s) {
+        const float* in = X;
+        for (int l = 0; l < nlayer; ++l) {
+            const int ni = in_w[l], no = out_w[l];
+            // Z[b][o] = sum_k in[b][k] W[o][k]
+            HIPCHK(launch_gemm_f32(B, no, ni, in, ni, 1, params.p + woff[l], 1, ni, Z[l].p, no, 0, s));
+            HIPCHK(launch_bias_act(B, no, params.p + boff[l], Z[l].p, A[l].p, l < nlayer - 1 ? 1 : 0, s));
+            in = A[l].p;
+        }
+    }
+
+    // Backward pass after forward(B, X) and the loss gradient in dZ (output layer).
+    void backward(int B, const float* X, hipStream_t s) {
+        for (int l = nlayer - 1; l >= 0; --l) {
+            const int ni = in_w[l], no = out_w[l];
+            const float* in = (l == 0) ? X : A[l - 1].p;
+            // dW[o][k] = sum_b dZ[b][o] in[b][k]
+            HIPCHK(launch_gemm_f32(no, ni, B, dZ.p, 1, no, in, ni, 1, grads.p + woff[l], ni, 0, s));
+            HIPCHK(launch_colsum(B, no, dZ.p, grads.p + boff[l], s));
+            if (l == 0) break;
+            // dA[b][k] = sum_o dZ[b][o] W[o][k]
+            HIPCHK(launch_gemm_f32(B, ni, no, dZ.p, no, 1, params.p + woff[l], ni, 1, dA.p, ni, 0, s));
+            HIPCHK(launch_gelu_back((size_t)B * ni, Z[l - 1].p, dA.p, dZ.p, s));
+        }
+    }
+
+    // Mean-squared-error loss of the batch (sum of part[] / (B n_out)) and its gradient.
+    void loss_grad(int B, const float* X, const float* Y, hipStream_t s) {
+        forward(B, X, s);
+        HIPCHK(launch_mse(B, n_out, A[nlayer - 1].p, Y, dZ.p, part.p, s));
+        backward(B, X, s);
+    }
+
+    double read_loss(int B, hipStream_t s) {
+        const int np = mse_parts(B, n_out);
+        std::vector<float> h((size_t)np);
+        HIPCHK(hipMemcpyAsync(h.data(), part.p, np * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        double acc = 0.0;
+        for (float v : h) acc += v;
+        return acc / ((double)B * n_out);
+    }
+
+    void adam(float lr, hipStream_t s) {
+        ++adam_t;
+        HIPCHK(launch_adam(nparam, params.p, grads.p, am.p, av.p, lr, 0.9f, 0.999f, 1.0e-8f, adam_t, s));
+        packed = false;
+    }
+
+    void pack(hipStream_t s) {
+        if (packed) return;
+        size_t nfrag = 0;
+        std::vector<size_t> foff(nlayer);
+        for (int l = 0; l < nlayer; ++l) {
+            foff[l] = nfrag;
+            nfrag += (size_t)((out_w[l] + 15) / 16) * ((in_w[l] + 31) / 32) * 64 * 8;
+        }
+        frag.reserve(nfrag);
+        size_t nb = 0;
+        for (int l = 0; l < nlayer; ++l) nb += out_w[l];
+        bias_cat.reserve(nb);
+        size_t bo = 0;
+        for (int l = 0; l < nlayer; ++l) {
+            HIPCHK(launch_mlp_pack(params.p + woff[l], out_w[l], in_w[l], l > 0 ? 1 : 0, frag.p + foff[l], s));
+            HIPCHK(hipMemcpyAsync(bias_cat.p + bo, params.p + boff[l], out_w[l] * sizeof(float),
+                                  hipMemcpyDeviceToDevice, s));
+            bo += out_w[l];
+        }
+        packed = true;
+    }
+};
+
+Mlp& get_mlp(int h);
+// BF16 inference of n cells: x SoA [n_in][n] -> y SoA [n_out][n] (device arrays).
+void mlp_infer(Mlp& m, int n, const float* x, float* y, hipStream_t s);
+
+}  // namespace gdtk

@@ -1,0 +1,350 @@
+// mlp_kernels.hip -- the chemistry surrogate network on gfx950: FP32 training kernels and a
+// BF16 MFMA inference kernel (no counterpart in the reference).
+//
+// Network: x[n_in] -> (W_l x + b_l, GELU) for the hidden layers of width H -> W_out h + b_out.
+// GELU(z) = z/2 (1 + erf(z/sqrt 2)).  Activations are cell-major [batch][width] in training.
+//
+// Inference keeps a wave's 16 cells in registers through every layer.  Each layer is
+// computed transposed, Y^T = W X^T, with v_mfma_f32_16x16x32_bf16:
+//   A operand (16 x 32) = a 16-row block of W,  lane l holds A[l%16][slot (l/16, j)], j < 8
+//   B operand (32 x 16) = 32 inputs x 16 cells,  lane l holds B[slot (l/16, j)][l%16]
+//   D (16 x 16) = 16 outputs x 16 cells,          lane l holds D[4 (l/16) + i][l%16], i < 4
+// The sum over K pairs slot (g, j) of A with slot (g, j) of B, so any assignment of input
+// indices to slots works if A and B agree.  The B fragment of k-step s of the next layer is
+// taken straight from this layer's D registers of row blocks 2s and 2s+1 (after bias, GELU
+// and rounding to BF16): slot (g, j) holds input 32 s + 4 g + j for j < 4 and
+// 32 s + 16 + 4 g + (j - 4) for j >= 4.  The packed weights of that layer use the same slot
+// map, so activations never leave registers.  The first layer's slots are k = 8 g + j over
+// the zero-padded inputs.  Packed fragments are 64 lanes x 8 BF16 = 1 KiB, one 16-byte load
+// per lane.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "mlp_launch.hpp"
+
+namespace gdtk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+    // d/dz [z/2 (1 + erf(z/sqrt2))] = (1 + erf(z/sqrt2))/2 + z exp(-z^2/2)/sqrt(2 pi)
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned int u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// Slot map of layer inputs: k-step s, lane group g, element j -> input index.
+__device__ __host__ inline int slot_first(int s, int g, int j) { return 32 * s + 8 * g + j; }
+__device__ __host__ inline int slot_chain(int s, int g, int j) { return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
+
+//---------------------------------------------------------------------------------
+// Lane-layout probe: D = A B for one 16x32 A and 32x16 B (row-major FP32 inputs, exact in
+// BF16), through the MFMA instruction and the slot map above with slots k = 8 g + j.
+__global__ void k_mfma_probe(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ D) {
+    const int l = threadIdx.x;
+    const int r = l & 15, g = l >> 4;
+    u16x8 a, b;
+    for (int j = 0; j < 8; ++j) {
+        a[j] = f2bf(A[r * 32 + slot_first(0, g, j)]);
+        b[j] = f2bf(B[slot_first(0, g, j) * 16 + r]);
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc, 0,
+                                                  0, 0);
+    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = acc[i];
+}
+
+hipError_t launch_mfma_probe(const float* A, const float* B, float* D, hipStream_t s) {
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, A, B, D);
+    return hipGetLastError();
+}
+
+//---------------------------------------------------------------------------------
+// Packing of FP32 weights W[out][in] (row-major) into BF16 fragments:
+// frag[((rb * nks) + s) * 64 + lane][j] = W[16 rb + l%16][slot(s, l/16, j)], zero outside.
+__global__ void k_mlp_pack(const float* __restrict__ W, int n_out, int n_in, int nrb, int nks, int chain,
+                           unsigned short* __restrict__ frag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = nrb * nks * 64 * 8;
+    if (t >= total) return;
+    const int j = t & 7, lane = (t >> 3) & 63, fs = t >> 9;
+    const int s = fs % nks, rb = fs / nks;
+    const int row = 16 * rb + (lane & 15), g = lane >> 4;
+    const int k = chain ? slot_chain(s, g, j) : slot_first(s, g, j);
+    const float v = (row < n_out && k < n_in) ? W[(size_t)row * n_in + k] : 0.0f;
+    frag[t] = f2bf(v);
+}
+
+hipError_t launch_mlp_pack(const float* W, int n_out, int n_in, int chain, unsigned short* frag, hipStream_t s) {
+    const int nrb = (n_out + 15) / 16, nks = (n_in + 31) / 32;
+    const int total = nrb * nks * 64 * 8;
+    hipLaunchKernelGGL(k_mlp_pack, dim3((total + 255) / 256), dim3(256), 0, s, W, n_out, n_in, nrb, nks, chain, frag);
+    return hipGetLastError();
+}
+
+//---------------------------------------------------------------------------------
+// Inference: x SoA [n_in][n] FP32 -> y SoA [n_out][n] FP32.  H hidden width (multiple of
+// 32), NH hidden layers, n_in <= 32, n_out <= 16.  frag: packed layers in order; bias:
+// FP32 biases in order.
+template <int H, int NH>
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, int n_in, int n,
+                                                   const unsigned short* __restrict__ frag,
+                                                   const float* __restrict__ bias, int n_out, float* __restrict__ y) {
+    constexpr int NRB = H / 16, NKS = H / 32;
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int cell = wave * 16 + r;
+    const bool live = cell < n;
+    const bf16x8* F = reinterpret_cast<const bf16x8*>(frag);
+
+    // Layer 1 B fragment: inputs 8 g + j of this lane's cell.
+    u16x8 xb;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = 8 * g + j;
+        xb[j] = (live && k < n_in) ? f2bf(x[(size_t)k * n + cell]) : (unsigned short)0;
+    }
+    bf16x8 act[NKS];
+    f32x4 acc[NRB];
+    // first hidden layer (one k-step over the padded inputs)
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+        f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[rb * 64 + lane], __builtin_bit_cast(bf16x8, xb), z, 0, 0, 0);
+    }
+    const bf16x8* Fl = F + NRB * 64;
+    const float* bl = bias;
+    for (int layer = 0; layer < NH; ++layer) {
+        // bias + GELU + BF16 -> B fragments of the next layer
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) {
+            u16x8 v;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = f2bf(gelu(acc[2 * s][i] + bl[32 * s + 4 * g + i]));
+                v[4 + i] = f2bf(gelu(acc[2 * s + 1][i] + bl[32 * s + 16 + 4 * g + i]));
+            }
+            act[s] = __builtin_bit_cast(bf16x8, v);
+        }
+        bl += H;
+        if (layer == NH - 1) break;
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) acc[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < NKS; ++s) {
+#pragma unroll
+            for (int rb = 0; rb < NRB; ++rb)
+                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fl[(rb * NKS + s) * 64 + lane], act[s], acc[rb], 0, 0, 0);
+        }
+        Fl += NRB * NKS * 64;
+    }
+    // output layer: one 16-row block
+    f32x4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NKS; ++s) o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fl[s * 64 + lane], act[s], o, 0, 0, 0);
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * g + i;
+            if (row < n_out) y[(size_t)row * n + cell] = o[i] + bl[row];
+        }
+    }
+}
+
+hipError_t launch_mlp_infer(int H, int NH, const float* x, int n_in, int n, const unsigned short* frag,
+                            const float* bias, int n_out, float* y, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + 63) / 64);  // 4 waves x 16 cells per workgroup
+    if (H == 256 && NH == 3) {
+        hipLaunchKernelGGL((k_mlp_infer<256, 3>), dim3(grid), dim3(256), 0, s, x, n_in, n, frag, bias, n_out, y);
+    } else if (H == 64 && NH == 2) {
+        hipLaunchKernelGGL((k_mlp_infer<64, 2>), dim3(grid), dim3(256), 0, s, x, n_in, n, frag, bias, n_out, y);
+    } else {
+        return hipErrorNotSupported;
+    }
+    return hipGetLastError();
+}
+
+//---------------------------------------------------------------------------------
+// FP32 GEMM for training: C[M][N] (+)= sum_k A(m,k) B(k,n) with
+// A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], C row-major with leading dim ldc.
+// 64x64 tiles, 256 threads with 4x4 outputs each, k tiles of 16 through LDS.
+constexpr int GT = 64, GK = 16;
+
+__global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, const float* __restrict__ A, int sam, int sak,
+                                                  const float* __restrict__ B, int sbk, int sbn, float* __restrict__ C,
+                                                  int ldc, int accumulate) {
+    __shared__ float As[GK][GT + 1];
+    __shared__ float Bs[GK][GT + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    float c[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) c[a][b] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        for (int e = threadIdx.x; e < GK * GT; e += 256) {
+            const int kk = e / GT, mm = e % GT;
+            const int m = m0 + mm, k = k0 + kk;
+            As[kk][mm] = (m < M && k < K) ? A[(size_t)m * sam + (size_t)k * sak] : 0.0f;
+            const int nn = mm, nidx = n0 + nn;
+            Bs[kk][nn] = (nidx < N && k < K) ? B[(size_t)k * sbk + (size_t)nidx * sbn] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = As[kk][ty * 4 + q];
+                b[q] = Bs[kk][tx * 4 + q];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) c[p][q] += a[p] * b[q];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int m = m0 + ty * 4 + p;
+        if (m >= M) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int nidx = n0 + tx * 4 + q;
+            if (nidx >= N) continue;
+            float* dst = C + (size_t)m * ldc + nidx;
+            *dst = accumulate ? *dst + c[p][q] : c[p][q];
+        }
+    }
+}
+
+hipError_t launch_gemm_f32(int M, int N, int K, const float* A, int sam, int sak, const float* B, int sbk, int sbn,
+                           float* C, int ldc, int accumulate, hipStream_t s) {
+    if (M <= 0 || N <= 0) return hipSuccess;
+    dim3 grid((unsigned)((N + GT - 1) / GT), (unsigned)((M + GT - 1) / GT));
+    hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate);
+    return hipGetLastError();
+}
+
+// Z[b][j] += bias[j]; A = GELU(Z) (act != 0) or A = Z.
+__global__ void k_bias_act(int rows, int cols, const float* __restrict__ bias, float* __restrict__ Z,
+                           float* __restrict__ Aout, int act) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)rows * cols) return;
+    const float z = Z[t] + bias[t % cols];
+    Z[t] = z;
+    Aout[t] = act ? gelu(z) : z;
+}
+
+hipError_t launch_bias_act(int rows, int cols, const float* bias, float* Z, float* A, int act, hipStream_t s) {
+    const size_t n = (size_t)rows * cols;
+    hipLaunchKernelGGL(k_bias_act, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, cols, bias, Z, A, act);
+    return hipGetLastError();
+}
+
+// dZ = dA * GELU'(Z)
+__global__ void k_gelu_back(size_t n, const float* __restrict__ Z, const float* __restrict__ dA, float* __restrict__ dZ) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dZ[t] = dA[t] * gelu_grad(Z[t]);
+}
+
+hipError_t launch_gelu_back(size_t n, const float* Z, const float* dA, float* dZ, hipStream_t s) {
+    hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, Z, dA, dZ);
+    return hipGetLastError();
+}
+
+// Mean-squared error over rows x cols with per-row weights 1: dY = 2 (Yp - Yt) / (rows cols);
+// the loss sum is accumulated per workgroup into part[blockIdx].
+__global__ void k_mse(int rows, int cols, const float* __restrict__ Yp, const float* __restrict__ Yt,
+                      float* __restrict__ dY, float* __restrict__ part) {
+    __shared__ float red[256];
+    const size_t n = (size_t)rows * cols;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float e = 0.0f;
+    if (t < n) {
+        const float d = Yp[t] - Yt[t];
+        dY[t] = 2.0f * d / (float)n;
+        e = d * d;
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+int mse_parts(int rows, int cols) { return (int)(((size_t)rows * cols + 255) / 256); }
+
+hipError_t launch_mse(int rows, int cols, const float* Yp, const float* Yt, float* dY, float* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_mse, dim3((unsigned)mse_parts(rows, cols)), dim3(256), 0, s, rows, cols, Yp, Yt, dY, part);
+    return hipGetLastError();
+}
+
+// db[j] = sum_b dZ[b][j]
+__global__ void k_colsum(int rows, int cols, const float* __restrict__ dZ, float* __restrict__ db) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    float s = 0.0f;
+    for (int b = 0; b < rows; ++b) s += dZ[(size_t)b * cols + j];
+    db[j] = s;
+}
+
+hipError_t launch_colsum(int rows, int cols, const float* dZ, float* db, hipStream_t s) {
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, s, rows, cols, dZ, db);
+    return hipGetLastError();
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; t is the 1-based step.
+__global__ void k_adam(size_t n, float* __restrict__ p, const float* __restrict__ grad, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = grad[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+hipError_t launch_adam(size_t n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                       int t, hipStream_t s) {
+    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p, g, m, v, lr, b1, b2, eps, c1,
+                       c2);
+    return hipGetLastError();
+}
+
+// Gather rows of a SoA [width][n] array into a cell-major batch [B][width]: out[b][f] =
+// src[f*n + idx[b]].
+__global__ void k_gather_batch(int B, int width, int n, const int* __restrict__ idx, const float* __restrict__ src,
+                               float* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * width) return;
+    const int b = t / width, f = t % width;
+    out[t] = src[(size_t)f * n + idx[b]];
+}
+
+hipError_t launch_gather_batch(int B, int width, int n, const int* idx, const float* src, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_batch, dim3((unsigned)((B * width + 255) / 256)), dim3(256), 0, s, B, width, n, idx, src,
+                       out);
+    return hipGetLastError();
+}
+
+}  // namespace gdtk

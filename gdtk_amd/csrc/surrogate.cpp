@@ -1,0 +1,203 @@
+// surrogate.cpp -- host side of the learned chemistry step (C ABI in include/gdtk_hip.h).
+//
+// gdtk_hip_surrogate_step_device is a drop-in for gdtk_hip_chem_step_device (the same
+// per-cell outputs) that answers most cells with two trained networks and an exact
+// element-conservation projection, and hands the rest (outside the trained range, networks
+// in disagreement, infeasible projection, failed T(u) iteration) to the alpha-QSS
+// integrator of the chemistry handle.  No counterpart in the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/gdtk_hip.h"
+#include "chem_host.hpp"
+#include "host_common.hpp"
+#include "mlp_host.hpp"
+#include "surrogate_launch.hpp"
+
+namespace gdtk {
+
+static_assert(GDTK_SURR_MAX_FEAT == SURR_MAX_FEAT && GDTK_SURR_MAX_ELEM == SURR_MAX_ELEM, "record sizes");
+
+struct Surrogate {
+    int chem = -1, net_a = -1, net_b = -1;
+    int nsp = 0;
+    SurrCfg cfg{};
+    DevArr<float> x, ya, yb;
+    DevArr<double> massf_new, disagree;
+    DevArr<int> path, fb_idx;
+    DevArr<unsigned> fb_count;
+    DevArr<double> c_rho, c_T, c_u, c_p, c_massf, c_dtc;
+    DevArr<int> c_st;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int last_n = 0, last_m = 0, last_frozen = 0;
+    bool have = false;
+
+    ~Surrogate() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+
+    void step(int n, double dt, const double* rho, const double* T, const double* u, const double* p,
+              const double* massf, const double* dtc, double* T_out, double* u_out, double* p_out, double* massf_out,
+              double* dtc_out, int* status, int* path_out, hipStream_t s) {
+        if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
+        if (!(dt > 0.0)) throw Error(GDTK_ERR_ARG, "dt must be positive");
+        Chem& ch = get_chem(chem);
+        Mlp& ma = get_mlp(net_a);
+        Mlp& mb = get_mlp(net_b);
+        const size_t N = (size_t)std::max(n, 1);
+        x.reserve((size_t)(nsp + 3) * N);
+        ya.reserve((size_t)nsp * N);
+        yb.reserve((size_t)nsp * N);
+        massf_new.reserve((size_t)nsp * N);
+        disagree.reserve(N);
+        path.reserve(N);
+        fb_idx.reserve(N);
+        fb_count.reserve(1);
+        int* pth = path_out ? path_out : path.p;
+        HIPCHK(hipEventRecord(ev[0], s));
+        HIPCHK(launch_surr_clear(fb_count.p, s));
+        HIPCHK(launch_surr_features(nsp, n, rho, T, massf, dt, cfg, x.p, s));
+        mlp_infer(ma, n, x.p, ya.p, s);
+        mlp_infer(mb, n, x.p, yb.p, s);
+        HIPCHK(hipEventRecord(ev[1], s));
+        HIPCHK(launch_surr_apply(nsp, n, T, x.p, ya.p, yb.p, massf, cfg, massf_new.p, pth, fb_idx.p, fb_count.p,
+                                 disagree.p, s));
+        HIPCHK(launch_surr_finish(nsp, n, rho, T, u, p, massf, dtc, massf_new.p, ch.dev, T_out, u_out, p_out, massf_out,
+                                  dtc_out, status, pth, fb_idx.p, fb_count.p, s));
+        HIPCHK(hipEventRecord(ev[2], s));
+        unsigned m = 0;
+        HIPCHK(hipMemcpyAsync(&m, fb_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (m > (unsigned)n) throw Error(GDTK_ERR_HIP, "fallback queue overflow");
+        if (m > 0) {
+            const size_t M = m;
+            c_rho.reserve(M);
+            c_T.reserve(M);
+            c_u.reserve(M);
+            c_p.reserve(M);
+            c_dtc.reserve(M);
+            c_massf.reserve((size_t)nsp * M);
+            c_st.reserve(M);
+            HIPCHK(launch_surr_gather(nsp, (int)m, n, fb_idx.p, rho, T, u, p, massf, dtc, c_rho.p, c_T.p, c_u.p, c_p.p,
+                                      c_massf.p, c_dtc.p, s));
+            ch.step_device((int)m, dt, c_rho.p, c_T.p, c_u.p, c_p.p, c_massf.p, c_dtc.p, c_T.p, c_u.p, c_p.p, c_massf.p,
+                           c_dtc.p, c_st.p, s);
+            HIPCHK(launch_surr_scatter(nsp, (int)m, n, fb_idx.p, c_T.p, c_u.p, c_p.p, c_massf.p, c_dtc.p, c_st.p, T_out,
+                                       u_out, p_out, massf_out, dtc_out, status, s));
+        }
+        HIPCHK(hipEventRecord(ev[3], s));
+        last_n = n;
+        last_m = (int)m;
+        have = true;
+    }
+
+    // {cells, cells queued for the integrator, ms features + networks, ms projection + T(u),
+    //  ms integrator (gather, alpha-QSS, scatter), max network disagreement}
+    int stats(double* out, int nout) {
+        if (!have) throw Error(GDTK_ERR_ARG, "no surrogate step has run");
+        HIPCHK(hipEventSynchronize(ev[3]));
+        float ms[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+        double dmax = 0.0;
+        if (last_n > 0) {
+            std::vector<double> d((size_t)last_n);
+            HIPCHK(hipMemcpy(d.data(), disagree.p, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (double v : d)
+                if (v > dmax) dmax = v;
+        }
+        const double v[6] = {(double)last_n, (double)last_m, ms[0], ms[1], ms[2], dmax};
+        const int m = std::min(nout, 6);
+        for (int k = 0; k < m; ++k) out[k] = v[k];
+        return m;
+    }
+};
+
+static std::mutex g_surr_mu;
+static std::vector<std::unique_ptr<Surrogate>> g_surrs;
+
+static Surrogate& get_surr(int h) {
+    std::lock_guard<std::mutex> lk(g_surr_mu);
+    if (h < 0 || h >= (int)g_surrs.size() || !g_surrs[h]) throw Error(GDTK_ERR_ARG, "bad surrogate handle");
+    return *g_surrs[h];
+}
+
+}  // namespace gdtk
+
+using namespace gdtk;
+
+extern "C" {
+
+int gdtk_hip_surrogate_create(int chem, int net_a, int net_b, const gdtk_surrogate_cfg* cfg, int* h) {
+    return guarded([&] {
+        if (!cfg || !h) throw Error(GDTK_ERR_ARG, "null argument");
+        Chem& ch = get_chem(chem);
+        Mlp& ma = get_mlp(net_a);
+        Mlp& mb = get_mlp(net_b);
+        const int nsp = ch.nsp;
+        if (ma.n_in != nsp + 3 || mb.n_in != nsp + 3 || ma.n_out != nsp || mb.n_out != nsp)
+            throw Error(GDTK_ERR_ARG, "networks must map nsp + 3 features to nsp outputs");
+        if (cfg->n_elem < 1 || cfg->n_elem > SURR_MAX_ELEM) throw Error(GDTK_ERR_ARG, "n_elem must be 1..4");
+        if (!(cfg->x_abs_max > 0.0) || !(cfg->disagree_tol >= 0.0)) throw Error(GDTK_ERR_ARG, "bad thresholds");
+        auto s = std::make_unique<Surrogate>();
+        s->chem = chem;
+        s->net_a = net_a;
+        s->net_b = net_b;
+        s->nsp = nsp;
+        for (int k = 0; k < SURR_MAX_FEAT; ++k) {
+            s->cfg.x_mean[k] = cfg->x_mean[k];
+            s->cfg.x_inv_std[k] = cfg->x_inv_std[k];
+        }
+        for (int k = 0; k < CH_MAX_SP; ++k) s->cfg.y_scale[k] = cfg->y_scale[k];
+        s->cfg.x_abs_max = cfg->x_abs_max;
+        s->cfg.disagree_tol = cfg->disagree_tol;
+        s->cfg.T_frozen = ch.host.T_frozen;
+        s->cfg.n_elem = cfg->n_elem;
+        for (int e = 0; e < SURR_MAX_ELEM; ++e)
+            for (int k = 0; k < CH_MAX_SP; ++k) s->cfg.elem[e][k] = cfg->elem[e][k];
+        for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+        std::lock_guard<std::mutex> lk(g_surr_mu);
+        g_surrs.push_back(std::move(s));
+        *h = (int)g_surrs.size() - 1;
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_surrogate_destroy(int h) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_surr_mu);
+        if (h < 0 || h >= (int)g_surrs.size() || !g_surrs[h]) throw Error(GDTK_ERR_ARG, "bad surrogate handle");
+        HIPCHK(hipDeviceSynchronize());
+        g_surrs[h].reset();
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_surrogate_step_device(int h, int n, const double* rho, const double* T, const double* u, const double* p,
+                                   const double* massf, const double* dt_chem, double dt, double* T_out, double* u_out,
+                                   double* p_out, double* massf_out, double* dt_chem_out, int* status, int* path,
+                                   void* stream) {
+    return guarded([&] {
+        Surrogate& s = get_surr(h);
+        if (n > 0 && (!rho || !T || !u || !p || !massf || !dt_chem || !T_out || !u_out || !p_out || !massf_out ||
+                      !dt_chem_out || !status))
+            throw Error(GDTK_ERR_ARG, "null array");
+        s.step(n, dt, rho, T, u, p, massf, dt_chem, T_out, u_out, p_out, massf_out, dt_chem_out, status, path,
+               static_cast<hipStream_t>(stream));
+        return s.last_m;
+    });
+}
+
+int gdtk_hip_surrogate_stats(int h, double* stats, int nstats) {
+    return guarded([&] {
+        if (!stats || nstats < 1) throw Error(GDTK_ERR_ARG, "null stats");
+        return get_surr(h).stats(stats, nstats);
+    });
+}
+
+}  // extern "C"

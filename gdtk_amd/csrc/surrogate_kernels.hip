@@ -1,0 +1,334 @@
+// surrogate_kernels.hip -- the learned chemistry step around the BF16 network (no
+// counterpart in the reference).
+//
+// Per cell: features (log T, log rho, Y[nsp], log dt, standardised) -> two networks ->
+// mean mass-fraction change -> exact element-conservation projection with an active set that
+// keeps every mass fraction >= 0 -> T, u, p by the gas model's Newton T(u) (as the
+// reference's thermochemical_increment ends, fvcell.d:906-925).  Cells whose features lie
+// outside the trained range, whose two networks disagree, or whose projection cannot be
+// made feasible are queued (compacted by an atomic counter) for the alpha-QSS integrator.
+#include <hip/hip_runtime.h>
+
+#include "chem_device.hpp"
+#include "surrogate_launch.hpp"
+
+namespace gdtk {
+
+template <int NSP>
+__global__ __launch_bounds__(256) void k_surr_features(int n, const double* __restrict__ rho,
+                                                       const double* __restrict__ T,
+                                                       const double* __restrict__ massf, double dt, SurrCfg cfg,
+                                                       float* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f[NSP + 3];
+    f[0] = log(T[i]);
+    f[1] = log(rho[i]);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) f[2 + s] = massf[(size_t)s * n + i];
+    f[NSP + 2] = log(dt);
+#pragma unroll
+    for (int k = 0; k < NSP + 3; ++k) x[(size_t)k * n + i] = (float)((f[k] - cfg.x_mean[k]) * cfg.x_inv_std[k]);
+}
+
+// Active-set least-squares projection: the change d closest to the network's prediction
+// d0 with E d = 0 (every element's mass conserved; the rows of E sum to the species' unit
+// mass, so total mass is conserved too) and Y0 + d >= 0.  Species driven below zero are
+// fixed at d = -Y0 and the rest re-projected, at most NSP rounds.  Returns false if the
+// free species cannot carry the element balance.
+template <int NSP>
+__device__ __forceinline__ bool project(const SurrCfg& cfg, const double* Y0, const double* d0, double* d) {
+    constexpr int ME = SURR_MAX_ELEM;
+    const int ne = cfg.n_elem;
+    bool fixed[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) fixed[s] = false;
+    for (int round = 0; round <= NSP; ++round) {
+        // G lam = rhs with G = E_U E_U^T (free species U), rhs = E (d0 on U, -Y0 on fixed)
+        double G[ME][ME], rhs[ME], L[ME][ME], y[ME], lam[ME];
+#pragma unroll
+        for (int e = 0; e < ME; ++e) {
+            rhs[e] = 0.0;
+            y[e] = 0.0;
+            lam[e] = 0.0;
+#pragma unroll
+            for (int f = 0; f < ME; ++f) {
+                G[e][f] = (e == f) ? 1.0 : 0.0;
+                L[e][f] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < ME; ++e) {
+            if (e >= ne) continue;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) rhs[e] += cfg.elem[e][s] * (fixed[s] ? -Y0[s] : d0[s]);
+#pragma unroll
+            for (int f = 0; f < ME; ++f) {
+                if (f >= ne) continue;
+                double gsum = 0.0;
+#pragma unroll
+                for (int s = 0; s < NSP; ++s)
+                    if (!fixed[s]) gsum += cfg.elem[e][s] * cfg.elem[f][s];
+                G[e][f] = gsum;
+            }
+        }
+        // Cholesky G = L L^T (unused rows keep the identity)
+#pragma unroll
+        for (int c = 0; c < ME; ++c) {
+            double diag = G[c][c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) diag -= L[c][k] * L[c][k];
+            if (!(diag > 1.0e-14)) return false;
+            L[c][c] = sqrt(diag);
+#pragma unroll
+            for (int r = c + 1; r < ME; ++r) {
+                double v = G[r][c];
+#pragma unroll
+                for (int k = 0; k < c; ++k) v -= L[r][k] * L[c][k];
+                L[r][c] = v / L[c][c];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ME; ++r) {
+            double v = rhs[r];
+#pragma unroll
+            for (int k = 0; k < r; ++k) v -= L[r][k] * y[k];
+            y[r] = v / L[r][r];
+        }
+#pragma unroll
+        for (int r = ME - 1; r >= 0; --r) {
+            double v = y[r];
+#pragma unroll
+            for (int k = r + 1; k < ME; ++k) v -= L[k][r] * lam[k];
+            lam[r] = v / L[r][r];
+        }
+        bool clean = true;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+            if (fixed[s]) {
+                d[s] = -Y0[s];
+                continue;
+            }
+            double corr = 0.0;
+#pragma unroll
+            for (int e = 0; e < ME; ++e)
+                if (e < ne) corr += cfg.elem[e][s] * lam[e];
+            d[s] = d0[s] - corr;
+            if (Y0[s] + d[s] < 0.0) {
+                fixed[s] = true;
+                clean = false;
+            }
+        }
+        if (clean) return true;
+    }
+    return false;
+}
+
+// path: 0 surrogate, 1 queued for the integrator, 2 frozen (T <= T_frozen: left unchanged,
+// fvcell.d:870).
+template <int NSP>
+__global__ __launch_bounds__(256) void k_surr_apply(int n, const double* __restrict__ T, const float* __restrict__ x,
+                                                    const float* __restrict__ ya, const float* __restrict__ yb,
+                                                    const double* __restrict__ massf, SurrCfg cfg,
+                                                    double* __restrict__ massf_new, int* __restrict__ path,
+                                                    int* __restrict__ fb_idx, unsigned* __restrict__ fb_count,
+                                                    double* __restrict__ disagree) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (T[i] <= cfg.T_frozen) {
+        path[i] = 2;
+        disagree[i] = 0.0;
+        return;
+    }
+    bool fallback = false;
+#pragma unroll
+    for (int k = 0; k < NSP + 3; ++k)
+        if (!(fabsf(x[(size_t)k * n + i]) <= (float)cfg.x_abs_max)) fallback = true;
+    double Y0[NSP], d0[NSP], d[NSP];
+    double dis = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        Y0[s] = massf[(size_t)s * n + i];
+        const double a = ya[(size_t)s * n + i], b = yb[(size_t)s * n + i];
+        d0[s] = 0.5 * (a + b) * cfg.y_scale[s];
+        dis = fmax(dis, fabs(a - b) * cfg.y_scale[s]);
+    }
+    disagree[i] = dis;
+    if (!(dis <= cfg.disagree_tol)) fallback = true;
+    if (!fallback && !project<NSP>(cfg, Y0, d0, d)) fallback = true;
+    if (!fallback) {
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) massf_new[(size_t)s * n + i] = Y0[s] + d[s];
+    }
+    path[i] = fallback ? 1 : 0;
+    if (fallback) fb_idx[atomicAdd(fb_count, 1u)] = i;
+}
+
+// T, u, p of the surrogate cells by Newton T(u) at the new composition (therm_perf_gas.d);
+// a Newton failure sends the cell to the integrator as well.
+template <int NSP>
+__global__ __launch_bounds__(256) void k_surr_finish(int n, const double* __restrict__ rho, const double* T,
+                                                     const double* u, const double* p_in, const double* massf_in,
+                                                     const double* dtc_in,
+                                                     const double* __restrict__ massf_new, const ChemDev* __restrict__ cd,
+                                                     double* T_out, double* u_out, double* p_out, double* massf_out,
+                                                     double* dtc_out, int* status, int* __restrict__ path,
+                                                     int* __restrict__ fb_idx, unsigned* __restrict__ fb_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int pth = path[i];
+    if (pth == 1) return;
+    double mf[NSP];
+    double Tn = T[i], un = u[i];
+    const double dc = dtc_in[i];
+    if (pth == 2) {
+        // frozen: the state is passed through (outputs may be the input arrays)
+        const double pv = p_in[i];
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) mf[s] = massf_in[(size_t)s * n + i];
+        T_out[i] = Tn;
+        u_out[i] = un;
+        p_out[i] = pv;
+        dtc_out[i] = dc;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) massf_out[(size_t)s * n + i] = mf[s];
+        status[i] = 0;
+        return;
+    }
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) mf[s] = massf_new[(size_t)s * n + i];
+    if (!tpg_update_temperature<NSP>(*cd, mf, Tn, un)) {
+        path[i] = 1;
+        fb_idx[atomicAdd(fb_count, 1u)] = i;
+        return;
+    }
+    const double Rm = tpg_Rmix<NSP>(*cd, mf);
+    p_out[i] = rho[i] * Rm * Tn;
+    T_out[i] = Tn;
+    u_out[i] = un;
+    dtc_out[i] = dc;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) massf_out[(size_t)s * n + i] = mf[s];
+    status[i] = 0;
+}
+
+// Gather / scatter of the queued cells: compact arrays of m cells <-> the full arrays.
+template <int NSP>
+__global__ void k_surr_gather(int m, int n, const int* __restrict__ idx, const double* __restrict__ rho,
+                              const double* __restrict__ T, const double* __restrict__ u, const double* __restrict__ p,
+                              const double* __restrict__ massf, const double* __restrict__ dtc, double* crho, double* cT,
+                              double* cu, double* cp, double* cmassf, double* cdtc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int i = idx[k];
+    crho[k] = rho[i];
+    cT[k] = T[i];
+    cu[k] = u[i];
+    cp[k] = p[i];
+    cdtc[k] = dtc[i];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) cmassf[(size_t)s * m + k] = massf[(size_t)s * n + i];
+}
+
+template <int NSP>
+__global__ void k_surr_scatter(int m, int n, const int* __restrict__ idx, const double* cT, const double* cu,
+                               const double* cp, const double* cmassf, const double* cdtc, const int* cst,
+                               double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
+                               int* status) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int i = idx[k];
+    T_out[i] = cT[k];
+    u_out[i] = cu[k];
+    p_out[i] = cp[k];
+    dtc_out[i] = cdtc[k];
+    status[i] = cst[k];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) massf_out[(size_t)s * n + i] = cmassf[(size_t)s * m + k];
+}
+
+__global__ void k_surr_clear(unsigned* c) {
+    if (threadIdx.x == 0) c[0] = 0u;
+}
+
+static inline unsigned g256(int n) { return (unsigned)((n + 255) / 256); }
+
+#define GDTK_SURR_NSP(F)  \
+    switch (nsp) {        \
+    case 1: F(1)          \
+    case 2: F(2)          \
+    case 3: F(3)          \
+    case 4: F(4)          \
+    case 5: F(5)          \
+    case 6: F(6)          \
+    case 7: F(7)          \
+    case 8: F(8)          \
+    default: return hipErrorInvalidValue; \
+    }
+
+hipError_t launch_surr_clear(unsigned* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_surr_clear, dim3(1), dim3(64), 0, s, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_surr_features(int nsp, int n, const double* rho, const double* T, const double* massf, double dt,
+                                const SurrCfg& cfg, float* x, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define F(N)                                                                                                    \
+    hipLaunchKernelGGL((k_surr_features<N>), dim3(g256(n)), dim3(256), 0, s, n, rho, T, massf, dt, cfg, x); \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_apply(int nsp, int n, const double* T, const float* x, const float* ya, const float* yb,
+                             const double* massf, const SurrCfg& cfg, double* massf_new, int* path, int* fb_idx,
+                             unsigned* fb_count, double* disagree, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define F(N)                                                                                                            \
+    hipLaunchKernelGGL((k_surr_apply<N>), dim3(g256(n)), dim3(256), 0, s, n, T, x, ya, yb, massf, cfg, massf_new, path, \
+                       fb_idx, fb_count, disagree);                                                                     \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_finish(int nsp, int n, const double* rho, const double* T, const double* u, const double* p,
+                              const double* massf, const double* dtc, const double* massf_new, const ChemDev* cd,
+                              double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
+                              int* status, int* path, int* fb_idx, unsigned* fb_count, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define F(N)                                                                                                           \
+    hipLaunchKernelGGL((k_surr_finish<N>), dim3(g256(n)), dim3(256), 0, s, n, rho, T, u, p, massf, dtc, massf_new, cd, \
+                       T_out, u_out, p_out, massf_out, dtc_out, status, path, fb_idx, fb_count);                       \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_gather(int nsp, int m, int n, const int* idx, const double* rho, const double* T,
+                              const double* u, const double* p, const double* massf, const double* dtc, double* crho,
+                              double* cT, double* cu, double* cp, double* cmassf, double* cdtc, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+#define F(N)                                                                                                        \
+    hipLaunchKernelGGL((k_surr_gather<N>), dim3(g256(m)), dim3(256), 0, s, m, n, idx, rho, T, u, p, massf, dtc, crho, \
+                       cT, cu, cp, cmassf, cdtc);                                                                   \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_scatter(int nsp, int m, int n, const int* idx, const double* cT, const double* cu,
+                               const double* cp, const double* cmassf, const double* cdtc, const int* cst,
+                               double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
+                               int* status, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+#define F(N)                                                                                                         \
+    hipLaunchKernelGGL((k_surr_scatter<N>), dim3(g256(m)), dim3(256), 0, s, m, n, idx, cT, cu, cp, cmassf, cdtc, cst, \
+                       T_out, u_out, p_out, massf_out, dtc_out, status);                                             \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+}  // namespace gdtk

@@ -242,6 +242,70 @@ int gdtk_hip_chem_autotune(int chem, int n, const double* rho, const double* T, 
 int gdtk_hip_tpg_update(int chem, int mode, int n, double* rho, double* p, double* T, double* u, double* a,
                         const double* massf);
 
+/* ---------------------------------------------------------------------------------
+ * Learned chemistry surrogate (no counterpart in the reference: the building blocks of a
+ * drop-in for gdtk_hip_chem_step_device).  A small MLP maps standardised features of a cell
+ * (log T, log rho, Y[nsp], log dt) to its mass-fraction change over dt; it is trained on
+ * this library's own alpha-QSS results with the FP32 kernels below, and evaluated in BF16
+ * on the matrix cores with FP32 accumulation. */
+typedef struct gdtk_mlp_desc {
+    int n_in;            /* 1..32 */
+    int hidden;          /* 256 (3 hidden layers) or 64 (2 hidden layers) */
+    int n_hidden_layers;
+    int n_out;           /* 1..16 */
+} gdtk_mlp_desc;
+
+/* He-normal initial weights from seed, zero biases. */
+int gdtk_hip_mlp_create(const gdtk_mlp_desc* desc, unsigned long long seed, int* net);
+int gdtk_hip_mlp_destroy(int net);
+/* Parameter count (>= 0) or a negative error.  Layout: per layer W [out][in] row-major, then b [out]. */
+int gdtk_hip_mlp_nparams(int net);
+int gdtk_hip_mlp_get_params(int net, float* params, int n);
+int gdtk_hip_mlp_set_params(int net, const float* params, int n);
+/* FP32 forward pass on host arrays x [batch][n_in] -> y [batch][n_out] (cell-major). */
+int gdtk_hip_mlp_forward(int net, int batch, const float* x, float* y);
+/* Mean-squared-error loss of a host batch and its gradient (grad [nparams], may be NULL). */
+int gdtk_hip_mlp_loss_grad(int net, int batch, const float* x, const float* y, float* loss, float* grad);
+/* Adam training on device arrays x [n_in][n], y [n_out][n] (species-major, FP32): shuffled
+ * minibatches, `epochs` passes; loss_hist[epoch] (may be NULL) = mean batch loss.  Returns the
+ * number of epochs run. */
+int gdtk_hip_mlp_train(int net, int n, const float* x, const float* y, int batch, int epochs, float lr,
+                       unsigned long long seed, float* loss_hist, int nhist);
+/* BF16 matrix-core inference on device arrays x [n_in][n] -> y [n_out][n] on `stream`. */
+int gdtk_hip_mlp_infer(int net, int n, const float* x, float* y, void* stream);
+/* The matrix-core lane layout used by the inference kernel, on one 16x32 by 32x16 product
+ * (host arrays, row-major; values exact in BF16): D = A B. */
+int gdtk_hip_mfma_probe(const float* A, const float* B, float* D);
+
+#define GDTK_SURR_MAX_FEAT 11
+#define GDTK_SURR_MAX_ELEM 4
+typedef struct gdtk_surrogate_cfg {
+    double x_mean[GDTK_SURR_MAX_FEAT];     /* features: log T, log rho, Y[0..nsp-1], log dt */
+    double x_inv_std[GDTK_SURR_MAX_FEAT];
+    double y_scale[GDTK_MAX_SPECIES];      /* mass-fraction change = network output * y_scale */
+    double x_abs_max;                      /* a standardised feature beyond this queues the cell */
+    double disagree_tol;                   /* max |dY_a - dY_b| beyond this queues the cell */
+    double elem[GDTK_SURR_MAX_ELEM][GDTK_MAX_SPECIES]; /* element mass per unit species mass */
+    int n_elem;
+} gdtk_surrogate_cfg;
+
+/* Two networks (nsp + 3 inputs, nsp outputs) and the chemistry handle that serves the
+ * queued cells. */
+int gdtk_hip_surrogate_create(int chem, int net_a, int net_b, const gdtk_surrogate_cfg* cfg, int* surrogate);
+int gdtk_hip_surrogate_destroy(int surrogate);
+/* Same arguments and per-cell results as gdtk_hip_chem_step_device, plus path [n] (may be
+ * NULL): 0 network, 1 alpha-QSS integrator, 2 frozen.  Network cells: mean of the two
+ * networks' changes, projected so that every element's mass is conserved and no mass
+ * fraction is negative, then T by Newton from the unchanged u and p = rho R T; dt_chem is
+ * passed through.  Returns the number of cells sent to the integrator. */
+int gdtk_hip_surrogate_step_device(int surrogate, int n, const double* rho, const double* T, const double* u,
+                                   const double* p, const double* massf, const double* dt_chem, double dt,
+                                   double* T_out, double* u_out, double* p_out, double* massf_out,
+                                   double* dt_chem_out, int* status, int* path, void* stream);
+/* {cells, cells sent to the integrator, ms features + networks, ms projection + T(u), ms
+ * integrator, max network disagreement} of the last step. */
+int gdtk_hip_surrogate_stats(int surrogate, double* stats, int nstats);
+
 /* Message of the last failing call on this thread. */
 const char* gdtk_hip_last_error(void);
 

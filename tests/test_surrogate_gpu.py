@@ -1,0 +1,168 @@
+"""GPU checks of the learned chemistry surrogate against the oracle (oracle/mlp_oracle.py)
+and the alpha-QSS integrator.
+
+- the matrix-core lane layout of the inference kernel (exact small-integer products);
+- FP32 forward pass and gradients of the training kernels against the float64 oracle;
+- BF16 inference against the oracle evaluated with the same BF16 roundings (inputs,
+  weights and every activation rounded to BF16, FP32-or-better accumulation);
+- training reduces the loss;
+- the surrogate step: every network answer conserves each element's mass and keeps the
+  mass fractions non-negative, frozen cells pass through, and cells handed to the
+  integrator are bit-identical to the integrator's own results.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16(a):
+    """Round to the nearest BF16 (ties to even), returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_mfma_probe_layout():
+    from gdtk_amd.surrogate import mfma_probe
+    rng = np.random.default_rng(0)
+    A = rng.integers(-4, 5, (16, 32)).astype(np.float64)
+    B = rng.integers(-4, 5, (32, 16)).astype(np.float64)
+    D = mfma_probe(A, B)
+    assert np.array_equal(D.astype(np.float64), A @ B)
+
+
+@pytest.mark.parametrize("hidden,n_hidden", [(64, 2), (256, 3)])
+def test_forward_and_gradient_match_oracle(hidden, n_hidden):
+    from gdtk_amd.surrogate import Mlp
+    net = Mlp(8, 5, hidden=hidden, n_hidden=n_hidden, seed=3)
+    lay, npar = MO.layout(8, hidden, n_hidden, 5)
+    assert net.nparams == npar
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(77, 8))
+    Y = rng.normal(size=(77, 5))
+    y_ref, _, _ = MO.forward(p, lay, X)
+    y_gpu = net.forward(X)
+    assert np.max(np.abs(y_gpu - y_ref)) <= 1e-5 * max(1.0, np.max(np.abs(y_ref)))
+    loss_ref, g_ref = MO.loss_grad(p, lay, X, Y)
+    loss, g = net.loss_grad(X, Y)
+    assert abs(loss - loss_ref) <= 1e-5 * loss_ref
+    for (i, o, w, b) in lay:
+        for lo, hi in ((w, w + o * i), (b, b + o)):
+            scale = np.max(np.abs(g_ref[lo:hi]))
+            assert np.max(np.abs(g[lo:hi] - g_ref[lo:hi])) <= 1e-4 * scale, (lo, hi)
+
+
+def _bf16_forward(p, lay, X):
+    """The inference kernel's arithmetic: BF16 operands, exact products, wide sums."""
+    Ws = MO.unpack(p, lay)
+    h = _bf16(X)
+    for l, (W, b) in enumerate(Ws):
+        z = h @ _bf16(W).T + b.astype(np.float32).astype(np.float64)
+        if l < len(Ws) - 1:
+            h = _bf16(MO.gelu(z))
+        else:
+            h = z
+    return h
+
+
+def test_bf16_inference_matches_rounded_oracle():
+    import torch
+    from gdtk_amd.surrogate import Mlp
+    net = Mlp(8, 5, seed=11)
+    lay, _ = MO.layout(8, 256, 3, 5)
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(2)
+    n = 1000  # not a multiple of 64: exercises the partial last wave
+    X = rng.normal(size=(n, 8)).astype(np.float32)
+    dev = torch.device("cuda:0")
+    xs = torch.tensor(np.ascontiguousarray(X.T), device=dev)
+    ys = torch.empty((5, n), dtype=torch.float32, device=dev)
+    net.infer(n, xs.data_ptr(), ys.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    y = ys.cpu().numpy().T.astype(np.float64)
+    y_bf = _bf16_forward(p, lay, X.astype(np.float64))
+    y_ref, _, _ = MO.forward(p, lay, X.astype(np.float64))
+    scale = np.max(np.abs(y_ref))
+    # same roundings: differences only from FP32 accumulation order and a rare BF16 tie
+    assert np.max(np.abs(y - y_bf)) <= 2e-3 * scale, np.max(np.abs(y - y_bf)) / scale
+    # and BF16 overall stays within a few per cent of the FP64 network
+    assert np.max(np.abs(y - y_ref)) <= 5e-2 * scale
+
+
+def test_training_reduces_loss():
+    import torch
+    from gdtk_amd.surrogate import Mlp
+    rng = np.random.default_rng(5)
+    n = 1 << 15
+    X = rng.normal(size=(8, n))
+    Y = np.vstack([np.sin(X[0]), X[1] * X[2], np.tanh(X[3] + X[4]), 0.5 * X[5] ** 2 - 0.5, X[6] - X[7]])
+    dev = torch.device("cuda:0")
+    xs = torch.tensor(X, dtype=torch.float32, device=dev)
+    ys = torch.tensor(Y, dtype=torch.float32, device=dev)
+    net = Mlp(8, 5, seed=1)
+    l0 = MO.loss_grad(net.get_params().astype(np.float64), MO.layout(8, 256, 3, 5)[0], X[:, :2048].T,
+                      Y[:, :2048].T)[0]
+    hist = net.train(n, xs.data_ptr(), ys.data_ptr(), batch=1024, epochs=4, lr=1e-3, seed=3)
+    print("loss", l0, hist, flush=True)
+    assert np.all(np.isfinite(hist))
+    assert hist[-1] < 0.1 * l0
+
+
+def test_surrogate_step_conservation_and_fallback():
+    """A quickly trained surrogate on the air workload: structural guarantees (not accuracy,
+    which the benchmark reports with the full training set)."""
+    import torch
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, gas_record, mech_record
+    from gdtk_amd.chemcases import air_cells
+    from gdtk_amd.surrogate import air_element_matrix, integrator_samples, train_air_surrogate
+    gd = gdata.load("tpg-air-5sp")
+    ch = Chemistry(gas_record(gd), mech_record(gdata.load("park-air-5sp-1T")))
+    # permissive thresholds: this checks the step's structure, so most cells should take
+    # the network path even though two epochs of training leave the networks apart
+    sur, info = train_air_surrogate(ch, gd, n_train=1 << 16, epochs=(2,), lrs=(2e-3,), batch=2048,
+                                    x_abs_max=50.0, disagree_tol=1.0)
+    n = 20000
+    c = air_cells(n, seed=99)
+    c["T"][:100] = np.linspace(200.0, 300.0, 100)  # frozen cells
+    dev = torch.device("cuda:0")
+    t_in, ref, ref_st = integrator_samples(ch, c, 1.0e-7, dev)
+    out = {k: torch.empty_like(t_in[k]) for k in ("T", "u", "p", "massf")}
+    dtc = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
+    dto = torch.empty_like(dtc)
+    st = torch.empty(n, dtype=torch.int32, device=dev)
+    path = torch.empty(n, dtype=torch.int32, device=dev)
+    m = sur.step_device(n, t_in["rho"].data_ptr(), t_in["T"].data_ptr(), t_in["u"].data_ptr(), t_in["p"].data_ptr(),
+                        t_in["massf"].data_ptr(), dtc.data_ptr(), 1.0e-7, out["T"].data_ptr(), out["u"].data_ptr(),
+                        out["p"].data_ptr(), out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), path.data_ptr(),
+                        torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    P = path.cpu().numpy()
+    s = sur.stats()
+    print("surrogate", s, "queued", m, flush=True)
+    assert m == s["fallback"] == int(np.count_nonzero(P == 1))
+    Y0 = c["massf"].reshape(5, n)
+    Y = out["massf"].cpu().numpy().reshape(5, n)
+    E = air_element_matrix(gd["species"], gd["M"])
+    net_cells = P == 0
+    assert net_cells.sum() > 0.5 * n
+    dY = Y[:, net_cells] - Y0[:, net_cells]
+    assert np.max(np.abs(E @ dY)) < 1e-14
+    assert np.min(Y[:, net_cells]) >= 0.0
+    assert np.max(np.abs(Y[:, net_cells].sum(axis=0) - 1.0)) < 1e-13
+    # frozen cells pass through
+    assert np.all(P[:100] == 2)
+    for k in ("T", "u", "p"):
+        assert np.array_equal(out[k].cpu().numpy()[:100], c[k][:100])
+    # integrator cells equal the integrator's own results bit for bit
+    fb = P == 1
+    refY = ref["massf"].cpu().numpy().reshape(5, n)
+    assert np.array_equal(Y[:, fb], refY[:, fb])
+    assert np.array_equal(out["T"].cpu().numpy()[fb], ref["T"].cpu().numpy()[fb])
+    assert np.array_equal(st.cpu().numpy()[fb], ref_st.cpu().numpy()[fb])
+    err = np.max(np.abs(Y[:, net_cells] - refY[:, net_cells]))
+    print("max |dY| error of network cells (two epochs of training)", err, flush=True)
