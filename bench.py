@@ -283,7 +283,8 @@ def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, log=No
     dY = (out["massf"] - ref["massf"]).abs().reshape(5, ncells).max(dim=0).values.cpu().numpy()
     Tref = ref["T"].cpu().numpy()
     Terr = np.abs(out["T"].cpu().numpy() - Tref) / Tref
-    ok = ref_st.cpu().numpy() == 0
+    rs, ss = ref_st.cpu().numpy(), st.cpu().numpy()
+    ok = (rs == 0) & (ss == 0)  # accuracy over cells both complete
     achieved = 2 * MLP_FLOPS_PER_CELL * ncells / (s["ms_network"] * 1e-3) / 1e12
     return {
         "metric": "chem cells/s (surrogate + alpha-QSS fallback)",
@@ -295,7 +296,10 @@ def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, log=No
         "accuracy": {"max_abs_dY": float(dY[ok].max()), "max_rel_T_err": float(Terr[ok].max()),
                      "p99_abs_dY": float(np.percentile(dY[ok], 99)),
                      "fallback_fraction": float(np.count_nonzero(P == 1)) / ncells,
-                     "network_cells": int(np.count_nonzero(P == 0)), "frozen_cells": int(np.count_nonzero(P == 2))},
+                     "network_cells": int(np.count_nonzero(P == 0)), "frozen_cells": int(np.count_nonzero(P == 2)),
+                     "failed_cells": int(np.count_nonzero(ss != 0)), "integrator_failed_cells": int(np.count_nonzero(rs != 0)),
+                     "false_failures": int(np.count_nonzero((rs == 0) & (ss != 0))),
+                     "false_successes": int(np.count_nonzero((rs != 0) & (ss == 0)))},
         "training": {"samples": info["samples"], "sample_s": info["sample_s"], "train_s": info["train_s"],
                      "final_loss": [float(h[-1]) for h in info["loss"]]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -1,8 +1,11 @@
 // mlp_kernels.hip -- the chemistry surrogate network on gfx950: FP32 training kernels and a
 // BF16 MFMA inference kernel (no counterpart in the reference).
 //
-// Network: x[n_in] -> (W_l x + b_l, GELU) for the hidden layers of width H -> W_out h + b_out.
-// GELU(z) = z/2 (1 + erf(z/sqrt 2)).  Activations are cell-major [batch][width] in training.
+// Network: x[n_in] -> (W_l x + b_l, GELU) for the hidden layers of width H -> W_out h + b_out,
+// with GELU in its tanh form (Hendrycks & Gimpel 2016):
+//   GELU(z) = z/2 (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) = z / (1 + exp(-2 u)),
+//   u = sqrt(2/pi) (z + 0.044715 z^3).
+// Activations are cell-major [batch][width] in training.
 //
 // Inference keeps a wave's 16 cells in registers through every layer.  Each layer is
 // computed transposed, Y^T = W X^T, with v_mfma_f32_16x16x32_bf16:
@@ -30,10 +33,21 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+constexpr float GELU_K = 0.7978845608028654f;   // sqrt(2/pi)
+constexpr float GELU_C = 0.044715f;
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + tanhf(GELU_K * (z + GELU_C * z * z * z))); }
 __device__ __forceinline__ float gelu_grad(float z) {
-    // d/dz [z/2 (1 + erf(z/sqrt2))] = (1 + erf(z/sqrt2))/2 + z exp(-z^2/2)/sqrt(2 pi)
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    // d/dz = (1 + t)/2 + z/2 (1 - t^2) sqrt(2/pi) (1 + 3 c z^2), t = tanh(u)
+    const float t = tanhf(GELU_K * (z + GELU_C * z * z * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * GELU_K * (1.0f + 3.0f * GELU_C * z * z);
+}
+// The same function for inference: z / (1 + 2^(z (A + B z^2))) with the hardware exp2 and
+// reciprocal (A = -2 log2(e) sqrt(2/pi), B = 0.044715 A); the result is rounded to BF16.
+__device__ __forceinline__ float gelu_fast(float z) {
+    constexpr float A = -2.0f * 1.4426950408889634f * 0.7978845608028654f;
+    constexpr float B = 0.044715f * A;
+    const float w = z * (A + B * (z * z));
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(w));
 }
 
 // FP32 -> BF16, round to nearest even (finite inputs).
@@ -96,68 +110,130 @@ hipError_t launch_mlp_pack(const float* W, int n_out, int n_in, int chain, unsig
 // Inference: x SoA [n_in][n] FP32 -> y SoA [n_out][n] FP32.  H hidden width (multiple of
 // 32), NH hidden layers, n_in <= 32, n_out <= 16.  frag: packed layers in order; bias:
 // FP32 biases in order.
+//
+// A workgroup of 4 waves serves 128 cells, each wave two 16-cell column blocks (CB = 2), so
+// every weight fragment read feeds two MFMAs.  The H x H layers stream their fragments
+// through LDS one k-step at a time (NRB fragments = 16 KiB for H = 256), double-buffered:
+// the global loads of k-step s+1 are in flight while k-step s is multiplied.  The small
+// first and last layers read their fragments straight from global memory (L2).
+constexpr int INF_WAVES = 4, INF_CB = 2;
+constexpr int INF_CELLS = INF_WAVES * INF_CB * 16;
+
 template <int H, int NH>
-__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, int n_in, int n,
-                                                   const unsigned short* __restrict__ frag,
-                                                   const float* __restrict__ bias, int n_out, float* __restrict__ y) {
+__global__ __launch_bounds__(256, 1) void k_mlp_infer(const float* __restrict__ x, int n_in, int n,
+                                                      const unsigned short* __restrict__ frag,
+                                                      const float* __restrict__ bias, int n_out, float* __restrict__ y) {
     constexpr int NRB = H / 16, NKS = H / 32;
+    constexpr int FRAG_PER_STEP = NRB * 64;                 // 16-byte fragments per k-step
+    constexpr int LOADS = FRAG_PER_STEP / 256;               // per thread
+    static_assert(FRAG_PER_STEP % 256 == 0, "k-step fragments must split over the workgroup");
+    __shared__ bf16x8 lds[2][FRAG_PER_STEP];
+    __shared__ float bias_s[NH * H + 16];
+    for (int k = threadIdx.x; k < NH * H + n_out; k += 256) bias_s[k] = bias[k];
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int cell = wave * 16 + r;
-    const bool live = cell < n;
+    const int cell0 = blockIdx.x * INF_CELLS + wave * (INF_CB * 16);
     const bf16x8* F = reinterpret_cast<const bf16x8*>(frag);
 
-    // Layer 1 B fragment: inputs 8 g + j of this lane's cell.
-    u16x8 xb;
+    bf16x8 act[INF_CB][NKS];
+    f32x4 acc[INF_CB][NRB];
+    // first layer: one k-step over the zero-padded inputs (slots k = 8 g + j)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int k = 8 * g + j;
-        xb[j] = (live && k < n_in) ? f2bf(x[(size_t)k * n + cell]) : (unsigned short)0;
+    for (int cb = 0; cb < INF_CB; ++cb) {
+        const int cell = cell0 + cb * 16 + r;
+        u16x8 xb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 8 * g + j;
+            xb[j] = (cell < n && k < n_in) ? f2bf(x[(size_t)k * n + cell]) : (unsigned short)0;
+        }
+        act[cb][0] = __builtin_bit_cast(bf16x8, xb);
     }
-    bf16x8 act[NKS];
-    f32x4 acc[NRB];
-    // first hidden layer (one k-step over the padded inputs)
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb) {
-        f32x4 z = {0.f, 0.f, 0.f, 0.f};
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(F[rb * 64 + lane], __builtin_bit_cast(bf16x8, xb), z, 0, 0, 0);
+        const bf16x8 a = F[rb * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < INF_CB; ++cb)
+            acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, act[cb][0], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
     const bf16x8* Fl = F + NRB * 64;
-    const float* bl = bias;
+    __syncthreads();  // biases in LDS
+    const float* bl = bias_s;
     for (int layer = 0; layer < NH; ++layer) {
         // bias + GELU + BF16 -> B fragments of the next layer
 #pragma unroll
         for (int s = 0; s < NKS; ++s) {
-            u16x8 v;
+            const float4 b0 = *reinterpret_cast<const float4*>(bl + 32 * s + 4 * g);
+            const float4 b1 = *reinterpret_cast<const float4*>(bl + 32 * s + 16 + 4 * g);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                v[i] = f2bf(gelu(acc[2 * s][i] + bl[32 * s + 4 * g + i]));
-                v[4 + i] = f2bf(gelu(acc[2 * s + 1][i] + bl[32 * s + 16 + 4 * g + i]));
+            for (int cb = 0; cb < INF_CB; ++cb) {
+                bf16x8 v;
+                v[0] = (__bf16)gelu_fast(acc[cb][2 * s][0] + b0.x);
+                v[1] = (__bf16)gelu_fast(acc[cb][2 * s][1] + b0.y);
+                v[2] = (__bf16)gelu_fast(acc[cb][2 * s][2] + b0.z);
+                v[3] = (__bf16)gelu_fast(acc[cb][2 * s][3] + b0.w);
+                v[4] = (__bf16)gelu_fast(acc[cb][2 * s + 1][0] + b1.x);
+                v[5] = (__bf16)gelu_fast(acc[cb][2 * s + 1][1] + b1.y);
+                v[6] = (__bf16)gelu_fast(acc[cb][2 * s + 1][2] + b1.z);
+                v[7] = (__bf16)gelu_fast(acc[cb][2 * s + 1][3] + b1.w);
+                act[cb][s] = v;
             }
-            act[s] = __builtin_bit_cast(bf16x8, v);
         }
         bl += H;
         if (layer == NH - 1) break;
 #pragma unroll
-        for (int rb = 0; rb < NRB; ++rb) acc[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int cb = 0; cb < INF_CB; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < NRB; ++rb) acc[cb][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // k-step s of this layer: fragments Fl[(rb * NKS + s) * 64 + lane] for rb < NRB
+        bf16x8 pre[LOADS];
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) {
+            const int f = threadIdx.x + 256 * q;  // f = rb * 64 + lane'
+            pre[q] = Fl[((f >> 6) * NKS + 0) * 64 + (f & 63)];
+        }
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) lds[0][threadIdx.x + 256 * q] = pre[q];
+        __syncthreads();
 #pragma unroll
         for (int s = 0; s < NKS; ++s) {
+            const int buf = s & 1;
+            if (s + 1 < NKS) {
 #pragma unroll
-            for (int rb = 0; rb < NRB; ++rb)
-                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fl[(rb * NKS + s) * 64 + lane], act[s], acc[rb], 0, 0, 0);
+                for (int q = 0; q < LOADS; ++q) {
+                    const int f = threadIdx.x + 256 * q;
+                    pre[q] = Fl[((f >> 6) * NKS + (s + 1)) * 64 + (f & 63)];
+                }
+            }
+#pragma unroll
+            for (int rb = 0; rb < NRB; ++rb) {
+                const bf16x8 a = lds[buf][rb * 64 + lane];
+#pragma unroll
+                for (int cb = 0; cb < INF_CB; ++cb)
+                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, act[cb][s], acc[cb][rb], 0, 0, 0);
+            }
+            if (s + 1 < NKS) {
+#pragma unroll
+                for (int q = 0; q < LOADS; ++q) lds[buf ^ 1][threadIdx.x + 256 * q] = pre[q];
+            }
+            __syncthreads();
         }
         Fl += NRB * NKS * 64;
     }
-    // output layer: one 16-row block
-    f32x4 o = {0.f, 0.f, 0.f, 0.f};
+    // output layer: one 16-row block, fragments from global
 #pragma unroll
-    for (int s = 0; s < NKS; ++s) o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fl[s * 64 + lane], act[s], o, 0, 0, 0);
-    if (live) {
+    for (int cb = 0; cb < INF_CB; ++cb) {
+        f32x4 o = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 4 * g + i;
-            if (row < n_out) y[(size_t)row * n + cell] = o[i] + bl[row];
+        for (int s = 0; s < NKS; ++s) o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Fl[s * 64 + lane], act[cb][s], o, 0, 0, 0);
+        const int cell = cell0 + cb * 16 + r;
+        if (cell < n) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 4 * g + i;
+                if (row < n_out) y[(size_t)row * n + cell] = o[i] + bl[row];
+            }
         }
     }
 }
@@ -165,7 +241,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, 
 hipError_t launch_mlp_infer(int H, int NH, const float* x, int n_in, int n, const unsigned short* frag,
                             const float* bias, int n_out, float* y, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    const unsigned grid = (unsigned)((n + 63) / 64);  // 4 waves x 16 cells per workgroup
+    const unsigned grid = (unsigned)((n + INF_CELLS - 1) / INF_CELLS);
     if (H == 256 && NH == 3) {
         hipLaunchKernelGGL((k_mlp_infer<256, 3>), dim3(grid), dim3(256), 0, s, x, n_in, n, frag, bias, n_out, y);
     } else if (H == 64 && NH == 2) {

@@ -125,7 +125,7 @@ __device__ __forceinline__ bool project(const SurrCfg& cfg, const double* Y0, co
 }
 
 // path: 0 surrogate, 1 queued for the integrator, 2 frozen (T <= T_frozen: left unchanged,
-// fvcell.d:870).
+// fvcell.d:870), 3 failed (set by k_surr_finish).
 template <int NSP>
 __global__ __launch_bounds__(256) void k_surr_apply(int n, const double* __restrict__ T, const float* __restrict__ x,
                                                     const float* __restrict__ ya, const float* __restrict__ yb,
@@ -164,8 +164,11 @@ __global__ __launch_bounds__(256) void k_surr_apply(int n, const double* __restr
     if (fallback) fb_idx[atomicAdd(fb_count, 1u)] = i;
 }
 
-// T, u, p of the surrogate cells by Newton T(u) at the new composition (therm_perf_gas.d);
-// a Newton failure sends the cell to the integrator as well.
+// T, u, p of the surrogate cells by Newton T(u) at the new composition (therm_perf_gas.d).
+// A composition that admits no temperature is the reference's failure mode of
+// thermochemical_increment (update_thermo_from_rhou throws after the chemistry, fvcell.d:
+// 906-925): the cell is reported failed (path 3, status -1, state unchanged, dt_chem -1)
+// without running the integrator on it.
 template <int NSP>
 __global__ __launch_bounds__(256) void k_surr_finish(int n, const double* __restrict__ rho, const double* T,
                                                      const double* u, const double* p_in, const double* massf_in,
@@ -198,8 +201,18 @@ __global__ __launch_bounds__(256) void k_surr_finish(int n, const double* __rest
 #pragma unroll
     for (int s = 0; s < NSP; ++s) mf[s] = massf_new[(size_t)s * n + i];
     if (!tpg_update_temperature<NSP>(*cd, mf, Tn, un)) {
-        path[i] = 1;
-        fb_idx[atomicAdd(fb_count, 1u)] = i;
+        path[i] = 3;
+        const double pv = p_in[i];
+        T_out[i] = T[i];
+        u_out[i] = u[i];
+        p_out[i] = pv;
+        dtc_out[i] = -1.0;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) {
+            const double m0 = massf_in[(size_t)s * n + i];
+            massf_out[(size_t)s * n + i] = m0;
+        }
+        status[i] = -1;
         return;
     }
     const double Rm = tpg_Rmix<NSP>(*cd, mf);

@@ -165,7 +165,8 @@ class Surrogate:
 
     def step_device(self, n, rho, T, u, p, massf, dt_chem, dt, T_out, u_out, p_out, massf_out, dt_chem_out, status,
                     path=None, stream=None):
-        """Device pointers (ints); returns the number of cells sent to the integrator."""
+        """Device pointers (ints); returns the number of cells sent to the integrator.
+        path (optional int32 array): 0 network, 1 integrator, 2 frozen, 3 failed."""
         return _check(_lib().gdtk_hip_surrogate_step_device(self.h, n, rho, T, u, p, massf, dt_chem, dt, T_out, u_out,
                                                             p_out, massf_out, dt_chem_out, status, path, stream))
 

@@ -294,10 +294,12 @@ typedef struct gdtk_surrogate_cfg {
 int gdtk_hip_surrogate_create(int chem, int net_a, int net_b, const gdtk_surrogate_cfg* cfg, int* surrogate);
 int gdtk_hip_surrogate_destroy(int surrogate);
 /* Same arguments and per-cell results as gdtk_hip_chem_step_device, plus path [n] (may be
- * NULL): 0 network, 1 alpha-QSS integrator, 2 frozen.  Network cells: mean of the two
- * networks' changes, projected so that every element's mass is conserved and no mass
+ * NULL): 0 network, 1 alpha-QSS integrator, 2 frozen, 3 failed.  Network cells: mean of the
+ * two networks' changes, projected so that every element's mass is conserved and no mass
  * fraction is negative, then T by Newton from the unchanged u and p = rho R T; dt_chem is
- * passed through.  Returns the number of cells sent to the integrator. */
+ * passed through.  A network composition that admits no temperature is reported failed
+ * (status -1, state unchanged, dt_chem -1), the reference's failure mode for such cells.
+ * Returns the number of cells sent to the integrator. */
 int gdtk_hip_surrogate_step_device(int surrogate, int n, const double* rho, const double* T, const double* u,
                                    const double* p, const double* massf, const double* dt_chem, double dt,
                                    double* T_out, double* u_out, double* p_out, double* massf_out,

@@ -3,8 +3,8 @@
 TEST INFRASTRUCTURE ONLY (the oracle): imported by tests/ and never by the product path.
 The surrogate has no counterpart in the reference; this module states what the GPU
 kernels compute so that they can be checked:
-  - forward: h = GELU(W_l h + b_l) over the hidden layers, y = W_out h + b_out, with
-    GELU(z) = z/2 (1 + erf(z/sqrt 2));
+  - forward: h = GELU(W_l h + b_l) over the hidden layers, y = W_out h + b_out, with GELU
+    in its tanh form, GELU(z) = z/2 (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)));
   - mean-squared-error loss over the batch and outputs, and its gradient by the chain rule;
   - the element-conservation projection with an active set (every element's mass kept,
     no negative mass fraction).
@@ -16,9 +16,6 @@ from __future__ import annotations
 import math
 
 import numpy as np
-
-_erf = np.vectorize(math.erf, otypes=[np.float64])
-
 
 def layout(n_in, hidden, n_hidden, n_out):
     """[(in, out, w_offset, b_offset)] per layer and the parameter count, in the library's
@@ -37,12 +34,17 @@ def unpack(params, lay):
     return [(p[w:w + o * i].reshape(o, i), p[b:b + o]) for (i, o, w, b) in lay]
 
 
+_K = math.sqrt(2.0 / math.pi)
+_C = 0.044715
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + _erf(z / math.sqrt(2.0)))
+    return 0.5 * z * (1.0 + np.tanh(_K * (z + _C * z ** 3)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + _erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+    t = np.tanh(_K * (z + _C * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * _K * (1.0 + 3.0 * _C * z * z)
 
 
 def forward(params, lay, X):
