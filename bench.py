@@ -242,20 +242,22 @@ MLP_FLOPS_PER_CELL = 2 * (8 * 256 + 256 * 256 + 256 * 256 + 256 * 5)  # 268 800 
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, log=None):
+def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, tols=(2.0e-4, 5.0e-4, 2.0e-3), log=None):
     """The learned step on the same cells as the alpha-QSS measurement: two networks trained
-    (set-up, untimed) on n_train integrator samples of independently seeded cells, then K
-    timed steps from the same resident inputs, compared cell by cell with the integrator."""
+    (set-up, untimed) on n_train integrator samples of independently seeded cells, then, for
+    each disagreement tolerance, K timed steps from the same resident inputs compared cell
+    by cell with the integrator.  The first tolerance whose max |dY| error is <= 1e-3 is the
+    reported operating point (the largest tolerance if none is)."""
     import numpy as np
     import torch
 
     from gdtk_amd import data as gdata
     from gdtk_amd.chem import Chemistry, gas_record, mech_record
     from gdtk_amd.chemcases import air_cells
-    from gdtk_amd.surrogate import integrator_samples, train_air_surrogate
+    from gdtk_amd.surrogate import integrator_samples, train_air_surrogate, with_tolerance
     gd = gdata.load("tpg-air-5sp")
     ch = Chemistry(gas_record(gd), mech_record(gdata.load("park-air-5sp-1T")))
-    sur, info = train_air_surrogate(ch, gd, n_train=n_train, dt=dt, seed=77, log=log)
+    sur0, info = train_air_surrogate(ch, gd, n_train=n_train, dt=dt, seed=77, log=log)
     dev = torch.device("cuda:0")
     cells = air_cells(ncells, seed=seed)
     t_in, ref, ref_st = integrator_samples(ch, cells, dt, dev)
@@ -269,44 +271,60 @@ def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, log=No
             t_in["massf"].data_ptr(), dtc.data_ptr(), dt, out["T"].data_ptr(), out["u"].data_ptr(),
             out["p"].data_ptr(), out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), path.data_ptr(),
             stream.cuda_stream)
-    sur.step_device(ncells, *args)  # warm-up (packs the BF16 weights)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(steps):
-        sur.step_device(ncells, *args)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    s = sur.stats()
-    P = path.cpu().numpy()
-    dY = (out["massf"] - ref["massf"]).abs().reshape(5, ncells).max(dim=0).values.cpu().numpy()
     Tref = ref["T"].cpu().numpy()
-    Terr = np.abs(out["T"].cpu().numpy() - Tref) / Tref
-    rs, ss = ref_st.cpu().numpy(), st.cpu().numpy()
-    ok = (rs == 0) & (ss == 0)  # accuracy over cells both complete
-    achieved = 2 * MLP_FLOPS_PER_CELL * ncells / (s["ms_network"] * 1e-3) / 1e12
+    rs = ref_st.cpu().numpy()
+    sweep = []
+    for tol in tols:
+        sur = with_tolerance(sur0, tol)
+        sur.step_device(ncells, *args)  # warm-up (packs the BF16 weights)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            sur.step_device(ncells, *args)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        s = sur.stats()
+        P = path.cpu().numpy()
+        ss = st.cpu().numpy()
+        dY = (out["massf"] - ref["massf"]).abs().reshape(5, ncells).max(dim=0).values.cpu().numpy()
+        Terr = np.abs(out["T"].cpu().numpy() - Tref) / Tref
+        ok = (rs == 0) & (ss == 0)  # accuracy over cells both complete
+        okT = ok & (Tref > 300.0)   # T error where the integrator's T is physical
+        achieved = 2 * MLP_FLOPS_PER_CELL * ncells / (s["ms_network"] * 1e-3) / 1e12
+        sweep.append({
+            "disagree_tol": tol, "value": ncells / (ms * 1e-3), "ms_per_step": ms,
+            "max_abs_dY": float(dY[ok].max()), "p999_abs_dY": float(np.percentile(dY[ok], 99.9)),
+            "p99_abs_dY": float(np.percentile(dY[ok], 99)), "max_rel_T_err": float(Terr[okT].max()),
+            "p999_rel_T_err": float(np.percentile(Terr[okT], 99.9)),
+            "fallback_fraction": float(np.count_nonzero(P == 1)) / ncells,
+            "network_cells": int(np.count_nonzero(P == 0)), "failed_cells": int(np.count_nonzero(ss != 0)),
+            "integrator_failed_cells": int(np.count_nonzero(rs != 0)),
+            "false_failures": int(np.count_nonzero((rs == 0) & (ss != 0))),
+            "false_successes": int(np.count_nonzero((rs != 0) & (ss == 0))),
+            "kernels": {k: s[k] for k in ("ms_network", "ms_project", "ms_integrator")},
+            "mfma_tflops": achieved})
+        if log:
+            log(f"surrogate tol {tol:g}: {sweep[-1]['value']:.3e} cells/s, fallback "
+                f"{sweep[-1]['fallback_fraction']:.3f}, max dY {sweep[-1]['max_abs_dY']:.2e}")
+        sur.close()
+    pick = next((r for r in sweep if r["max_abs_dY"] <= 1.0e-3), sweep[-1])
     return {
         "metric": "chem cells/s (surrogate + alpha-QSS fallback)",
-        "value": ncells / (ms * 1e-3),
+        "value": pick["value"],
         "unit": "cells/s",
-        "ms_per_step": ms,
+        "ms_per_step": pick["ms_per_step"],
         "steps": steps,
-        "dtype": "bf16 network, f64 projection/T(u)/fallback",
-        "accuracy": {"max_abs_dY": float(dY[ok].max()), "max_rel_T_err": float(Terr[ok].max()),
-                     "p99_abs_dY": float(np.percentile(dY[ok], 99)),
-                     "fallback_fraction": float(np.count_nonzero(P == 1)) / ncells,
-                     "network_cells": int(np.count_nonzero(P == 0)), "frozen_cells": int(np.count_nonzero(P == 2)),
-                     "failed_cells": int(np.count_nonzero(ss != 0)), "integrator_failed_cells": int(np.count_nonzero(rs != 0)),
-                     "false_failures": int(np.count_nonzero((rs == 0) & (ss != 0))),
-                     "false_successes": int(np.count_nonzero((rs != 0) & (ss == 0)))},
+        "dtype": "bf16 network (fp32 accumulate), f64 projection/T(u)/fallback",
+        "operating_point": pick,
+        "sweep": sweep,
         "training": {"samples": info["samples"], "sample_s": info["sample_s"], "train_s": info["train_s"],
                      "final_loss": [float(h[-1]) for h in info["loss"]]},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_mlp_infer<256,3> x2 + features", "avg_launch_ms": s["ms_network"],
+        "roofline": {"bound": "mfma", "achieved": pick["mfma_tflops"], "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": pick["mfma_tflops"] / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_mlp_infer<256,3> x2 + features", "avg_launch_ms": pick["kernels"]["ms_network"],
                      "flops_per_cell": 2 * MLP_FLOPS_PER_CELL},
-        "kernels": {k: s[k] for k in ("ms_network", "ms_project", "ms_integrator")},
     }
 
 

@@ -112,10 +112,20 @@ int gdtk_hip_mlp_set_params(int h, const float* in, int n) {
         HIPCHK(hipStreamSynchronize(m.stream));
         HIPCHK(hipMemcpy(m.params.p, in, m.nparam * sizeof(float), hipMemcpyHostToDevice));
         m.packed = false;
+        m.wq_fresh = false;
         m.adam_t = 0;
         HIPCHK(hipMemset(m.am.p, 0, m.nparam * sizeof(float)));
         HIPCHK(hipMemset(m.av.p, 0, m.nparam * sizeof(float)));
         HIPCHK(hipDeviceSynchronize());
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_mlp_set_bf16_training(int h, int on) {
+    return guarded([&] {
+        Mlp& m = get_mlp(h);
+        m.bf16_train = on ? 1 : 0;
+        m.wq_fresh = false;
         return GDTK_OK;
     });
 }

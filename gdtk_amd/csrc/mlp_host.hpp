@@ -18,6 +18,12 @@ struct Mlp {
     size_t nparam = 0;
     DevArr<float> params, grads, am, av;
     int adam_t = 0;
+    // BF16-aware training: the forward pass uses BF16-rounded inputs, weights (wq, refreshed
+    // after every update) and hidden activations, as the inference kernel does; gradients
+    // pass the roundings unchanged and update the FP32 masters.
+    int bf16_train = 0;
+    DevArr<float> wq;
+    bool wq_fresh = false;
     // BF16 inference fragments and the concatenated biases
     DevArr<unsigned short> frag;
     DevArr<float> bias_cat;
@@ -25,7 +31,7 @@ struct Mlp {
     // training batch buffers
     int bcap = 0;
     std::vector<DevArr<float>> Z, A;
-    DevArr<float> xb, yb, dA, dZ, part, xh, yh;
+    DevArr<float> xb, yb, dA, dZ, part, xh, yh, gparts, cparts;
     DevArr<int> idx;
     hipStream_t stream = nullptr;
 
@@ -68,31 +74,58 @@ struct Mlp {
         dA.alloc((size_t)B * wmax);
         dZ.alloc((size_t)B * wmax);
         part.alloc((size_t)mse_parts(B, n_out));
+        size_t gp = 0;
+        for (int l = 0; l < nlayer; ++l)
+            gp = std::max(gp, (size_t)gemm_split(out_w[l], in_w[l], B) * out_w[l] * in_w[l]);
+        gparts.alloc(gp);
+        cparts.alloc((size_t)colsum_parts(B) * wmax);
+    }
+
+    const float* weights(hipStream_t s) {
+        if (!bf16_train) return params.p;
+        if (!wq_fresh) {
+            wq.reserve(nparam);
+            HIPCHK(launch_round_bf16(nparam, params.p, wq.p, s));
+            wq_fresh = true;
+        }
+        return wq.p;
     }
 
     // Forward pass of a cell-major batch X [B][n_in]; output A[last] = Z[last].
     void forward(int B, const float* X, hipStream_t s) {
         const float* in = X;
+        if (bf16_train) {
+            HIPCHK(launch_round_bf16((size_t)B * n_in, X, xq_buf(B), s));
+            in = xq.p;
+        }
+        const float* Wt = weights(s);
         for (int l = 0; l < nlayer; ++l) {
             const int ni = in_w[l], no = out_w[l];
-            // Z[b][o] = sum_k in[b][k] W[o][k]
-            HIPCHK(launch_gemm_f32(B, no, ni, in, ni, 1, params.p + woff[l], 1, ni, Z[l].p, no, 0, s));
-            HIPCHK(launch_bias_act(B, no, params.p + boff[l], Z[l].p, A[l].p, l < nlayer - 1 ? 1 : 0, s));
+            // Z[b][o] = sum_k in[b][k] W[o][k]; the biases stay FP32 (as in inference)
+            HIPCHK(launch_gemm_f32(B, no, ni, in, ni, 1, Wt + woff[l], 1, ni, Z[l].p, no, 0, s));
+            HIPCHK(launch_bias_act(B, no, params.p + boff[l], Z[l].p, A[l].p, l < nlayer - 1 ? 1 : 0, bf16_train, s));
             in = A[l].p;
         }
+    }
+    DevArr<float> xq;
+    float* xq_buf(int B) {
+        xq.reserve((size_t)B * n_in);
+        return xq.p;
     }
 
     // Backward pass after forward(B, X) and the loss gradient in dZ (output layer).
     void backward(int B, const float* X, hipStream_t s) {
+        const float* Wt = weights(s);
         for (int l = nlayer - 1; l >= 0; --l) {
             const int ni = in_w[l], no = out_w[l];
-            const float* in = (l == 0) ? X : A[l - 1].p;
-            // dW[o][k] = sum_b dZ[b][o] in[b][k]
-            HIPCHK(launch_gemm_f32(no, ni, B, dZ.p, 1, no, in, ni, 1, grads.p + woff[l], ni, 0, s));
-            HIPCHK(launch_colsum(B, no, dZ.p, grads.p + boff[l], s));
+            const float* in = (l == 0) ? (bf16_train ? xq.p : X) : A[l - 1].p;
+            // dW[o][k] = sum_b dZ[b][o] in[b][k]  (split over the batch)
+            HIPCHK(launch_gemm_f32_split(no, ni, B, dZ.p, 1, no, in, ni, 1, grads.p + woff[l], ni, gparts.p,
+                                         gemm_split(no, ni, B), s));
+            HIPCHK(launch_colsum(B, no, dZ.p, grads.p + boff[l], cparts.p, s));
             if (l == 0) break;
             // dA[b][k] = sum_o dZ[b][o] W[o][k]
-            HIPCHK(launch_gemm_f32(B, ni, no, dZ.p, no, 1, params.p + woff[l], ni, 1, dA.p, ni, 0, s));
+            HIPCHK(launch_gemm_f32(B, ni, no, dZ.p, no, 1, Wt + woff[l], ni, 1, dA.p, ni, 0, s));
             HIPCHK(launch_gelu_back((size_t)B * ni, Z[l - 1].p, dA.p, dZ.p, s));
         }
     }
@@ -118,6 +151,7 @@ struct Mlp {
         ++adam_t;
         HIPCHK(launch_adam(nparam, params.p, grads.p, am.p, av.p, lr, 0.9f, 0.999f, 1.0e-8f, adam_t, s));
         packed = false;
+        wq_fresh = false;
     }
 
     void pack(hipStream_t s) {

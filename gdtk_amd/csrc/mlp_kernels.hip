@@ -258,25 +258,31 @@ hipError_t launch_mlp_infer(int H, int NH, const float* x, int n_in, int n, cons
 // 64x64 tiles, 256 threads with 4x4 outputs each, k tiles of 16 through LDS.
 constexpr int GT = 64, GK = 16;
 
+// Split K: workgroup z covers k in [z kc, (z+1) kc) and writes its partial product to
+// C + z * M * ldc (summed afterwards in z order by k_sum_parts, so results do not depend on
+// scheduling).
 __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, const float* __restrict__ A, int sam, int sak,
                                                   const float* __restrict__ B, int sbk, int sbn, float* __restrict__ C,
-                                                  int ldc, int accumulate) {
+                                                  int ldc, int accumulate, int kc) {
     __shared__ float As[GK][GT + 1];
     __shared__ float Bs[GK][GT + 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int kbeg = blockIdx.z * kc;
+    const int kend = (kbeg + kc < K) ? kbeg + kc : K;
+    C += (size_t)blockIdx.z * M * ldc;
     float c[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) c[a][b] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += GK) {
+    for (int k0 = kbeg; k0 < kend; k0 += GK) {
         for (int e = threadIdx.x; e < GK * GT; e += 256) {
             const int kk = e / GT, mm = e % GT;
             const int m = m0 + mm, k = k0 + kk;
-            As[kk][mm] = (m < M && k < K) ? A[(size_t)m * sam + (size_t)k * sak] : 0.0f;
+            As[kk][mm] = (m < M && k < kend) ? A[(size_t)m * sam + (size_t)k * sak] : 0.0f;
             const int nn = mm, nidx = n0 + nn;
-            Bs[kk][nn] = (nidx < N && k < K) ? B[(size_t)k * sbk + (size_t)nidx * sbn] : 0.0f;
+            Bs[kk][nn] = (nidx < N && k < kend) ? B[(size_t)k * sbk + (size_t)nidx * sbn] : 0.0f;
         }
         __syncthreads();
 #pragma unroll
@@ -308,27 +314,75 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, const flo
     }
 }
 
+// C[m][n] = sum_z P[z][m][n] (row-major, leading dimension ldc), z ascending.
+__global__ void k_sum_parts(int M, int N, int ldc, int nz, const float* __restrict__ P, float* __restrict__ C) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)M * N) return;
+    const size_t m = t / N, nn = t % N;
+    float acc = 0.0f;
+    for (int z = 0; z < nz; ++z) acc += P[(size_t)z * M * ldc + m * ldc + nn];
+    C[m * ldc + nn] = acc;
+}
+
 hipError_t launch_gemm_f32(int M, int N, int K, const float* A, int sam, int sak, const float* B, int sbk, int sbn,
                            float* C, int ldc, int accumulate, hipStream_t s) {
     if (M <= 0 || N <= 0) return hipSuccess;
-    dim3 grid((unsigned)((N + GT - 1) / GT), (unsigned)((M + GT - 1) / GT));
-    hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate);
+    dim3 grid((unsigned)((N + GT - 1) / GT), (unsigned)((M + GT - 1) / GT), 1);
+    hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, accumulate, K);
     return hipGetLastError();
 }
 
-// Z[b][j] += bias[j]; A = GELU(Z) (act != 0) or A = Z.
+int gemm_split(int M, int N, int K) {
+    const int tiles = ((N + GT - 1) / GT) * ((M + GT - 1) / GT);
+    int z = 256 / (tiles > 0 ? tiles : 1);
+    z = z < 1 ? 1 : (z > 64 ? 64 : z);
+    while (z > 1 && K / z < 4 * GK) z >>= 1;
+    return z;
+}
+
+hipError_t launch_gemm_f32_split(int M, int N, int K, const float* A, int sam, int sak, const float* B, int sbk,
+                                 int sbn, float* C, int ldc, float* parts, int nz, hipStream_t s) {
+    if (M <= 0 || N <= 0) return hipSuccess;
+    if (nz <= 1) return launch_gemm_f32(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, 0, s);
+    const int kc = ((K + nz - 1) / nz + GK - 1) / GK * GK;
+    dim3 grid((unsigned)((N + GT - 1) / GT), (unsigned)((M + GT - 1) / GT), (unsigned)nz);
+    hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, parts, ldc, 0, kc);
+    const size_t mn = (size_t)M * N;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, M, N, ldc, nz, parts, C);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ float round_bf16(float f) { return __uint_as_float((unsigned)f2bf(f) << 16); }
+
+// Z[b][j] += bias[j]; A = GELU(Z) (act != 0) or A = Z.  With bf16 != 0 the GELU output is
+// rounded to BF16, as the inference kernel rounds it (BF16-aware training; the gradient
+// passes the rounding unchanged).
 __global__ void k_bias_act(int rows, int cols, const float* __restrict__ bias, float* __restrict__ Z,
-                           float* __restrict__ Aout, int act) {
+                           float* __restrict__ Aout, int act, int bf16) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)rows * cols) return;
     const float z = Z[t] + bias[t % cols];
     Z[t] = z;
-    Aout[t] = act ? gelu(z) : z;
+    const float a = act ? gelu(z) : z;
+    Aout[t] = (act && bf16) ? round_bf16(a) : a;
 }
 
-hipError_t launch_bias_act(int rows, int cols, const float* bias, float* Z, float* A, int act, hipStream_t s) {
+hipError_t launch_bias_act(int rows, int cols, const float* bias, float* Z, float* A, int act, int bf16,
+                           hipStream_t s) {
     const size_t n = (size_t)rows * cols;
-    hipLaunchKernelGGL(k_bias_act, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, cols, bias, Z, A, act);
+    hipLaunchKernelGGL(k_bias_act, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, cols, bias, Z, A, act,
+                       bf16);
+    return hipGetLastError();
+}
+
+// out[i] = BF16(in[i]) (as FP32 values)
+__global__ void k_round_bf16(size_t n, const float* __restrict__ in, float* __restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = round_bf16(in[t]);
+}
+
+hipError_t launch_round_bf16(size_t n, const float* in, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_round_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, in, out);
     return hipGetLastError();
 }
 
@@ -372,17 +426,26 @@ hipError_t launch_mse(int rows, int cols, const float* Yp, const float* Yt, floa
     return hipGetLastError();
 }
 
-// db[j] = sum_b dZ[b][j]
-__global__ void k_colsum(int rows, int cols, const float* __restrict__ dZ, float* __restrict__ db) {
+// db[j] = sum_b dZ[b][j]: chunks of COLSUM_ROWS rows summed per workgroup into part[chunk][j],
+// then the chunks summed in order.
+constexpr int COLSUM_ROWS = 128;
+__global__ void k_colsum_part(int rows, int cols, const float* __restrict__ dZ, float* __restrict__ part) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cols) return;
-    float s = 0.0f;
-    for (int b = 0; b < rows; ++b) s += dZ[(size_t)b * cols + j];
-    db[j] = s;
+    const int b0 = blockIdx.y * COLSUM_ROWS;
+    const int b1 = (b0 + COLSUM_ROWS < rows) ? b0 + COLSUM_ROWS : rows;
+    float acc = 0.0f;
+    for (int b = b0; b < b1; ++b) acc += dZ[(size_t)b * cols + j];
+    part[(size_t)blockIdx.y * cols + j] = acc;
 }
 
-hipError_t launch_colsum(int rows, int cols, const float* dZ, float* db, hipStream_t s) {
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, s, rows, cols, dZ, db);
+int colsum_parts(int rows) { return (rows + COLSUM_ROWS - 1) / COLSUM_ROWS; }
+
+hipError_t launch_colsum(int rows, int cols, const float* dZ, float* db, float* part, hipStream_t s) {
+    const int nch = colsum_parts(rows);
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((cols + 63) / 64), (unsigned)nch), dim3(64), 0, s, rows, cols, dZ,
+                       part);
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s, 1, cols, cols, nch, part, db);
     return hipGetLastError();
 }
 

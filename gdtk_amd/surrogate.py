@@ -47,6 +47,7 @@ def _lib():
         L.gdtk_hip_mlp_get_params.argtypes = [C.c_int, _fp, C.c_int]
         L.gdtk_hip_mlp_set_params.argtypes = [C.c_int, _fp, C.c_int]
         L.gdtk_hip_mlp_forward.argtypes = [C.c_int, C.c_int, _fp, _fp]
+        L.gdtk_hip_mlp_set_bf16_training.argtypes = [C.c_int, C.c_int]
         L.gdtk_hip_mlp_loss_grad.argtypes = [C.c_int, C.c_int, _fp, _fp, _fp, _fp]
         L.gdtk_hip_mlp_train.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
                                          C.c_ulonglong, _fp, C.c_int]
@@ -117,6 +118,9 @@ class Mlp:
         _check(_lib().gdtk_hip_mlp_loss_grad(self.h, X.shape[0], X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp),
                                              C.byref(loss), g.ctypes.data_as(_fp)))
         return loss.value, g
+
+    def set_bf16_training(self, on=True):
+        _check(_lib().gdtk_hip_mlp_set_bf16_training(self.h, 1 if on else 0))
 
     def train(self, n, x_ptr, y_ptr, batch=4096, epochs=1, lr=1e-3, seed=0):
         """Adam on device arrays x [n_in][n], y [n_out][n] (FP32 pointers); loss per epoch."""
@@ -195,8 +199,8 @@ def integrator_samples(chem, cells, dt, device):
     return t, out, st
 
 
-def train_air_surrogate(chem, gas_json, n_train=1 << 21, dt=1.0e-7, seed=7, epochs=(6, 4, 3), lrs=(2e-3, 5e-4, 1e-4),
-                        batch=4096, x_abs_max=6.0, disagree_tol=2.0e-4, log=None):
+def train_air_surrogate(chem, gas_json, n_train=1 << 22, dt=1.0e-7, seed=7, epochs=(8, 6, 4), lrs=(2e-3, 5e-4, 1e-4),
+                        batch=4096, x_abs_max=6.0, disagree_tol=5.0e-4, bf16_aware=False, log=None):
     """Samples the integrator on n_train cells of the air workload, trains two networks
     (different initialisations and shuffles) and returns (Surrogate, info)."""
     import torch
@@ -223,6 +227,7 @@ def train_air_surrogate(chem, gas_json, n_train=1 << 21, dt=1.0e-7, seed=7, epoc
     t1 = time.perf_counter()
     for k, s in enumerate((seed * 101 + 1, seed * 101 + 2)):
         net = Mlp(nsp + 3, nsp, seed=s)
+        net.set_bf16_training(bf16_aware)
         h = []
         for ep, lr in zip(epochs, lrs):
             h += list(net.train(m, X.data_ptr(), Yt.data_ptr(), batch=batch, epochs=ep, lr=lr, seed=s + len(h)))
@@ -249,3 +254,13 @@ def train_air_surrogate(chem, gas_json, n_train=1 << 21, dt=1.0e-7, seed=7, epoc
     info = dict(samples=m, sample_s=t_samp, train_s=t_train, loss=hist, x_mean=x_mean.cpu().numpy(),
                 x_std=x_std.cpu().numpy(), y_scale=y_scale.cpu().numpy())
     return sur, info
+
+
+def with_tolerance(sur, disagree_tol, x_abs_max=None):
+    """A second surrogate over the same networks and chemistry with other thresholds."""
+    cfg = SurrogateCfg()
+    C.memmove(C.byref(cfg), C.byref(sur.cfg), C.sizeof(SurrogateCfg))
+    cfg.disagree_tol = disagree_tol
+    if x_abs_max is not None:
+        cfg.x_abs_max = x_abs_max
+    return Surrogate(sur.chem, sur.net_a, sur.net_b, cfg)

@@ -262,6 +262,10 @@ int gdtk_hip_mlp_destroy(int net);
 int gdtk_hip_mlp_nparams(int net);
 int gdtk_hip_mlp_get_params(int net, float* params, int n);
 int gdtk_hip_mlp_set_params(int net, const float* params, int n);
+/* BF16-aware training (on != 0): the training forward pass rounds inputs, weights and hidden
+ * activations to BF16 as the inference kernel does; gradients pass the roundings unchanged
+ * and update the FP32 parameters.  Off by default. */
+int gdtk_hip_mlp_set_bf16_training(int net, int on);
 /* FP32 forward pass on host arrays x [batch][n_in] -> y [batch][n_out] (cell-major). */
 int gdtk_hip_mlp_forward(int net, int batch, const float* x, float* y);
 /* Mean-squared-error loss of a host batch and its gradient (grad [nparams], may be NULL). */

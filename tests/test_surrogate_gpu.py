@@ -93,6 +93,31 @@ def test_bf16_inference_matches_rounded_oracle():
     assert np.max(np.abs(y - y_ref)) <= 5e-2 * scale
 
 
+def test_bf16_aware_forward_matches_inference():
+    """With BF16-aware training on, the training forward pass computes the inference
+    kernel's function (BF16 operands and activations) up to FP32 accumulation order."""
+    import torch
+    from gdtk_amd.surrogate import Mlp
+    net = Mlp(8, 5, seed=13)
+    net.set_bf16_training(True)
+    lay, _ = MO.layout(8, 256, 3, 5)
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(3)
+    n = 256
+    X = rng.normal(size=(n, 8)).astype(np.float32)
+    y_train = net.forward(X).astype(np.float64)
+    dev = torch.device("cuda:0")
+    xs = torch.tensor(np.ascontiguousarray(X.T), device=dev)
+    ys = torch.empty((5, n), dtype=torch.float32, device=dev)
+    net.infer(n, xs.data_ptr(), ys.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    y_inf = ys.cpu().numpy().T.astype(np.float64)
+    y_bf = _bf16_forward(p, lay, X.astype(np.float64))
+    scale = np.max(np.abs(y_bf))
+    assert np.max(np.abs(y_train - y_bf)) <= 2e-3 * scale
+    assert np.max(np.abs(y_inf - y_train)) <= 2e-3 * scale
+
+
 def test_training_reduces_loss():
     import torch
     from gdtk_amd.surrogate import Mlp
