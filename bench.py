@@ -328,6 +328,56 @@ def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, tols=(
     }
 
 
+def bench_batched(nblk, steps, warmup, flux="ausmdv", log=None):
+    """Batched uneven blocks: nblk[0] x nblk[1] x nblk[2] blocks with edges drawn from 16..192
+    cells (seeded), joined by exchange boundaries, each with its own Mach 2-10 free stream;
+    every stage is one ghost-cell launch and one persistent stage launch over all blocks,
+    each step one captured graph.  Returns a sub-record with its own throughput and the
+    stage kernel's roofline."""
+    from gdtk_amd import Simulation, cases
+    t0 = time.perf_counter()
+    blocks = cases.batched_uneven(nblk=nblk)
+    ncells = sum(b.ncells for b in blocks)
+    sim = Simulation(3, flux=flux)
+    for q, b in enumerate(blocks):
+        sim.add_block(b)
+        if log and q % 16 == 15:
+            log(f"batched: {q + 1} blocks set up, {time.perf_counter() - t0:.1f} s")
+    if log:
+        log(f"batched: {len(blocks)} blocks, {ncells} cells, set-up {time.perf_counter() - t0:.1f} s")
+    tune = sim.autotune(dt=0.25 * min(sim.block_dt(q, 1.0, 0.5, False)[0] for q in range(len(blocks))), reps=3)
+    if log:
+        log(f"batched autotune {tune}")
+    if warmup:
+        sim.run(warmup)
+    t0 = time.perf_counter()
+    _, st = sim.run(steps)
+    el = time.perf_counter() - t0
+    sim.profile(True)
+    sim.run(min(5, steps))
+    ks = sim.kernel_stats()
+    sim.profile(False)
+    dom = max(("face_flux", "fused_stage"), key=lambda k: ks[k]["ms"])
+    kd = ks[dom]
+    avg_ms = kd["ms"] / max(kd["launches"], 1)
+    bpu = stage_bytes_per_cell(3) if dom == "fused_stage" else face_bytes_per_face(3)
+    achieved = bpu * kd["items_per_launch"] / (avg_ms * 1e-3) / 1e9
+    sim.close()
+    return {
+        "value": ncells * steps / el, "unit": "cell-updates/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
+        "config": {"workload": f"batched uneven blocks {nblk[0]}x{nblk[1]}x{nblk[2]}, edges 16-192, exchange, "
+                               f"Mach 2-10 per block, {flux}, pc RK2",
+                   "blocks": len(blocks), "cells": ncells, "stage_kernels": tune["best"], "kchunk": tune["kchunk"],
+                   "graph": True, "final_dt": st["dt"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": f"k_fused_batch<3,{1 if flux == 'efm' else 0},*>"
+                     if dom == "fused_stage" else dom, "avg_launch_ms": avg_ms,
+                     "items_per_launch": kd["items_per_launch"]},
+        "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1)} for k, v in ks.items()
+                    if v["launches"]},
+    }
+
+
 def copy_probe(nbytes=1 << 30):
     """A plain device copy of known size for HBM counter calibration (FETCH_SIZE/WRITE_SIZE)."""
     import torch
@@ -355,6 +405,10 @@ def main():
     ap.add_argument("--chem-only", action="store_true", help="measure only the chemistry workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
     ap.add_argument("--surrogate-train", type=int, default=1 << 22, help="integrator samples to train on")
+    ap.add_argument("--batched", default="4,4,4",
+                    help="blocks per direction of the batched uneven-blocks workload ('0' skips it)")
+    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--batched-only", action="store_true", help="measure only the batched uneven-blocks workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -376,6 +430,10 @@ def main():
 
     if args.chem_only:
         print(json.dumps(chem_legs(), default=float), flush=True)
+        return
+    nblk = tuple(int(x) for x in args.batched.split(","))
+    if args.batched_only:
+        print(json.dumps(bench_batched(nblk, args.batched_steps, 2, args.flux, log), default=float), flush=True)
         return
     nic, njc, nkc = (int(x) for x in args.size.split(","))
     from gdtk_amd import Simulation, cases
@@ -407,7 +465,7 @@ def main():
     if dom == "face_flux":
         bpu, unit_name, kname = face_bytes_per_face(3), "face", "k_face_flux<3,%d>" % (1 if args.flux == "efm" else 0)
     else:
-        bpu, unit_name, kname = stage_bytes_per_cell(3), "cell", "k_fused_stage<3,%d,*>" % (1 if args.flux == "efm" else 0)
+        bpu, unit_name, kname = stage_bytes_per_cell(3), "cell", "k_fused_batch<3,%d,*>" % (1 if args.flux == "efm" else 0)
     achieved = bpu * kd["items_per_launch"] / (avg_ms * 1e-3) / 1e9
     traffic = measured_traffic(kname, kd["items_per_launch"])
     out = {
@@ -438,6 +496,9 @@ def main():
             out["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": str(e)}
+    sim.close()
+    if nblk != (0,):
+        out["batched"] = bench_batched(nblk, args.batched_steps, 2, args.flux, log)
     if args.chem_cells > 0:
         out["chem"] = chem_legs()
     print(json.dumps(out, default=float), flush=True)

@@ -43,6 +43,7 @@ class BlockCase:
     inflow: np.ndarray = field(default_factory=lambda: np.zeros((6, 5)))  # [p,T,vx,vy,vz] per face
     other_block: list = field(default_factory=lambda: [-1] * 6)
     other_face: list = field(default_factory=lambda: [-1] * 6)
+    other_orientation: list = field(default_factory=lambda: [0] * 6)
 
     @property
     def ncells(self) -> int:
@@ -176,5 +177,103 @@ def perturb_vertices(case: BlockCase, amp=0.15, seed=7) -> BlockCase:
         dz = (v[-1, 0, 0, 2] - v[0, 0, 0, 2]) / case.nkc
         v[..., 2] += amp * dz * bump * np.sin(2.0 * np.pi * (r + s) + ph[4]) * np.cos(2.0 * np.pi * t + ph[5])
     out = BlockCase(case.dims, case.nic, case.njc, case.nkc, v.reshape(-1, 3), case.p, case.T, case.vel,
-                    list(case.bc), case.inflow.copy(), list(case.other_block), list(case.other_face))
+                    list(case.bc), case.inflow.copy(), list(case.other_block), list(case.other_face),
+                    list(case.other_orientation))
+    return out
+
+
+def split_block(case: BlockCase, icuts, jcuts, kcuts=(0, 1)) -> list:
+    """Cut a block into a grid of blocks joined by full-face-copy exchange boundaries.
+
+    icuts/jcuts/kcuts are cell-index boundaries, e.g. (0, 5, 12) cuts 12 cells in i into
+    5 + 7.  Blocks are numbered (bk*nbj + bj)*nbi + bi; outer faces keep the original
+    boundary conditions and inflow states, inner faces exchange with their neighbours
+    (orientation 0: index axes aligned).
+    """
+    icuts, jcuts, kcuts = list(icuts), list(jcuts), list(kcuts)
+    if case.dims == 2:
+        kcuts = [0, 1]
+    assert icuts[0] == 0 and icuts[-1] == case.nic and jcuts[0] == 0 and jcuts[-1] == case.njc
+    assert kcuts[0] == 0 and kcuts[-1] == case.nkc
+    nbi, nbj, nbk = len(icuts) - 1, len(jcuts) - 1, len(kcuts) - 1
+    niv, njv = case.nic + 1, case.njc + 1
+    nkv = case.nkc + 1 if case.dims == 3 else 1
+    V = case.vertices.reshape(nkv, njv, niv, 3)
+    cell = lambda a: a.reshape(case.nkc, case.njc, case.nic, *a.shape[1:])
+    P, Tt, Vel = cell(case.p), cell(case.T), cell(case.vel)
+    bid = lambda bi, bj, bk: (bk * nbj + bj) * nbi + bi
+    out = []
+    for bk in range(nbk):
+        for bj in range(nbj):
+            for bi in range(nbi):
+                i0, i1 = icuts[bi], icuts[bi + 1]
+                j0, j1 = jcuts[bj], jcuts[bj + 1]
+                k0, k1 = kcuts[bk], kcuts[bk + 1]
+                kv = slice(k0, k1 + 1) if case.dims == 3 else slice(0, 1)
+                v = V[kv, j0:j1 + 1, i0:i1 + 1].reshape(-1, 3).copy()
+                sl = (slice(k0, k1), slice(j0, j1), slice(i0, i1))
+                b = BlockCase(case.dims, i1 - i0, j1 - j0, k1 - k0, v, P[sl].reshape(-1).copy(),
+                              Tt[sl].reshape(-1).copy(), Vel[sl].reshape(-1, 3).copy(), list(case.bc),
+                              case.inflow.copy())
+                nbr = {WEST: (bi > 0, bid(bi - 1, bj, bk)), EAST: (bi < nbi - 1, bid(bi + 1, bj, bk)),
+                       SOUTH: (bj > 0, bid(bi, bj - 1, bk)), NORTH: (bj < nbj - 1, bid(bi, bj + 1, bk)),
+                       BOTTOM: (bk > 0, bid(bi, bj, bk - 1)), TOP: (bk < nbk - 1, bid(bi, bj, bk + 1))}
+                for side, (inner, other) in nbr.items():
+                    if inner:
+                        b.bc[side] = BC_EXCHANGE
+                        b.other_block[side] = other
+                        b.other_face[side] = side ^ 1
+                        b.other_orientation[side] = 0
+                out.append(b)
+    return out
+
+
+def batched_uneven(nblk=(4, 4, 4), lo=16, hi=192, seed=2024, p0=2.0e3, T0=220.0, mach=(2.0, 10.0)) -> list:
+    """Batched uneven blocks: a grid of nblk[0] x nblk[1] x nblk[2] structured blocks whose
+    edge lengths per index direction are drawn uniformly from lo..hi cells (seeded; one draw
+    per block column, row and layer, so that neighbouring faces match), on unit cells.
+
+    Each block starts from its own ideal-air free stream, p0 and T0 with a Mach number drawn
+    uniformly from `mach`, flowing along +x.  The west faces of the first block column are
+    supersonic inflows of their block's free stream, the other outer faces simple outflows
+    (flux), and every inner face a full-face-copy exchange with the neighbour block.
+    """
+    rng = np.random.default_rng(seed)
+    ni = rng.integers(lo, hi + 1, nblk[0])
+    nj = rng.integers(lo, hi + 1, nblk[1])
+    nk = rng.integers(lo, hi + 1, nblk[2])
+    x0 = np.concatenate([[0], np.cumsum(ni)])
+    y0 = np.concatenate([[0], np.cumsum(nj)])
+    z0 = np.concatenate([[0], np.cumsum(nk)])
+    machs = rng.uniform(mach[0], mach[1], nblk[0] * nblk[1] * nblk[2])
+    R = R_UNIVERSAL / IDEAL_AIR_MMASS
+    a0 = np.sqrt(IDEAL_AIR_GAMMA * R * T0)
+    nbi, nbj, nbk = nblk
+    bid = lambda bi, bj, bk: (bk * nbj + bj) * nbi + bi
+    out = []
+    for bk in range(nbk):
+        for bj in range(nbj):
+            for bi in range(nbi):
+                q = bid(bi, bj, bk)
+                nic, njc, nkc = int(ni[bi]), int(nj[bj]), int(nk[bk])
+                v = box_vertices(nic, njc, nkc, float(nic), float(njc), float(nkc), dims=3,
+                                 x0=float(x0[bi]), y0=float(y0[bj]), z0=float(z0[bk]))
+                n = nic * njc * nkc
+                u = machs[q] * a0
+                vel = np.zeros((n, 3))
+                vel[:, 0] = u
+                b = BlockCase(3, nic, njc, nkc, v, np.full(n, p0), np.full(n, T0), vel,
+                              [BC_OUTFLOW_SIMPLE_FLUX] * 6)
+                if bi == 0:
+                    b.bc[WEST] = BC_INFLOW_SUPERSONIC
+                    b.inflow[WEST] = [p0, T0, u, 0.0, 0.0]
+                nbr = {WEST: (bi > 0, bid(bi - 1, bj, bk)), EAST: (bi < nbi - 1, bid(bi + 1, bj, bk)),
+                       SOUTH: (bj > 0, bid(bi, bj - 1, bk)), NORTH: (bj < nbj - 1, bid(bi, bj + 1, bk)),
+                       BOTTOM: (bk > 0, bid(bi, bj, bk - 1)), TOP: (bk < nbk - 1, bid(bi, bj, bk + 1))}
+                for side, (inner, other) in nbr.items():
+                    if inner:
+                        b.bc[side] = BC_EXCHANGE
+                        b.other_block[side] = other
+                        b.other_face[side] = side ^ 1
+                out.append(b)
     return out

@@ -54,4 +54,64 @@ __host__ __device__ inline int64_t pidx(const BlockDev& b, int i, int j, int k) 
     return (int64_t)(i + NGHOST) + (int64_t)b.NI * ((int64_t)(j + NGHOST) + (int64_t)b.NJ * (int64_t)(k + (b.dims == 3 ? NGHOST : 0)));
 }
 
+// In-face coordinates of a block side: west/east (j, k), south/north (i, k), bottom/top
+// (i, j); extents A x B (B = 1 on the sides of a 2-D block).
+__host__ __device__ inline void side_extents(int side, int nic, int njc, int nkc, int& A, int& B) {
+    if (side <= 1) { A = njc; B = nkc; }
+    else if (side <= 3) { A = nic; B = nkc; }
+    else { A = nic; B = njc; }
+}
+
+// Source cell (active-cell indices of the source block) of ghost layer n at in-face
+// position (a, b) of a full-face-copy exchange, following the case tables of
+// set_up_cell_mapping_phase0 (src/eilmer/bc/ghost_cell_effect/full_face_copy.d:696-1480):
+// orientation 0 keeps (a, b), orientation 2 reverses both, orientations 1 and 3 are the
+// quarter turns (b, B'-1-a) and (A'-1-b, a) -- in that order when the destination side's
+// in-face coordinates and its outward normal are right-handed (east, south, top), in the
+// opposite order otherwise -- and the first source coordinate is reversed when both sides
+// have the same handedness.  (A', B') are the source side's extents; 2-D sides use only
+// the first coordinate.  Source layer n lies n cells inside the source side.
+__host__ __device__ inline void ffc_source(int dims, int dside, int sside, int orient, int a, int b, int n, int snic,
+                                           int snjc, int snkc, int& i, int& j, int& k) {
+    const bool dright = (dside == 1) || (dside == 2) || (dside == 5);
+    const bool sright = (sside == 1) || (sside == 2) || (sside == 5);
+    int A, B;
+    side_extents(sside, snic, snjc, snkc, A, B);
+    int p, q;
+    if (dims == 2 || orient == 0) { p = a; q = (dims == 2) ? 0 : b; }
+    else if (orient == 2) { p = A - 1 - a; q = B - 1 - b; }
+    else if ((orient == 1) == dright) { p = b; q = B - 1 - a; }
+    else { p = A - 1 - b; q = a; }
+    if (dright == sright) p = A - 1 - p;
+    switch (sside) {
+    case 0: i = n; j = p; k = q; break;
+    case 1: i = snic - 1 - n; j = p; k = q; break;
+    case 2: j = n; i = p; k = q; break;
+    case 3: j = snjc - 1 - n; i = p; k = q; break;
+    case 4: k = n; i = p; j = q; break;
+    default: k = snkc - 1 - n; i = p; j = q; break;
+    }
+}
+
+// One side of one block in the batched ghost-cell launch: its boundary kind, the
+// exchange source (block, side, orientation) and the side's first ghost cell in the
+// launch (ghost cells ordered (face position, layer) within a side).
+struct SideDev {
+    int blk, side, kind;
+    int oblk, oside, orient;
+    int na, nb;
+    int64_t start;
+};
+
+// Batched stage launch over every block: per-block device records for this stage (their
+// buffers already point at this stage's input and output copies), the prefix sums of the
+// blocks' tile counts, and the per-XCD tile queues (8 counters, TILE_QSTRIDE apart).
+constexpr int TILE_QSTRIDE = 32;
+struct StageBatch {
+    const BlockDev* tab;
+    const int* tile_start;
+    unsigned* queue;
+    int nblocks, total_tiles, per_xcd, KC;
+};
+
 }  // namespace gdtk

@@ -13,46 +13,33 @@
 
 namespace gdtk {
 
-// Ghost-cell effects, one thread per (boundary face, layer):
+// Padded index of ghost layer `layer` at in-face position (a, bb) of a block side.
+__device__ __forceinline__ int64_t ghost_pidx(const BlockDev& b, int side, int a, int bb, int layer) {
+    if (side == 0) return pidx(b, -1 - layer, a, bb);
+    if (side == 1) return pidx(b, b.nic + layer, a, bb);
+    if (side == 2) return pidx(b, a, -1 - layer, bb);
+    if (side == 3) return pidx(b, a, b.njc + layer, bb);
+    if (side == 4) return pidx(b, a, bb, -1 - layer);
+    return pidx(b, a, bb, b.nkc + layer);
+}
+
+// Ghost-cell effect of one ghost cell (layer `layer` at in-face position (a, bb) of side
+// `side`) for the boundary kinds that read the block's own cells:
 // internal_copy_then_reflect.d:111-134, flow_state_copy.d:92-107,
 // extrapolate_copy.d:124-140 (xOrder 0).
-__global__ void k_ghost_fill(BlockDev b, double* fs) {
-    const int nsides = (b.dims == 3) ? 6 : 4;
-    int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // Per side: number of faces.
-    int64_t nW = (int64_t)b.njc * b.nkc, nS = (int64_t)b.nic * b.nkc, nB = (int64_t)b.nic * b.njc;
-    int64_t counts[6] = {nW, nW, nS, nS, nB, nB};
-    int side = -1;
-    int64_t r = tid;
-    for (int s = 0; s < nsides; ++s) {
-        int64_t c = counts[s] * NGHOST;
-        if (r < c) { side = s; break; }
-        r -= c;
-    }
-    if (side < 0) return;
-    int layer = (int)(r % NGHOST);
-    int64_t face = r / NGHOST;
+__device__ __forceinline__ void ghost_one(const BlockDev& b, double* fs, int side, int a, int bb, int layer) {
     int i0, j0, k0;  // face location in the side
     int d;           // index direction normal to the side
     int fi, fj, fk;  // face index
     if (side <= 1) {
-        d = 0;
-        j0 = (int)(face % b.njc);
-        k0 = (int)(face / b.njc);
+        d = 0; j0 = a; k0 = bb; i0 = 0;
         fi = (side == 0) ? 0 : b.nic; fj = j0; fk = k0;
-        i0 = 0;
     } else if (side <= 3) {
-        d = 1;
-        i0 = (int)(face % b.nic);
-        k0 = (int)(face / b.nic);
+        d = 1; i0 = a; k0 = bb; j0 = 0;
         fi = i0; fj = (side == 2) ? 0 : b.njc; fk = k0;
-        j0 = 0;
     } else {
-        d = 2;
-        i0 = (int)(face % b.nic);
-        j0 = (int)(face / b.nic);
+        d = 2; i0 = a; j0 = bb; k0 = 0;
         fi = i0; fj = j0; fk = (side == 4) ? 0 : b.nkc;
-        k0 = 0;
     }
     const bool lo = (side % 2) == 0;
     const int nd = (d == 0) ? b.nic : (d == 1) ? b.njc : b.nkc;
@@ -86,6 +73,67 @@ __global__ void k_ghost_fill(BlockDev b, double* fs) {
         fs[g + FS_VX * np] = w.x;
         fs[g + FS_VY * np] = w.y;
         fs[g + FS_VZ * np] = w.z;
+    }
+}
+
+// Ghost-cell effects of one block, one thread per (boundary face, layer).
+__global__ void k_ghost_fill(BlockDev b, double* fs) {
+    const int nsides = (b.dims == 3) ? 6 : 4;
+    int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nW = (int64_t)b.njc * b.nkc, nS = (int64_t)b.nic * b.nkc, nB = (int64_t)b.nic * b.njc;
+    int64_t counts[6] = {nW, nW, nS, nS, nB, nB};
+    int side = -1;
+    int64_t r = tid;
+    for (int s = 0; s < nsides; ++s) {
+        int64_t c = counts[s] * NGHOST;
+        if (r < c) { side = s; break; }
+        r -= c;
+    }
+    if (side < 0) return;
+    int A, B;
+    side_extents(side, b.nic, b.njc, b.nkc, A, B);
+    const int layer = (int)(r % NGHOST);
+    const int64_t face = r / NGHOST;
+    ghost_one(b, fs, side, (int)(face % A), (int)(face / A), layer);
+}
+
+// Ghost cells of every side of every block in one launch: full-face-copy exchange from the
+// source block's cells (simcore_exchange.d:96-135 -> full_face_copy.d:1888-1900, the
+// same-process copy of the whole flow state) and the other ghost-cell effects.  Each block
+// record's fs_in is the flow-state copy this stage reads, so the ghost cells are written
+// there.  mode 1 copies the exchange sides' cell lengths instead (the geometry exchange
+// of full_face_copy.d:1576-1625, done once).
+__global__ void k_ghost_batch(const BlockDev* __restrict__ tab, const SideDev* __restrict__ sides, int nsides,
+                              int64_t total, int mode) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int lo = 0, hi = nsides - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sides[mid].start <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const SideDev sd = sides[lo];
+    const int64_t r = t - sd.start;
+    const int layer = (int)(r % NGHOST);
+    const int64_t face = r / NGHOST;
+    const int a = (int)(face % sd.na), bb = (int)(face / sd.na);
+    const BlockDev& b = tab[sd.blk];
+    if (sd.kind == BC_EXCHANGE) {
+        const BlockDev& o = tab[sd.oblk];
+        int i, j, k;
+        ffc_source(b.dims, sd.side, sd.oside, sd.orient, a, bb, layer, o.nic, o.njc, o.nkc, i, j, k);
+        const int64_t src = pidx(o, i, j, k);
+        const int64_t dst = ghost_pidx(b, sd.side, a, bb, layer);
+        if (mode == 0) {
+            double* fs = const_cast<double*>(b.fs_in);
+            for (int f = 0; f < FS_NBASE; ++f) fs[dst + f * b.npad] = o.fs_in[src + f * o.npad];
+        } else {
+            double* len = const_cast<double*>(b.len);
+            for (int f = 0; f < 3; ++f) len[dst + f * b.npad] = o.len[src + f * o.npad];
+        }
+    } else if (mode == 0) {
+        ghost_one(b, const_cast<double*>(b.fs_in), sd.side, a, bb, layer);
     }
 }
 
@@ -362,6 +410,13 @@ hipError_t launch_cell_update(const BlockDev& b, const IdealGasDev& gas, int sta
         if (stage == 1) hipLaunchKernelGGL((k_cell_update<2, 1>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
         else hipLaunchKernelGGL((k_cell_update<2, 2>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsides, int64_t total, int mode,
+                              hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ghost_batch, dim3(nblk(total, 256)), dim3(256), 0, s, tab, sides, nsides, total, mode);
     return hipGetLastError();
 }
 

@@ -13,8 +13,12 @@
 // its six faces in the reference order W, E, S, N, B, T (fvcell.d:835-845), so results
 // match the face-per-thread path bit for bit.  The top-face flux of plane k is the
 // bottom-face flux of plane k+1 (registers).
-// Workgroups are numbered so that each XCD receives a contiguous range of tiles, which
-// keeps the halo re-reads of neighbouring tiles in the same L2.
+// One persistent launch per stage covers every block of the simulation: the tiles of all
+// blocks form one list (block by block, i fastest), split into eight contiguous ranges, one
+// per XCD (workgroup id mod 8), so that neighbouring tiles, whose halos overlap, are
+// processed by the same L2.  A workgroup takes the next tile of its XCD's range from a
+// queue counter and, when that range is empty, takes tiles from the other ranges, so
+// blocks and tiles of uneven size keep every CU busy to the end.
 #include <hip/hip_runtime.h>
 
 #include "kernel_common.hpp"
@@ -50,26 +54,26 @@ __device__ __forceinline__ void face_FA(const BlockDev& b, const IdealGasDev& ga
     for (int q = 0; q < C::n; ++q) FA[q] = F[q] * g.area;
 }
 
+// LDS of one tile: the reconstruction inputs of a plane with its halo, and the F*A
+// exchange arrays -- i-faces [FT_Y][FT_X+1][NQ] and j-faces [FT_Y+1][FT_X][NQ], written by
+// the face jobs as they finish so that no flux is held in registers across jobs.
+template <int NQ>
+struct TileLds {
+    double S[SL_N][FH_Y][FH_X];
+    double FAi[FT_Y * (FT_X + 1) * NQ];
+    double FAj[(FT_Y + 1) * FT_X * NQ];
+};
+
+// One tile (64 x 4 cells, k planes [bz*KC, bz*KC + KC)) of one block.
 template <int DIM, int FC, int STAGE>
-__global__ __launch_bounds__(FT_X * FT_Y, GDTK_FUSED_WAVES_PER_EU) void k_fused_stage(
-    BlockDev b, IdealGasDev gas, double dt, double eps, CellLimits lim, int KC, int gx, int gy, int gz, int per_xcd) {
+__device__ __forceinline__ void fused_tile(const BlockDev& b, const IdealGasDev& gas, double dt, double eps,
+                                           const CellLimits& lim, int KC, int bx, int by, int bz,
+                                           TileLds<CQ<DIM>::n>& L) {
     using C = CQ<DIM>;
     constexpr int NQ = C::n;
-    __shared__ double S[SL_N][FH_Y][FH_X];
-    // F*A exchange: i-faces [FT_Y][FT_X+1][NQ] and j-faces [FT_Y+1][FT_X][NQ], written by the
-    // face jobs as they finish so that no flux is held in registers across jobs.
-    __shared__ double FAi[FT_Y * (FT_X + 1) * NQ];
-    __shared__ double FAj[(FT_Y + 1) * FT_X * NQ];
-
-    // XCD-aware numbering: hardware workgroup w runs on XCD (w % 8); give XCD x the
-    // contiguous logical range [x*per_xcd, (x+1)*per_xcd).
-    const int hw = blockIdx.x;
-    const int logical = (hw % 8) * per_xcd + hw / 8;
-    const int total = gx * gy * gz;
-    if (logical >= total) return;
-    const int bx = logical % gx;
-    const int by = (logical / gx) % gy;
-    const int bz = logical / (gx * gy);
+    auto& S = L.S;
+    double* FAi = L.FAi;
+    double* FAj = L.FAj;
 
     const int ti = threadIdx.x, tj = threadIdx.y;
     const int tid = tj * FT_X + ti;
@@ -212,19 +216,78 @@ __global__ __launch_bounds__(FT_X * FT_Y, GDTK_FUSED_WAVES_PER_EU) void k_fused_
     }
 }
 
-static int g_kchunk = 16;
+// Persistent stage kernel over every block (see the header comment).
+template <int DIM, int FC, int STAGE>
+__global__ __launch_bounds__(FT_X * FT_Y, GDTK_FUSED_WAVES_PER_EU) void k_fused_batch(
+    StageBatch sb, IdealGasDev gas, const double* __restrict__ dtp, double eps, CellLimits lim) {
+    __shared__ TileLds<CQ<DIM>::n> L;
+    __shared__ int s_tile;
+    const int xcd = blockIdx.x & 7;
+    const double dt = *dtp;
+    const bool leader = (threadIdx.x == 0) && (threadIdx.y == 0);
+    int q = 0;  // queues tried so far (leader only)
+    while (true) {
+        if (leader) {
+            int t = -1;
+            for (; q < 8; ++q) {
+                const int x = (xcd + q) & 7;
+                const int lo = x * sb.per_xcd;
+                const int hi = min(lo + sb.per_xcd, sb.total_tiles);
+                if (lo >= hi) continue;
+                const unsigned got = atomicAdd(&sb.queue[x * TILE_QSTRIDE], 1u);
+                if ((long long)lo + got < (long long)hi) {
+                    t = lo + (int)got;
+                    break;
+                }
+            }
+            s_tile = t;
+        }
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        if (t < 0) break;
+        // Block of tile t: the last block whose first tile is <= t.
+        int lo = 0, hi = sb.nblocks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sb.tile_start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int blk = __builtin_amdgcn_readfirstlane(lo);
+        const BlockDev& b = sb.tab[blk];
+        const int lt = t - sb.tile_start[blk];
+        const int gx = (b.nic + FT_X - 1) / FT_X, gy = (b.njc + FT_Y - 1) / FT_Y;
+        fused_tile<DIM, FC, STAGE>(b, gas, dt, eps, lim, (DIM == 3) ? sb.KC : 1, lt % gx, (lt / gx) % gy,
+                                   lt / (gx * gy), L);
+        __syncthreads();  // the tile's LDS and s_tile are free again
+    }
+}
 
-hipError_t launch_fused_stage(const BlockDev& b, const IdealGasDev& gas, int stage, int flux_calc, double dt,
-                              double eps, const CellLimits& lim, hipStream_t s) {
-    const int KC = (b.dims == 3) ? g_kchunk : 1;
+int fused_tiles(const BlockDev& b, int kc) {
     const int gx = (b.nic + FT_X - 1) / FT_X, gy = (b.njc + FT_Y - 1) / FT_Y;
-    const int gz = (b.dims == 3) ? (b.nkc + KC - 1) / KC : 1;
-    const int total = gx * gy * gz;
-    const int per_xcd = (total + 7) / 8;
-    dim3 grid(8 * per_xcd), blk(FT_X, FT_Y);
-#define GDTK_FUSED(D, F, ST) \
-    hipLaunchKernelGGL((k_fused_stage<D, F, ST>), grid, blk, 0, s, b, gas, dt, eps, lim, KC, gx, gy, gz, per_xcd)
-    if (b.dims == 3) {
+    const int gz = (b.dims == 3) ? (b.nkc + kc - 1) / kc : 1;
+    return gx * gy * gz;
+}
+
+int fused_batch_grid(int dims, int flux_calc) {
+    int dev = 0, ncu = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    hipError_t e;
+#define GDTK_OCC(D, F) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fused_batch<D, F, 2>, FT_X * FT_Y, 0)
+    if (dims == 3) { if (flux_calc == 1) GDTK_OCC(3, 1); else GDTK_OCC(3, 0); }
+    else { if (flux_calc == 1) GDTK_OCC(2, 1); else GDTK_OCC(2, 0); }
+#undef GDTK_OCC
+    if (e != hipSuccess || occ < 1) occ = 1;
+    return ncu * occ;
+}
+
+hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flux_calc, const double* dtp,
+                              double eps, const CellLimits& lim, const IdealGasDev& gas, int grid, hipStream_t s) {
+    if (sb.total_tiles <= 0) return hipSuccess;
+    grid = (grid + 7) / 8 * 8;
+    dim3 g(grid), blk(FT_X, FT_Y);
+#define GDTK_FUSED(D, F, ST) hipLaunchKernelGGL((k_fused_batch<D, F, ST>), g, blk, 0, s, sb, gas, dtp, eps, lim)
+    if (dims == 3) {
         if (flux_calc == 1) { if (stage == 1) GDTK_FUSED(3, 1, 1); else GDTK_FUSED(3, 1, 2); }
         else { if (stage == 1) GDTK_FUSED(3, 0, 1); else GDTK_FUSED(3, 0, 2); }
     } else {
@@ -235,8 +298,27 @@ hipError_t launch_fused_stage(const BlockDev& b, const IdealGasDev& gas, int sta
     return hipGetLastError();
 }
 
-void set_fused_kchunk(int kc) {
-    if (kc >= 1) g_kchunk = kc;
+// Clears n unsigned counters (queue and cell counters) with a kernel, so that the
+// clearing can be part of a captured step.
+__global__ void k_clear_u32(unsigned* p, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0u;
+}
+
+hipError_t launch_clear_u32(unsigned* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// Writes one double on the device (the step's dt), ordered on the stream.
+__global__ void k_set_f64(double* p, double v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+hipError_t launch_set_f64(double* p, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
 }
 
 }  // namespace gdtk

@@ -27,10 +27,18 @@ hipError_t launch_dt(const BlockDev& b, double dt_current, double cfl, double* p
 hipError_t launch_encode(const BlockDev& b, double* U, hipStream_t s);
 hipError_t launch_decode(const BlockDev& b, const IdealGasDev& gas, double* U, double* fs, double low_T,
                          hipStream_t s);
-// Fused LDS-tiled stage kernel (fused_stage.hip); returns hipErrorNotSupported for
-// shapes it does not handle.
-void set_fused_kchunk(int kc);
-hipError_t launch_fused_stage(const BlockDev& b, const IdealGasDev& gas, int stage, int flux_calc, double dt,
-                              double eps, const CellLimits& lim, hipStream_t s);
+// Ghost cells of all blocks in one launch (exchange and the other ghost-cell effects);
+// mode 1 exchanges the cell lengths of the exchange sides instead.
+hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsides, int64_t total, int mode,
+                              hipStream_t s);
+// Fused LDS-tiled stage kernel over all blocks (fused_stage.hip): tiles of one block for
+// k chunks of kc planes, the persistent grid size for the device, and the launch (dt is
+// read from device memory).
+int fused_tiles(const BlockDev& b, int kc);
+int fused_batch_grid(int dims, int flux_calc);
+hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flux_calc, const double* dtp,
+                              double eps, const CellLimits& lim, const IdealGasDev& gas, int grid, hipStream_t s);
+hipError_t launch_clear_u32(unsigned* p, int n, hipStream_t s);
+hipError_t launch_set_f64(double* p, double v, hipStream_t s);
 
 }  // namespace gdtk

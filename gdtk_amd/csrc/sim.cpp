@@ -50,11 +50,20 @@ struct Block {
     BlockDev dev{};
     DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], dtpart, dtout;
     int* counters = nullptr;
-    int cur = 0;   // which fs/U copy is current (input of the next stage)
     ~Block() {
         if (counters) (void)hipFree(counters);
     }
     int64_t ncell() const { return (int64_t)nic * njc * nkc; }
+};
+
+// Counters of the batched stage launches, per stage s = 1, 2: 8 per-XCD tile-queue
+// counters (TILE_QSTRIDE apart), then 4 cell counters per block ([0] decode failures,
+// [1] invalid cells).
+struct StageCounters {
+    int nblocks = 0;
+    int region() const { return 8 * TILE_QSTRIDE + 4 * nblocks; }
+    int queue(int stage) const { return (stage - 1) * region(); }
+    int cells(int stage, int blk) const { return (stage - 1) * region() + 8 * TILE_QSTRIDE + 4 * blk; }
 };
 
 struct Sim {
@@ -63,6 +72,25 @@ struct Sim {
     CellLimits lim;
     std::vector<std::unique_ptr<Block>> blocks;
     hipStream_t stream = nullptr;
+    // Which flow-state copy is the input of the next stage, and which U copy holds U[0]
+    // (the reference swaps U[0] and U[2] after a step, simcore_gasdynamic_step.d:1557-1562).
+    int cur = 0, u0 = 0;
+    // Batched launches over all blocks, set up once all blocks exist: per-block device
+    // records for each (stage, cur, u0), the ghost-side table, tile prefix sums, counters,
+    // the device copy of dt and the captured steps (one per u0).
+    bool finalized = false;
+    DevArr<BlockDev> tabs;
+    DevArr<SideDev> sides;
+    int nsides = 0;
+    int64_t nghost = 0;
+    DevArr<int> tile_start;
+    int total_tiles = 0, kchunk = 16, tiles_kc = -1;
+    DevArr<unsigned> ctr;
+    StageCounters sc;
+    DevArr<double> dt_dev;
+    int grid = 0, wg_per_cu = 0;
+    bool use_graph = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     // SimState (simcore.d)
     double time = 0.0, dt_global = 0.0, dt_allow = 0.0, cfl_max = 0.0;
     int step = 0, attempts_last = 0;
@@ -96,6 +124,7 @@ struct Sim {
     }
 
     ~Sim() {
+        drop_graphs();
         blocks.clear();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -106,14 +135,220 @@ struct Sim {
 
     void refresh_dev(Block& b) {
         BlockDev& d = b.dev;
-        d.fs_in = b.fs[b.cur].p;
-        d.fs_out = b.fs[1 - b.cur].p;
-        d.U0 = b.U[0].p;
-        d.Unew = b.U[1].p;
+        d.fs_in = b.fs[cur].p;
+        d.fs_out = b.fs[1 - cur].p;
+        d.U0 = b.U[u0].p;
+        d.Unew = b.U[1 - u0].p;
         d.dUdt0 = b.dUdt0.p;
+        d.counters = b.counters;
+    }
+
+    void drop_graphs() {
+        for (auto& g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+
+    static int tab_index(int stage, int c, int u) { return ((stage - 1) * 2 + c) * 2 + u; }
+    const BlockDev* tab(int stage, int c, int u) const {
+        return tabs.p + (size_t)tab_index(stage, c, u) * blocks.size();
+    }
+
+    // Exchange connections (full_face_copy.d:140-695, check_cell_mapping): the source
+    // side must exist and hold the same number of cells, in the orientation's arrangement;
+    // every ghost cell's source must lie inside the source block (the mapping is affine
+    // in the in-face position, so its corners decide).
+    void check_exchange(const Block& b, int side) const {
+        const gdtk_block_desc& d = b.desc;
+        const int nsd = (cfg.dimensions == 3) ? 6 : 4;
+        const int ob = d.other_block[side], os = d.other_face[side], oo = d.other_orientation[side];
+        if (ob < 0 || ob >= (int)blocks.size()) throw Error(GDTK_ERR_ARG, "exchange: bad other block");
+        if (os < 0 || os >= nsd) throw Error(GDTK_ERR_ARG, "exchange: bad other face");
+        if (oo < 0 || oo > 3 || (cfg.dimensions == 2 && oo != 0)) throw Error(GDTK_ERR_ARG, "exchange: bad orientation");
+        const Block& o = *blocks[ob];
+        int A, B, oA, oB;
+        side_extents(side, b.nic, b.njc, b.nkc, A, B);
+        side_extents(os, o.nic, o.njc, o.nkc, oA, oB);
+        const bool turned = (cfg.dimensions == 3) && (oo % 2 == 1);
+        const bool ok = (cfg.dimensions == 2) ? (A == oA) : (turned ? (A == oB && B == oA) : (A == oA && B == oB));
+        if (!ok) throw Error(GDTK_ERR_ARG, "exchange: the two faces hold different numbers of cells");
+        for (int ca = 0; ca < 2; ++ca)
+            for (int cb = 0; cb < 2; ++cb)
+                for (int n = 0; n < NGHOST; ++n) {
+                    int i, j, k;
+                    ffc_source(cfg.dimensions, side, os, oo, ca ? A - 1 : 0, cb ? B - 1 : 0, n, o.nic, o.njc, o.nkc, i,
+                               j, k);
+                    if (i < 0 || i >= o.nic || j < 0 || j >= o.njc || k < 0 || k >= o.nkc)
+                        throw Error(GDTK_ERR_ARG, "exchange: mapping leaves the source block");
+                }
+    }
+
+    void build_tiles(int kc) {
+        std::vector<int> ts(blocks.size() + 1, 0);
+        for (size_t q = 0; q < blocks.size(); ++q) ts[q + 1] = ts[q] + fused_tiles(blocks[q]->dev, kc);
+        total_tiles = ts.back();
+        tile_start.alloc(ts.size());
+        HIPCHK(hipMemcpy(tile_start.p, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice));
+        tiles_kc = kc;
+        drop_graphs();
+    }
+
+    void build_tabs() {
+        const size_t nb = blocks.size();
+        std::vector<BlockDev> h(8 * nb);
+        for (int st = 1; st <= 2; ++st)
+            for (int c = 0; c < 2; ++c)
+                for (int u = 0; u < 2; ++u)
+                    for (size_t q = 0; q < nb; ++q) {
+                        Block& b = *blocks[q];
+                        BlockDev d = b.dev;
+                        d.fs_in = b.fs[c].p;
+                        d.fs_out = b.fs[1 - c].p;
+                        d.U0 = b.U[u].p;
+                        d.Unew = b.U[1 - u].p;
+                        d.dUdt0 = b.dUdt0.p;
+                        d.counters = reinterpret_cast<int*>(ctr.p + sc.cells(st, (int)q));
+                        h[(size_t)tab_index(st, c, u) * nb + q] = d;
+                    }
+        tabs.alloc(h.size());
+        HIPCHK(hipMemcpy(tabs.p, h.data(), h.size() * sizeof(BlockDev), hipMemcpyHostToDevice));
+    }
+
+    // Sets up the batched launches once every block exists (the reference does the same
+    // after building all blocks: set_up_cell_mapping, exchange_geometry).
+    void finalize() {
+        if (finalized) return;
+        if (blocks.empty()) throw Error(GDTK_ERR_ARG, "no blocks");
+        const int nsd = (cfg.dimensions == 3) ? 6 : 4;
+        std::vector<SideDev> hs;
+        int64_t start = 0;
+        bool any_exchange = false;
+        for (auto& bp : blocks) {
+            Block& b = *bp;
+            for (int side = 0; side < nsd; ++side) {
+                SideDev sd{};
+                sd.blk = b.id;
+                sd.side = side;
+                sd.kind = b.desc.bc_kind[side];
+                sd.oblk = sd.oside = -1;
+                sd.orient = 0;
+                if (sd.kind == GDTK_BC_EXCHANGE) {
+                    check_exchange(b, side);
+                    sd.oblk = b.desc.other_block[side];
+                    sd.oside = b.desc.other_face[side];
+                    sd.orient = b.desc.other_orientation[side];
+                    any_exchange = true;
+                }
+                side_extents(side, b.nic, b.njc, b.nkc, sd.na, sd.nb);
+                sd.start = start;
+                start += (int64_t)sd.na * sd.nb * NGHOST;
+                hs.push_back(sd);
+            }
+        }
+        nsides = (int)hs.size();
+        nghost = start;
+        sides.alloc(hs.size());
+        HIPCHK(hipMemcpy(sides.p, hs.data(), hs.size() * sizeof(SideDev), hipMemcpyHostToDevice));
+        sc.nblocks = (int)blocks.size();
+        ctr.alloc((size_t)2 * sc.region());
+        dt_dev.alloc(1);
+        for (auto& bp : blocks) refresh_dev(*bp);
+        build_tabs();
+        build_tiles(kchunk);
+        grid = fused_batch_grid(cfg.dimensions, cfg.flux_calculator);
+        if (grid < 8) throw Error(GDTK_ERR_HIP, "cannot size the stage launch");
+        // Geometry exchange: ghost cells of exchange sides take their source cells' lengths.
+        if (any_exchange) {
+            HIPCHK(launch_ghost_batch(tab(1, 0, 0), sides.p, nsides, nghost, 1, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        finalized = true;
+    }
+
+    int launch_grid() const {
+        if (wg_per_cu <= 0) return grid;
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        return std::max(8, ncu * wg_per_cu);
+    }
+
+    // Batched stage over all blocks: ghost cells (exchange + BCs), then the fused stage
+    // kernel; dt is read from dt_dev.  Counters of the stage must be clear.
+    void launch_batched_stage(int stage, int c, int u) {
+        const BlockDev* t = tab(stage, c, u);
+        const double nc = (double)total_cells();
+        timed(3, (double)nghost, [&] { return launch_ghost_batch(t, sides.p, nsides, nghost, 0, stream); });
+        StageBatch sb;
+        sb.tab = t;
+        sb.tile_start = tile_start.p;
+        sb.queue = ctr.p + sc.queue(stage);
+        sb.nblocks = (int)blocks.size();
+        sb.total_tiles = total_tiles;
+        sb.per_xcd = (total_tiles + 7) / 8;
+        sb.KC = kchunk;
+        const double eps = 1.0e-12;  // epsilon_van_albada, globalconfig.d:1079
+        timed(2, nc, [&] {
+            return launch_fused_batch(sb, cfg.dimensions, stage, cfg.flux_calculator, dt_dev.p, eps, lim, gas,
+                                      launch_grid(), stream);
+        });
+    }
+
+    int64_t total_cells() const {
+        int64_t n = 0;
+        for (auto& bp : blocks) n += bp->ncell();
+        return n;
+    }
+
+    // Reads the cell counters of the given stages: (decode failures summed over blocks,
+    // invalid cells of the worst block) per stage.
+    void read_counters(int s_lo, int s_hi, int nf[3], int ni[3]) {
+        std::vector<unsigned> h((size_t)2 * sc.region());
+        HIPCHK(hipMemcpyAsync(h.data(), ctr.p, h.size() * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int st = s_lo; st <= s_hi; ++st) {
+            nf[st] = 0;
+            ni[st] = 0;
+            for (int q = 0; q < sc.nblocks; ++q) {
+                nf[st] += (int)h[sc.cells(st, q)];
+                ni[st] = std::max(ni[st], (int)h[sc.cells(st, q) + 1]);
+            }
+        }
+    }
+
+    // The whole pc step of all blocks as one captured graph (per U parity): clear the
+    // counters, then ghost cells + fused stage for stage 1 and stage 2.
+    hipGraphExec_t step_graph(int u) {
+        if (graph[u]) return graph[u];
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        bool saved = profile;
+        profile = false;
+        try {
+            HIPCHK(launch_clear_u32(ctr.p, 2 * sc.region(), stream));
+            launch_batched_stage(1, 0, u);
+            launch_batched_stage(2, 1, u);
+        } catch (...) {
+            profile = saved;
+            hipGraph_t dummy = nullptr;
+            (void)hipStreamEndCapture(stream, &dummy);
+            if (dummy) (void)hipGraphDestroy(dummy);
+            throw;
+        }
+        profile = saved;
+        HIPCHK(hipStreamEndCapture(stream, &g));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+        graph[u] = ex;
+        return ex;
     }
 
     void add_block(const gdtk_block_desc& desc, int* id) {
+        if (finalized) throw Error(GDTK_ERR_ARG, "blocks cannot be added after the first stage");
         if (desc.nic < 2 || desc.njc < 2 || (cfg.dimensions == 3 && desc.nkc < 2) ||
             (cfg.dimensions == 2 && desc.nkc != 1))
             throw Error(GDTK_ERR_ARG, "block extents must be >= 2 cells per active direction (nkc == 1 in 2-D)");
@@ -194,10 +429,9 @@ struct Sim {
                 b->fgeom[q].alloc(geo.face[q].size());
                 HIPCHK(hipMemcpy(b->fgeom[q].p, geo.face[q].data(), geo.face[q].size() * sizeof(double),
                                  hipMemcpyHostToDevice));
-                b->flux[q].alloc((size_t)d.nface[q] * b->ncq);
             }
             d.fgeom[q] = b->fgeom[q].p;
-            d.flux[q] = b->flux[q].p;
+            d.flux[q] = nullptr;  // face-flux arrays: allocated when the face-per-thread kernels run
         }
         d.len = b->len.p;
         d.vol = b->vol.p;
@@ -214,7 +448,6 @@ struct Sim {
                 double row[8] = {rho, desc.inflow[s][2], desc.inflow[s][3], desc.inflow[s][4], u, p, T, a};
                 for (int f = 0; f < 8; ++f) d.inflow[s][f] = row[f];
             }
-            if (desc.bc_kind[s] == GDTK_BC_EXCHANGE) throw Error(GDTK_ERR_UNSUPPORTED, "exchange BC not built yet");
         }
         HIPCHK(hipMalloc(&b->counters, 4 * sizeof(int)));
         HIPCHK(hipMemset(b->counters, 0, 4 * sizeof(int)));
@@ -251,12 +484,12 @@ struct Sim {
                     double row[8] = {rho, vx[c], vy[c], vz[c], u, pp, TT, a};
                     for (int f = 0; f < 8; ++f) h[pc + f * np] = row[f];
                 }
-        b.cur = 0;
         refresh_dev(b);
-        HIPCHK(hipMemcpy(b.fs[0].p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipMemcpy(b.fs[cur].p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemsetAsync(b.counters, 0, 4 * sizeof(int), stream));
-        HIPCHK(launch_encode(b.dev, b.U[0].p, stream));
-        HIPCHK(launch_decode(b.dev, gas, b.U[0].p, b.fs[0].p, lim.low_T, stream));
+        HIPCHK(launch_encode(b.dev, b.U[u0].p, stream));
+        HIPCHK(launch_decode(b.dev, gas, b.U[u0].p, b.fs[cur].p, lim.low_T, stream));
         int cnt[4];
         HIPCHK(hipMemcpyAsync(cnt, b.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -269,8 +502,8 @@ struct Sim {
         const size_t np = (size_t)d.npad;
         const double* src = nullptr;
         if (field >= GDTK_FIELD_DUDT0 && field < GDTK_FIELD_DUDT0 + b.ncq) src = b.dUdt0.p + (field - GDTK_FIELD_DUDT0) * np;
-        else if (field >= GDTK_FIELD_U0 && field < GDTK_FIELD_U0 + b.ncq) src = b.U[0].p + (field - GDTK_FIELD_U0) * np;
-        else if (field >= 0 && field < FS_NBASE) src = b.fs[b.cur].p + field * np;
+        else if (field >= GDTK_FIELD_U0 && field < GDTK_FIELD_U0 + b.ncq) src = b.U[u0].p + (field - GDTK_FIELD_U0) * np;
+        else if (field >= 0 && field < FS_NBASE) src = b.fs[cur].p + field * np;
         else if (field == GDTK_FIELD_VOLUME) src = b.vol.p;
         else if (field >= GDTK_FIELD_ILENGTH && field <= GDTK_FIELD_KLENGTH) src = b.len.p + (field - GDTK_FIELD_ILENGTH) * np;
         else throw Error(GDTK_ERR_ARG, "bad field");
@@ -289,23 +522,50 @@ struct Sim {
 
     bool use_fused() const { return variant >= 0 ? variant == 1 : cfg.use_fused != 0; }
 
-    // Launch one stage on every block; returns without synchronising.
+    void ensure_flux(Block& b) {
+        const int nd = (b.dims == 3) ? 3 : 2;
+        for (int q = 0; q < nd; ++q)
+            if (!b.flux[q].p) {
+                b.flux[q].alloc((size_t)b.dev.nface[q] * b.ncq);
+                b.dev.flux[q] = b.flux[q].p;
+            }
+    }
+
+    // Launches one stage on every block (counters must be clear); returns without
+    // synchronising.  Fused variant: the batched launches over all blocks (dt written to
+    // dt_dev first); face-per-thread variant: per block, ghost fill, face fluxes per
+    // direction and the cell update.
     void launch_stage(int stage, double dt) {
+        if (use_fused()) {
+            finalize();
+            if (tiles_kc != kchunk) build_tiles(kchunk);
+            HIPCHK(launch_set_f64(dt_dev.p, dt, stream));
+            launch_batched_stage(stage, cur, u0);
+            return;
+        }
         const double eps = 1.0e-12;  // epsilon_van_albada, globalconfig.d:1079
+        finalize();
+        // Exchange sides read the other blocks' cells: all blocks' ghost cells first.
+        timed(3, (double)nghost, [&] { return launch_ghost_batch(tab(stage, cur, u0), sides.p, nsides, nghost, 0, stream); });
         for (auto& bp : blocks) {
             Block& b = *bp;
+            ensure_flux(b);
             refresh_dev(b);
             const double nc = (double)b.ncell();
-            timed(3, nc, [&] { return launch_ghost_fill(b.dev, b.fs[b.cur].p, stream); });
-            if (use_fused()) {
-                timed(2, nc, [&] { return launch_fused_stage(b.dev, gas, stage, cfg.flux_calculator, dt, eps, lim, stream); });
-            } else {
-                const int nd = (b.dims == 3) ? 3 : 2;
-                for (int d = 0; d < nd; ++d)
-                    timed(0, (double)b.dev.nface[d],
-                          [&] { return launch_face_flux_dir(b.dev, gas, cfg.flux_calculator, eps, d, stream); });
-                timed(1, nc, [&] { return launch_cell_update(b.dev, gas, stage, dt, lim, stream); });
-            }
+            const int nd = (b.dims == 3) ? 3 : 2;
+            for (int d = 0; d < nd; ++d)
+                timed(0, (double)b.dev.nface[d],
+                      [&] { return launch_face_flux_dir(b.dev, gas, cfg.flux_calculator, eps, d, stream); });
+            timed(1, nc, [&] { return launch_cell_update(b.dev, gas, stage, dt, lim, stream); });
+        }
+    }
+
+    void clear_stage_counters(int stage) {
+        if (use_fused()) {
+            finalize();
+            HIPCHK(launch_clear_u32(ctr.p + sc.queue(stage), sc.region(), stream));
+        } else {
+            for (auto& bp : blocks) HIPCHK(hipMemsetAsync(bp->counters, 0, 4 * sizeof(int), stream));
         }
     }
 
@@ -319,15 +579,16 @@ struct Sim {
         const int kcs[] = {4, 8, 16, 32, 64};
         bool saved_profile = profile;
         profile = false;
-        // Stage 2 on the same input writes only the scratch copies (fs_out, U[1]), so
-        // a stage-1 + stage-2 pair can be timed without advancing the solution.
         auto time_variant = [&](int var, int kc) {
             variant = var;
-            if (kc > 0) set_fused_kchunk(kc);
+            if (kc > 0) kchunk = kc;
+            clear_stage_counters(1);
             launch_stage(1, dt);  // warm-up
             HIPCHK(hipEventRecord(ev0, stream));
             for (int r = 0; r < reps; ++r) {
+                clear_stage_counters(1);
                 launch_stage(1, dt);
+                clear_stage_counters(2);
                 launch_stage(2, dt);
             }
             HIPCHK(hipEventRecord(ev1, stream));
@@ -351,7 +612,7 @@ struct Sim {
             if (t < best) { best = t; best_var = 1; best_kc = 1; }
         }
         variant = best_var;
-        if (best_var == 1) set_fused_kchunk(best_kc);
+        if (best_var == 1) kchunk = best_kc;
         HIPCHK(hipStreamSynchronize(stream));
         profile = saved_profile;
         result.insert(result.begin(), {(double)best_var, (double)best_kc, best});
@@ -360,20 +621,28 @@ struct Sim {
 
     // Runs one stage on all blocks and reports (decode failures, invalid cells).
     void stage_checked(int stage, double dt, int& n_fail, int& n_invalid) {
-        for (auto& bp : blocks) HIPCHK(hipMemsetAsync(bp->counters, 0, 4 * sizeof(int), stream));
+        clear_stage_counters(stage);
         launch_stage(stage, dt);
         n_fail = 0;
         n_invalid = 0;
-        std::vector<int> cnt(4 * blocks.size());
-        for (size_t q = 0; q < blocks.size(); ++q)
-            HIPCHK(hipMemcpyAsync(&cnt[4 * q], blocks[q]->counters, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        for (size_t q = 0; q < blocks.size(); ++q) {
-            n_fail += cnt[4 * q];
-            n_invalid = std::max(n_invalid, cnt[4 * q + 1]);
+        if (use_fused()) {
+            int nf[3], ni[3];
+            read_counters(stage, stage, nf, ni);
+            n_fail = nf[stage];
+            n_invalid = ni[stage];
+        } else {
+            std::vector<int> cnt(4 * blocks.size());
+            for (size_t q = 0; q < blocks.size(); ++q)
+                HIPCHK(hipMemcpyAsync(&cnt[4 * q], blocks[q]->counters, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            for (size_t q = 0; q < blocks.size(); ++q) {
+                n_fail += cnt[4 * q];
+                n_invalid = std::max(n_invalid, cnt[4 * q + 1]);
+            }
         }
         // The stage output becomes the current flow state.
-        for (auto& bp : blocks) bp->cur = 1 - bp->cur;
+        cur = 1 - cur;
+        for (auto& bp : blocks) refresh_dev(*bp);
     }
 
     // Public single stage (tests): stage 1 leaves fs = decode(U1), stage 2 fs = decode(U2);
@@ -387,12 +656,13 @@ struct Sim {
         return GDTK_OK;
     }
 
-    void block_dt(Block& b, double dt_current, double cfl, bool check_cfl, double out[3]) {
+    void launch_block_dt(Block& b, double dt_current, double cfl) {
         refresh_dev(b);
         HIPCHK(launch_dt(b.dev, dt_current, cfl, b.dtpart.p, b.dtout.p, stream));
-        double h[4];
-        HIPCHK(hipMemcpyAsync(h, b.dtout.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // fluidblock.d:987-1084 from the reduced {dt_allow, cfl_max, last signal}.
+    static void finish_block_dt(const double h[4], bool check_cfl, double out[3]) {
         double dt_allow = h[0], cflm = h[1], last_signal = h[2];
         const double cfl_allow = 1.2, cfl_adjust = 0.5;  // fluidblock.d:1011, 1002
         if (check_cfl && (cflm < 0.0 || cflm > cfl_allow)) {
@@ -404,13 +674,28 @@ struct Sim {
         out[2] = 0.0;
     }
 
-    // simcore_gasdynamic_step.d:60-159
+    void block_dt(Block& b, double dt_current, double cfl, bool check_cfl, double out[3]) {
+        launch_block_dt(b, dt_current, cfl);
+        double h[4];
+        HIPCHK(hipMemcpyAsync(h, b.dtout.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        finish_block_dt(h, check_cfl, out);
+    }
+
+    // simcore_gasdynamic_step.d:60-159: every block's estimate is launched, then read back
+    // with one synchronisation.
     void determine_time_step_size() {
         if ((step % cfg.cfl_count) != 0) return;
+        const size_t nb = blocks.size();
+        std::vector<double> h(4 * nb);
+        for (auto& bp : blocks) launch_block_dt(*bp, dt_global, cfg.cfl_value);
+        for (size_t q = 0; q < nb; ++q)
+            HIPCHK(hipMemcpyAsync(&h[4 * q], blocks[q]->dtout.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         double da = std::numeric_limits<double>::max(), cm = 0.0;
-        for (auto& bp : blocks) {
+        for (size_t q = 0; q < nb; ++q) {
             double r[3];
-            block_dt(*bp, dt_global, cfg.cfl_value, step > 0, r);
+            finish_block_dt(&h[4 * q], step > 0, r);
             da = std::min(da, r[0]);
             cm = std::max(cm, r[1]);
         }
@@ -425,28 +710,61 @@ struct Sim {
         }
     }
 
+    // Restart of a failed attempt from U[0]: the flow state is restored by decoding U[0]
+    // (the reference retries from whatever state its cells hold).
+    void restore_from_U0() {
+        cur = 0;
+        for (auto& bp : blocks) {
+            Block& b = *bp;
+            refresh_dev(b);
+            HIPCHK(launch_decode(b.dev, gas, b.U[u0].p, b.fs[0].p, lim.low_T, stream));
+        }
+    }
+
+    // One attempt of the pc step as the captured graph: both stages run, then the counters
+    // decide as the reference's stage-by-stage checks would (a failed stage 1 makes stage
+    // 2's results irrelevant; the retry restarts from U[0], which the attempt leaves
+    // untouched).  Returns true when the attempt failed.
+    bool graph_attempt() {
+        finalize();
+        if (tiles_kc != kchunk) build_tiles(kchunk);
+        hipGraphExec_t ex = step_graph(u0);
+        HIPCHK(launch_set_f64(dt_dev.p, dt_global, stream));
+        HIPCHK(hipGraphLaunch(ex, stream));
+        int nf[3], ni[3];
+        read_counters(1, 2, nf, ni);
+        for (int st = 1; st <= 2; ++st) {
+            if (nf[st] > 0) return true;
+            if (ni[st] > cfg.max_invalid_cells)
+                throw Error(GDTK_ERR_TOO_MANY_BAD_CELLS, "Too many bad cells during explicit gasdynamic update.");
+        }
+        return false;
+    }
+
     // simcore_gasdynamic_step.d:906-1575 for the pc scheme.
     void gasdynamic_step() {
         double t0 = time;
         int attempt = 0;
         bool failed = false;
+        const bool graphed = use_fused() && use_graph && !profile && cur == 0;
         do {
             ++attempt;
             failed = false;
+            if (graphed) {
+                failed = graph_attempt();
+                if (failed) {
+                    dt_global = dt_global * 0.2;
+                    restore_from_U0();
+                }
+                continue;
+            }
             for (int s = 1; s <= 2; ++s) {
                 int nf, ni;
                 stage_checked(s, dt_global, nf, ni);
                 if (nf > 0) {
                     failed = true;
                     dt_global = dt_global * 0.2;
-                    // Restart from U[0]: restore the flow state by decoding U[0]
-                    // (the reference retries from whatever state its cells hold).
-                    for (auto& bp : blocks) {
-                        Block& b = *bp;
-                        b.cur = 0;
-                        refresh_dev(b);
-                        HIPCHK(launch_decode(b.dev, gas, b.U[0].p, b.fs[0].p, lim.low_T, stream));
-                    }
+                    restore_from_U0();
                     break;
                 }
                 if (ni > cfg.max_invalid_cells)
@@ -455,16 +773,10 @@ struct Sim {
         } while (failed && attempt < cfg.max_attempts_for_step);
         attempts_last = attempt;
         if (failed) throw Error(GDTK_ERR_STEP_FAILED, "Explicit update failed after max attempts; giving up.");
-        // swap(U[0], U[end]) (simcore_gasdynamic_step.d:1557-1562); after two stages the
-        // current flow state is again in fs[0] when cur toggled twice.
-        for (auto& bp : blocks) {
-            std::swap(bp->U[0].p, bp->U[1].p);
-            if (bp->cur != 0) {
-                std::swap(bp->fs[0].p, bp->fs[1].p);
-                bp->cur = 0;
-            }
-            refresh_dev(*bp);
-        }
+        // swap(U[0], U[end]) (simcore_gasdynamic_step.d:1557-1562).  The two stages leave
+        // the flow state in the copy the step started from.
+        u0 = 1 - u0;
+        for (auto& bp : blocks) refresh_dev(*bp);
         time = t0 + dt_global;
     }
 
@@ -650,12 +962,44 @@ int gdtk_hip_sim_autotune(int sim, double dt, int reps, double* out, int nout) {
     });
 }
 
+int gdtk_hip_exchange_source(int dims, int face, int other_face, int orientation, int a, int b, int n, int nic,
+                             int njc, int nkc, int* ijk) {
+    return guarded([&] {
+        if (!ijk) throw Error(GDTK_ERR_ARG, "null argument");
+        const int nsd = (dims == 3) ? 6 : 4;
+        if ((dims != 2 && dims != 3) || face < 0 || face >= nsd || other_face < 0 || other_face >= nsd ||
+            orientation < 0 || orientation > 3 || n < 0 || n >= NGHOST)
+            throw Error(GDTK_ERR_ARG, "bad exchange description");
+        int i, j, k;
+        ffc_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc, i, j, k);
+        ijk[0] = i;
+        ijk[1] = j;
+        ijk[2] = k;
+        return GDTK_OK;
+    });
+}
+
 int gdtk_hip_sim_set_tuning(int sim, int key, int value) {
     return guarded([&] {
-        (void)get_sim(sim);
+        Sim& s = get_sim(sim);
         if (key == GDTK_TUNE_KCHUNK) {
             if (value < 1 || value > 4096) throw Error(GDTK_ERR_ARG, "k chunk out of range");
-            set_fused_kchunk(value);
+            s.kchunk = value;
+            return GDTK_OK;
+        }
+        if (key == GDTK_TUNE_GRAPH) {
+            s.use_graph = value != 0;
+            return GDTK_OK;
+        }
+        if (key == GDTK_TUNE_WG_PER_CU) {
+            if (value < 0 || value > 32) throw Error(GDTK_ERR_ARG, "workgroups per CU out of range");
+            s.wg_per_cu = value;
+            s.drop_graphs();
+            return GDTK_OK;
+        }
+        if (key == GDTK_TUNE_VARIANT) {
+            if (value < -1 || value > 1) throw Error(GDTK_ERR_ARG, "variant must be -1, 0 or 1");
+            s.variant = value;
             return GDTK_OK;
         }
         throw Error(GDTK_ERR_ARG, "unknown tuning key");

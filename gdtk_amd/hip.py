@@ -30,7 +30,7 @@ EXPORTS = [
     "gdtk_hip_sim_create", "gdtk_hip_sim_destroy", "gdtk_hip_sim_add_block", "gdtk_hip_block_set_initial",
     "gdtk_hip_block_get_field", "gdtk_hip_stage", "gdtk_hip_block_dt", "gdtk_hip_sim_run",
     "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
-    "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error",
+    "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error", "gdtk_hip_exchange_source",
     "gdtk_hip_chem_create", "gdtk_hip_chem_destroy", "gdtk_hip_chem_step", "gdtk_hip_chem_step_device",
     "gdtk_hip_chem_stats", "gdtk_hip_chem_substeps", "gdtk_hip_chem_set_tuning", "gdtk_hip_chem_autotune",
     "gdtk_hip_tpg_update",
@@ -57,7 +57,7 @@ class SimConfig(C.Structure):
 class BlockDesc(C.Structure):
     _fields_ = [("nic", C.c_int), ("njc", C.c_int), ("nkc", C.c_int), ("vertices", dp),
                 ("bc_kind", C.c_int * 6), ("inflow", (C.c_double * 5) * 6),
-                ("other_block", C.c_int * 6), ("other_face", C.c_int * 6)]
+                ("other_block", C.c_int * 6), ("other_face", C.c_int * 6), ("other_orientation", C.c_int * 6)]
 
 
 def lib_path() -> str:
@@ -98,6 +98,7 @@ def lib():
         L.gdtk_hip_sim_kernel_stats.argtypes = [C.c_int, dp, C.c_int]
         L.gdtk_hip_sim_set_tuning.argtypes = [C.c_int, C.c_int, C.c_int]
         L.gdtk_hip_sim_autotune.argtypes = [C.c_int, C.c_double, C.c_int, dp, C.c_int]
+        L.gdtk_hip_exchange_source.argtypes = [C.c_int] * 10 + [ip]
         _lib = L
     return _lib
 
@@ -147,6 +148,7 @@ class Simulation:
             d.bc_kind[s] = int(case.bc[s])
             d.other_block[s] = int(case.other_block[s])
             d.other_face[s] = int(case.other_face[s])
+            d.other_orientation[s] = int(getattr(case, "other_orientation", [0] * 6)[s])
             for q in range(5):
                 d.inflow[s][q] = float(case.inflow[s][q])
         bid = C.c_int(-1)
@@ -198,6 +200,18 @@ class Simulation:
     def set_kchunk(self, kc):
         _check(lib().gdtk_hip_sim_set_tuning(self.h, 1, int(kc)))
 
+    def set_graph(self, enable):
+        """Run each step's attempt as one captured HIP graph (default) or kernel by kernel."""
+        _check(lib().gdtk_hip_sim_set_tuning(self.h, 2, 1 if enable else 0))
+
+    def set_wg_per_cu(self, n):
+        """Workgroups per CU of the persistent stage launch (0: occupancy of the kernel)."""
+        _check(lib().gdtk_hip_sim_set_tuning(self.h, 3, int(n)))
+
+    def set_variant(self, variant):
+        """-1: from the configuration, 0: face-per-thread kernels, 1: fused stage kernel."""
+        _check(lib().gdtk_hip_sim_set_tuning(self.h, 4, int(variant)))
+
     KERNEL_KINDS = ("face_flux", "cell_update", "fused_stage", "ghost_fill")
 
     def kernel_stats(self):
@@ -217,3 +231,13 @@ class Simulation:
         cands = [dict(variant="fused" if out[q] == 1 else "face", kchunk=int(out[q + 1]), ms=out[q + 2])
                  for q in range(3, n, 3)]
         return dict(best="fused" if out[0] == 1 else "face", kchunk=int(out[1]), ms=out[2], candidates=cands)
+
+
+def exchange_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc):
+    """(i, j, k) of the source cell of ghost layer n at in-face position (a, b) of an
+    exchange face, in the other block of nic x njc x nkc cells (the mapping the exchange
+    kernel uses; runs on the host)."""
+    out = (C.c_int * 3)()
+    _check(lib().gdtk_hip_exchange_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc,
+                                          C.cast(out, ip)))
+    return tuple(out)

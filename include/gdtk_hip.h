@@ -72,14 +72,21 @@ typedef struct gdtk_sim_config {
     int use_fused;             /* 1: fused LDS-tiled stage kernel where the block shape allows */
 } gdtk_sim_config;
 
-/* One structured block (src/eilmer/sfluidblock.d:68-110). */
+/* One structured block (src/eilmer/sfluidblock.d:68-110).  Faces are numbered as the
+ * reference's Face enum (src/geom/elements/nomenclature.d:12-20): west 0, east 1,
+ * south 2, north 3, bottom 4, top 5. */
 typedef struct gdtk_block_desc {
     int nic, njc, nkc;         /* nkc = 1 in 2-D */
     const double* vertices;    /* niv*njv*nkv x 3 coordinates */
     int bc_kind[6];            /* per face west, east, south, north, bottom, top */
     double inflow[6][5];       /* p, T, vx, vy, vz of an inflow face (FlowState(p,T,vel)) */
-    int other_block[6];        /* exchange faces: neighbour block id and face */
+    /* Exchange faces (GDTK_BC_EXCHANGE, the same-process full-face copy of
+     * src/eilmer/bc/ghost_cell_effect/full_face_copy.d:67-125): the neighbour block id,
+     * its face and the orientation 0..3 of the connection (src/eilmer/blk_conn.lua conventions; 0 in
+     * 2-D).  Connections are checked when the first stage runs, once every block exists. */
+    int other_block[6];
     int other_face[6];
+    int other_orientation[6];
 } gdtk_block_desc;
 
 /* Replaces the GlobalConfig/SimState set-up of init_simulation (src/eilmer/simcore.d:83-720). */
@@ -101,7 +108,8 @@ int gdtk_hip_block_get_field(int sim, int block, int field, double* out);
 /* One stage of the explicit update over all blocks (simcore_gasdynamic_step.d:946-1426):
  * exchange, ghost-cell effects, convective_flux_phase0 (sfluidblock.d:2188), boundary flux
  * effects, time_derivatives, the pc stage update and decode_conserved.  Returns
- * GDTK_ERR_BAD_CELL where the reference would set step_failed. */
+ * GDTK_ERR_BAD_CELL where the reference would set step_failed.  All blocks are updated by
+ * one launch of the ghost-cell kernel and one persistent launch of the stage kernel. */
 int gdtk_hip_stage(int sim, int stage, double dt);
 
 /* FluidBlock.determine_time_step_size (fluidblock.d:987-1084): out = {dt_allow, cfl_max, 0}. */
@@ -133,9 +141,24 @@ int gdtk_hip_sim_kernel_stats(int sim, double* stats, int nstats);
  * candidate, ms per stage}.  The flow state and U[0] are unchanged. */
 int gdtk_hip_sim_autotune(int sim, double dt, int reps, double* out, int nout);
 
-/* Launch-shape knobs of the fused stage kernel (chosen by the autotuner):
- * GDTK_TUNE_KCHUNK = planes in k marched by one workgroup. */
+/* Source cell of an exchange ghost cell, as the exchange kernel maps it (the case tables
+ * of set_up_cell_mapping_phase0, full_face_copy.d:696-1480): ghost layer n at in-face
+ * position (a, b) of `face` -- west/east (j, k), south/north (i, k), bottom/top (i, j) --
+ * connected to `other_face` of a block of nic x njc x nkc cells with `orientation`.
+ * Writes the source cell's (i, j, k) to ijk[3].  Host-only; needs no GPU. */
+int gdtk_hip_exchange_source(int dims, int face, int other_face, int orientation, int a, int b, int n, int nic,
+                             int njc, int nkc, int* ijk);
+
+/* Launch knobs (no reference counterpart; KCHUNK and VARIANT are chosen by the autotuner):
+ * GDTK_TUNE_KCHUNK   planes in k marched by one workgroup of the fused stage kernel;
+ * GDTK_TUNE_GRAPH    1 (default): each step's attempt runs as one captured HIP graph
+ *                    (both stages of all blocks), 0: launched kernel by kernel;
+ * GDTK_TUNE_WG_PER_CU workgroups per CU of the persistent stage launch (0: occupancy);
+ * GDTK_TUNE_VARIANT  -1 (from use_fused), 0 face-per-thread kernels, 1 fused kernel. */
 #define GDTK_TUNE_KCHUNK 1
+#define GDTK_TUNE_GRAPH 2
+#define GDTK_TUNE_WG_PER_CU 3
+#define GDTK_TUNE_VARIANT 4
 int gdtk_hip_sim_set_tuning(int sim, int key, int value);
 
 /* ---------------------------------------------------------------------------------

@@ -1082,29 +1082,127 @@ void Simulation::determine_time_step_size() {
     }
 }
 
-// simcore_exchange.d:96-135 -> full_face_copy.d:1888-1900 (same process).
-void Simulation::exchange_ghost_cell_boundary_data() {
-    // Implemented with the multi-block configuration (exchange BCs).
+// Source cell of a full-face copy: restates the case tables of
+// set_up_cell_mapping_phase0 (full_face_copy.d:696-869 in 2-D, 870-1480 in 3-D).
+// Each face has in-face coordinates (a, b): west/east (j, k), south/north (i, k),
+// bottom/top (i, j); in 2-D the single coordinate a is j on west/east and i on
+// south/north.  The tables map the destination face's (a, b) to the source face's (p, q):
+// orientation 0 keeps (a, b), orientation 2 reverses both, and orientations 1 and 3 are
+// the quarter turns (b, B-1-a) and (A-1-b, a), in that order when the destination
+// face's coordinates and its outward normal form a right-handed set (east, south, top)
+// and in the opposite order otherwise; then p is reversed, p -> A-1-p, when both faces
+// have the same handedness.  (A, B) are the source face's extents.  Source layer n lies
+// n cells inside the source face.
+namespace {
+const int kFaceHand[6] = {-1, +1, +1, -1, -1, +1};  // W, E, S, N, B, T
+
+void face_extents(const SFluidBlock* b, int face, int& A, int& B) {
+    if (face <= 1) { A = b->njc; B = b->nkc; }
+    else if (face <= 3) { A = b->nic; B = b->nkc; }
+    else { A = b->nic; B = b->njc; }
+}
+}  // namespace
+
+void ffc_source_index(int dims, int dface, int sface, int orient, int a, int bb, int n, int snic, int snjc,
+                      int snkc, int& i, int& j, int& k) {
+    int A, B;
+    if (sface <= 1) { A = snjc; B = snkc; }
+    else if (sface <= 3) { A = snic; B = snkc; }
+    else { A = snic; B = snjc; }
+    int p, q;
+    if (dims == 2) {
+        p = a; q = 0;
+    } else if (orient == 0) {
+        p = a; q = bb;
+    } else if (orient == 2) {
+        p = A - 1 - a; q = B - 1 - bb;
+    } else if ((orient == 1) == (kFaceHand[dface] > 0)) {
+        p = bb; q = B - 1 - a;
+    } else {
+        p = A - 1 - bb; q = a;
+    }
+    if (kFaceHand[dface] == kFaceHand[sface]) p = A - 1 - p;
+    switch (sface) {
+    case 0: i = n; j = p; k = q; break;
+    case 1: i = snic - 1 - n; j = p; k = q; break;
+    case 2: j = n; i = p; k = q; break;
+    case 3: j = snjc - 1 - n; i = p; k = q; break;
+    case 4: k = n; i = p; j = q; break;
+    default: k = snkc - 1 - n; i = p; j = q; break;
+    }
+}
+
+namespace {
+FVCell* ffc_source(SFluidBlock* dst, int dface, SFluidBlock* src, int sface, int orient, int a, int bb, int n) {
+    int i, j, k;
+    ffc_source_index(dst->cfg.dimensions, dface, sface, orient, a, bb, n, src->nic, src->njc, src->nkc, i, j, k);
+    if (i < 0 || i >= src->nic || j < 0 || j >= src->njc || k < 0 || k >= src->nkc)
+        throw FlowSolverException("Full-face copy maps outside the source block.");
+    return src->get_cell(i, j, k);
+}
+}  // namespace
+
+// check_cell_mapping + set_up_cell_mapping_phase0..2 (full_face_copy.d:140-1480) and
+// exchange_geometry_phase2 (:1576-1625, same process): each ghost cell of an exchange
+// face is paired with its source cell and takes that cell's lengths (copy_values_from
+// with CopyDataOption.grid, fvcell.d; lengths are copied as they are, not permuted).
+void Simulation::set_up_exchange() {
+    const int nfaces = (cfg.dimensions == 3) ? 6 : 4;
     for (auto b : blocks) {
-        for (int face = 0; face < ((b->cfg.dimensions == 3) ? 6 : 4); ++face) {
+        for (int face = 0; face < nfaces; ++face) {
             BoundaryCondition& B = b->bc[face];
             if (B.kind != BCKind::exchange) continue;
-            SFluidBlock* o = blocks.at(B.other_block);
-            // Full-face copy with aligned orientation: ghost layer n on this face takes the
-            // interior cell n from the other block's matching face.
+            if (B.other_block < 0 || B.other_block >= (int)blocks.size() || B.other_face < 0 ||
+                B.other_face >= nfaces || B.other_orientation < 0 || B.other_orientation > 3)
+                throw FlowSolverException("Incorrect boundary connection.");
+            SFluidBlock* o = blocks[B.other_block];
+            int A, Bx, oA, oB;
+            face_extents(b, face, A, Bx);
+            face_extents(o, B.other_face, oA, oB);
+            bool turned = (cfg.dimensions == 3) && (B.other_orientation % 2 == 1);
+            if (cfg.dimensions == 2 ? (A != oA) : (turned ? (A != oB || Bx != oA) : (A != oA || Bx != oB)))
+                throw FlowSolverException("Exchange faces have inconsistent numbers of cells.");
+            B.ghost_cells.clear();
+            B.mapped_cells.clear();
             const int ng = b->n_ghost_cell_layers;
-            for (size_t i = 0; i < B.faces.size(); ++i) {
-                FVInterface* f = B.faces[i];
-                FVInterface* of = o->bc[B.other_face].faces.at(i);
-                int oos = o->bc[B.other_face].outsigns[i];
-                for (int n = 0; n < ng; ++n) {
-                    FVCell* ghost = (B.outsigns[i] == 1) ? f->right_cells[n] : f->left_cells[n];
-                    FVCell* src = (oos == 1) ? of->left_cells[n] : of->right_cells[n];
-                    ghost->fs = src->fs;
+            for (int bb = 0; bb < (cfg.dimensions == 3 ? Bx : 1); ++bb)
+                for (int a = 0; a < A; ++a) {
+                    FVInterface* f;
+                    switch (face) {
+                    case 0: f = b->get_ifi(0, a, bb); break;
+                    case 1: f = b->get_ifi(b->nic, a, bb); break;
+                    case 2: f = b->get_ifj(a, 0, bb); break;
+                    case 3: f = b->get_ifj(a, b->njc, bb); break;
+                    case 4: f = b->get_ifk(a, bb, 0); break;
+                    default: f = b->get_ifk(a, bb, b->nkc); break;
+                    }
+                    for (int n = 0; n < ng; ++n) {
+                        FVCell* g = (face % 2 == 1) ? f->right_cells[n] : f->left_cells[n];
+                        FVCell* m = ffc_source(b, face, o, B.other_face, B.other_orientation, a, bb, n);
+                        B.ghost_cells.push_back(g);
+                        B.mapped_cells.push_back(m);
+                        g->iLength = m->iLength;
+                        g->jLength = m->jLength;
+                        g->kLength = m->kLength;
+                        g->volume = m->volume;
+                        g->pos = m->pos;
+                    }
                 }
-            }
         }
     }
+    exchange_ready = true;
+}
+
+// simcore_exchange.d:96-135 -> full_face_copy.d:1888-1900 (same process): every ghost
+// cell of an exchange face takes its source cell's flow state.
+void Simulation::exchange_ghost_cell_boundary_data() {
+    if (!exchange_ready) set_up_exchange();
+    for (auto b : blocks)
+        for (int face = 0; face < 6; ++face) {
+            BoundaryCondition& B = b->bc[face];
+            if (B.kind != BCKind::exchange) continue;
+            for (size_t i = 0; i < B.ghost_cells.size(); ++i) B.ghost_cells[i]->fs = B.mapped_cells[i]->fs;
+        }
 }
 
 // One stage of simcore_gasdynamic_step.d:946-1545 for the pc scheme, without retries.

@@ -193,8 +193,10 @@ struct BoundaryCondition {
     FlowState inflow;                 // for inflow_supersonic
     std::vector<FVInterface*> faces;
     std::vector<int> outsigns;
-    // exchange: other block and face (same-process full-face copy)
-    int other_block = -1, other_face = -1;
+    // exchange: other block, face and orientation (same-process full-face copy), and the
+    // cell mapping built once all blocks exist (full_face_copy.d:67-125, 696-1480)
+    int other_block = -1, other_face = -1, other_orientation = 0;
+    std::vector<FVCell*> ghost_cells, mapped_cells;
 };
 
 // src/eilmer/conservedquantities.d:147-193 (transient ordering)
@@ -270,6 +272,11 @@ private:
     FVCell* new_ghost();
 };
 
+// Source cell (i, j, k) of ghost layer n at in-face position (a, b) of face dface,
+// exchanged from face sface of a block of snic x snjc x snkc cells (full_face_copy.d).
+void ffc_source_index(int dims, int dface, int sface, int orient, int a, int b, int n, int snic, int snjc,
+                      int snkc, int& i, int& j, int& k);
+
 // Pure functions exposed for unit tests (same code the block uses).
 void ausmdv(const FlowState& L, const FlowState& R, std::vector<double>& F, const CQI& cqi,
             const GasModel& gm, bool entropy_fix, double factor = 1.0);
@@ -292,9 +299,11 @@ public:
     Config cfg;
     std::vector<SFluidBlock*> blocks;
     SimState st;
+    bool exchange_ready = false;
     explicit Simulation(const Config& c) : cfg(c) { st.dt_global = cfg.dt_init; }
     ~Simulation();
     void determine_time_step_size();
+    void set_up_exchange();   // cell mapping + geometry exchange, once all blocks exist
     void exchange_ghost_cell_boundary_data();
     void gasdynamic_step();   // simcore_gasdynamic_step.d:906-1575 (pc, 2 stages)
     void stage(int stage, double dt);  // one stage, no retry logic (for unit tests)

@@ -38,7 +38,8 @@ def lib():
         L.orc_sim_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                      C.c_int, C.c_int]
         L.orc_sim_destroy.argtypes = [C.c_void_p]
-        L.orc_sim_add_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, ip, dp, ip, ip]
+        L.orc_ffc_source.argtypes = [C.c_int] * 10 + [ip]
+        L.orc_sim_add_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, ip, dp, ip, ip, ip]
         L.orc_block_set_initial.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp]
         L.orc_block_get.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.orc_block_faces.argtypes = [C.c_void_p, C.c_int, dp]
@@ -127,7 +128,8 @@ class Sim:
         inf, infp = _d(inflow)
         ob, obp = _i(case.other_block)
         of, ofp = _i(case.other_face)
-        b = lib().orc_sim_add_block(self.h, case.nic, case.njc, case.nkc, vp, bcp, infp, obp, ofp)
+        oo, oop = _i(case.other_orientation)
+        b = lib().orc_sim_add_block(self.h, case.nic, case.njc, case.nkc, vp, bcp, infp, obp, ofp, oop)
         if b < 0:
             raise RuntimeError(err())
         arrs = [_d(case.p), _d(case.T), _d(case.vel[:, 0]), _d(case.vel[:, 1]), _d(case.vel[:, 2])]
@@ -344,3 +346,10 @@ def chem_step_batch(gas_rec, mech_rec, rho, T, u, p, massf, dt_chem, dt, nthread
                                              att.ctypes.data_as(ip), nthreads)
     return dict(T=T, u=u, p=p, massf=massf, dt_chem=dt_chem, status=status, substeps=sub, attempts=att,
                 failed=failed)
+
+
+def ffc_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc):
+    """(i, j, k) of the full-face-copy source cell of ghost layer n at (a, b) of `face`."""
+    out = (C.c_int * 3)()
+    lib().orc_ffc_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc, C.cast(out, ip))
+    return tuple(out)

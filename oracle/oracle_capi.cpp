@@ -88,10 +88,12 @@ static FlowState make_fs(const GasModel* gm, const double* pTv, const double* ma
     return fs;
 }
 
-// bc_kinds[6]: BCKind values; other_block/other_face used by exchange faces.
+// bc_kinds[6]: BCKind values; other_block/other_face/other_orientation used by exchange
+// faces (other_orientation may be null: orientation 0).
 // inflow: 6 x (5 + nsp) doubles, one row per face (only read for inflow faces).
 int orc_sim_add_block(void* h, int nic, int njc, int nkc, const double* vtx, const int* bc_kinds,
-                      const double* inflow, const int* other_block, const int* other_face) {
+                      const double* inflow, const int* other_block, const int* other_face,
+                      const int* other_orientation) {
     Sim* s = static_cast<Sim*>(h);
     try {
         const Config& c = s->sim->cfg;
@@ -109,6 +111,7 @@ int orc_sim_add_block(void* h, int nic, int njc, int nkc, const double* vtx, con
             if (bcs[b].kind == BCKind::exchange) {
                 bcs[b].other_block = other_block[b];
                 bcs[b].other_face = other_face[b];
+                bcs[b].other_orientation = other_orientation ? other_orientation[b] : 0;
             }
         }
         int id = (int)s->sim->blocks.size();
@@ -118,6 +121,11 @@ int orc_sim_add_block(void* h, int nic, int njc, int nkc, const double* vtx, con
         g_err = e.what();
         return -1;
     }
+}
+
+void orc_ffc_source(int dims, int dface, int sface, int orient, int a, int b, int n, int snic, int snjc, int snkc,
+                    int* ijk) {
+    ffc_source_index(dims, dface, sface, orient, a, b, n, snic, snjc, snkc, ijk[0], ijk[1], ijk[2]);
 }
 
 // Per interior cell: p, T, vx, vy, vz, and massf (ncell x nsp, may be null for nsp == 1).
