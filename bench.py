@@ -368,6 +368,7 @@ def bench_batched(nblk, steps, warmup, flux="ausmdv", log=None):
         "config": {"workload": f"batched uneven blocks {nblk[0]}x{nblk[1]}x{nblk[2]}, edges 16-192, exchange, "
                                f"Mach 2-10 per block, {flux}, pc RK2",
                    "blocks": len(blocks), "cells": ncells, "stage_kernels": tune["best"], "kchunk": tune["kchunk"],
+                   "tile_x": tune.get("tile_x"),
                    "graph": True, "final_dt": st["dt"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "kernel": f"k_fused_batch<3,{1 if flux == 'efm' else 0},*>"
@@ -483,7 +484,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Mach-6 3-D box {nic}x{njc}x{nkc}, ideal air, l2r2+van Albada, {args.flux}, pc RK2",
                    "cells": case.ncells, "flux": args.flux, "stage_kernels": tune["best"],
-                   "kchunk": tune["kchunk"], "final_dt": st["dt"]},
+                   "kchunk": tune["kchunk"], "tile_x": tune.get("tile_x"), "final_dt": st["dt"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
                      "avg_launch_ms": avg_ms, "bytes_per_" + unit_name: bpu,

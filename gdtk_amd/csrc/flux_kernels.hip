@@ -76,27 +76,6 @@ __device__ __forceinline__ void ghost_one(const BlockDev& b, double* fs, int sid
     }
 }
 
-// Ghost-cell effects of one block, one thread per (boundary face, layer).
-__global__ void k_ghost_fill(BlockDev b, double* fs) {
-    const int nsides = (b.dims == 3) ? 6 : 4;
-    int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t nW = (int64_t)b.njc * b.nkc, nS = (int64_t)b.nic * b.nkc, nB = (int64_t)b.nic * b.njc;
-    int64_t counts[6] = {nW, nW, nS, nS, nB, nB};
-    int side = -1;
-    int64_t r = tid;
-    for (int s = 0; s < nsides; ++s) {
-        int64_t c = counts[s] * NGHOST;
-        if (r < c) { side = s; break; }
-        r -= c;
-    }
-    if (side < 0) return;
-    int A, B;
-    side_extents(side, b.nic, b.njc, b.nkc, A, B);
-    const int layer = (int)(r % NGHOST);
-    const int64_t face = r / NGHOST;
-    ghost_one(b, fs, side, (int)(face % A), (int)(face / A), layer);
-}
-
 // Ghost cells of every side of every block in one launch: full-face-copy exchange from the
 // source block's cells (simcore_exchange.d:96-135 -> full_face_copy.d:1888-1900, the
 // same-process copy of the whole flow state) and the other ghost-cell effects.  Each block
@@ -140,10 +119,9 @@ __global__ void k_ghost_batch(const BlockDev* __restrict__ tab, const SideDev* _
 // One thread per face of index direction d: reconstruction + flux calculator, or the
 // simple-outflow boundary flux effect on faces whose BC computes the flux itself.
 template <int DIM, int FC>
-__global__ void k_face_flux(BlockDev b, IdealGasDev gas, int d, double eps) {
+__device__ __forceinline__ void face_flux_one(const BlockDev& b, const IdealGasDev& gas, int d, double eps,
+                                              int64_t f) {
     using C = CQ<DIM>;
-    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= b.nface[d]) return;
     int fi = (int)(f % b.nf[d][0]);
     int64_t rest = f / b.nf[d][0];
     int fj = (int)(rest % b.nf[d][1]);
@@ -181,16 +159,34 @@ __global__ void k_face_flux(BlockDev b, IdealGasDev gas, int d, double eps) {
     for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
 }
 
+// Faces of direction d of every block in one launch.  Each block's faces start at a
+// multiple of the workgroup size in the launch (start[q], q = 0..nblocks), so every
+// workgroup serves one block and reads that block's record with scalar loads.
+template <int DIM, int FC>
+__global__ __launch_bounds__(256, 4) void k_face_flux_batch(const BlockDev* __restrict__ tab,
+                                                         const int64_t* __restrict__ start, int nblocks, int d,
+                                                         double eps, IdealGasDev gas) {
+    const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= w0) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const BlockDev& b = tab[q];
+    const int64_t f = w0 - start[q] + threadIdx.x;
+    if (f >= b.nface[d]) return;
+    face_flux_one<DIM, FC>(b, gas, d, eps, f);
+}
+
 // Per interior cell: time derivatives (fvcell.d:824-854), the pc stage update
 // (simcore_gasdynamic_step.d:1225-1321), decode (fvcell.d:586-821) and the
 // invalid-cell check (fluidblock.d:607-676).
 template <int DIM, int STAGE>
-__global__ void k_cell_update(BlockDev b, IdealGasDev gas, double dt, double low_T, double max_vel, double min_T,
-                              double max_T) {
+__device__ __forceinline__ void cell_update_one(const BlockDev& b, const IdealGasDev& gas, double dt, double low_T,
+                                                double max_vel, double min_T, double max_T, int64_t t) {
     using C = CQ<DIM>;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t ncell = (int64_t)b.nic * b.njc * b.nkc;
-    if (t >= ncell) return;
     int i = (int)(t % b.nic);
     int64_t rest = t / b.nic;
     int j = (int)(rest % b.njc);
@@ -220,6 +216,27 @@ __global__ void k_cell_update(BlockDev b, IdealGasDev gas, double dt, double low
         si[jq] = s;
     }
     update_decode_store<DIM, STAGE>(b, gas, c, si, dt, low_T, max_vel, min_T, max_T);
+}
+
+// Cells of every block in one launch (block ranges aligned as in k_face_flux_batch); dt
+// is read from device memory.
+template <int DIM, int STAGE>
+__global__ __launch_bounds__(256) void k_cell_update_batch(const BlockDev* __restrict__ tab,
+                                                           const int64_t* __restrict__ start, int nblocks,
+                                                           const double* __restrict__ dtp, CellLimits lim,
+                                                           IdealGasDev gas) {
+    const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= w0) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const BlockDev& b = tab[q];
+    const int64_t t = w0 - start[q] + threadIdx.x;
+    if (t >= (int64_t)b.nic * b.njc * b.nkc) return;
+    cell_update_one<DIM, STAGE>(b, gas, *dtp, lim.low_T, lim.max_vel, lim.min_T, lim.max_T, t);
 }
 
 // Cell signal frequency (fvcell.d:975-1058) and per-workgroup reductions for the
@@ -364,51 +381,31 @@ __global__ void k_decode(BlockDev b, IdealGasDev gas, double* U, double* fs, dou
 
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-hipError_t launch_ghost_fill(const BlockDev& b, double* fs, hipStream_t s) {
-    int64_t nW = (int64_t)b.njc * b.nkc, nS = (int64_t)b.nic * b.nkc, nB = (int64_t)b.nic * b.njc;
-    int64_t n = 2 * (nW + nS) * NGHOST + ((b.dims == 3) ? 2 * nB * NGHOST : 0);
-    hipLaunchKernelGGL(k_ghost_fill, dim3(nblk(n, 256)), dim3(256), 0, s, b, fs);
-    return hipGetLastError();
-}
-
-hipError_t launch_face_flux(const BlockDev& b, const IdealGasDev& gas, int flux_calc, double eps, hipStream_t s) {
-    int nd = (b.dims == 3) ? 3 : 2;
-    for (int d = 0; d < nd; ++d) {
-        dim3 grid(nblk(b.nface[d], 256)), blk(256);
-        if (b.dims == 3) {
-            if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<3, 1>), grid, blk, 0, s, b, gas, d, eps);
-            else hipLaunchKernelGGL((k_face_flux<3, 0>), grid, blk, 0, s, b, gas, d, eps);
-        } else {
-            if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<2, 1>), grid, blk, 0, s, b, gas, d, eps);
-            else hipLaunchKernelGGL((k_face_flux<2, 0>), grid, blk, 0, s, b, gas, d, eps);
-        }
+hipError_t launch_face_flux_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
+                                 int flux_calc, double eps, int d, const IdealGasDev& gas, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    dim3 grid(nblk(total, 256)), blk(256);
+    if (dims == 3) {
+        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux_batch<3, 1>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
+        else hipLaunchKernelGGL((k_face_flux_batch<3, 0>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
+    } else {
+        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux_batch<2, 1>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
+        else hipLaunchKernelGGL((k_face_flux_batch<2, 0>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_face_flux_dir(const BlockDev& b, const IdealGasDev& gas, int flux_calc, double eps, int d,
-                                hipStream_t s) {
-    dim3 grid(nblk(b.nface[d], 256)), blk(256);
-    if (b.dims == 3) {
-        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<3, 1>), grid, blk, 0, s, b, gas, d, eps);
-        else hipLaunchKernelGGL((k_face_flux<3, 0>), grid, blk, 0, s, b, gas, d, eps);
+hipError_t launch_cell_update_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
+                                   int stage, const double* dtp, const CellLimits& lim, const IdealGasDev& gas,
+                                   hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    dim3 grid(nblk(total, 256)), blk(256);
+    if (dims == 3) {
+        if (stage == 1) hipLaunchKernelGGL((k_cell_update_batch<3, 1>), grid, blk, 0, s, tab, start, nblocks, dtp, lim, gas);
+        else hipLaunchKernelGGL((k_cell_update_batch<3, 2>), grid, blk, 0, s, tab, start, nblocks, dtp, lim, gas);
     } else {
-        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux<2, 1>), grid, blk, 0, s, b, gas, d, eps);
-        else hipLaunchKernelGGL((k_face_flux<2, 0>), grid, blk, 0, s, b, gas, d, eps);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_cell_update(const BlockDev& b, const IdealGasDev& gas, int stage, double dt, const CellLimits& lim,
-                              hipStream_t s) {
-    int64_t n = (int64_t)b.nic * b.njc * b.nkc;
-    dim3 grid(nblk(n, 256)), blk(256);
-    if (b.dims == 3) {
-        if (stage == 1) hipLaunchKernelGGL((k_cell_update<3, 1>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
-        else hipLaunchKernelGGL((k_cell_update<3, 2>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
-    } else {
-        if (stage == 1) hipLaunchKernelGGL((k_cell_update<2, 1>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
-        else hipLaunchKernelGGL((k_cell_update<2, 2>), grid, blk, 0, s, b, gas, dt, lim.low_T, lim.max_vel, lim.min_T, lim.max_T);
+        if (stage == 1) hipLaunchKernelGGL((k_cell_update_batch<2, 1>), grid, blk, 0, s, tab, start, nblocks, dtp, lim, gas);
+        else hipLaunchKernelGGL((k_cell_update_batch<2, 2>), grid, blk, 0, s, tab, start, nblocks, dtp, lim, gas);
     }
     return hipGetLastError();
 }

@@ -1,15 +1,17 @@
 // fused_stage.hip -- one kernel per predictor-corrector stage: reconstruction, fluxes,
 // time derivatives, stage update, decode and validity check of every cell of a block.
 //
-// Mapping (gfx950, 64-wide waves): a workgroup of 64 x 4 threads owns a tile of 64 (i) x
-// 4 (j) cells and marches up a chunk of KC planes in k (2.5-D blocking).  For each plane
+// Mapping (gfx950, 64-wide waves): a workgroup of 256 threads owns a tile of TX (i) x TY
+// (j) cells, TX * TY = 256 (64 x 4, 32 x 8 or 16 x 16, chosen by the autotuner: narrow
+// tiles waste fewer lanes on blocks whose i extent is not a multiple of 64 and compute
+// fewer tile-boundary faces), and marches up a chunk of KC planes in k (2.5-D blocking).  For each plane
 // the reconstruction inputs of the tile plus a two-cell halo in i and j are staged in LDS
 // (coalesced along i); the k-direction stencil of a thread's own column is read through
 // the caches (each plane arrives from HBM once and is reused from L1/L2 by the next
 // iterations).  Each thread computes the west (i), south (j) and top (k) face of its
 // cell, all with one copy of the face code in a non-unrolled job loop to keep register
-// pressure low; the tile's east and north boundary faces are computed by lanes of waves 0
-// and 1.  F*A of every i/j face goes to LDS as soon as it is computed and each cell sums
+// pressure low; the tile's east and north boundary faces are computed by the first
+// TY + TX threads.  F*A of every i/j face goes to LDS as soon as it is computed and each cell sums
 // its six faces in the reference order W, E, S, N, B, T (fvcell.d:835-845), so results
 // match the face-per-thread path bit for bit.  The top-face flux of plane k is the
 // bottom-face flux of plane k+1 (registers).
@@ -26,7 +28,9 @@
 
 namespace gdtk {
 
-constexpr int FT_X = 64, FT_Y = 4, FH_X = FT_X + 4, FH_Y = FT_Y + 4;
+// Tile shapes: TX x TY cells with TX * TY = FT_THREADS, one thread per cell; the
+// autotuner picks TX (16, 32 or 64).
+constexpr int FT_THREADS = 256;
 enum : int { SL_RHO = 0, SL_VX, SL_VY, SL_VZ, SL_U, SL_A, SL_N };
 
 #ifndef GDTK_FUSED_WAVES_PER_EU
@@ -57,34 +61,39 @@ __device__ __forceinline__ void face_FA(const BlockDev& b, const IdealGasDev& ga
 // LDS of one tile: the reconstruction inputs of a plane with its halo, and the F*A
 // exchange arrays -- i-faces [FT_Y][FT_X+1][NQ] and j-faces [FT_Y+1][FT_X][NQ], written by
 // the face jobs as they finish so that no flux is held in registers across jobs.
-template <int NQ>
+template <int NQ, int TX>
 struct TileLds {
-    double S[SL_N][FH_Y][FH_X];
-    double FAi[FT_Y * (FT_X + 1) * NQ];
-    double FAj[(FT_Y + 1) * FT_X * NQ];
+    static constexpr int TY = FT_THREADS / TX, HX = TX + 4, HY = TY + 4;
+    double S[SL_N][HY][HX];
+    double FAi[TY * (TX + 1) * NQ];
+    double FAj[(TY + 1) * TX * NQ];
 };
 
-// One tile (64 x 4 cells, k planes [bz*KC, bz*KC + KC)) of one block.
-template <int DIM, int FC, int STAGE>
+// One tile (TX x TY cells, k planes [bz*KC, bz*KC + KC)) of one block.
+template <int DIM, int FC, int STAGE, int TX>
 __device__ __forceinline__ void fused_tile(const BlockDev& b, const IdealGasDev& gas, double dt, double eps,
                                            const CellLimits& lim, int KC, int bx, int by, int bz,
-                                           TileLds<CQ<DIM>::n>& L) {
+                                           TileLds<CQ<DIM>::n, TX>& L) {
     using C = CQ<DIM>;
     constexpr int NQ = C::n;
+    constexpr int FT_X = TX, FT_Y = FT_THREADS / TX, FH_X = FT_X + 4, FH_Y = FT_Y + 4;
     auto& S = L.S;
     double* FAi = L.FAi;
     double* FAj = L.FAj;
 
-    const int ti = threadIdx.x, tj = threadIdx.y;
-    const int tid = tj * FT_X + ti;
+    const int tid = threadIdx.x;
+    const int ti = tid % FT_X, tj = tid / FT_X;
     const int i0 = bx * FT_X, j0 = by * FT_Y;
     const int k0 = (DIM == 3) ? bz * KC : 0;
     const int ncol = min(FT_X, b.nic - i0), nrow = min(FT_Y, b.njc - j0);
     const int kend = (DIM == 3) ? min(k0 + KC, b.nkc) : 1;
     const int i = i0 + ti, j = j0 + tj;
     const bool act = (ti < ncol) && (tj < nrow);
-    const bool doE = (tj == 0) && (ti < nrow);
-    const bool doN = (tj == 1) && (ti < ncol);
+    // The tile's east faces (one per row) go to threads 0..FT_Y-1, its north faces (one
+    // per column) to threads FT_Y..FT_Y+FT_X-1.
+    const bool eastJob = tid < FT_Y;
+    const bool doE = eastJob && (tid < nrow);
+    const bool doN = !eastJob && (tid - FT_Y < ncol) && (tid < FT_Y + FT_X);
     const int64_t np = b.npad;
     const double* fs = b.fs_in;
 
@@ -113,7 +122,7 @@ __device__ __forceinline__ void fused_tile(const BlockDev& b, const IdealGasDev&
             __syncthreads();
         }
         // Face jobs: 0 west face (i-face at i), 1 south face (j-face at j), 2 the tile's
-        // east (wave 0) or north (wave 1) boundary faces, all from LDS; 3 the top face
+        // east or north boundary faces, all from LDS; 3 the top face
         // (k-face at k+1) from the cached flow-state planes.  One copy of the face code.
         double FAt[NQ];
         const int job_lo = pre ? 3 : 0;
@@ -133,13 +142,14 @@ __device__ __forceinline__ void fused_tile(const BlockDev& b, const IdealGasDev&
                 hx0 = ti + 2; hy0 = tj; sy = 1;
                 of = (j == 0 && b.bc_kind[2] == BC_OUTFLOW_SIMPLE_FLUX) ? -1 : 0;
             } else if (job == 2) {
-                if (tj == 0) {
-                    doit = doE; d = 0; fi = i0 + ncol; fj = j0 + ti; fk = k;
-                    hx0 = ncol; hy0 = ti + 2; sx = 1;
+                if (eastJob) {
+                    doit = doE; d = 0; fi = i0 + ncol; fj = j0 + tid; fk = k;
+                    hx0 = ncol; hy0 = tid + 2; sx = 1;
                     of = (fi == b.nic && b.bc_kind[1] == BC_OUTFLOW_SIMPLE_FLUX) ? 1 : 0;
                 } else {
-                    doit = doN; d = 1; fi = i0 + ti; fj = j0 + nrow; fk = k;
-                    hx0 = ti + 2; hy0 = nrow; sy = 1;
+                    const int c = tid - FT_Y;
+                    doit = doN; d = 1; fi = i0 + c; fj = j0 + nrow; fk = k;
+                    hx0 = c + 2; hy0 = nrow; sy = 1;
                     of = (fj == b.njc && b.bc_kind[3] == BC_OUTFLOW_SIMPLE_FLUX) ? 1 : 0;
                 }
             } else {
@@ -217,14 +227,15 @@ __device__ __forceinline__ void fused_tile(const BlockDev& b, const IdealGasDev&
 }
 
 // Persistent stage kernel over every block (see the header comment).
-template <int DIM, int FC, int STAGE>
-__global__ __launch_bounds__(FT_X * FT_Y, GDTK_FUSED_WAVES_PER_EU) void k_fused_batch(
+template <int DIM, int FC, int STAGE, int TX>
+__global__ __launch_bounds__(FT_THREADS, GDTK_FUSED_WAVES_PER_EU) void k_fused_batch(
     StageBatch sb, IdealGasDev gas, const double* __restrict__ dtp, double eps, CellLimits lim) {
-    __shared__ TileLds<CQ<DIM>::n> L;
+    constexpr int TY = FT_THREADS / TX;
+    __shared__ TileLds<CQ<DIM>::n, TX> L;
     __shared__ int s_tile;
     const int xcd = blockIdx.x & 7;
     const double dt = *dtp;
-    const bool leader = (threadIdx.x == 0) && (threadIdx.y == 0);
+    const bool leader = threadIdx.x == 0;
     int q = 0;  // queues tried so far (leader only)
     while (true) {
         if (leader) {
@@ -255,38 +266,53 @@ __global__ __launch_bounds__(FT_X * FT_Y, GDTK_FUSED_WAVES_PER_EU) void k_fused_
         const int blk = __builtin_amdgcn_readfirstlane(lo);
         const BlockDev& b = sb.tab[blk];
         const int lt = t - sb.tile_start[blk];
-        const int gx = (b.nic + FT_X - 1) / FT_X, gy = (b.njc + FT_Y - 1) / FT_Y;
-        fused_tile<DIM, FC, STAGE>(b, gas, dt, eps, lim, (DIM == 3) ? sb.KC : 1, lt % gx, (lt / gx) % gy,
-                                   lt / (gx * gy), L);
+        const int gx = (b.nic + TX - 1) / TX, gy = (b.njc + TY - 1) / TY;
+        fused_tile<DIM, FC, STAGE, TX>(b, gas, dt, eps, lim, (DIM == 3) ? sb.KC : 1, lt % gx, (lt / gx) % gy,
+                                       lt / (gx * gy), L);
         __syncthreads();  // the tile's LDS and s_tile are free again
     }
 }
 
-int fused_tiles(const BlockDev& b, int kc) {
-    const int gx = (b.nic + FT_X - 1) / FT_X, gy = (b.njc + FT_Y - 1) / FT_Y;
+int fused_tiles(const BlockDev& b, int kc, int tx) {
+    const int ty = FT_THREADS / tx;
+    const int gx = (b.nic + tx - 1) / tx, gy = (b.njc + ty - 1) / ty;
     const int gz = (b.dims == 3) ? (b.nkc + kc - 1) / kc : 1;
     return gx * gy * gz;
 }
 
-int fused_batch_grid(int dims, int flux_calc) {
-    int dev = 0, ncu = 0, occ = 0;
+bool fused_tile_x_ok(int tx) { return tx == 16 || tx == 32 || tx == 64; }
+
+template <int DIM, int FC, int TX>
+static int occupancy_of() {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fused_batch<DIM, FC, 2, TX>, FT_THREADS, 0) !=
+            hipSuccess ||
+        occ < 1)
+        occ = 1;
+    return occ;
+}
+
+int fused_batch_grid(int dims, int flux_calc, int tx) {
+    int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    hipError_t e;
-#define GDTK_OCC(D, F) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fused_batch<D, F, 2>, FT_X * FT_Y, 0)
-    if (dims == 3) { if (flux_calc == 1) GDTK_OCC(3, 1); else GDTK_OCC(3, 0); }
-    else { if (flux_calc == 1) GDTK_OCC(2, 1); else GDTK_OCC(2, 0); }
+    int occ = 1;
+#define GDTK_OCC(TXV)                                                                            \
+    occ = (dims == 3) ? (flux_calc == 1 ? occupancy_of<3, 1, TXV>() : occupancy_of<3, 0, TXV>()) \
+                      : (flux_calc == 1 ? occupancy_of<2, 1, TXV>() : occupancy_of<2, 0, TXV>())
+    if (tx == 16) GDTK_OCC(16);
+    else if (tx == 32) GDTK_OCC(32);
+    else GDTK_OCC(64);
 #undef GDTK_OCC
-    if (e != hipSuccess || occ < 1) occ = 1;
     return ncu * occ;
 }
 
-hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flux_calc, const double* dtp,
-                              double eps, const CellLimits& lim, const IdealGasDev& gas, int grid, hipStream_t s) {
-    if (sb.total_tiles <= 0) return hipSuccess;
-    grid = (grid + 7) / 8 * 8;
-    dim3 g(grid), blk(FT_X, FT_Y);
-#define GDTK_FUSED(D, F, ST) hipLaunchKernelGGL((k_fused_batch<D, F, ST>), g, blk, 0, s, sb, gas, dtp, eps, lim)
+template <int TX>
+static void launch_fused_tx(const StageBatch& sb, int dims, int stage, int flux_calc, const double* dtp, double eps,
+                            const CellLimits& lim, const IdealGasDev& gas, dim3 g, hipStream_t s) {
+    dim3 blk(FT_THREADS);
+#define GDTK_FUSED(D, F, ST) \
+    hipLaunchKernelGGL((k_fused_batch<D, F, ST, TX>), g, blk, 0, s, sb, gas, dtp, eps, lim)
     if (dims == 3) {
         if (flux_calc == 1) { if (stage == 1) GDTK_FUSED(3, 1, 1); else GDTK_FUSED(3, 1, 2); }
         else { if (stage == 1) GDTK_FUSED(3, 0, 1); else GDTK_FUSED(3, 0, 2); }
@@ -295,6 +321,18 @@ hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flu
         else { if (stage == 1) GDTK_FUSED(2, 0, 1); else GDTK_FUSED(2, 0, 2); }
     }
 #undef GDTK_FUSED
+}
+
+hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flux_calc, const double* dtp,
+                              double eps, const CellLimits& lim, const IdealGasDev& gas, int grid, int tx,
+                              hipStream_t s) {
+    if (sb.total_tiles <= 0) return hipSuccess;
+    if (!fused_tile_x_ok(tx)) return hipErrorInvalidValue;
+    grid = (grid + 7) / 8 * 8;
+    dim3 g(grid);
+    if (tx == 16) launch_fused_tx<16>(sb, dims, stage, flux_calc, dtp, eps, lim, gas, g, s);
+    else if (tx == 32) launch_fused_tx<32>(sb, dims, stage, flux_calc, dtp, eps, lim, gas, g, s);
+    else launch_fused_tx<64>(sb, dims, stage, flux_calc, dtp, eps, lim, gas, g, s);
     return hipGetLastError();
 }
 

@@ -84,13 +84,18 @@ struct Sim {
     int nsides = 0;
     int64_t nghost = 0;
     DevArr<int> tile_start;
-    int total_tiles = 0, kchunk = 16, tiles_kc = -1;
+    int total_tiles = 0, kchunk = 16, tile_x = 64, tiles_kc = -1, tiles_tx = -1;
     DevArr<unsigned> ctr;
     StageCounters sc;
     DevArr<double> dt_dev;
     int grid = 0, wg_per_cu = 0;
     bool use_graph = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    long long graph_sig[2] = {-1, -1};
+    // Face-per-thread launches over all blocks: first launch index of each block's faces
+    // (per direction) and cells, aligned to the 256-thread workgroup.
+    DevArr<int64_t> fstart[3], cstart;
+    int64_t ftotal[3] = {0, 0, 0}, ctotal = 0;
     // SimState (simcore.d)
     double time = 0.0, dt_global = 0.0, dt_allow = 0.0, cfl_max = 0.0;
     int step = 0, attempts_last = 0;
@@ -144,11 +149,55 @@ struct Sim {
     }
 
     void drop_graphs() {
-        for (auto& g : graph)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
+        for (int q = 0; q < 2; ++q) {
+            if (graph[q]) (void)hipGraphExecDestroy(graph[q]);
+            graph[q] = nullptr;
+            graph_sig[q] = -1;
+        }
+    }
+
+    // What a captured step depends on besides the buffers: the kernel variant and the
+    // fused kernel's launch shape.
+    long long launch_signature() const {
+        return use_fused() ? 1000000LL + 1000LL * kchunk + tile_x + 100000000LL * wg_per_cu : 0LL;
+    }
+
+    void build_starts() {
+        auto aligned = [](int64_t n) { return (n + 255) / 256 * 256; };
+        const size_t nb = blocks.size();
+        const int nd = (cfg.dimensions == 3) ? 3 : 2;
+        for (int d = 0; d < 3; ++d) {
+            std::vector<int64_t> st(nb + 1, 0);
+            if (d < nd)
+                for (size_t q = 0; q < nb; ++q) st[q + 1] = st[q] + aligned(blocks[q]->dev.nface[d]);
+            ftotal[d] = st[nb];
+            fstart[d].alloc(st.size());
+            HIPCHK(hipMemcpy(fstart[d].p, st.data(), st.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        }
+        std::vector<int64_t> st(nb + 1, 0);
+        for (size_t q = 0; q < nb; ++q) st[q + 1] = st[q] + aligned(blocks[q]->ncell());
+        ctotal = st[nb];
+        cstart.alloc(st.size());
+        HIPCHK(hipMemcpy(cstart.p, st.data(), st.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
+
+    // Face-flux arrays of the face-per-thread kernels, allocated the first time they run.
+    void ensure_flux_all() {
+        bool grew = false;
+        for (auto& bp : blocks) {
+            Block& b = *bp;
+            const int nd = (b.dims == 3) ? 3 : 2;
+            for (int q = 0; q < nd; ++q)
+                if (!b.flux[q].p) {
+                    b.flux[q].alloc((size_t)b.dev.nface[q] * b.ncq);
+                    b.dev.flux[q] = b.flux[q].p;
+                    grew = true;
+                }
+        }
+        if (grew) {
+            build_tabs();
+            drop_graphs();
+        }
     }
 
     static int tab_index(int stage, int c, int u) { return ((stage - 1) * 2 + c) * 2 + u; }
@@ -185,14 +234,27 @@ struct Sim {
                 }
     }
 
-    void build_tiles(int kc) {
+    // Tile list of the fused stage launch for k chunks of kc planes and tiles tx cells wide,
+    // and the persistent grid for that kernel.
+    void build_tiles(int kc, int tx) {
         std::vector<int> ts(blocks.size() + 1, 0);
-        for (size_t q = 0; q < blocks.size(); ++q) ts[q + 1] = ts[q] + fused_tiles(blocks[q]->dev, kc);
+        for (size_t q = 0; q < blocks.size(); ++q) {
+            const int64_t t = (int64_t)ts[q] + fused_tiles(blocks[q]->dev, kc, tx);
+            if (t > (int64_t)1 << 30) throw Error(GDTK_ERR_UNSUPPORTED, "too many tiles for one launch");
+            ts[q + 1] = (int)t;
+        }
         total_tiles = ts.back();
         tile_start.alloc(ts.size());
         HIPCHK(hipMemcpy(tile_start.p, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice));
         tiles_kc = kc;
+        tiles_tx = tx;
+        grid = fused_batch_grid(cfg.dimensions, cfg.flux_calculator, tx);
+        if (grid < 8) throw Error(GDTK_ERR_HIP, "cannot size the stage launch");
         drop_graphs();
+    }
+
+    void ensure_tiles() {
+        if (tiles_kc != kchunk || tiles_tx != tile_x) build_tiles(kchunk, tile_x);
     }
 
     void build_tabs() {
@@ -256,9 +318,8 @@ struct Sim {
         dt_dev.alloc(1);
         for (auto& bp : blocks) refresh_dev(*bp);
         build_tabs();
-        build_tiles(kchunk);
-        grid = fused_batch_grid(cfg.dimensions, cfg.flux_calculator);
-        if (grid < 8) throw Error(GDTK_ERR_HIP, "cannot size the stage launch");
+        build_starts();
+        build_tiles(kchunk, tile_x);
         // Geometry exchange: ghost cells of exchange sides take their source cells' lengths.
         if (any_exchange) {
             HIPCHK(launch_ghost_batch(tab(1, 0, 0), sides.p, nsides, nghost, 1, stream));
@@ -276,24 +337,45 @@ struct Sim {
     }
 
     // Batched stage over all blocks: ghost cells (exchange + BCs), then the fused stage
-    // kernel; dt is read from dt_dev.  Counters of the stage must be clear.
+    // kernel, or the face-per-thread kernels (one launch per direction, then the cell
+    // update); dt is read from dt_dev.  Counters of the stage must be clear.
     void launch_batched_stage(int stage, int c, int u) {
         const BlockDev* t = tab(stage, c, u);
         const double nc = (double)total_cells();
+        const double eps = 1.0e-12;  // epsilon_van_albada, globalconfig.d:1079
+        const int nb = (int)blocks.size();
         timed(3, (double)nghost, [&] { return launch_ghost_batch(t, sides.p, nsides, nghost, 0, stream); });
+        if (!use_fused()) {
+            const int nd = (cfg.dimensions == 3) ? 3 : 2;
+            for (int d = 0; d < nd; ++d)
+                timed(0, (double)nfaces_total(d), [&] {
+                    return launch_face_flux_batch(t, fstart[d].p, nb, ftotal[d], cfg.dimensions, cfg.flux_calculator,
+                                                  eps, d, gas, stream);
+                });
+            timed(1, nc, [&] {
+                return launch_cell_update_batch(t, cstart.p, nb, ctotal, cfg.dimensions, stage, dt_dev.p, lim, gas,
+                                                stream);
+            });
+            return;
+        }
         StageBatch sb;
         sb.tab = t;
         sb.tile_start = tile_start.p;
         sb.queue = ctr.p + sc.queue(stage);
-        sb.nblocks = (int)blocks.size();
+        sb.nblocks = nb;
         sb.total_tiles = total_tiles;
         sb.per_xcd = (total_tiles + 7) / 8;
         sb.KC = kchunk;
-        const double eps = 1.0e-12;  // epsilon_van_albada, globalconfig.d:1079
         timed(2, nc, [&] {
             return launch_fused_batch(sb, cfg.dimensions, stage, cfg.flux_calculator, dt_dev.p, eps, lim, gas,
-                                      launch_grid(), stream);
+                                      launch_grid(), tile_x, stream);
         });
+    }
+
+    int64_t nfaces_total(int d) const {
+        int64_t n = 0;
+        for (auto& bp : blocks) n += bp->dev.nface[d];
+        return n;
     }
 
     int64_t total_cells() const {
@@ -319,9 +401,14 @@ struct Sim {
     }
 
     // The whole pc step of all blocks as one captured graph (per U parity): clear the
-    // counters, then ghost cells + fused stage for stage 1 and stage 2.
+    // counters, then ghost cells + stage kernels for stage 1 and stage 2.
     hipGraphExec_t step_graph(int u) {
-        if (graph[u]) return graph[u];
+        const long long sig = launch_signature();
+        if (graph[u] && graph_sig[u] == sig) return graph[u];
+        if (graph[u]) {
+            (void)hipGraphExecDestroy(graph[u]);
+            graph[u] = nullptr;
+        }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         bool saved = profile;
@@ -344,6 +431,7 @@ struct Sim {
         (void)hipGraphDestroy(g);
         HIPCHK(e);
         graph[u] = ex;
+        graph_sig[u] = sig;
         return ex;
     }
 
@@ -522,66 +610,43 @@ struct Sim {
 
     bool use_fused() const { return variant >= 0 ? variant == 1 : cfg.use_fused != 0; }
 
-    void ensure_flux(Block& b) {
-        const int nd = (b.dims == 3) ? 3 : 2;
-        for (int q = 0; q < nd; ++q)
-            if (!b.flux[q].p) {
-                b.flux[q].alloc((size_t)b.dev.nface[q] * b.ncq);
-                b.dev.flux[q] = b.flux[q].p;
-            }
+    // Prepares the launches of the current variant.
+    void prepare_launches() {
+        finalize();
+        if (use_fused()) ensure_tiles();
+        else ensure_flux_all();
     }
 
     // Launches one stage on every block (counters must be clear); returns without
-    // synchronising.  Fused variant: the batched launches over all blocks (dt written to
-    // dt_dev first); face-per-thread variant: per block, ghost fill, face fluxes per
-    // direction and the cell update.
+    // synchronising.  dt is written to dt_dev first.
     void launch_stage(int stage, double dt) {
-        if (use_fused()) {
-            finalize();
-            if (tiles_kc != kchunk) build_tiles(kchunk);
-            HIPCHK(launch_set_f64(dt_dev.p, dt, stream));
-            launch_batched_stage(stage, cur, u0);
-            return;
-        }
-        const double eps = 1.0e-12;  // epsilon_van_albada, globalconfig.d:1079
-        finalize();
-        // Exchange sides read the other blocks' cells: all blocks' ghost cells first.
-        timed(3, (double)nghost, [&] { return launch_ghost_batch(tab(stage, cur, u0), sides.p, nsides, nghost, 0, stream); });
-        for (auto& bp : blocks) {
-            Block& b = *bp;
-            ensure_flux(b);
-            refresh_dev(b);
-            const double nc = (double)b.ncell();
-            const int nd = (b.dims == 3) ? 3 : 2;
-            for (int d = 0; d < nd; ++d)
-                timed(0, (double)b.dev.nface[d],
-                      [&] { return launch_face_flux_dir(b.dev, gas, cfg.flux_calculator, eps, d, stream); });
-            timed(1, nc, [&] { return launch_cell_update(b.dev, gas, stage, dt, lim, stream); });
-        }
+        prepare_launches();
+        HIPCHK(launch_set_f64(dt_dev.p, dt, stream));
+        launch_batched_stage(stage, cur, u0);
     }
 
     void clear_stage_counters(int stage) {
-        if (use_fused()) {
-            finalize();
-            HIPCHK(launch_clear_u32(ctr.p + sc.queue(stage), sc.region(), stream));
-        } else {
-            for (auto& bp : blocks) HIPCHK(hipMemsetAsync(bp->counters, 0, 4 * sizeof(int), stream));
-        }
+        finalize();
+        HIPCHK(launch_clear_u32(ctr.p + sc.queue(stage), sc.region(), stream));
     }
 
     // Autotuner: times both stages of every candidate launch shape on the current flow
     // state (stage 1 reads fs and U0 and writes the other fs copy and dUdt0; stage 2 then
     // reads the same fs and writes fs_out and U[1]: the state the next step starts from,
     // fs[cur] and U[0], is untouched) and keeps the fastest.  Candidates: face-per-thread
-    // kernels, and the fused kernel with k chunks of 4..64 planes.
+    // kernels, and the fused kernel for tiles 64, 32 and 16 cells wide with k chunks of
+    // 8..64 planes (3-D).  Returns {best variant, best k chunk, best ms, best tile width,
+    // then (variant, k chunk, ms, tile width) per candidate}.
     std::vector<double> autotune(double dt, int reps) {
         std::vector<double> result;
-        const int kcs[] = {4, 8, 16, 32, 64};
+        const int kcs[] = {8, 16, 32, 64};
+        const int txs[] = {64, 32, 16};
         bool saved_profile = profile;
         profile = false;
-        auto time_variant = [&](int var, int kc) {
+        auto time_variant = [&](int var, int kc, int tx) {
             variant = var;
             if (kc > 0) kchunk = kc;
+            if (tx > 0) tile_x = tx;
             clear_stage_counters(1);
             launch_stage(1, dt);  // warm-up
             HIPCHK(hipEventRecord(ev0, stream));
@@ -597,25 +662,28 @@ struct Sim {
             HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
             return (double)ms / (2 * reps);
         };
-        double best = time_variant(0, 0);
-        int best_var = 0, best_kc = 0;
-        result.push_back(0); result.push_back(0); result.push_back(best);
-        if (cfg.dimensions == 3) {
-            for (int kc : kcs) {
-                double t = time_variant(1, kc);
-                result.push_back(1); result.push_back(kc); result.push_back(t);
-                if (t < best) { best = t; best_var = 1; best_kc = kc; }
+        double best = time_variant(0, 0, 0);
+        int best_var = 0, best_kc = 0, best_tx = 0;
+        auto add = [&](int var, int kc, int tx, double t) {
+            result.push_back(var); result.push_back(kc); result.push_back(t); result.push_back(tx);
+            if (t < best) { best = t; best_var = var; best_kc = kc; best_tx = tx; }
+        };
+        result.push_back(0); result.push_back(0); result.push_back(best); result.push_back(0);
+        for (int tx : txs) {
+            if (cfg.dimensions == 3) {
+                for (int kc : kcs) add(1, kc, tx, time_variant(1, kc, tx));
+            } else {
+                add(1, 1, tx, time_variant(1, 1, tx));
             }
-        } else {
-            double t = time_variant(1, 1);
-            result.push_back(1); result.push_back(1); result.push_back(t);
-            if (t < best) { best = t; best_var = 1; best_kc = 1; }
         }
         variant = best_var;
-        if (best_var == 1) kchunk = best_kc;
+        if (best_var == 1) {
+            kchunk = best_kc;
+            tile_x = best_tx;
+        }
         HIPCHK(hipStreamSynchronize(stream));
         profile = saved_profile;
-        result.insert(result.begin(), {(double)best_var, (double)best_kc, best});
+        result.insert(result.begin(), {(double)best_var, (double)best_kc, best, (double)best_tx});
         return result;
     }
 
@@ -623,23 +691,10 @@ struct Sim {
     void stage_checked(int stage, double dt, int& n_fail, int& n_invalid) {
         clear_stage_counters(stage);
         launch_stage(stage, dt);
-        n_fail = 0;
-        n_invalid = 0;
-        if (use_fused()) {
-            int nf[3], ni[3];
-            read_counters(stage, stage, nf, ni);
-            n_fail = nf[stage];
-            n_invalid = ni[stage];
-        } else {
-            std::vector<int> cnt(4 * blocks.size());
-            for (size_t q = 0; q < blocks.size(); ++q)
-                HIPCHK(hipMemcpyAsync(&cnt[4 * q], blocks[q]->counters, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            for (size_t q = 0; q < blocks.size(); ++q) {
-                n_fail += cnt[4 * q];
-                n_invalid = std::max(n_invalid, cnt[4 * q + 1]);
-            }
-        }
+        int nf[3], ni[3];
+        read_counters(stage, stage, nf, ni);
+        n_fail = nf[stage];
+        n_invalid = ni[stage];
         // The stage output becomes the current flow state.
         cur = 1 - cur;
         for (auto& bp : blocks) refresh_dev(*bp);
@@ -726,8 +781,7 @@ struct Sim {
     // 2's results irrelevant; the retry restarts from U[0], which the attempt leaves
     // untouched).  Returns true when the attempt failed.
     bool graph_attempt() {
-        finalize();
-        if (tiles_kc != kchunk) build_tiles(kchunk);
+        prepare_launches();
         hipGraphExec_t ex = step_graph(u0);
         HIPCHK(launch_set_f64(dt_dev.p, dt_global, stream));
         HIPCHK(hipGraphLaunch(ex, stream));
@@ -746,7 +800,7 @@ struct Sim {
         double t0 = time;
         int attempt = 0;
         bool failed = false;
-        const bool graphed = use_fused() && use_graph && !profile && cur == 0;
+        const bool graphed = use_graph && !profile && cur == 0;
         do {
             ++attempt;
             failed = false;
@@ -995,6 +1049,11 @@ int gdtk_hip_sim_set_tuning(int sim, int key, int value) {
             if (value < 0 || value > 32) throw Error(GDTK_ERR_ARG, "workgroups per CU out of range");
             s.wg_per_cu = value;
             s.drop_graphs();
+            return GDTK_OK;
+        }
+        if (key == GDTK_TUNE_TILE_X) {
+            if (!fused_tile_x_ok(value)) throw Error(GDTK_ERR_ARG, "tile width must be 16, 32 or 64");
+            s.tile_x = value;
             return GDTK_OK;
         }
         if (key == GDTK_TUNE_VARIANT) {

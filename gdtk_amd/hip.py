@@ -208,6 +208,10 @@ class Simulation:
         """Workgroups per CU of the persistent stage launch (0: occupancy of the kernel)."""
         _check(lib().gdtk_hip_sim_set_tuning(self.h, 3, int(n)))
 
+    def set_tile_x(self, tx):
+        """Tile width of the fused stage kernel: 64 (x 4), 32 (x 8) or 16 (x 16) cells."""
+        _check(lib().gdtk_hip_sim_set_tuning(self.h, 5, int(tx)))
+
     def set_variant(self, variant):
         """-1: from the configuration, 0: face-per-thread kernels, 1: fused stage kernel."""
         _check(lib().gdtk_hip_sim_set_tuning(self.h, 4, int(variant)))
@@ -228,9 +232,10 @@ class Simulation:
         n = lib().gdtk_hip_sim_autotune(self.h, dt, reps, out.ctypes.data_as(dp), 64)
         if n < 0:
             _check(n)
-        cands = [dict(variant="fused" if out[q] == 1 else "face", kchunk=int(out[q + 1]), ms=out[q + 2])
-                 for q in range(3, n, 3)]
-        return dict(best="fused" if out[0] == 1 else "face", kchunk=int(out[1]), ms=out[2], candidates=cands)
+        cands = [dict(variant="fused" if out[q] == 1 else "face", kchunk=int(out[q + 1]), ms=out[q + 2],
+                      tile_x=int(out[q + 3])) for q in range(4, n, 4)]
+        return dict(best="fused" if out[0] == 1 else "face", kchunk=int(out[1]), ms=out[2], tile_x=int(out[3]),
+                    candidates=cands)
 
 
 def exchange_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc):

@@ -135,10 +135,11 @@ int gdtk_hip_sim_profile(int sim, int enable);
 int gdtk_hip_sim_kernel_stats(int sim, double* stats, int nstats);
 
 /* Project autotuner (no reference counterpart): times both stages of each launch-shape
- * candidate (face-per-thread kernels; fused kernel with k chunks 4..64) on the current
- * state, keeps the fastest for the rest of the run and returns the number of values
- * written to out: {best_variant, best_kchunk, best_ms, then (variant, kchunk, ms) per
- * candidate, ms per stage}.  The flow state and U[0] are unchanged. */
+ * candidate (face-per-thread kernels; fused kernel with tiles 64, 32 and 16 cells wide and
+ * k chunks of 8..64 planes) on the current state, keeps the fastest for the rest of the run
+ * and returns the number of values written to out: {best_variant, best_kchunk, best_ms,
+ * best_tile_x, then (variant, kchunk, ms, tile_x) per candidate, ms per stage}.  The flow
+ * state and U[0] are unchanged. */
 int gdtk_hip_sim_autotune(int sim, double dt, int reps, double* out, int nout);
 
 /* Source cell of an exchange ghost cell, as the exchange kernel maps it (the case tables
@@ -154,11 +155,13 @@ int gdtk_hip_exchange_source(int dims, int face, int other_face, int orientation
  * GDTK_TUNE_GRAPH    1 (default): each step's attempt runs as one captured HIP graph
  *                    (both stages of all blocks), 0: launched kernel by kernel;
  * GDTK_TUNE_WG_PER_CU workgroups per CU of the persistent stage launch (0: occupancy);
- * GDTK_TUNE_VARIANT  -1 (from use_fused), 0 face-per-thread kernels, 1 fused kernel. */
+ * GDTK_TUNE_VARIANT  -1 (from use_fused), 0 face-per-thread kernels, 1 fused kernel;
+ * GDTK_TUNE_TILE_X   tile width of the fused kernel: 64 (x 4), 32 (x 8) or 16 (x 16) cells. */
 #define GDTK_TUNE_KCHUNK 1
 #define GDTK_TUNE_GRAPH 2
 #define GDTK_TUNE_WG_PER_CU 3
 #define GDTK_TUNE_VARIANT 4
+#define GDTK_TUNE_TILE_X 5
 int gdtk_hip_sim_set_tuning(int sim, int key, int value);
 
 /* ---------------------------------------------------------------------------------

@@ -105,13 +105,14 @@ def test_graph_and_variants_bitwise():
                 assert np.array_equal(r[q][n], runs[0][q][n]), (q, n)
 
 
-@pytest.mark.parametrize("kc", [4, 16, 64])
-def test_kchunk_and_grid_do_not_change_results(kc):
+@pytest.mark.parametrize("kc,tx", [(4, 64), (16, 32), (64, 16), (8, 16)])
+def test_launch_shape_does_not_change_results(kc, tx):
     blocks = cases.batched_uneven(nblk=(2, 2, 2), lo=8, hi=40, seed=9)
     ref = _gpu(blocks, 3)
     ref.run(3)
     g = _gpu(blocks, 3)
     g.set_kchunk(kc)
+    g.set_tile_x(tx)
     g.set_wg_per_cu(1)
     g.run(3)
     for q in range(len(blocks)):
