@@ -6,7 +6,8 @@
 //   pc(i,j,k) = (i+2) + NI*((j+2) + NJ*(k+2)) for active-cell indices i,j,k (ghost layer n
 //   on the west side sits at i = -1-n, on the east side at i = nic+n).  With ghosts at
 //   those positions every face stencil (L1, L0, R0, R1) is the same index pattern.
-//   Flow state: FS_NBASE field planes of ncell_pad doubles (two copies, ping-pong).
+//   Flow state: nfs field planes of ncell_pad doubles (two copies, ping-pong): rho, vx,
+//   vy, vz, u, p, T, a, and for a multi-species gas massf[nsp] and rho_s[nsp].
 //   Conserved quantities U: ncq planes (two copies, ping-pong as the reference swaps
 //   U[0] and U[2]); dUdt0: ncq planes.
 //   Cell lengths iLength/jLength/kLength: 3 planes (ghosts carry mirrored lengths,
@@ -28,13 +29,18 @@ enum BCKind : int {
 };
 
 constexpr int NGHOST = 2;
+// Flow-state planes: rho, vx, vy, vz, u, p, T, a, then for a multi-species gas the mass
+// fractions and the species densities (nsp planes each).
+constexpr int FS_BASE_PLANES = 8;
+constexpr int FS_MAX_PLANES = FS_BASE_PLANES + 2 * 8;
 
 struct BlockDev {
     int dims, nic, njc, nkc;
     int NI, NJ, NK;
     int64_t npad;                 // NI*NJ*NK
     int ncq;
-    const double* fs_in;          // FS_NBASE planes
+    int nsp, nfs;                 // species (1: single-species gas) and flow-state planes
+    const double* fs_in;          // nfs planes
     double* fs_out;
     const double* U0;             // ncq planes
     double* Unew;
@@ -46,7 +52,7 @@ struct BlockDev {
     int nf[3][3];                 // face-array extents per direction
     int64_t nface[3];
     int bc_kind[6];
-    double inflow[6][8];          // rho, vx, vy, vz, u, p, T, a
+    double inflow[6][FS_MAX_PLANES];  // an inflow face's flow state, plane order
     int* counters;                // [0] decode failures, [1] invalid cells
 };
 

@@ -34,14 +34,12 @@ Chem& get_chem(int h) {
     return *g_chems[h];
 }
 
-// Gas and mechanism records -> the device table, with the checks the reference's input
-// readers make (species indices, segment counts) and the options the kernels implement.
-static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDev& cd) {
+// Gas record -> the gas part of the device table (molecular masses, species gas constants,
+// CEA curves with their end-point values), with the checks of the reference's readers.
+void build_tpg_gas(const gdtk_tpg_gas& g, ChemDev& cd) {
     std::memset(&cd, 0, sizeof(cd));
     if (g.nsp < 1 || g.nsp > CH_MAX_SP) throw Error(GDTK_ERR_ARG, "nsp must be 1..8");
-    if (m.nr < 0 || m.nr > CH_MAX_R) throw Error(GDTK_ERR_ARG, "nr must be 0..32");
     cd.nsp = g.nsp;
-    cd.nr = m.nr;
     for (int i = 0; i < g.nsp; ++i) {
         if (!(g.M[i] > 0.0)) throw Error(GDTK_ERR_ARG, "molecular mass must be positive");
         cd.M[i] = g.M[i];
@@ -67,6 +65,14 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
         c.s_low = cea_s(c, c.T_low);
         c.s_high = cea_s(c, c.T_high);
     }
+}
+
+// Gas and mechanism records -> the device table, with the checks the reference's input
+// readers make (species indices, segment counts) and the options the kernels implement.
+static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDev& cd) {
+    build_tpg_gas(g, cd);
+    if (m.nr < 0 || m.nr > CH_MAX_R) throw Error(GDTK_ERR_ARG, "nr must be 0..32");
+    cd.nr = m.nr;
     for (int ir = 0; ir < m.nr; ++ir) {
         const gdtk_reaction& r = m.r[ir];
         ReacDev& d = cd.r[ir];

@@ -10,6 +10,9 @@
 
 namespace gdtk {
 
+// Gas record -> the gas part of a device table (chem.cpp).
+void build_tpg_gas(const gdtk_tpg_gas& g, ChemDev& cd);
+
 struct Chem {
     ChemDev host{};
     ChemDev* dev = nullptr;

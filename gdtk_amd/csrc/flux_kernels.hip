@@ -57,12 +57,12 @@ __device__ __forceinline__ void ghost_one(const BlockDev& b, double* fs, int sid
     int64_t g = at(g_at);
     const int64_t np = b.npad;
     if (kind == BC_INFLOW_SUPERSONIC) {
-        for (int f = 0; f < FS_NBASE; ++f) fs[g + f * np] = b.inflow[side][f];
+        for (int f = 0; f < b.nfs; ++f) fs[g + f * np] = b.inflow[side][f];
         return;
     }
     if (kind == BC_EXCHANGE) return;
     int64_t s = at(src_at);
-    for (int f = 0; f < FS_NBASE; ++f) fs[g + f * np] = fs[s + f * np];
+    for (int f = 0; f < b.nfs; ++f) fs[g + f * np] = fs[s + f * np];
     if (kind == BC_SLIP_WALL) {
         // ghost_cell.d:33-43
         FaceGeom fg = load_face(b.fgeom[d], b.nface[d], face_index(b, d, fi, fj, fk));
@@ -106,7 +106,7 @@ __global__ void k_ghost_batch(const BlockDev* __restrict__ tab, const SideDev* _
         const int64_t dst = ghost_pidx(b, sd.side, a, bb, layer);
         if (mode == 0) {
             double* fs = const_cast<double*>(b.fs_in);
-            for (int f = 0; f < FS_NBASE; ++f) fs[dst + f * b.npad] = o.fs_in[src + f * o.npad];
+            for (int f = 0; f < b.nfs; ++f) fs[dst + f * b.npad] = o.fs_in[src + f * o.npad];
         } else {
             double* len = const_cast<double*>(b.len);
             for (int f = 0; f < 3; ++f) len[dst + f * b.npad] = o.len[src + f * o.npad];

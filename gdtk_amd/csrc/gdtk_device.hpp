@@ -98,25 +98,30 @@ struct L2R2 {
     }
 };
 
-// A reconstructed face state (single species).
-struct FaceState {
+// A reconstructed face state; massf is used for NSP > 1 (multi-species gas).
+template <int NSP>
+struct FaceStateT {
     double rho, p, T, u, a;
     Vec3d vel;
+    double massf[NSP];
 };
+using FaceState = FaceStateT<1>;
 
-// Conserved-quantity layout for the transient solver (conservedquantities.d:147-193).
-template <int DIM>
+// Conserved-quantity layout for the transient solver (conservedquantities.d:147-193):
+// mass, momentum, total energy, then the species densities of a multi-species gas.
+template <int DIM, int NSP = 1>
 struct CQ {
     static constexpr int mass = 0, xMom = 1, yMom = 2;
     static constexpr int zMom = (DIM == 3) ? 3 : -1;
     static constexpr int totEnergy = (DIM == 3) ? 4 : 3;
-    static constexpr int n = (DIM == 3) ? 5 : 4;
+    static constexpr int species = (DIM == 3) ? 5 : 4;
+    static constexpr int n = ((DIM == 3) ? 5 : 4) + ((NSP > 1) ? NSP : 0);
 };
 
 // fluxcalc.d:474-647, factor == 1, accumulating into F (which starts at zero).
-template <int DIM>
-__device__ __forceinline__ void ausmdv(const FaceState& L, const FaceState& R, double* F) {
-    using C = CQ<DIM>;
+template <int DIM, int NSP = 1>
+__device__ __forceinline__ void ausmdv(const FaceStateT<NSP>& L, const FaceStateT<NSP>& R, double* F) {
+    using C = CQ<DIM, NSP>;
     const double factor = 1.0;
     double rL = L.rho, pL = L.p;
     double pLrL = pL / rL;
@@ -171,6 +176,10 @@ __device__ __forceinline__ void ausmdv(const FaceState& L, const FaceState& R, d
     F[C::yMom] += (ru_half * vU) * factor;
     if (DIM == 3) F[C::zMom] += (ru_half * wU) * factor;
     F[C::totEnergy] += factor * ru_half * HU;
+    if constexpr (NSP > 1) {
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) F[C::species + i] += factor * ru_half * (fromL ? L.massf[i] : R.massf[i]);
+    }
     // Entropy fix (apply_entropy_fix defaults to true, globalconfig.d:1088).
     const double C_EFIX = 0.125;
     bool caseA = ((uL - aL) < 0.0) && ((uR - aR) > 0.0);
@@ -184,6 +193,10 @@ __device__ __forceinline__ void ausmdv(const FaceState& L, const FaceState& R, d
         F[C::yMom] -= factor * d_ua * (rR * vR - rL * vL);
         if (DIM == 3) F[C::zMom] -= factor * d_ua * (rR * wR - rL * wL);
         F[C::totEnergy] -= factor * d_ua * (rR * HR - rL * HL);
+        if constexpr (NSP > 1) {
+#pragma unroll
+            for (int i = 0; i < NSP; ++i) F[C::species + i] -= factor * d_ua * (rR * R.massf[i] - rL * L.massf[i]);
+        }
     }
 }
 
@@ -211,10 +224,11 @@ __device__ __forceinline__ void exxef(double sn, double& exx, double& ef) {
     ef = copysign(ef1, sn);
 }
 
-// fluxcalc.d:1131-1277, factor == 1.  Cv is the ideal-gas constant Cv for both sides.
-template <int DIM>
-__device__ __forceinline__ void efmflx(const FaceState& L, const FaceState& R, double cvL, double cvR, double* F) {
-    using C = CQ<DIM>;
+// fluxcalc.d:1131-1277, factor == 1.  cvL, cvR: the gas model's Cv of each side.
+template <int DIM, int NSP = 1>
+__device__ __forceinline__ void efmflx(const FaceStateT<NSP>& L, const FaceStateT<NSP>& R, double cvL, double cvR,
+                                       double* F) {
+    using C = CQ<DIM, NSP>;
     const double factor = 1.0;
     const double PHI = 1.0;
     const double dtwspi = 0.282094792;
@@ -263,6 +277,11 @@ __device__ __forceinline__ void efmflx(const FaceState& L, const FaceState& R, d
     if (DIM == 3) F[C::zMom] += factor * (fmsL * vqL + fmsR * vqR);
     F[C::totEnergy] += factor * ((wL * rhoL * vnL) * (hvsqL + hL) + (wR * rhoR * vnR) * (hvsqR + hR) +
                                  (dL * cmpL * rhoL) * (hvsqL + con * rtL) + (dR * cmpR * rhoR) * (hvsqR + con * rtR));
+    if constexpr (NSP > 1) {
+        const bool fromL = mass_flux > 0.0;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) F[C::species + i] += mass_flux * (fromL ? L.massf[i] : R.massf[i]);
+    }
 }
 
 // One stencil cell as read by the reconstruction.
@@ -331,11 +350,12 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
     if (DIM == 3) F[C::zMom] = m.z;
 }
 
-// boundary_flux_effect.d:586-645, the simple-outflow flux from the interior cell.
-template <int DIM>
+// boundary_flux_effect.d:586-645, the simple-outflow flux from the interior cell (massf:
+// its mass fractions, read for NSP > 1).
+template <int DIM, int NSP = 1>
 __device__ __forceinline__ void outflow_flux(const FaceGeom& g, int outsign, double rho, double u, double p, Vec3d vel,
-                                             double* F) {
-    using C = CQ<DIM>;
+                                             double* F, const double* massf = nullptr) {
+    using C = CQ<DIM, NSP>;
     double vn = vel.x * g.n.x + vel.y * g.n.y + vel.z * g.n.z;
     double mass_flux = rho * vn;
     if ((outsign * mass_flux) > 0.0) {
@@ -345,6 +365,10 @@ __device__ __forceinline__ void outflow_flux(const FaceGeom& g, int outsign, dou
         if (DIM == 3) F[C::zMom] = p * g.n.z + vel.z * mass_flux;
         double utot = u + 0.5 * (vel.x * vel.x + vel.y * vel.y + vel.z * vel.z);
         F[C::totEnergy] = mass_flux * utot + p * vn;
+        if constexpr (NSP > 1) {
+#pragma unroll
+            for (int i = 0; i < NSP; ++i) F[C::species + i] = mass_flux * massf[i];
+        }
     } else {
         F[C::mass] = 0.0;
         F[C::xMom] = g.n.x * p;
@@ -352,6 +376,10 @@ __device__ __forceinline__ void outflow_flux(const FaceGeom& g, int outsign, dou
         if (DIM == 3) F[C::zMom] = g.n.z * p;
         double gdotn = 0.0 * g.n.x + 0.0 * g.n.y + 0.0 * g.n.z;
         F[C::totEnergy] = p * gdotn;
+        if constexpr (NSP > 1) {
+#pragma unroll
+            for (int i = 0; i < NSP; ++i) F[C::species + i] = 0.0;
+        }
     }
 }
 

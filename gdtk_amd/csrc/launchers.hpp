@@ -29,6 +29,18 @@ hipError_t launch_face_flux_batch(const BlockDev* tab, const int64_t* start, int
 hipError_t launch_cell_update_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
                                    int stage, const double* dtp, const CellLimits& lim, const IdealGasDev& gas,
                                    hipStream_t s);
+// Multi-species thermally perfect gas (flux_species.hip), nsp = 2 or 5: the same batched
+// face and cell-update launches, and one block's encode/decode at start-up.
+struct ChemDev;
+bool species_supported(int nsp);
+hipError_t launch_face_flux_species(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
+                                   int flux_calc, double eps, int d, int nsp, const ChemDev* cd, hipStream_t s);
+hipError_t launch_cell_update_species(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
+                                     int stage, const double* dtp, const CellLimits& lim, int nsp, const ChemDev* cd,
+                                     hipStream_t s);
+hipError_t launch_encode_species(const BlockDev& b, double* U, hipStream_t s);
+hipError_t launch_decode_species(const BlockDev& b, const ChemDev* cd, double* U, double* fs, double low_T,
+                                 hipStream_t s);
 // Ghost cells of all blocks in one launch (exchange and the other ghost-cell effects);
 // mode 1 exchanges the cell lengths of the exchange sides instead.
 hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsides, int64_t total, int mode,

@@ -23,6 +23,8 @@
 #include "gdtk_device.hpp"
 #include "geometry.hpp"
 #include "host_common.hpp"
+#include "chem_device.hpp"
+#include "chem_host.hpp"
 #include "launchers.hpp"
 
 namespace gdtk {
@@ -48,7 +50,7 @@ struct Block {
     gdtk_block_desc desc;
     int dims, nic, njc, nkc, ncq;
     BlockDev dev{};
-    DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], dtpart, dtout;
+    DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], dtpart, dtout, fs_save;
     int* counters = nullptr;
     ~Block() {
         if (counters) (void)hipFree(counters);
@@ -70,6 +72,11 @@ struct Sim {
     gdtk_sim_config cfg;
     IdealGasDev gas;
     CellLimits lim;
+    // Multi-species thermally perfect gas (gdtk_hip_sim_set_gas_tpg): species count and the
+    // gas table (species gas constants, CEA curves) on host and device.
+    int nsp = 1;
+    ChemDev tpg{};
+    DevArr<ChemDev> tpg_dev;
     std::vector<std::unique_ptr<Block>> blocks;
     hipStream_t stream = nullptr;
     // Which flow-state copy is the input of the next stage, and which U copy holds U[0]
@@ -136,7 +143,39 @@ struct Sim {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    int ncq() const { return cfg.dimensions == 3 ? 5 : 4; }
+    int ncq() const { return (cfg.dimensions == 3 ? 5 : 4) + (nsp > 1 ? nsp : 0); }
+    int nfs() const { return FS_BASE_PLANES + (nsp > 1 ? 2 * nsp : 0); }
+
+    // FlowState(p, T, vel, massf) of the thermally perfect mixture: update_thermo_from_pT
+    // and update_sound_speed (therm_perf_gas.d:230-234, 394-404; perf_gas_mix_eos.d:43-76),
+    // as one row of flow-state planes.
+    void tpg_state_pT(double p, double T, const double* vel, const double* massf, double* row) const {
+        double Rm = 0.0, u = 0.0, Cp = 0.0, Cv = 0.0;
+        for (int i = 0; i < nsp; ++i) Rm += massf[i] * tpg.R[i];
+        const double rho = p / (Rm * T);
+        for (int i = 0; i < nsp; ++i) u += massf[i] * (cea_h(tpg.cea[i], T) - tpg.R[i] * T);
+        for (int i = 0; i < nsp; ++i) Cp += massf[i] * cea_Cp(tpg.cea[i], T);
+        for (int i = 0; i < nsp; ++i) Cv += massf[i] * (cea_Cp(tpg.cea[i], T) - tpg.R[i]);
+        const double gamma = Cp / Cv;
+        double Rs = 0.0;
+        for (int i = 0; i < nsp; ++i) Rs += massf[i] * tpg.R[i];
+        const double a = std::sqrt(gamma * (Rs * T));
+        const double base[8] = {rho, vel[0], vel[1], vel[2], u, p, T, a};
+        for (int f = 0; f < FS_BASE_PLANES; ++f) row[f] = base[f];
+        for (int i = 0; i < nsp; ++i) {
+            row[FS_BASE_PLANES + i] = massf[i];
+            row[FS_BASE_PLANES + nsp + i] = rho * massf[i];
+        }
+    }
+
+    void set_gas_tpg(const gdtk_tpg_gas& g) {
+        if (!blocks.empty()) throw Error(GDTK_ERR_ARG, "the gas must be set before blocks are added");
+        if (!species_supported(g.nsp)) throw Error(GDTK_ERR_UNSUPPORTED, "multi-species flux path: 2 or 5 species");
+        build_tpg_gas(g, tpg);
+        nsp = g.nsp;
+        tpg_dev.alloc(1);
+        HIPCHK(hipMemcpy(tpg_dev.p, &tpg, sizeof(ChemDev), hipMemcpyHostToDevice));
+    }
 
     void refresh_dev(Block& b) {
         BlockDev& d = b.dev;
@@ -349,10 +388,16 @@ struct Sim {
             const int nd = (cfg.dimensions == 3) ? 3 : 2;
             for (int d = 0; d < nd; ++d)
                 timed(0, (double)nfaces_total(d), [&] {
+                    if (nsp > 1)
+                        return launch_face_flux_species(t, fstart[d].p, nb, ftotal[d], cfg.dimensions,
+                                                        cfg.flux_calculator, eps, d, nsp, tpg_dev.p, stream);
                     return launch_face_flux_batch(t, fstart[d].p, nb, ftotal[d], cfg.dimensions, cfg.flux_calculator,
                                                   eps, d, gas, stream);
                 });
             timed(1, nc, [&] {
+                if (nsp > 1)
+                    return launch_cell_update_species(t, cstart.p, nb, ctotal, cfg.dimensions, stage, dt_dev.p, lim,
+                                                      nsp, tpg_dev.p, stream);
                 return launch_cell_update_batch(t, cstart.p, nb, ctotal, cfg.dimensions, stage, dt_dev.p, lim, gas,
                                                 stream);
             });
@@ -463,8 +508,10 @@ struct Sim {
         d.NK = (cfg.dimensions == 3) ? desc.nkc + 2 * NGHOST : 1;
         d.npad = (int64_t)d.NI * d.NJ * d.NK;
         d.ncq = b->ncq;
+        d.nsp = nsp;
+        d.nfs = nfs();
         const size_t np = (size_t)d.npad;
-        for (int q = 0; q < 2; ++q) b->fs[q].alloc(np * FS_NBASE);
+        for (int q = 0; q < 2; ++q) b->fs[q].alloc(np * (size_t)d.nfs);
         for (int q = 0; q < 2; ++q) b->U[q].alloc(np * b->ncq);
         b->dUdt0.alloc(np * b->ncq);
         // Padded lengths and volume, ghosts mirrored (sfluidblock.d:884-1000).
@@ -530,6 +577,10 @@ struct Sim {
                 // (ideal_gas.d:89-97, 136-143).
                 double p = desc.inflow[s][0], T = desc.inflow[s][1];
                 if (!(T > 0.0) || !(p > 0.0)) throw Error(GDTK_ERR_ARG, "inflow state needs positive p and T");
+                if (nsp > 1) {
+                    tpg_state_pT(p, T, &desc.inflow[s][2], desc.inflow_massf[s], d.inflow[s]);
+                    continue;
+                }
                 double rho = p / (T * gas.R);
                 double u = gas.Cv * T;
                 double a = std::sqrt(gas.gamma * gas.R * T);
@@ -552,12 +603,19 @@ struct Sim {
         return *blocks[id];
     }
 
-    void set_initial(int bid, const double* p, const double* T, const double* vx, const double* vy, const double* vz) {
+    // Initial flow state of a block's cells (simcore.d:326-330): FlowState(p, T, vel[,
+    // massf]) per cell, U[0] = encode(fs), fs = decode(U[0]).  massf is species-major
+    // [nsp][ncell] for a multi-species gas and ignored otherwise.
+    void set_initial(int bid, const double* p, const double* T, const double* vx, const double* vy, const double* vz,
+                     const double* massf) {
         Block& b = blk(bid);
         BlockDev& d = b.dev;
         const size_t np = (size_t)d.npad;
-        std::vector<double> h(np * FS_NBASE, 0.0);
+        if (nsp > 1 && !massf) throw Error(GDTK_ERR_ARG, "a multi-species gas needs initial mass fractions");
+        std::vector<double> h(np * (size_t)d.nfs, 0.0);
         const int kg = (cfg.dimensions == 3) ? NGHOST : 0;
+        const size_t ncell = (size_t)b.ncell();
+        double row[FS_MAX_PLANES], mf[CH_MAX_SP];
         for (int k = 0; k < b.nkc; ++k)
             for (int j = 0; j < b.njc; ++j)
                 for (int i = 0; i < b.nic; ++i) {
@@ -565,19 +623,31 @@ struct Sim {
                     size_t pc = (size_t)(i + NGHOST) + (size_t)d.NI * ((size_t)(j + NGHOST) + (size_t)d.NJ * (size_t)(k + kg));
                     double pp = p[c], TT = T[c];
                     if (!(TT > 0.0) || !(pp > 0.0)) throw Error(GDTK_ERR_ARG, "initial state needs positive p and T");
-                    // FlowState(p, T, vel) (ideal_gas.d:89-97, 136-143)
-                    double rho = pp / (TT * gas.R);
-                    double u = gas.Cv * TT;
-                    double a = std::sqrt(gas.gamma * gas.R * TT);
-                    double row[8] = {rho, vx[c], vy[c], vz[c], u, pp, TT, a};
-                    for (int f = 0; f < 8; ++f) h[pc + f * np] = row[f];
+                    if (nsp > 1) {
+                        const double vel[3] = {vx[c], vy[c], vz[c]};
+                        for (int q = 0; q < nsp; ++q) mf[q] = massf[q * ncell + c];
+                        tpg_state_pT(pp, TT, vel, mf, row);
+                    } else {
+                        // FlowState(p, T, vel) (ideal_gas.d:89-97, 136-143)
+                        double rho = pp / (TT * gas.R);
+                        double u = gas.Cv * TT;
+                        double a = std::sqrt(gas.gamma * gas.R * TT);
+                        const double base[8] = {rho, vx[c], vy[c], vz[c], u, pp, TT, a};
+                        for (int f = 0; f < 8; ++f) row[f] = base[f];
+                    }
+                    for (int f = 0; f < d.nfs; ++f) h[pc + f * np] = row[f];
                 }
         refresh_dev(b);
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipMemcpy(b.fs[cur].p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemsetAsync(b.counters, 0, 4 * sizeof(int), stream));
-        HIPCHK(launch_encode(b.dev, b.U[u0].p, stream));
-        HIPCHK(launch_decode(b.dev, gas, b.U[u0].p, b.fs[cur].p, lim.low_T, stream));
+        if (nsp > 1) {
+            HIPCHK(launch_encode_species(b.dev, b.U[u0].p, stream));
+            HIPCHK(launch_decode_species(b.dev, tpg_dev.p, b.U[u0].p, b.fs[cur].p, lim.low_T, stream));
+        } else {
+            HIPCHK(launch_encode(b.dev, b.U[u0].p, stream));
+            HIPCHK(launch_decode(b.dev, gas, b.U[u0].p, b.fs[cur].p, lim.low_T, stream));
+        }
         int cnt[4];
         HIPCHK(hipMemcpyAsync(cnt, b.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -591,7 +661,11 @@ struct Sim {
         const double* src = nullptr;
         if (field >= GDTK_FIELD_DUDT0 && field < GDTK_FIELD_DUDT0 + b.ncq) src = b.dUdt0.p + (field - GDTK_FIELD_DUDT0) * np;
         else if (field >= GDTK_FIELD_U0 && field < GDTK_FIELD_U0 + b.ncq) src = b.U[u0].p + (field - GDTK_FIELD_U0) * np;
-        else if (field >= 0 && field < FS_NBASE) src = b.fs[cur].p + field * np;
+        else if (field >= 0 && field < FS_BASE_PLANES) src = b.fs[cur].p + field * np;
+        else if (nsp > 1 && field >= GDTK_FIELD_MASSF && field < GDTK_FIELD_MASSF + nsp)
+            src = b.fs[cur].p + (FS_BASE_PLANES + field - GDTK_FIELD_MASSF) * np;
+        else if (nsp > 1 && field >= GDTK_FIELD_RHO_S && field < GDTK_FIELD_RHO_S + nsp)
+            src = b.fs[cur].p + (FS_BASE_PLANES + nsp + field - GDTK_FIELD_RHO_S) * np;
         else if (field == GDTK_FIELD_VOLUME) src = b.vol.p;
         else if (field >= GDTK_FIELD_ILENGTH && field <= GDTK_FIELD_KLENGTH) src = b.len.p + (field - GDTK_FIELD_ILENGTH) * np;
         else throw Error(GDTK_ERR_ARG, "bad field");
@@ -608,7 +682,12 @@ struct Sim {
                 }
     }
 
-    bool use_fused() const { return variant >= 0 ? variant == 1 : cfg.use_fused != 0; }
+    // The fused stage kernel serves the single-species gas; a multi-species gas runs the
+    // face-per-thread kernels.
+    bool use_fused() const {
+        if (nsp > 1) return false;
+        return variant >= 0 ? variant == 1 : cfg.use_fused != 0;
+    }
 
     // Prepares the launches of the current variant.
     void prepare_launches() {
@@ -670,6 +749,7 @@ struct Sim {
         };
         result.push_back(0); result.push_back(0); result.push_back(best); result.push_back(0);
         for (int tx : txs) {
+            if (nsp > 1) break;  // the fused kernel serves the single-species gas only
             if (cfg.dimensions == 3) {
                 for (int kc : kcs) add(1, kc, tx, time_variant(1, kc, tx));
             } else {
@@ -767,12 +847,35 @@ struct Sim {
 
     // Restart of a failed attempt from U[0]: the flow state is restored by decoding U[0]
     // (the reference retries from whatever state its cells hold).
+    // For a multi-species gas the decode iterates from the cell's temperature, so decoding
+    // U[0] again would not return the same state bit for bit: the flow state the step
+    // started from is kept in fs_save instead and copied back.
     void restore_from_U0() {
+        if (nsp > 1) {
+            for (auto& bp : blocks) {
+                Block& b = *bp;
+                HIPCHK(hipMemcpyAsync(b.fs[0].p, b.fs_save.p, b.fs_save.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                      stream));
+            }
+            cur = 0;
+            for (auto& bp : blocks) refresh_dev(*bp);
+            return;
+        }
         cur = 0;
         for (auto& bp : blocks) {
             Block& b = *bp;
             refresh_dev(b);
             HIPCHK(launch_decode(b.dev, gas, b.U[u0].p, b.fs[0].p, lim.low_T, stream));
+        }
+    }
+
+    void save_step_start() {
+        if (nsp <= 1) return;
+        for (auto& bp : blocks) {
+            Block& b = *bp;
+            if (!b.fs_save.p) b.fs_save.alloc((size_t)b.dev.npad * b.dev.nfs);
+            HIPCHK(hipMemcpyAsync(b.fs_save.p, b.fs[cur].p, b.fs_save.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                  stream));
         }
     }
 
@@ -798,6 +901,7 @@ struct Sim {
     // simcore_gasdynamic_step.d:906-1575 for the pc scheme.
     void gasdynamic_step() {
         double t0 = time;
+        save_step_start();
         int attempt = 0;
         bool failed = false;
         const bool graphed = use_graph && !profile && cur == 0;
@@ -923,7 +1027,24 @@ int gdtk_hip_block_set_initial(int sim, int block, const double* p, const double
                                const double* vy, const double* vz) {
     return guarded([&] {
         if (!p || !T || !vx || !vy || !vz) throw Error(GDTK_ERR_ARG, "null argument");
-        get_sim(sim).set_initial(block, p, T, vx, vy, vz);
+        get_sim(sim).set_initial(block, p, T, vx, vy, vz, nullptr);
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_block_set_initial_massf(int sim, int block, const double* p, const double* T, const double* vx,
+                                     const double* vy, const double* vz, const double* massf) {
+    return guarded([&] {
+        if (!p || !T || !vx || !vy || !vz || !massf) throw Error(GDTK_ERR_ARG, "null argument");
+        get_sim(sim).set_initial(block, p, T, vx, vy, vz, massf);
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_sim_set_gas_tpg(int sim, const gdtk_tpg_gas* gas) {
+    return guarded([&] {
+        if (!gas) throw Error(GDTK_ERR_ARG, "null argument");
+        get_sim(sim).set_gas_tpg(*gas);
         return GDTK_OK;
     });
 }

@@ -31,6 +31,7 @@ EXPORTS = [
     "gdtk_hip_block_get_field", "gdtk_hip_stage", "gdtk_hip_block_dt", "gdtk_hip_sim_run",
     "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
     "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error", "gdtk_hip_exchange_source",
+    "gdtk_hip_sim_set_gas_tpg", "gdtk_hip_block_set_initial_massf",
     "gdtk_hip_chem_create", "gdtk_hip_chem_destroy", "gdtk_hip_chem_step", "gdtk_hip_chem_step_device",
     "gdtk_hip_chem_stats", "gdtk_hip_chem_substeps", "gdtk_hip_chem_set_tuning", "gdtk_hip_chem_autotune",
     "gdtk_hip_tpg_update",
@@ -57,7 +58,8 @@ class SimConfig(C.Structure):
 class BlockDesc(C.Structure):
     _fields_ = [("nic", C.c_int), ("njc", C.c_int), ("nkc", C.c_int), ("vertices", dp),
                 ("bc_kind", C.c_int * 6), ("inflow", (C.c_double * 5) * 6),
-                ("other_block", C.c_int * 6), ("other_face", C.c_int * 6), ("other_orientation", C.c_int * 6)]
+                ("other_block", C.c_int * 6), ("other_face", C.c_int * 6), ("other_orientation", C.c_int * 6),
+                ("inflow_massf", (C.c_double * 8) * 6)]
 
 
 def lib_path() -> str:
@@ -99,6 +101,8 @@ def lib():
         L.gdtk_hip_sim_set_tuning.argtypes = [C.c_int, C.c_int, C.c_int]
         L.gdtk_hip_sim_autotune.argtypes = [C.c_int, C.c_double, C.c_int, dp, C.c_int]
         L.gdtk_hip_exchange_source.argtypes = [C.c_int] * 10 + [ip]
+        L.gdtk_hip_sim_set_gas_tpg.argtypes = [C.c_int, C.c_void_p]
+        L.gdtk_hip_block_set_initial_massf.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -118,15 +122,21 @@ class Simulation:
     """A set of structured blocks advanced by the HIP explicit update."""
 
     def __init__(self, dims, flux="ausmdv", cfl=0.5, dt_init=1.0e-3, dt_max=1.0e-3, cfl_count=10,
-                 max_attempts_for_step=3, max_invalid_cells=0, mMass=0.02896, gamma=1.4, fused=True):
+                 max_attempts_for_step=3, max_invalid_cells=0, mMass=0.02896, gamma=1.4, fused=True, tpg=None):
+        """tpg: a gdtk_amd.chem.TPGGas record for a multi-species thermally perfect gas
+        (replaces the ideal gas given by mMass and gamma)."""
         self.dims = dims
-        self.ncq = 5 if dims == 3 else 4
+        self.nsp = int(tpg.nsp) if tpg is not None else 1
+        self.ncq = (5 if dims == 3 else 4) + (self.nsp if self.nsp > 1 else 0)
         cfg = SimConfig(dims, {"ausmdv": 0, "efm": 1}[flux], 2, cfl, dt_init, dt_max, cfl_count,
                         max_attempts_for_step, max_invalid_cells, mMass, gamma, 1 if fused else 0)
         h = C.c_int(-1)
         _check(lib().gdtk_hip_sim_create(C.byref(cfg), C.byref(h)))
         self.h = h.value
         self.cases = []
+        if tpg is not None:
+            self._tpg = tpg
+            _check(lib().gdtk_hip_sim_set_gas_tpg(self.h, C.byref(tpg)))
 
     def close(self):
         if getattr(self, "h", -1) >= 0 and _lib is not None:
@@ -151,10 +161,17 @@ class Simulation:
             d.other_orientation[s] = int(getattr(case, "other_orientation", [0] * 6)[s])
             for q in range(5):
                 d.inflow[s][q] = float(case.inflow[s][q])
+            if self.nsp > 1 and getattr(case, "inflow_massf", None) is not None:
+                for q in range(self.nsp):
+                    d.inflow_massf[s][q] = float(case.inflow_massf[s][q])
         bid = C.c_int(-1)
         _check(lib().gdtk_hip_sim_add_block(self.h, C.byref(d), C.byref(bid)))
         arrs = [_d(case.p), _d(case.T), _d(case.vel[:, 0]), _d(case.vel[:, 1]), _d(case.vel[:, 2])]
-        _check(lib().gdtk_hip_block_set_initial(self.h, bid.value, *[a[1] for a in arrs]))
+        if self.nsp > 1:
+            mf = _d(np.asarray(case.massf).reshape(-1, self.nsp).T)  # species-major
+            _check(lib().gdtk_hip_block_set_initial_massf(self.h, bid.value, *[a[1] for a in arrs], mf[1]))
+        else:
+            _check(lib().gdtk_hip_block_set_initial(self.h, bid.value, *[a[1] for a in arrs]))
         self.cases.append(case)
         return bid.value
 
@@ -163,6 +180,10 @@ class Simulation:
         out = np.empty(self.cases[b].ncells)
         _check(lib().gdtk_hip_block_get_field(self.h, b, f, out.ctypes.data_as(dp)))
         return out
+
+    def get_massf(self, b):
+        """Mass fractions (ncells, nsp) of a multi-species gas."""
+        return np.stack([self.get(b, 300 + s) for s in range(self.nsp)], axis=1)
 
     def get_U(self, b):
         return np.stack([self.get(b, 100 + j) for j in range(self.ncq)], axis=1)

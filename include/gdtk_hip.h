@@ -28,6 +28,9 @@ extern "C" {
 #define GDTK_ERR_STEP_FAILED (-5)     /* step failed after max_attempts_for_step */
 #define GDTK_ERR_UNSUPPORTED (-6)
 
+/* Largest species count of a gas or mechanism record. */
+#define GDTK_MAX_SPECIES 8
+
 /* Flux calculators, a subset of src/eilmer/globalconfig.d:270-290. */
 #define GDTK_FLUX_AUSMDV 0
 #define GDTK_FLUX_EFM 1
@@ -54,6 +57,8 @@ extern "C" {
 #define GDTK_FIELD_KLENGTH 11
 #define GDTK_FIELD_U0 100   /* + conserved-quantity index: U[0][j] */
 #define GDTK_FIELD_DUDT0 200 /* + conserved-quantity index: dUdt[0][j] */
+#define GDTK_FIELD_MASSF 300 /* + species index: mass fraction (multi-species gas) */
+#define GDTK_FIELD_RHO_S 400 /* + species index: species density (multi-species gas) */
 
 /* Simulation-wide options: the subset of GlobalConfig (src/eilmer/globalconfig.d:936-1294)
  * read on this path.  Gas: ideal gas from mMass and gamma (src/gas/ideal_gas.d:30-72). */
@@ -87,6 +92,8 @@ typedef struct gdtk_block_desc {
     int other_block[6];
     int other_face[6];
     int other_orientation[6];
+    /* Mass fractions of an inflow face's state (multi-species gas only). */
+    double inflow_massf[6][GDTK_MAX_SPECIES];
 } gdtk_block_desc;
 
 /* Replaces the GlobalConfig/SimState set-up of init_simulation (src/eilmer/simcore.d:83-720). */
@@ -101,6 +108,19 @@ int gdtk_hip_sim_add_block(int sim, const gdtk_block_desc* desc, int* block_id);
  * (simcore.d:326-330): FlowState(p, T, vel) per cell, U[0] = encode(fs), fs = decode(U[0]). */
 int gdtk_hip_block_set_initial(int sim, int block, const double* p, const double* T, const double* vx,
                                const double* vy, const double* vz);
+
+/* Multi-species thermally perfect gas for the flux path (ThermallyPerfectGas,
+ * src/gas/therm_perf_gas.d, with its mixture EOS): replaces the ideal gas of the
+ * configuration; call before adding blocks.  The conserved quantities then carry the
+ * species densities (conservedquantities.d:147-193), the reconstruction interpolates them
+ * (onedinterp.d:855-895) and decode/encode go through the mixture's Newton T(u).  2 or 5
+ * species; the face-per-thread stage kernels serve this gas. */
+int gdtk_hip_sim_set_gas_tpg(int sim, const struct gdtk_tpg_gas* gas);
+
+/* gdtk_hip_block_set_initial with mass fractions, species-major massf[isp*ncell + cell]:
+ * FlowState(p, T, vel, massf) per cell (multi-species gas). */
+int gdtk_hip_block_set_initial_massf(int sim, int block, const double* p, const double* T, const double* vx,
+                                     const double* vy, const double* vz, const double* massf);
 
 /* Copies one field of all active cells to host memory (GDTK_FIELD_*). */
 int gdtk_hip_block_get_field(int sim, int block, int field, double* out);
@@ -173,7 +193,6 @@ int gdtk_hip_sim_set_tuning(int sim, int key, int value);
  * anonymous-collider reactions with Arrhenius or equilibrium-constant rates
  * (src/kinetics/reaction.d, rate_constant.d); ODE step alpha-QSS or RKF
  * (src/kinetics/chemistry_update.d). */
-#define GDTK_MAX_SPECIES 8
 #define GDTK_MAX_REACTIONS 32
 #define GDTK_ODE_ALPHA_QSS 0
 #define GDTK_ODE_RKF 1

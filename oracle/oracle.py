@@ -125,6 +125,8 @@ class Sim:
         inflow[:, :5] = case.inflow
         if nsp == 1:
             inflow[:, 5] = 1.0
+        elif getattr(case, "inflow_massf", None) is not None:
+            inflow[:, 5:] = case.inflow_massf
         inf, infp = _d(inflow)
         ob, obp = _i(case.other_block)
         of, ofp = _i(case.other_face)
@@ -133,7 +135,10 @@ class Sim:
         if b < 0:
             raise RuntimeError(err())
         arrs = [_d(case.p), _d(case.T), _d(case.vel[:, 0]), _d(case.vel[:, 1]), _d(case.vel[:, 2])]
-        rc = lib().orc_block_set_initial(self.h, b, *[a[1] for a in arrs], None)
+        mf = None
+        if nsp > 1:
+            mf = _d(np.asarray(case.massf).reshape(-1, nsp))  # cell-major
+        rc = lib().orc_block_set_initial(self.h, b, *[a[1] for a in arrs], mf[1] if mf is not None else None)
         if rc != 0:
             raise RuntimeError(err())
         self.blocks.append(case)
