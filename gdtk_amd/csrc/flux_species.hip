@@ -172,3 +172,90 @@ hipError_t launch_decode_species(const BlockDev& b, const ChemDev* cd, double* U
 }
 
 }  // namespace gdtk
+
+namespace gdtk {
+
+// Operator-split chemistry of a simulation (simcore.d:1619-1638): the cells of every block
+// gathered into dense arrays for the batched reactor, and the results written back as
+// thermochemical_increment leaves them (fvcell.d:858-972): T, u, p and the mass fractions
+// of the flow state, and U[0]'s species densities rho * massf.  The sound speed and the
+// cell's species densities are left as they are, as in the reference.
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void k_cells_chem(const BlockDev* __restrict__ tab, const int64_t* __restrict__ start,
+                                                    const int64_t* __restrict__ coff, int nblocks, int nsp, int64_t n,
+                                                    double* rho, double* T, double* u, double* p, double* massf,
+                                                    const double* T_o, const double* u_o, const double* p_o,
+                                                    const double* massf_o, const double* dtc_o, double* dtc,
+                                                    const int* status, unsigned* nfail) {
+    const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= w0) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const BlockDev& b = tab[q];
+    const int64_t t = w0 - start[q] + threadIdx.x;
+    if (t >= (int64_t)b.nic * b.njc * b.nkc) return;
+    const int i = (int)(t % b.nic);
+    const int64_t rest = t / b.nic;
+    const int64_t c = pidx(b, i, (int)(rest % b.njc), (int)(rest / b.njc));
+    const int64_t np = b.npad;
+    const int64_t x = coff[q] + t;
+    double* fs = const_cast<double*>(b.fs_in);
+    if (!SCATTER) {
+        rho[x] = fs[c + FS_RHO * np];
+        T[x] = fs[c + FS_T * np];
+        u[x] = fs[c + FS_U * np];
+        p[x] = fs[c + FS_P * np];
+        for (int s = 0; s < nsp; ++s) massf[(int64_t)s * n + x] = fs[c + (FS_BASE_PLANES + s) * np];
+        return;
+    }
+    if (status[x] != 0) atomicAdd(nfail, 1u);
+    fs[c + FS_T * np] = T_o[x];
+    fs[c + FS_U * np] = u_o[x];
+    fs[c + FS_P * np] = p_o[x];
+    const double r = fs[c + FS_RHO * np];
+    double* U = const_cast<double*>(b.U0);
+    const int sp0 = (b.dims == 3) ? 5 : 4;
+    for (int s = 0; s < nsp; ++s) {
+        const double y = massf_o[(int64_t)s * n + x];
+        fs[c + (FS_BASE_PLANES + s) * np] = y;
+        U[c + (sp0 + s) * np] = r * y;
+    }
+    dtc[x] = dtc_o[x];
+}
+
+hipError_t launch_cells_gather(const BlockDev* tab, const int64_t* start, const int64_t* coff, int nblocks,
+                               int64_t total, int nsp, int64_t n, double* rho, double* T, double* u, double* p,
+                               double* massf, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cells_chem<false>, dim3(nblk256(total)), dim3(256), 0, s, tab, start, coff, nblocks, nsp, n,
+                       rho, T, u, p, massf, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_scatter(const BlockDev* tab, const int64_t* start, const int64_t* coff, int nblocks,
+                                int64_t total, int nsp, int64_t n, const double* T_o, const double* u_o,
+                                const double* p_o, const double* massf_o, const double* dtc_o, double* dtc,
+                                const int* status, unsigned* nfail, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cells_chem<true>, dim3(nblk256(total)), dim3(256), 0, s, tab, start, coff, nblocks, nsp, n,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, T_o, u_o, p_o, massf_o, dtc_o, dtc, status, nfail);
+    return hipGetLastError();
+}
+
+// Fills n doubles with v.
+__global__ void k_fill_f64(double* p, int64_t n, double v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk256(n)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace gdtk

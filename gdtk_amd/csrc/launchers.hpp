@@ -41,6 +41,17 @@ hipError_t launch_cell_update_species(const BlockDev* tab, const int64_t* start,
 hipError_t launch_encode_species(const BlockDev& b, double* U, hipStream_t s);
 hipError_t launch_decode_species(const BlockDev& b, const ChemDev* cd, double* U, double* fs, double low_T,
                                  hipStream_t s);
+// Operator-split chemistry: every block's cells to dense arrays (rho, T, u, p [n], massf
+// [nsp][n]) and the reactor's results back (flow state and U[0] species densities); the
+// scatter counts failed cells into nfail.  coff[q] = first dense index of block q.
+hipError_t launch_cells_gather(const BlockDev* tab, const int64_t* start, const int64_t* coff, int nblocks,
+                               int64_t total, int nsp, int64_t n, double* rho, double* T, double* u, double* p,
+                               double* massf, hipStream_t s);
+hipError_t launch_cells_scatter(const BlockDev* tab, const int64_t* start, const int64_t* coff, int nblocks,
+                                int64_t total, int nsp, int64_t n, const double* T_o, const double* u_o,
+                                const double* p_o, const double* massf_o, const double* dtc_o, double* dtc,
+                                const int* status, unsigned* nfail, hipStream_t s);
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
 // Ghost cells of all blocks in one launch (exchange and the other ghost-cell effects);
 // mode 1 exchanges the cell lengths of the exchange sides instead.
 hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsides, int64_t total, int mode,

@@ -77,6 +77,17 @@ struct Sim {
     int nsp = 1;
     ChemDev tpg{};
     DevArr<ChemDev> tpg_dev;
+    // Operator-split chemistry (gdtk_hip_sim_set_chemistry): the reactor handle, or a
+    // surrogate handle serving it, dense per-cell arrays and each cell's dt_chem.
+    int chem_h = -1, surr_h = -1;
+    int64_t ncell_all = 0;
+    DevArr<double> cin, cout_, dtc;
+    DevArr<int> cstatus;
+    DevArr<int64_t> coff;
+    DevArr<unsigned> cfail;
+    double chem_ms = 0.0;
+    long long chem_calls = 0, surr_queued = 0;
+    hipEvent_t cev0 = nullptr, cev1 = nullptr;
     std::vector<std::unique_ptr<Block>> blocks;
     hipStream_t stream = nullptr;
     // Which flow-state copy is the input of the next stage, and which U copy holds U[0]
@@ -137,6 +148,8 @@ struct Sim {
 
     ~Sim() {
         drop_graphs();
+        if (cev0) (void)hipEventDestroy(cev0);
+        if (cev1) (void)hipEventDestroy(cev1);
         blocks.clear();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -845,6 +858,71 @@ struct Sim {
         }
     }
 
+    void set_chemistry(int chem, int surrogate) {
+        if (nsp < 2) throw Error(GDTK_ERR_ARG, "chemistry needs a multi-species gas (gdtk_hip_sim_set_gas_tpg)");
+        Chem& c = get_chem(chem);
+        if (c.nsp != nsp) throw Error(GDTK_ERR_ARG, "the chemistry's species differ from the gas's");
+        chem_h = chem;
+        surr_h = surrogate;
+        if (!cev0) {
+            HIPCHK(hipEventCreate(&cev0));
+            HIPCHK(hipEventCreate(&cev1));
+        }
+    }
+
+    // chemistry_step (simcore.d:1619-1638) for the full_T_full_R splitting: every cell's
+    // thermochemical_increment over dt on the device (the batched reactor, or the surrogate
+    // that serves it), dt_chem kept per cell (initially -1, simcore.d:335).  A failed cell
+    // ends the run, as the reference's FlowSolverException does.
+    void chemistry_step(double dt) {
+        if (chem_h < 0) return;
+        finalize();
+        const int nb = (int)blocks.size();
+        if (!ncell_all) {
+            std::vector<int64_t> off(nb + 1, 0);
+            for (int q = 0; q < nb; ++q) off[q + 1] = off[q] + blocks[q]->ncell();
+            ncell_all = off[nb];
+            coff.alloc(off.size());
+            HIPCHK(hipMemcpy(coff.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+            const size_t n = (size_t)ncell_all;
+            cin.alloc((4 + (size_t)nsp) * n);
+            cout_.alloc((4 + (size_t)nsp) * n);
+            dtc.alloc(n);
+            cstatus.alloc(n);
+            cfail.alloc(1);
+            HIPCHK(launch_fill_f64(dtc.p, ncell_all, -1.0, stream));
+        }
+        const int64_t n = ncell_all;
+        double *rho = cin.p, *T = cin.p + n, *u = cin.p + 2 * n, *p = cin.p + 3 * n, *mf = cin.p + 4 * n;
+        double *To = cout_.p, *uo = cout_.p + n, *po = cout_.p + 2 * n, *mfo = cout_.p + 3 * n,
+               *dto = cout_.p + (3 + nsp) * n;
+        const BlockDev* t = tab(1, cur, u0);
+        HIPCHK(hipEventRecord(cev0, stream));
+        HIPCHK(launch_cells_gather(t, cstart.p, coff.p, nb, ctotal, nsp, n, rho, T, u, p, mf, stream));
+        int rc;
+        if (surr_h >= 0) {
+            rc = gdtk_hip_surrogate_step_device(surr_h, (int)n, rho, T, u, p, mf, dtc.p, dt, To, uo, po, mfo, dto,
+                                                cstatus.p, nullptr, stream);
+            if (rc >= 0) surr_queued += rc;
+        } else {
+            rc = gdtk_hip_chem_step_device(chem_h, (int)n, rho, T, u, p, mf, dtc.p, dt, To, uo, po, mfo, dto,
+                                           cstatus.p, stream);
+        }
+        if (rc < 0) throw Error(rc, g_last_error);
+        HIPCHK(launch_clear_u32(cfail.p, 1, stream));
+        HIPCHK(launch_cells_scatter(t, cstart.p, coff.p, nb, ctotal, nsp, n, To, uo, po, mfo, dto, dtc.p, cstatus.p,
+                                    cfail.p, stream));
+        HIPCHK(hipEventRecord(cev1, stream));
+        unsigned nf = 0;
+        HIPCHK(hipMemcpyAsync(&nf, cfail.p, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, cev0, cev1));
+        chem_ms += ms;
+        ++chem_calls;
+        if (nf > 0) throw Error(GDTK_ERR_STEP_FAILED, "The thermochemical_increment() failed.");
+    }
+
     // Restart of a failed attempt from U[0]: the flow state is restored by decoding U[0]
     // (the reference retries from whatever state its cells hold).
     // For a multi-species gas the decode iterates from the cell's temperature, so decoding
@@ -943,6 +1021,7 @@ struct Sim {
         while (true) {
             determine_time_step_size();
             gasdynamic_step();
+            chemistry_step(dt_global);
             step += 1;
             ++taken;
             if (time >= max_time) break;
@@ -1041,6 +1120,23 @@ int gdtk_hip_block_set_initial_massf(int sim, int block, const double* p, const 
     });
 }
 
+int gdtk_hip_sim_set_chemistry(int sim, int chem, int surrogate) {
+    return guarded([&] {
+        get_sim(sim).set_chemistry(chem, surrogate);
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_sim_chem_stats(int sim, double* stats, int nstats) {
+    return guarded([&] {
+        Sim& s = get_sim(sim);
+        const double v[3] = {(double)s.chem_calls, s.chem_ms, (double)s.surr_queued};
+        int m = std::min(nstats, 3);
+        for (int k = 0; k < m; ++k) stats[k] = v[k];
+        return m;
+    });
+}
+
 int gdtk_hip_sim_set_gas_tpg(int sim, const gdtk_tpg_gas* gas) {
     return guarded([&] {
         if (!gas) throw Error(GDTK_ERR_ARG, "null argument");
@@ -1086,6 +1182,7 @@ int gdtk_hip_sim_step_fixed_dt(int sim, double dt, double* state) {
         sp = &get_sim(sim);
         sp->dt_global = dt;
         sp->gasdynamic_step();
+        sp->chemistry_step(sp->dt_global);
         sp->step += 1;
         return GDTK_OK;
     });

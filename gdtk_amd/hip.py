@@ -31,7 +31,8 @@ EXPORTS = [
     "gdtk_hip_block_get_field", "gdtk_hip_stage", "gdtk_hip_block_dt", "gdtk_hip_sim_run",
     "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
     "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error", "gdtk_hip_exchange_source",
-    "gdtk_hip_sim_set_gas_tpg", "gdtk_hip_block_set_initial_massf",
+    "gdtk_hip_sim_set_gas_tpg", "gdtk_hip_block_set_initial_massf", "gdtk_hip_sim_set_chemistry",
+    "gdtk_hip_sim_chem_stats",
     "gdtk_hip_chem_create", "gdtk_hip_chem_destroy", "gdtk_hip_chem_step", "gdtk_hip_chem_step_device",
     "gdtk_hip_chem_stats", "gdtk_hip_chem_substeps", "gdtk_hip_chem_set_tuning", "gdtk_hip_chem_autotune",
     "gdtk_hip_tpg_update",
@@ -103,12 +104,20 @@ def lib():
         L.gdtk_hip_exchange_source.argtypes = [C.c_int] * 10 + [ip]
         L.gdtk_hip_sim_set_gas_tpg.argtypes = [C.c_int, C.c_void_p]
         L.gdtk_hip_block_set_initial_massf.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp]
+        L.gdtk_hip_sim_set_chemistry.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.gdtk_hip_sim_chem_stats.argtypes = [C.c_int, dp, C.c_int]
         _lib = L
     return _lib
 
 
 def _check(rc):
     if rc != GDTK_OK:
+        raise HipError(rc, lib().gdtk_hip_last_error().decode())
+    return rc
+
+
+def _check_n(rc):
+    if rc < 0:
         raise HipError(rc, lib().gdtk_hip_last_error().decode())
     return rc
 
@@ -220,6 +229,17 @@ class Simulation:
 
     def set_kchunk(self, kc):
         _check(lib().gdtk_hip_sim_set_tuning(self.h, 1, int(kc)))
+
+    def set_chemistry(self, chem, surrogate=None):
+        """Operator-split chemistry after every step: a gdtk_amd.chem.Chemistry, optionally
+        served by a gdtk_amd.surrogate.Surrogate built on it."""
+        self._chem = (chem, surrogate)
+        _check(lib().gdtk_hip_sim_set_chemistry(self.h, chem.h, surrogate.h if surrogate is not None else -1))
+
+    def chem_stats(self):
+        out = np.zeros(3)
+        _check_n(lib().gdtk_hip_sim_chem_stats(self.h, out.ctypes.data_as(dp), 3))
+        return dict(steps=int(out[0]), ms=out[1], surrogate_queued=int(out[2]))
 
     def set_graph(self, enable):
         """Run each step's attempt as one captured HIP graph (default) or kernel by kernel."""

@@ -122,6 +122,18 @@ int gdtk_hip_sim_set_gas_tpg(int sim, const struct gdtk_tpg_gas* gas);
 int gdtk_hip_block_set_initial_massf(int sim, int block, const double* p, const double* T, const double* vx,
                                      const double* vy, const double* vz, const double* massf);
 
+/* Operator-split finite-rate chemistry (config.reacting with the default full_T_full_R
+ * splitting, simcore.d:1089-1096): after every gasdynamic step, chemistry_step
+ * (simcore.d:1619-1638) runs FVCell.thermochemical_increment (fvcell.d:858-972) on every
+ * cell with dt_global and the cell's dt_chem (initially -1, simcore.d:335) through the
+ * chemistry handle `chem` (gdtk_hip_chem_create, same species as the gas), or through the
+ * learned surrogate `surrogate` (gdtk_hip_surrogate_create; -1: none).  A failed cell ends
+ * the step with GDTK_ERR_STEP_FAILED. */
+int gdtk_hip_sim_set_chemistry(int sim, int chem, int surrogate);
+/* {chemistry steps, total ms, cells the surrogate sent to the integrator}; returns the
+ * number of values written (<= 3). */
+int gdtk_hip_sim_chem_stats(int sim, double* stats, int nstats);
+
 /* Copies one field of all active cells to host memory (GDTK_FIELD_*). */
 int gdtk_hip_block_get_field(int sim, int block, int field, double* out);
 

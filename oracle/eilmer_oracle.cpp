@@ -20,6 +20,7 @@
 // Bit-level per-step field values of the D code cannot be pinned without a D compiler
 // (none in this image); see DESIGN.md.
 #include "eilmer_oracle.hpp"
+#include "chem_oracle.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -1270,12 +1271,30 @@ void Simulation::gasdynamic_step() {
     st.time = t0 + st.dt_global;
 }
 
+// simcore.d:1619-1638 -> fvcell.d:858-972 for every cell: the reactor over dt with the
+// cell's dt_chem (one retry with dt_chem = -1), the EOS from rho, u, then the species
+// densities of U[0] from the new mass fractions.  The sound speed and the cell's species
+// densities rho_s are not updated (the reference leaves them until the next decode).
+void Simulation::chemistry_step(double dt) {
+    for (auto b : blocks) {
+        const CQI& cqi = b->cqi;
+        for (auto c : b->cells) {
+            if (thermochemical_increment(*chem, *chem_gas, c->fs.gas, dt, c->dt_chem, T_frozen) != 0)
+                throw FlowSolverException("The thermochemical_increment() failed.");
+            if (cqi.n_species > 1)
+                for (int isp = 0; isp < cqi.n_species; ++isp)
+                    c->U[0][cqi.species + isp] = c->fs.gas.rho * c->fs.gas.massf[isp];
+        }
+    }
+}
+
 // simcore.d:1014-1377 (the parts of the loop on this path)
 int Simulation::run(int max_steps, double max_time) {
     int taken = 0;
     while (true) {
         determine_time_step_size();
         gasdynamic_step();
+        if (chem) chemistry_step(st.dt_global);
         st.step += 1;
         ++taken;
         if (st.time >= max_time) break;

@@ -169,6 +169,7 @@ struct FVCell {
     std::vector<int> outsign;
     bool data_is_bad = false;
     double signal = 0.0;
+    double dt_chem = -1.0;        // simcore.d:335 (set_cell_dt_chem(-1.0))
 };
 
 enum class IndexDirection { i, j, k };
@@ -294,12 +295,20 @@ struct SimState {
     int attempts_last_step = 0;
 };
 
+class ChemistryUpdate;
+class ThermallyPerfectGas;
+
 class Simulation {
 public:
     Config cfg;
     std::vector<SFluidBlock*> blocks;
     SimState st;
     bool exchange_ready = false;
+    // Operator-split finite-rate chemistry (config.reacting, full_T_full_R splitting).
+    ChemistryUpdate* chem = nullptr;
+    const ThermallyPerfectGas* chem_gas = nullptr;
+    double T_frozen = 300.0;
+    void chemistry_step(double dt);  // simcore.d:1619-1638
     explicit Simulation(const Config& c) : cfg(c) { st.dt_global = cfg.dt_init; }
     ~Simulation();
     void determine_time_step_size();

@@ -200,6 +200,13 @@ class Sim:
         lib().orc_block_dt(self.h, b, dt_current, cfl, 1 if check_cfl else 0, out.ctypes.data_as(dp))
         return out
 
+    def set_chemistry(self, chem):
+        """Operator-split finite-rate chemistry after every step (an oracle Chem object)."""
+        L = lib()
+        L.orc_sim_set_chemistry.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_sim_set_chemistry(self.h, chem.h)
+        self._chem = chem
+
     def set_dt(self, dt_global, step):
         lib().orc_sim_set_dt(self.h, dt_global, step)
 

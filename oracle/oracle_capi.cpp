@@ -429,6 +429,16 @@ void* orc_chem_create(const void* gasv, const void* mechv) {
     return new OChem{gas, new ChemistryUpdate(gas, defs, cfg), md->T_frozen};
 }
 
+// Operator-split chemistry of a simulation (its gas must be the chemistry's gas).
+int orc_sim_set_chemistry(void* h, void* chem) {
+    Sim* s = static_cast<Sim*>(h);
+    auto* c = static_cast<OChem*>(chem);
+    s->sim->chem = c->chem;
+    s->sim->chem_gas = c->gas;
+    s->sim->T_frozen = c->T_frozen;
+    return 0;
+}
+
 void orc_chem_destroy(void* h) {
     auto* c = static_cast<OChem*>(h);
     delete c->chem;
