@@ -379,6 +379,107 @@ def bench_batched(nblk, steps, warmup, flux="ausmdv", log=None):
     }
 
 
+def _cell_fields(sim, nblocks, names):
+    import numpy as np
+    return {n: np.concatenate([sim.get(q, n) for q in range(nblocks)]) for n in names}
+
+
+def bench_coupled(steps, snapshot_every=250, tols=(5.0e-4, 2.0e-3), log=None):
+    """The coupled 2-D Mach-10 cylinder (5-species air, 8 blocks of 256 x 128 cells): flux
+    steps with operator-split chemistry, first through the FP64 alpha-QSS integrator, then
+    through the learned surrogate trained on (state, dt, result) samples of that run's own
+    trajectory (cell states every `snapshot_every` steps, integrated at 0.5, 1 and 2 times
+    the flow time step).  Reports the cell-update rate of both runs, the chemistry cells/s
+    of each and their ratio, and the surrogate run's shock stand-off, stagnation
+    temperature and final-field max |dY| and T error against the integrator run."""
+    import numpy as np
+
+    from gdtk_amd import Simulation, cases
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, mech_record
+    from gdtk_amd.surrogate import train_surrogate, with_tolerance
+    blocks, rec, info = cases.cylinder_blocks()
+    nb = len(blocks)
+    ncell = sum(b.ncells for b in blocks)
+    gd = gdata.load("tpg-air-5sp")
+    ch = Chemistry(rec, mech_record(gdata.load("park-air-5sp-1T")))
+
+    def make(sur=None):
+        g = Simulation(2, tpg=rec, dt_init=1.0e-10, dt_max=1.0e-6)
+        for b in blocks:
+            g.add_block(b)
+        g.set_chemistry(ch, sur)
+        return g
+
+    def run(g, sample=None):
+        done, el, st = 0, 0.0, None
+        while done < steps:
+            t0 = time.perf_counter()
+            n, st = g.run(min(snapshot_every, steps - done))
+            el += time.perf_counter() - t0
+            done += n
+            if sample is not None:
+                sample.append(_cell_fields(g, nb, ["rho", "T", "u", "p", 300, 301, 302, 303, 304]))
+            if log:
+                log(f"coupled: {done} steps, t = {st['time']:.3e} s, {cases.stagnation_line(g, blocks)}")
+        return el, st
+
+    pool = []
+    gi = make()
+    el_i, st_i = run(gi, pool)
+    ref = _cell_fields(gi, nb, ["T", 300, 301, 302, 303, 304])
+    sl_i = cases.stagnation_line(gi, blocks)
+    cs_i = gi.chem_stats()
+    gi.close()
+    # trajectory samples: every cell of a snapshot that the reactor would integrate
+    # (T above T_frozen = 300 K, free-stream cells included: decoding leaves many of them a
+    # hair above 300 K)
+    cells = {k: [] for k in ("rho", "T", "u", "p", "massf")}
+    for snap in pool:
+        hot = snap["T"] > 300.0
+        for k in ("rho", "T", "u", "p"):
+            cells[k].append(snap[k][hot])
+        cells["massf"].append(np.stack([snap[300 + s][hot] for s in range(5)]))
+    cells = {k: (np.concatenate(v, axis=1) if k == "massf" else np.concatenate(v)) for k, v in cells.items()}
+    nsamp = cells["T"].size
+    cells["massf"] = np.ascontiguousarray(cells["massf"]).reshape(-1)
+    dt_typ = st_i["dt"]
+    sur, tinfo = train_surrogate(ch, gd, cells, (0.5 * dt_typ, dt_typ, 2.0 * dt_typ), log=log)
+    chem_rate_i = ncell * cs_i["steps"] / (cs_i["ms"] * 1e-3)
+    results = []
+    for tol in tols:
+        g = make(with_tolerance(sur, tol))
+        el_s, st_s = run(g)
+        out = _cell_fields(g, nb, ["T", 300, 301, 302, 303, 304])
+        sl_s = cases.stagnation_line(g, blocks)
+        cs_s = g.chem_stats()
+        g.close()
+        dY = max(float(np.max(np.abs(out[300 + s] - ref[300 + s]))) for s in range(5))
+        hot = ref["T"] > 350.0
+        Terr = np.abs(out["T"][hot] - ref["T"][hot]) / ref["T"][hot]
+        chem_rate_s = ncell * cs_s["steps"] / (cs_s["ms"] * 1e-3)
+        results.append({
+            "disagree_tol": tol, "cell_updates_per_s": ncell * steps / el_s, "chem_cells_per_s": chem_rate_s,
+            "chem_speedup_vs_alpha_qss": chem_rate_s / chem_rate_i,
+            "chem_ms_per_step": cs_s["ms"] / cs_s["steps"], "cells_to_integrator_per_step":
+                cs_s["surrogate_queued"] / cs_s["steps"],
+            "standoff_m": sl_s["standoff"], "standoff_rel_diff": sl_s["standoff"] / sl_i["standoff"] - 1.0,
+            "T_stag_K": sl_s["T_stag"], "T_stag_rel_diff": sl_s["T_stag"] / sl_i["T_stag"] - 1.0,
+            "max_abs_dY": dY, "T_rel_err_max": float(Terr.max()), "T_rel_err_p99": float(np.quantile(Terr, 0.99)),
+            "final_time": st_s["time"]})
+    return {
+        "config": {"workload": "2-D Mach-10 cylinder, R = 1 cm, p = 500 Pa, T = 300 K, 5-species air (Park 1-T), "
+                               "8 blocks of 256x128, AUSMDV, pc RK2 + operator-split chemistry",
+                   "cells": ncell, "steps": steps, "final_time": st_i["time"]},
+        "alpha_qss": {"cell_updates_per_s": ncell * steps / el_i, "chem_cells_per_s": chem_rate_i,
+                      "chem_ms_per_step": cs_i["ms"] / cs_i["steps"], "standoff_m": sl_i["standoff"],
+                      "T_stag_K": sl_i["T_stag"]},
+        "surrogate_training": {"samples": int(tinfo["samples"]), "trajectory_states": int(nsamp),
+                               "sample_s": tinfo["sample_s"], "train_s": tinfo["train_s"]},
+        "surrogate": results,
+    }
+
+
 def copy_probe(nbytes=1 << 30):
     """A plain device copy of known size for HBM counter calibration (FETCH_SIZE/WRITE_SIZE)."""
     import torch
@@ -410,6 +511,9 @@ def main():
                     help="blocks per direction of the batched uneven-blocks workload ('0' skips it)")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-only", action="store_true", help="measure only the batched uneven-blocks workload")
+    ap.add_argument("--coupled-steps", type=int, default=5000,
+                    help="steps of the coupled Mach-10 cylinder runs (0 skips them)")
+    ap.add_argument("--coupled-only", action="store_true", help="measure only the coupled cylinder")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -431,6 +535,9 @@ def main():
 
     if args.chem_only:
         print(json.dumps(chem_legs(), default=float), flush=True)
+        return
+    if args.coupled_only:
+        print(json.dumps(bench_coupled(args.coupled_steps, log=log), default=float), flush=True)
         return
     nblk = tuple(int(x) for x in args.batched.split(","))
     if args.batched_only:
@@ -502,6 +609,8 @@ def main():
         out["batched"] = bench_batched(nblk, args.batched_steps, 2, args.flux, log)
     if args.chem_cells > 0:
         out["chem"] = chem_legs()
+    if args.coupled_steps > 0:
+        out["coupled"] = bench_coupled(args.coupled_steps, log=log)
     print(json.dumps(out, default=float), flush=True)
 
 

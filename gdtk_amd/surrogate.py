@@ -199,24 +199,37 @@ def integrator_samples(chem, cells, dt, device):
     return t, out, st
 
 
-def train_air_surrogate(chem, gas_json, n_train=1 << 22, dt=1.0e-7, seed=7, epochs=(8, 6, 4), lrs=(2e-3, 5e-4, 1e-4),
-                        batch=4096, x_abs_max=6.0, disagree_tol=5.0e-4, bf16_aware=False, log=None):
+def train_air_surrogate(chem, gas_json, n_train=1 << 22, dt=1.0e-7, seed=7, **kw):
     """Samples the integrator on n_train cells of the air workload, trains two networks
     (different initialisations and shuffles) and returns (Surrogate, info)."""
+    from .chemcases import air_cells
+    return train_surrogate(chem, gas_json, air_cells(n_train, seed=seed), (dt,), seed=seed, **kw)
+
+
+def train_surrogate(chem, gas_json, cells, dts, seed=7, epochs=(8, 6, 4), lrs=(2e-3, 5e-4, 1e-4), batch=4096,
+                    x_abs_max=6.0, disagree_tol=5.0e-4, bf16_aware=False, log=None):
+    """Trains two networks (different initialisations and shuffles) on the GPU integrator's
+    results for the given cell states (dict rho, T, u, p, massf species-major) over each
+    chemistry interval in dts, and returns (Surrogate, info)."""
     import torch
 
-    from .chemcases import air_cells
     dev = torch.device("cuda:0")
     nsp = chem.nsp
     t0 = time.perf_counter()
-    cells = air_cells(n_train, seed=seed)
-    t_in, t_out, st = integrator_samples(chem, cells, dt, dev)
-    ok = (st == 0) & (t_in["T"] > 300.0)
-    Y0 = t_in["massf"].reshape(nsp, -1)
-    dY = (t_out["massf"].reshape(nsp, -1) - Y0)[:, ok]
-    f = torch.vstack([torch.log(t_in["T"])[None, :], torch.log(t_in["rho"])[None, :], Y0,
-                      torch.full((1, Y0.shape[1]), float(np.log(dt)), dtype=torch.float64, device=dev)])[:, ok]
-    m = int(ok.sum().item())
+    feats, dYs = [], []
+    for dt in dts:
+        t_in, t_out, st = integrator_samples(chem, cells, dt, dev)
+        ok = (st == 0) & (t_in["T"] > 300.0)
+        Y0 = t_in["massf"].reshape(nsp, -1)
+        dYs.append((t_out["massf"].reshape(nsp, -1) - Y0)[:, ok])
+        feats.append(torch.vstack([torch.log(t_in["T"])[None, :], torch.log(t_in["rho"])[None, :], Y0,
+                                   torch.full((1, Y0.shape[1]), float(np.log(dt)), dtype=torch.float64,
+                                              device=dev)])[:, ok])
+        del t_in, t_out, st
+    f = torch.cat(feats, dim=1)
+    dY = torch.cat(dYs, dim=1)
+    del feats, dYs
+    m = int(f.shape[1])
     x_mean = f.mean(dim=1)
     x_std = torch.clamp(f.std(dim=1), min=0.05)
     y_scale = torch.clamp(dY.std(dim=1), min=1e-6)

@@ -139,3 +139,35 @@ def test_reacting_species_box_against_oracle():
     # formed in the blob
     assert np.max(g.get_massf(0)[:, 4] - Y0[:, 4]) > 1e-4
     _check(g, o, 1, "reacting species box")
+
+
+def test_coupled_surrogate_with_zero_tolerance_is_the_integrator():
+    """The coupled cylinder with the surrogate in the chemistry step: at a disagreement
+    tolerance of zero every reacting cell goes to the integrator, so the run equals the
+    run with the integrator alone bit for bit (the surrogate path's wiring: gather, queue,
+    integrator, scatter, per-cell dt_chem)."""
+    from gdtk_amd import Simulation
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, mech_record
+    from gdtk_amd.surrogate import with_tolerance, train_air_surrogate
+    blocks, rec, _ = cases.cylinder_blocks(ni=48, nj_block=12, njb=4)
+    gd = gdata.load("tpg-air-5sp")
+    ch = Chemistry(rec, mech_record(gdata.load("park-air-5sp-1T")))
+    sur, _ = train_air_surrogate(ch, gd, n_train=1 << 14, epochs=(1,), lrs=(1e-3,), batch=2048)
+    sur0 = with_tolerance(sur, 0.0, x_abs_max=1.0e30)
+    out = []
+    for s in (None, sur0):
+        g = Simulation(2, tpg=rec, dt_init=1e-10, dt_max=1e-6)
+        for b in blocks:
+            g.add_block(b)
+        g.set_chemistry(ch, s)
+        g.run(120)
+        out.append([{n: g.get(q, n) for n in ("rho", "T", "p", 300, 304)} for q in range(len(blocks))])
+        cs = g.chem_stats()
+        print("coupled", "surrogate" if s else "integrator", cs, flush=True)
+        if s is not None:
+            assert cs["surrogate_queued"] > 0
+        g.close()
+    for q in range(len(blocks)):
+        for n in out[0][q]:
+            assert np.array_equal(out[0][q][n], out[1][q][n]), (q, n)
