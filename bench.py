@@ -571,7 +571,7 @@ def main():
     kd = ks[dom]
     avg_ms = kd["ms"] / max(kd["launches"], 1)
     if dom == "face_flux":
-        bpu, unit_name, kname = face_bytes_per_face(3), "face", "k_face_flux<3,%d>" % (1 if args.flux == "efm" else 0)
+        bpu, unit_name, kname = face_bytes_per_face(3), "face", "k_face_flux_batch<3,%d>" % (1 if args.flux == "efm" else 0)
     else:
         bpu, unit_name, kname = stage_bytes_per_cell(3), "cell", "k_fused_batch<3,%d,*>" % (1 if args.flux == "efm" else 0)
     achieved = bpu * kd["items_per_launch"] / (avg_ms * 1e-3) / 1e9
