@@ -362,6 +362,64 @@ def species_box(nic=48, njc=24, nsp=5, seed=3, mach=4.0, closed=False):
     return case, gas
 
 
+def coons_patch(south, north, west, east, r, s):
+    """Transfinite interpolation between four boundary curves given as functions of their
+    parameter in [0, 1] (south/north in r, west/east in s); returns (len(s), len(r), 2)
+    vertex positions."""
+    Sr, Nr = south(r)[None, :, :], north(r)[None, :, :]
+    Ws, Es = west(s)[:, None, :], east(s)[:, None, :]
+    R_ = r[None, :, None]
+    S_ = s[:, None, None]
+    p00, p10, p01, p11 = south(np.zeros(1))[0], south(np.ones(1))[0], north(np.zeros(1))[0], north(np.ones(1))[0]
+    return ((1 - S_) * Sr + S_ * Nr + (1 - R_) * Ws + R_ * Es
+            - ((1 - R_) * (1 - S_) * p00 + R_ * (1 - S_) * p10 + (1 - R_) * S_ * p01 + R_ * S_ * p11))
+
+
+def _line(p0, p1):
+    p0, p1 = np.asarray(p0, dtype=float), np.asarray(p1, dtype=float)
+    return lambda t: p0[None, :] + np.asarray(t, dtype=float)[:, None] * (p1 - p0)[None, :]
+
+
+def _polyline(points):
+    """Straight segments through `points`, parameterised by arc length."""
+    P = np.asarray(points, dtype=float)
+    L = np.concatenate([[0.0], np.cumsum(np.hypot(*np.diff(P, axis=0).T))])
+    L /= L[-1]
+
+    def f(t):
+        t = np.asarray(t, dtype=float)
+        return np.stack([np.interp(t, L, P[:, 0]), np.interp(t, L, P[:, 1])], axis=1)
+    return f
+
+
+def wedge_2d(nx=40, ny=40, mach=10.0, theta_deg=6.0, p_inf=1.0e5, T_inf=300.0):
+    """The reference's Mach-10 flow of ideal air over a 6-degree wedge
+    (examples/eilmer/2D/sharp-wedge-6-degrees/sg/wedge.lua:18-93): a 0.1 m run-up along the
+    x axis to the wedge tip at the origin, then the 0.3 m wedge surface; 0.1 m high; a
+    Coons patch of nx x ny cells.  West: supersonic inflow; south: slip wall; north and
+    east: simple outflow (OutFlowBC_Simple is the flux version, bc.lua:1625).  Every cell
+    starts at the inflow state.  The example runs the adaptive flux calculator."""
+    tb = np.tan(np.radians(theta_deg))
+    a, b, c = (-0.1, 0.0), (0.0, 0.0), (0.3, 0.3 * tb)
+    d, e = (-0.1, 0.1), (0.3, 0.1)
+    grid = coons_patch(_polyline([a, b, c]), _line(d, e), _line(a, d), _line(c, e),
+                       np.linspace(0.0, 1.0, nx + 1), np.linspace(0.0, 1.0, ny + 1))
+    v = np.zeros(((nx + 1) * (ny + 1), 3))
+    v[:, 0] = grid[..., 0].reshape(-1)
+    v[:, 1] = grid[..., 1].reshape(-1)
+    R = R_UNIVERSAL / IDEAL_AIR_MMASS
+    u_inf = mach * np.sqrt(IDEAL_AIR_GAMMA * R * T_inf)
+    n = nx * ny
+    vel = np.zeros((n, 3))
+    vel[:, 0] = u_inf
+    case = BlockCase(2, nx, ny, 1, v, np.full(n, p_inf), np.full(n, T_inf), vel,
+                     [BC_INFLOW_SUPERSONIC, BC_OUTFLOW_SIMPLE_FLUX, BC_SLIP_WALL, BC_OUTFLOW_SIMPLE_FLUX,
+                      BC_SLIP_WALL, BC_SLIP_WALL])
+    case.inflow = np.zeros((6, 5))
+    case.inflow[WEST] = [p_inf, T_inf, u_inf, 0.0, 0.0]
+    return case
+
+
 def billig_shock_x(y, R, mach):
     """Billig's shock-shape correlation for a circular cylinder in a gamma = 1.4 free stream
     (Billig, J. Spacecraft Rockets 4 (1967) 822): distance upstream of the cylinder centre

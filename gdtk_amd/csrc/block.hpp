@@ -12,6 +12,7 @@
 //   U[0] and U[2]); dUdt0: ncq planes.
 //   Cell lengths iLength/jLength/kLength: 3 planes (ghosts carry mirrored lengths,
 //   src/eilmer/sfluidblock.d:884-1000); volume: 1 plane.
+//   Shock indicator (adaptive flux calculator): one byte per padded cell.
 //   Face geometry per index direction d: 10 planes (n, t1, t2, area) over
 //   (nic+1)*njc*nkc (d=0), nic*(njc+1)*nkc (d=1), nic*njc*(nkc+1) (d=2), face (fi,fj,fk)
 //   at fi + nfi*(fj + nfj*fk).
@@ -54,6 +55,10 @@ struct BlockDev {
     int bc_kind[6];
     double inflow[6][FS_MAX_PLANES];  // an inflow face's flow state, plane order
     int* counters;                // [0] decode failures, [1] invalid cells
+    // Shock indicator per padded cell (0 or 1; adaptive flux calculator only, else null):
+    // interior cells hold the last detection, ghost cells the copy made by the ghost fill
+    // of the stage that detected.
+    unsigned char* shock;
 };
 
 __host__ __device__ inline int64_t pidx(const BlockDev& b, int i, int j, int k) {

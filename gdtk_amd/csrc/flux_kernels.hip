@@ -27,7 +27,10 @@ __device__ __forceinline__ int64_t ghost_pidx(const BlockDev& b, int side, int a
 // `side`) for the boundary kinds that read the block's own cells:
 // internal_copy_then_reflect.d:111-134, flow_state_copy.d:92-107,
 // extrapolate_copy.d:124-140 (xOrder 0).
-__device__ __forceinline__ void ghost_one(const BlockDev& b, double* fs, int side, int a, int bb, int layer) {
+// With `shock`, the ghost cell also takes its source's shock indicator (0 for an inflow), as
+// the reference's ghost cells copy the whole flow state, fs.S included.
+__device__ __forceinline__ void ghost_one(const BlockDev& b, double* fs, int side, int a, int bb, int layer,
+                                          bool shock) {
     int i0, j0, k0;  // face location in the side
     int d;           // index direction normal to the side
     int fi, fj, fk;  // face index
@@ -58,11 +61,13 @@ __device__ __forceinline__ void ghost_one(const BlockDev& b, double* fs, int sid
     const int64_t np = b.npad;
     if (kind == BC_INFLOW_SUPERSONIC) {
         for (int f = 0; f < b.nfs; ++f) fs[g + f * np] = b.inflow[side][f];
+        if (shock) b.shock[g] = 0;
         return;
     }
     if (kind == BC_EXCHANGE) return;
     int64_t s = at(src_at);
     for (int f = 0; f < b.nfs; ++f) fs[g + f * np] = fs[s + f * np];
+    if (shock) b.shock[g] = b.shock[s];
     if (kind == BC_SLIP_WALL) {
         // ghost_cell.d:33-43
         FaceGeom fg = load_face(b.fgeom[d], b.nface[d], face_index(b, d, fi, fj, fk));
@@ -81,7 +86,8 @@ __device__ __forceinline__ void ghost_one(const BlockDev& b, double* fs, int sid
 // same-process copy of the whole flow state) and the other ghost-cell effects.  Each block
 // record's fs_in is the flow-state copy this stage reads, so the ghost cells are written
 // there.  mode 1 copies the exchange sides' cell lengths instead (the geometry exchange
-// of full_face_copy.d:1576-1625, done once).
+// of full_face_copy.d:1576-1625, done once); mode 2 is mode 0 with the shock indicators
+// (the fill that precedes shock detection).
 __global__ void k_ghost_batch(const BlockDev* __restrict__ tab, const SideDev* __restrict__ sides, int nsides,
                               int64_t total, int mode) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,16 +110,90 @@ __global__ void k_ghost_batch(const BlockDev* __restrict__ tab, const SideDev* _
         ffc_source(b.dims, sd.side, sd.oside, sd.orient, a, bb, layer, o.nic, o.njc, o.nkc, i, j, k);
         const int64_t src = pidx(o, i, j, k);
         const int64_t dst = ghost_pidx(b, sd.side, a, bb, layer);
-        if (mode == 0) {
+        if (mode != 1) {
             double* fs = const_cast<double*>(b.fs_in);
             for (int f = 0; f < b.nfs; ++f) fs[dst + f * b.npad] = o.fs_in[src + f * o.npad];
+            if (mode == 2) b.shock[dst] = o.shock[src];
         } else {
             double* len = const_cast<double*>(b.len);
             for (int f = 0; f < 3; ++f) len[dst + f * b.npad] = o.len[src + f * o.npad];
         }
-    } else if (mode == 0) {
-        ghost_one(b, const_cast<double*>(b.fs_in), sd.side, a, bb, layer);
+    } else if (mode != 1) {
+        ghost_one(b, const_cast<double*>(b.fs_in), sd.side, a, bb, layer, mode == 2);
     }
+}
+
+// PJ shock detector (shockdetectors.d:21-113) for the face between cells cl and cr with
+// face frame g: compression (uR - uL)/a_min and shear max(|vL - vR|, |wL - wR|)/a_mean.
+__device__ __forceinline__ bool pj_shock(const double* fs, int64_t np, int64_t cl, int64_t cr, const FaceGeom& g,
+                                         double comp_tol, double shear_tol) {
+    const Vec3d vl{fs[cl + FS_VX * np], fs[cl + FS_VY * np], fs[cl + FS_VZ * np]};
+    const Vec3d vr{fs[cr + FS_VX * np], fs[cr + FS_VY * np], fs[cr + FS_VZ * np]};
+    const double aL = fs[cl + FS_A * np], aR = fs[cr + FS_A * np];
+    const double uL = vl.x * g.n.x + vl.y * g.n.y + vl.z * g.n.z;
+    const double uR = vr.x * g.n.x + vr.y * g.n.y + vr.z * g.n.z;
+    const double a_min = (aL < aR) ? aL : aR;
+    const double comp = ((uR - uL) / a_min);
+    const double vL = vl.x * g.t1.x + vl.y * g.t1.y + vl.z * g.t1.z;
+    const double vR = vr.x * g.t1.x + vr.y * g.t1.y + vr.z * g.t1.z;
+    const double wL = vl.x * g.t2.x + vl.y * g.t2.y + vl.z * g.t2.z;
+    const double wR = vr.x * g.t2.x + vr.y * g.t2.y + vr.z * g.t2.z;
+    const double sound_speed = 0.5 * (aL + aR);
+    const double shear_y = fabs(vL - vR) / sound_speed;
+    const double shear_z = fabs(wL - wR) / sound_speed;
+    const double shear = fmax(shear_y, shear_z);
+    return (shear < shear_tol) && (comp < comp_tol);
+}
+
+// Shock detection of every interior cell of every block (detect_shock_points then
+// shock_faces_to_cells, fluidblock.d:478-520, 568-580): a cell is shocked if the PJ
+// detector fires on any of its faces, or (damped_outflow) one of its faces lies on an
+// outflow_simple_extrapolate boundary.  The strict rule (fluidblock.d:582-605) is applied
+// where the flux kernels read the indicators: a face is shocked if either adjacent cell is
+// (a face the detector marks has its interior cells marked too).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_shock_detect_batch(const BlockDev* __restrict__ tab,
+                                                            const int64_t* __restrict__ start, int nblocks,
+                                                            double comp_tol, double shear_tol, int damped_outflow) {
+    const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= w0) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const BlockDev& b = tab[q];
+    const int64_t t = w0 - start[q] + threadIdx.x;
+    if (t >= (int64_t)b.nic * b.njc * b.nkc) return;
+    const int i = (int)(t % b.nic);
+    const int64_t rest = t / b.nic;
+    const int j = (int)(rest % b.njc);
+    const int k = (int)(rest / b.njc);
+    const int64_t c = pidx(b, i, j, k);
+    const int64_t np = b.npad;
+    const double* fs = b.fs_in;
+    bool shocked = false;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int pos = (d == 0) ? i : (d == 1) ? j : k;
+        const int nd = (d == 0) ? b.nic : (d == 1) ? b.njc : b.nkc;
+#pragma unroll
+        for (int hiside = 0; hiside < 2; ++hiside) {
+            const int side = 2 * d + hiside;
+            if (damped_outflow && (pos == (hiside ? nd - 1 : 0)) && b.bc_kind[side] == BC_OUTFLOW_SIMPLE_EXTRAPOLATE) {
+                shocked = true;
+                continue;
+            }
+            const int fi = i + (d == 0 ? hiside : 0), fj = j + (d == 1 ? hiside : 0), fk = k + (d == 2 ? hiside : 0);
+            const FaceGeom g = load_face(b.fgeom[d], b.nface[d], face_index(b, d, fi, fj, fk));
+            const int64_t other = pidx(b, i + (d == 0 ? 2 * hiside - 1 : 0), j + (d == 1 ? 2 * hiside - 1 : 0),
+                                       k + (d == 2 ? 2 * hiside - 1 : 0));
+            shocked |= hiside ? pj_shock(fs, np, c, other, g, comp_tol, shear_tol)
+                              : pj_shock(fs, np, other, c, g, comp_tol, shear_tol);
+        }
+    }
+    b.shock[c] = shocked ? 1 : 0;
 }
 
 // One thread per face of index direction d: reconstruction + flux calculator, or the
@@ -153,8 +233,10 @@ __device__ __forceinline__ void face_flux_one(const BlockDev& b, const IdealGasD
         StencilCell L1 = load_sc(fs, np, cL1), L0 = load_sc(fs, np, cL0), R0 = load_sc(fs, np, cR0),
                     R1 = load_sc(fs, np, cR1);
         FallbackFS fb{fs, np, cL0, cR0};
+        // Adaptive calculator: EFM where the face is shocked (strict rule), else AUSMDV.
+        const bool shocked = (FC == 2) && ((b.shock[cL0] | b.shock[cR0]) != 0);
         face_flux_l2r2<DIM, FC>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cR0 + FS_A * np], fb, len[cL1],
-                                len[cL0], len[cR0], len[cR1], eps, F);
+                                len[cL0], len[cR0], len[cR1], eps, F, shocked);
     }
     for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
 }
@@ -385,13 +467,30 @@ hipError_t launch_face_flux_batch(const BlockDev* tab, const int64_t* start, int
                                  int flux_calc, double eps, int d, const IdealGasDev& gas, hipStream_t s) {
     if (total <= 0) return hipSuccess;
     dim3 grid(nblk(total, 256)), blk(256);
+#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas)
     if (dims == 3) {
-        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux_batch<3, 1>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
-        else hipLaunchKernelGGL((k_face_flux_batch<3, 0>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
+        if (flux_calc == 2) GDTK_FACE(3, 2);
+        else if (flux_calc == 1) GDTK_FACE(3, 1);
+        else GDTK_FACE(3, 0);
     } else {
-        if (flux_calc == 1) hipLaunchKernelGGL((k_face_flux_batch<2, 1>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
-        else hipLaunchKernelGGL((k_face_flux_batch<2, 0>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas);
+        if (flux_calc == 2) GDTK_FACE(2, 2);
+        else if (flux_calc == 1) GDTK_FACE(2, 1);
+        else GDTK_FACE(2, 0);
     }
+#undef GDTK_FACE
+    return hipGetLastError();
+}
+
+hipError_t launch_shock_detect(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
+                               double comp_tol, double shear_tol, int damped_outflow, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    dim3 grid(nblk(total, 256)), blk(256);
+    if (dims == 3)
+        hipLaunchKernelGGL((k_shock_detect_batch<3>), grid, blk, 0, s, tab, start, nblocks, comp_tol, shear_tol,
+                           damped_outflow);
+    else
+        hipLaunchKernelGGL((k_shock_detect_batch<2>), grid, blk, 0, s, tab, start, nblocks, comp_tol, shear_tol,
+                           damped_outflow);
     return hipGetLastError();
 }
 

@@ -167,7 +167,9 @@ __device__ __forceinline__ bool face_flux_species(const BlockDev& b, const ChemD
         Rs.vel = to_local(to_global(Rs.vel, g), g);
 #pragma unroll
         for (int j = 0; j < C::n; ++j) F[j] = 0.0;
-        if (FC == 1)
+        // FC 2: adaptive_efm_ausmdv (fluxcalc.d:1315-1332), EFM where the face is shocked.
+        const bool shocked = (FC == 2) && ((b.shock[cL0] | b.shock[cR0]) != 0);
+        if (FC == 1 || shocked)
             efmflx<DIM, NSP>(Ls, Rs, tpg_Cv<NSP>(cd, Ls.massf, Ls.T), tpg_Cv<NSP>(cd, Rs.massf, Rs.T), F);
         else
             ausmdv<DIM, NSP>(Ls, Rs, F);

@@ -327,7 +327,7 @@ hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flu
                               double eps, const CellLimits& lim, const IdealGasDev& gas, int grid, int tx,
                               hipStream_t s) {
     if (sb.total_tiles <= 0) return hipSuccess;
-    if (!fused_tile_x_ok(tx)) return hipErrorInvalidValue;
+    if (!fused_tile_x_ok(tx) || (flux_calc != 0 && flux_calc != 1)) return hipErrorInvalidValue;
     grid = (grid + 7) / 8 * 8;
     dim3 g(grid);
     if (tx == 16) launch_fused_tx<16>(sb, dims, stage, flux_calc, dtp, eps, lim, gas, g, s);

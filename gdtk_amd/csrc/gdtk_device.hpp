@@ -307,7 +307,8 @@ template <int DIM, int FC>
 __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const FaceGeom& g, const StencilCell& L1,
                                                const StencilCell& L0, const StencilCell& R0, const StencilCell& R1,
                                                double aL0, double aR0, const FallbackFS& fb, double lenL1,
-                                               double lenL0, double lenR0, double lenR1, double eps, double* F) {
+                                               double lenL0, double lenR0, double lenR1, double eps, double* F,
+                                               bool shocked = false) {
     using C = CQ<DIM>;
     Vec3d vL1 = to_local(L1.vel, g), vL0 = to_local(L0.vel, g), vR0 = to_local(R0.vel, g), vR1 = to_local(R1.vel, g);
     L2R2 c;
@@ -336,7 +337,9 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
     Ls.vel = to_local(to_global(Ls.vel, g), g);
     Rs.vel = to_local(to_global(Rs.vel, g), g);
     for (int j = 0; j < C::n; ++j) F[j] = 0.0;
-    if (FC == 1) efmflx<DIM>(Ls, Rs, gas.Cv, gas.Cv, F);
+    // FC 2: adaptive_efm_ausmdv (fluxcalc.d:1315-1332) with the face's indicator 0 or 1, so
+    // one calculator runs with factor 1.
+    if (FC == 1 || (FC == 2 && shocked)) efmflx<DIM>(Ls, Rs, gas.Cv, gas.Cv, F);
     else ausmdv<DIM>(Ls, Rs, F);
     // fluxcalc.d:159-171 add grid-velocity terms, here with gvel = 0: each adds a signed
     // zero to a flux component.  Every component starts at +0 and is only ever changed by

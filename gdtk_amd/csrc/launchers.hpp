@@ -53,9 +53,14 @@ hipError_t launch_cells_scatter(const BlockDev* tab, const int64_t* start, const
                                 const int* status, unsigned* nfail, hipStream_t s);
 hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
 // Ghost cells of all blocks in one launch (exchange and the other ghost-cell effects);
-// mode 1 exchanges the cell lengths of the exchange sides instead.
+// mode 1 exchanges the cell lengths of the exchange sides instead; mode 2 also copies the
+// shock indicators.
+// Shock detection over all blocks' interior cells (PJ detector, strict rule applied by
+// the adaptive flux kernels), launched like the cell update.
 hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsides, int64_t total, int mode,
                               hipStream_t s);
+hipError_t launch_shock_detect(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
+                               double comp_tol, double shear_tol, int damped_outflow, hipStream_t s);
 // Fused LDS-tiled stage kernel over all blocks (fused_stage.hip): tiles of one block for
 // k chunks of kc planes, the persistent grid size for the device, and the launch (dt is
 // read from device memory).

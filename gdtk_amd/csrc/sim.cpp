@@ -51,6 +51,7 @@ struct Block {
     int dims, nic, njc, nkc, ncq;
     BlockDev dev{};
     DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], dtpart, dtout, fs_save;
+    DevArr<unsigned char> shock;  // padded cells (adaptive flux calculator)
     int* counters = nullptr;
     ~Block() {
         if (counters) (void)hipFree(counters);
@@ -118,12 +119,15 @@ struct Sim {
     double time = 0.0, dt_global = 0.0, dt_allow = 0.0, cfl_max = 0.0;
     int step = 0, attempts_last = 0;
     // Profiling: HIP events around each kernel launch of the stage, per kernel kind
-    // (0 face flux, 1 cell update, 2 fused stage, 3 ghost fill), on the library's stream.
+    // (0 face flux, 1 cell update, 2 fused stage, 3 ghost fill, 4 shock detection), on the
+    // library's stream.
     bool profile = false;
-    static constexpr int NKIND = 4;
-    double prof_ms[NKIND] = {0, 0, 0, 0};
-    long long prof_launches[NKIND] = {0, 0, 0, 0};
-    double prof_items[NKIND] = {0, 0, 0, 0};
+    static constexpr int NKIND = 5;
+    double prof_ms[NKIND] = {0, 0, 0, 0, 0};
+    long long prof_launches[NKIND] = {0, 0, 0, 0, 0};
+    double prof_items[NKIND] = {0, 0, 0, 0, 0};
+    // Shock-detector options (globalconfig.d:1061-1131) of the adaptive flux calculator.
+    gdtk_shock_detector sdet{-0.30, 0.20, 1, 0, 0, 1000000000};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Stage variant chosen by the autotuner: -1 not tuned (use cfg.use_fused), 0 face-per-
     // thread, 1 fused.
@@ -208,10 +212,18 @@ struct Sim {
         }
     }
 
-    // What a captured step depends on besides the buffers: the kernel variant and the
-    // fused kernel's launch shape.
+    bool adaptive() const { return cfg.flux_calculator == GDTK_FLUX_ADAPTIVE_EFM_AUSMDV; }
+
+    // Whether this step's stage 1 detects shocks (simcore_gasdynamic_step.d:1033-1035).
+    bool detect_now() const {
+        return adaptive() && (!sdet.frozen_shock_detector || sdet.shock_detector_freeze_step > step);
+    }
+
+    // What a captured step depends on besides the buffers: the kernel variant, the fused
+    // kernel's launch shape and whether stage 1 detects shocks.
     long long launch_signature() const {
-        return use_fused() ? 1000000LL + 1000LL * kchunk + tile_x + 100000000LL * wg_per_cu : 0LL;
+        return (use_fused() ? 1000000LL + 1000LL * kchunk + tile_x + 100000000LL * wg_per_cu : 0LL) +
+               (detect_now() ? 10000000000LL : 0LL);
     }
 
     void build_starts() {
@@ -396,7 +408,16 @@ struct Sim {
         const double nc = (double)total_cells();
         const double eps = 1.0e-12;  // epsilon_van_albada, globalconfig.d:1079
         const int nb = (int)blocks.size();
-        timed(3, (double)nghost, [&] { return launch_ghost_batch(t, sides.p, nsides, nghost, 0, stream); });
+        // Stage 1 of a step that detects shocks: the ghost fill also copies the indicators,
+        // then detect_shocks (simcore_gasdynamic_step.d:1030-1037).
+        const bool detect = (stage == 1) && detect_now();
+        timed(3, (double)nghost,
+              [&] { return launch_ghost_batch(t, sides.p, nsides, nghost, detect ? 2 : 0, stream); });
+        if (detect)
+            timed(4, nc, [&] {
+                return launch_shock_detect(t, cstart.p, nb, ctotal, cfg.dimensions, sdet.compression_tolerance,
+                                           sdet.shear_tolerance, sdet.damped_outflow, stream);
+            });
         if (!use_fused()) {
             const int nd = (cfg.dimensions == 3) ? 3 : 2;
             for (int d = 0; d < nd; ++d)
@@ -604,6 +625,11 @@ struct Sim {
         HIPCHK(hipMalloc(&b->counters, 4 * sizeof(int)));
         HIPCHK(hipMemset(b->counters, 0, 4 * sizeof(int)));
         d.counters = b->counters;
+        d.shock = nullptr;
+        if (adaptive()) {
+            b->shock.alloc(np);  // zero: the initial flow state's S (flowstate.d:93)
+            d.shock = b->shock.p;
+        }
         b->dtpart.alloc(2 * (size_t)dt_partial_count(d) + 2);
         b->dtout.alloc(4);
         refresh_dev(*b);
@@ -654,6 +680,7 @@ struct Sim {
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipMemcpy(b.fs[cur].p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemsetAsync(b.counters, 0, 4 * sizeof(int), stream));
+        if (b.shock.p) HIPCHK(hipMemsetAsync(b.shock.p, 0, b.shock.n, stream));
         if (nsp > 1) {
             HIPCHK(launch_encode_species(b.dev, b.U[u0].p, stream));
             HIPCHK(launch_decode_species(b.dev, tpg_dev.p, b.U[u0].p, b.fs[cur].p, lim.low_T, stream));
@@ -681,10 +708,16 @@ struct Sim {
             src = b.fs[cur].p + (FS_BASE_PLANES + nsp + field - GDTK_FIELD_RHO_S) * np;
         else if (field == GDTK_FIELD_VOLUME) src = b.vol.p;
         else if (field >= GDTK_FIELD_ILENGTH && field <= GDTK_FIELD_KLENGTH) src = b.len.p + (field - GDTK_FIELD_ILENGTH) * np;
-        else throw Error(GDTK_ERR_ARG, "bad field");
+        else if (field != GDTK_FIELD_SHOCK || !b.shock.p) throw Error(GDTK_ERR_ARG, "bad field");
         std::vector<double> h(np);
         HIPCHK(hipStreamSynchronize(stream));
-        HIPCHK(hipMemcpy(h.data(), src, np * sizeof(double), hipMemcpyDeviceToHost));
+        if (src) {
+            HIPCHK(hipMemcpy(h.data(), src, np * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<unsigned char> hs(np);
+            HIPCHK(hipMemcpy(hs.data(), b.shock.p, np, hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < np; ++q) h[q] = hs[q];
+        }
         const int kg = (cfg.dimensions == 3) ? NGHOST : 0;
         for (int k = 0; k < b.nkc; ++k)
             for (int j = 0; j < b.njc; ++j)
@@ -695,10 +728,11 @@ struct Sim {
                 }
     }
 
-    // The fused stage kernel serves the single-species gas; a multi-species gas runs the
-    // face-per-thread kernels.
+    // The fused stage kernel serves the single-species gas with the AUSMDV or EFM
+    // calculator; a multi-species gas and the adaptive calculator run the face-per-thread
+    // kernels.
     bool use_fused() const {
-        if (nsp > 1) return false;
+        if (nsp > 1 || adaptive()) return false;
         return variant >= 0 ? variant == 1 : cfg.use_fused != 0;
     }
 
@@ -735,6 +769,17 @@ struct Sim {
         const int txs[] = {64, 32, 16};
         bool saved_profile = profile;
         profile = false;
+        // Stage 1 may detect shocks: the indicators are restored afterwards, so that the next
+        // step's ghost fill copies the same values as without tuning.
+        std::vector<std::unique_ptr<DevArr<unsigned char>>> shock_keep;
+        if (detect_now()) {
+            finalize();
+            for (auto& bp : blocks) {
+                shock_keep.emplace_back(new DevArr<unsigned char>());
+                shock_keep.back()->alloc(bp->shock.n);
+                HIPCHK(hipMemcpyAsync(shock_keep.back()->p, bp->shock.p, bp->shock.n, hipMemcpyDeviceToDevice, stream));
+            }
+        }
         auto time_variant = [&](int var, int kc, int tx) {
             variant = var;
             if (kc > 0) kchunk = kc;
@@ -762,7 +807,7 @@ struct Sim {
         };
         result.push_back(0); result.push_back(0); result.push_back(best); result.push_back(0);
         for (int tx : txs) {
-            if (nsp > 1) break;  // the fused kernel serves the single-species gas only
+            if (nsp > 1 || adaptive()) break;  // the fused kernel: single-species gas, AUSMDV or EFM
             if (cfg.dimensions == 3) {
                 for (int kc : kcs) add(1, kc, tx, time_variant(1, kc, tx));
             } else {
@@ -774,6 +819,9 @@ struct Sim {
             kchunk = best_kc;
             tile_x = best_tx;
         }
+        for (size_t q = 0; q < shock_keep.size(); ++q)
+            HIPCHK(hipMemcpyAsync(blocks[q]->shock.p, shock_keep[q]->p, shock_keep[q]->n, hipMemcpyDeviceToDevice,
+                                  stream));
         HIPCHK(hipStreamSynchronize(stream));
         profile = saved_profile;
         result.insert(result.begin(), {(double)best_var, (double)best_kc, best, (double)best_tx});
@@ -1057,7 +1105,8 @@ int gdtk_hip_sim_create(const gdtk_sim_config* cfg, int* sim) {
     return guarded([&] {
         if (!cfg || !sim) throw Error(GDTK_ERR_ARG, "null argument");
         if (cfg->dimensions != 2 && cfg->dimensions != 3) throw Error(GDTK_ERR_ARG, "dimensions must be 2 or 3");
-        if (cfg->flux_calculator != GDTK_FLUX_AUSMDV && cfg->flux_calculator != GDTK_FLUX_EFM)
+        if (cfg->flux_calculator != GDTK_FLUX_AUSMDV && cfg->flux_calculator != GDTK_FLUX_EFM &&
+            cfg->flux_calculator != GDTK_FLUX_ADAPTIVE_EFM_AUSMDV)
             throw Error(GDTK_ERR_UNSUPPORTED, "flux calculator not implemented");
         if (cfg->interpolation_order != 2) throw Error(GDTK_ERR_UNSUPPORTED, "only interpolation_order 2 is implemented");
         if (!(cfg->mMass > 0.0) || !(cfg->gamma > 1.0)) throw Error(GDTK_ERR_ARG, "bad gas parameters");
@@ -1208,6 +1257,20 @@ int gdtk_hip_sim_profile(int sim, int enable) {
             s.prof_launches[q] = 0;
             s.prof_items[q] = 0.0;
         }
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_sim_set_shock_detector(int sim, const gdtk_shock_detector* opts) {
+    return guarded([&] {
+        Sim& s = get_sim(sim);
+        if (!opts) throw Error(GDTK_ERR_ARG, "null argument");
+        // detect_shock_points throws for a positive tolerance (fluidblock.d:484)
+        if (opts->compression_tolerance > 0.0) throw Error(GDTK_ERR_ARG, "compression_tolerance should be negative!");
+        if (!opts->strict_shock_detector)
+            throw Error(GDTK_ERR_UNSUPPORTED, "only the strict shock detector is implemented");
+        s.sdet = *opts;
+        s.drop_graphs();  // the tolerances are launch arguments of the captured steps
         return GDTK_OK;
     });
 }

@@ -34,6 +34,9 @@ extern "C" {
 /* Flux calculators, a subset of src/eilmer/globalconfig.d:270-290. */
 #define GDTK_FLUX_AUSMDV 0
 #define GDTK_FLUX_EFM 1
+/* adaptive_efm_ausmdv (fluxcalc.d:1315-1332, "adaptive"): EFM on faces the shock detector
+ * marks, AUSMDV elsewhere; turns shock detection on (globalconfig.d:2682-2690). */
+#define GDTK_FLUX_ADAPTIVE_EFM_AUSMDV 2
 
 /* Boundary conditions (src/eilmer/bc.lua:757, 1320, 1580, 1601, 1650). */
 #define GDTK_BC_SLIP_WALL 0
@@ -55,6 +58,7 @@ extern "C" {
 #define GDTK_FIELD_ILENGTH 9
 #define GDTK_FIELD_JLENGTH 10
 #define GDTK_FIELD_KLENGTH 11
+#define GDTK_FIELD_SHOCK 12 /* the cell's shock indicator fs.S (0 or 1; adaptive flux calculator) */
 #define GDTK_FIELD_U0 100   /* + conserved-quantity index: U[0][j] */
 #define GDTK_FIELD_DUDT0 200 /* + conserved-quantity index: dUdt[0][j] */
 #define GDTK_FIELD_MASSF 300 /* + species index: mass fraction (multi-species gas) */
@@ -137,6 +141,24 @@ int gdtk_hip_sim_chem_stats(int sim, double* stats, int nstats);
 /* Copies one field of all active cells to host memory (GDTK_FIELD_*). */
 int gdtk_hip_block_get_field(int sim, int block, int field, double* out);
 
+/* Shock-detector options of GlobalConfig (src/eilmer/globalconfig.d:1061-1131), read when the
+ * flux calculator is GDTK_FLUX_ADAPTIVE_EFM_AUSMDV.  Detection (detect_shocks,
+ * simcore_gasdynamic_step.d:3195-3224) runs in stage 1 of every step after the ghost-cell
+ * fill, with the PJ detector (shockdetectors.d:21-113), no smoothing and the strict rule
+ * (fluidblock.d:582-605). */
+typedef struct gdtk_shock_detector {
+    double compression_tolerance;   /* -0.30 */
+    double shear_tolerance;         /* 0.20 */
+    int strict_shock_detector;      /* 1 (0 is not supported) */
+    int damped_outflow;             /* 0: 1 marks outflow_simple_extrapolate faces shocked */
+    int frozen_shock_detector;      /* 0: 1 stops detection from shock_detector_freeze_step on */
+    int shock_detector_freeze_step; /* 1000000000 */
+} gdtk_shock_detector;
+
+/* Replaces the shock-detector fields of the GlobalConfig read (globalconfig.d:1957-1990).
+ * Call before the first stage; defaults as commented above. */
+int gdtk_hip_sim_set_shock_detector(int sim, const gdtk_shock_detector* opts);
+
 /* One stage of the explicit update over all blocks (simcore_gasdynamic_step.d:946-1426):
  * exchange, ghost-cell effects, convective_flux_phase0 (sfluidblock.d:2188), boundary flux
  * effects, time_derivatives, the pc stage update and decode_conserved.  Returns
@@ -162,7 +184,7 @@ int gdtk_hip_sim_set_dt(int sim, double dt_global, int step);
 /* Kernel timing on the library's stream: enable collects HIP events around every kernel
  * launch of a stage.  stats holds 3 values {launches, total_ms, items_per_launch} for
  * each kernel kind in the order face flux (items = faces), cell update (cells), fused
- * stage (cells), ghost fill (cells): nstats up to 12. */
+ * stage (cells), ghost fill (cells), shock detection (cells): nstats up to 15. */
 int gdtk_hip_sim_profile(int sim, int enable);
 int gdtk_hip_sim_kernel_stats(int sim, double* stats, int nstats);
 

@@ -937,6 +937,13 @@ void SFluidBlock::compute_interface_flux(FlowState& L, FlowState& R, FVInterface
     switch (cfg.flux_calculator) {
     case FluxCalc::ausmdv: ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix); break;
     case FluxCalc::efm: efmflx(L, R, f->F, cqi, *gmodel); break;
+    case FluxCalc::adaptive_efm_ausmdv: {
+        // fluxcalc.d:1315-1332: EFM weighted by the face's shock indicator, AUSMDV by the rest.
+        double alpha = f->S;
+        if (alpha > 0.0) efmflx(L, R, f->F, cqi, *gmodel, alpha);
+        if (alpha < 1.0) ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0 - alpha);
+        break;
+    }
     }
     auto& F = f->F;
     // Energy flux: F[E] += 0.5*F[mass]*v_sqr + (F[x]*gvx + F[y]*gvy + F[z]*gvz) with gvel = 0.
@@ -955,6 +962,91 @@ void SFluidBlock::compute_interface_flux(FlowState& L, FlowState& R, FVInterface
     double vy = ax * f->n.y + ay * f->t1.y + az * f->t2.y;
     double vz = ax * f->n.z + ay * f->t1.z + az * f->t2.z;
     F[cqi.xMom] = vx; F[cqi.yMom] = vy; Fz = vz;
+}
+
+// shockdetectors.d:21-113: compression and shear between the face's two cells, in the
+// face frame and relative to the sound speeds; 1 where the compression is stronger than
+// comp_tol and the shear weaker than shear_tol.
+double PJ_shock_detector(const FVInterface& f, double comp_tol, double shear_tol) {
+    const FVCell* cL = f.left_cell();
+    const FVCell* cR = f.right_cell();
+    double comp, shear;
+    if (cL && cR) {
+        double uL = cL->fs.vel.dot(f.n);
+        double uR = cR->fs.vel.dot(f.n);
+        double aL = cL->fs.gas.a;
+        double aR = cR->fs.gas.a;
+        double a_min = (aL < aR) ? aL : aR;
+        comp = ((uR - uL) / a_min);
+        double vL = cL->fs.vel.dot(f.t1);
+        double vR = cR->fs.vel.dot(f.t1);
+        double wL = cL->fs.vel.dot(f.t2);
+        double wR = cR->fs.vel.dot(f.t2);
+        double sound_speed = 0.5 * (aL + aR);
+        double shear_y = std::fabs(vL - vR) / sound_speed;
+        double shear_z = std::fabs(wL - wR) / sound_speed;
+        shear = std::fmax(shear_y, shear_z);
+    } else if (cL) {
+        // A wall on the right; the grid velocity is zero here.
+        double uL = cL->fs.vel.dot(f.n);
+        double aL = cL->fs.gas.a;
+        comp = ((-uL) / aL);
+        double vL = cL->fs.vel.dot(f.t1);
+        double wL = cL->fs.vel.dot(f.t2);
+        shear = std::fmax(std::fabs(vL) / aL, std::fabs(wL) / aL);
+    } else if (cR) {
+        double uR = cR->fs.vel.dot(f.n);
+        double aR = cR->fs.gas.a;
+        comp = (uR / aR);
+        double vR = cR->fs.vel.dot(f.t1);
+        double wR = cR->fs.vel.dot(f.t2);
+        shear = std::fmax(std::fabs(vR) / aR, std::fabs(wR) / aR);
+    } else {
+        return 0.0;
+    }
+    if ((shear < shear_tol) && (comp < comp_tol)) return 1.0;
+    return 0.0;
+}
+
+// fluidblock.d:478-520 (the PJ detector; damped_outflow marks the faces of
+// outflow_simple_extrapolate boundaries)
+void SFluidBlock::detect_shock_points() {
+    double comp_tol = cfg.compression_tolerance;
+    double shear_tol = cfg.shear_tolerance;
+    if (comp_tol > 0.0) throw std::runtime_error("compression_tolerance should be negative!");
+    for (auto f : faces) f->S = PJ_shock_detector(*f, comp_tol, shear_tol);
+    if (cfg.damped_outflow) {
+        for (int b = 0; b < ((cfg.dimensions == 3) ? 6 : 4); ++b)
+            if (bc[b].kind == BCKind::outflow_simple_extrapolate)
+                for (auto f : bc[b].faces) f->S = 1;
+    }
+}
+
+// fluidblock.d:568-580: a cell is shocked if any of its faces is.
+void SFluidBlock::shock_faces_to_cells() {
+    for (auto c : cells) {
+        c->fs.S = 0.0;
+        for (auto f : c->iface) c->fs.S = std::fmax(c->fs.S, f->S);
+    }
+}
+
+// fluidblock.d:582-605: a face is shocked if it or either of its cells is.  Ghost cells keep
+// the indicator they were given by the last ghost-cell fill.
+void SFluidBlock::enforce_strict_shock_detector() {
+    for (auto f : faces) {
+        if (f->S > 0.0) {
+            f->S = 1.0;
+            continue;
+        }
+        if (f->left_cell() && f->left_cell()->fs.S > 0.0) {
+            f->S = 1.0;
+            continue;
+        }
+        if (f->right_cell() && f->right_cell()->fs.S > 0.0) {
+            f->S = 1.0;
+            continue;
+        }
+    }
 }
 
 // sfluidblock.d:2188-2245
@@ -1194,6 +1286,16 @@ void Simulation::set_up_exchange() {
     exchange_ready = true;
 }
 
+// simcore_gasdynamic_step.d:3195-3224 without smoothing (shock_detector_smoothing 0).
+void Simulation::detect_shocks() {
+    for (auto b : blocks) {
+        b->detect_shock_points();
+        b->shock_faces_to_cells();
+    }
+    if (cfg.strict_shock_detector)
+        for (auto b : blocks) b->enforce_strict_shock_detector();
+}
+
 // simcore_exchange.d:96-135 -> full_face_copy.d:1888-1900 (same process): every ghost
 // cell of an exchange face takes its source cell's flow state.
 void Simulation::exchange_ghost_cell_boundary_data() {
@@ -1215,6 +1317,10 @@ void Simulation::stage(int stage, double dt) {
     }
     exchange_ghost_cell_boundary_data();
     for (auto b : blocks) b->apply_pre_recon(blocks);
+    // simcore_gasdynamic_step.d:1030-1037: detection in stage 1, after the ghost-cell fill.
+    if (cfg.do_shock_detect && stage == 1 &&
+        ((!cfg.frozen_shock_detector) || (cfg.shock_detector_freeze_step > st.step)))
+        detect_shocks();
     for (auto b : blocks) b->convective_flux_phase0(true);
     for (auto b : blocks) b->apply_post_conv_flux();
     for (auto b : blocks) {

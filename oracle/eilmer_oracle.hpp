@@ -81,6 +81,7 @@ struct GasState {
 struct FlowState {
     GasState gas;
     Vec3 vel;
+    double S = 0.0;  // shock indicator, flowstate.d:63 (copied with the flow state)
 };
 
 class GasModel {
@@ -113,7 +114,7 @@ public:
     double R(const GasState&) const override { return Rgas; }
 };
 
-enum class FluxCalc { ausmdv = 0, efm = 1 };
+enum class FluxCalc { ausmdv = 0, efm = 1, adaptive_efm_ausmdv = 2 };
 enum class BCKind { slip_wall = 0, inflow_supersonic = 1, outflow_simple_flux = 2,
                     outflow_simple_extrapolate = 3, exchange = 4 };
 
@@ -142,6 +143,15 @@ struct Config {
     // Reference-fidelity switch: rotate the stencil cells' velocities in place
     // (onedinterp.d:761-770, 983-986), as the reference does.
     bool mutate_cell_velocities = true;
+    // Shock detection (globalconfig.d:1061-1131): on for the adaptive flux calculators
+    // (configCheckPoint1, globalconfig.d:2682-2690); the PJ detector.
+    bool do_shock_detect = false;
+    double compression_tolerance = -0.30;    // globalconfig.d:1123
+    double shear_tolerance = 0.20;           // globalconfig.d:1108
+    bool strict_shock_detector = true;       // globalconfig.d:1064
+    bool damped_outflow = false;             // globalconfig.d:1061
+    bool frozen_shock_detector = false;      // globalconfig.d:1130
+    int shock_detector_freeze_step = 1000000000;  // globalconfig.d:1131
 };
 
 struct FVInterface;
@@ -185,6 +195,11 @@ struct FVInterface {
     bool is_on_boundary = false;
     int bc_id = -1;
     int i_bndry = -1;
+    double S = 0.0;  // the face flow state's shock indicator (fvinterface.d fs.S)
+    // left_cell/right_cell of sfluidblock.d:757-773: the adjacent cells, ghost cells included
+    // (every boundary kind here has ghost-cell data).
+    FVCell* left_cell() const { return left_cells.empty() ? nullptr : left_cells[0]; }
+    FVCell* right_cell() const { return right_cells.empty() ? nullptr : right_cells[0]; }
 };
 
 struct BoundaryCondition {
@@ -257,6 +272,9 @@ public:
     int count_invalid_cells(int ftl);
     void determine_time_step_size(double dt_current, double cfl_value, bool check_cfl, double out[3]);
     bool check_data(const FlowState& fs) const;
+    void detect_shock_points();              // fluidblock.d:478-520
+    void shock_faces_to_cells();             // fluidblock.d:568-580
+    void enforce_strict_shock_detector();    // fluidblock.d:582-605
 
     // Work states (fluidblock.d:93, 208-209)
     FlowState Lft, Rght;
@@ -284,6 +302,8 @@ void ausmdv(const FlowState& L, const FlowState& R, std::vector<double>& F, cons
 void efmflx(const FlowState& L, const FlowState& R, std::vector<double>& F, const CQI& cqi,
             const GasModel& gm, double factor = 1.0);
 void exxef(double sn, double& exx, double& ef);
+// shockdetectors.d:21-113
+double PJ_shock_detector(const FVInterface& f, double comp_tol, double shear_tol);
 double clip_to_limits(double q, double A, double B);
 void scale_mass_fractions(std::vector<double>& massf, double tolerance = 0.0,
                           double assert_error_tolerance = 0.1);
@@ -309,9 +329,14 @@ public:
     const ThermallyPerfectGas* chem_gas = nullptr;
     double T_frozen = 300.0;
     void chemistry_step(double dt);  // simcore.d:1619-1638
-    explicit Simulation(const Config& c) : cfg(c) { st.dt_global = cfg.dt_init; }
+    explicit Simulation(const Config& c) : cfg(c) {
+        st.dt_global = cfg.dt_init;
+        // configCheckPoint1, globalconfig.d:2682-2690
+        if (cfg.flux_calculator == FluxCalc::adaptive_efm_ausmdv) cfg.do_shock_detect = true;
+    }
     ~Simulation();
     void determine_time_step_size();
+    void detect_shocks();     // simcore_gasdynamic_step.d:3195-3224
     void set_up_exchange();   // cell mapping + geometry exchange, once all blocks exist
     void exchange_ghost_cell_boundary_data();
     void gasdynamic_step();   // simcore_gasdynamic_step.d:906-1575 (pc, 2 stages)

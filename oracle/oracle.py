@@ -38,6 +38,11 @@ def lib():
         L.orc_sim_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                      C.c_int, C.c_int]
         L.orc_sim_destroy.argtypes = [C.c_void_p]
+        L.orc_sim_set_shock_detector.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                                 C.c_int]
+        L.orc_block_face_S.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_pj_detector.restype = C.c_double
+        L.orc_pj_detector.argtypes = [dp, C.c_double, dp, C.c_double, dp, dp, dp, C.c_double, C.c_double]
         L.orc_ffc_source.argtypes = [C.c_int] * 10 + [ip]
         L.orc_sim_add_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, ip, dp, ip, ip, ip]
         L.orc_block_set_initial.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp]
@@ -91,19 +96,35 @@ class IdealGas:
         return dict(rho=st[0], p=st[1], T=st[2], u=st[3], a=st[4])
 
 
-FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11)
+FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11, S=12)
+
+
+FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2}
+# globalconfig.d:1061-1131
+SHOCK_DETECTOR_DEFAULTS = dict(compression_tolerance=-0.30, shear_tolerance=0.20, strict_shock_detector=True,
+                               damped_outflow=False, frozen_shock_detector=False,
+                               shock_detector_freeze_step=1_000_000_000)
 
 
 class Sim:
     """Whole-simulation oracle: blocks, BCs, the pc explicit update and the dt controller."""
 
     def __init__(self, gas, dims, flux="ausmdv", cfl=0.5, dt_init=1.0e-3, dt_max=1.0e-3, cfl_count=10,
-                 mutate_cell_velocities=True):
+                 mutate_cell_velocities=True, shock_detector=None):
+        """flux: "ausmdv", "efm" or "adaptive_efm_ausmdv" (EFM at detected shocks);
+        shock_detector: dict of the detector options compression_tolerance (-0.30),
+        shear_tolerance (0.20), strict_shock_detector (True), damped_outflow (False),
+        frozen_shock_detector (False), shock_detector_freeze_step (1e9)."""
         self.gas = gas
         self.dims = dims
-        fc = {"ausmdv": 0, "efm": 1}[flux]
+        fc = FLUX_CALCULATORS[flux]
         self.h = lib().orc_sim_create(gas.h, dims, fc, cfl, dt_init, dt_max, cfl_count,
                                       1 if mutate_cell_velocities else 0)
+        sd = dict(SHOCK_DETECTOR_DEFAULTS)
+        sd.update(shock_detector or {})
+        lib().orc_sim_set_shock_detector(self.h, sd["compression_tolerance"], sd["shear_tolerance"],
+                                         int(sd["strict_shock_detector"]), int(sd["damped_outflow"]),
+                                         int(sd["frozen_shock_detector"]), int(sd["shock_detector_freeze_step"]))
         self.blocks = []
 
     def close(self):
@@ -170,6 +191,12 @@ class Sim:
         lib().orc_block_faces(self.h, b, out.ctypes.data_as(dp))
         return out
 
+    def face_S(self, b):
+        """Shock indicator of every face of block b (face order ifi, ifj, ifk)."""
+        out = np.empty(self.faces(b).shape[0])
+        lib().orc_block_face_S(self.h, b, out.ctypes.data_as(dp))
+        return out
+
     def face_flux(self, b, ncq):
         n = self.faces(b).shape[0]
         out = np.empty((n, ncq))
@@ -209,6 +236,12 @@ class Sim:
 
     def set_dt(self, dt_global, step):
         lib().orc_sim_set_dt(self.h, dt_global, step)
+
+
+def pj_detector(velL, aL, velR, aR, n, t1, t2, comp_tol=-0.30, shear_tol=0.20):
+    """PJ shock detector (shockdetectors.d:21-113) on one face between two cells."""
+    v = [_d(np.asarray(x, dtype=float)) for x in (velL, velR, n, t1, t2)]
+    return lib().orc_pj_detector(v[0][1], aL, v[1][1], aR, v[2][1], v[3][1], v[4][1], comp_tol, shear_tol)
 
 
 def flux(gas, kind, dims, L, R, massfL=None, massfR=None):

@@ -58,7 +58,8 @@ void* orc_sim_create(void* gas, int dims, int flux_calc, double cfl, double dt_i
                      int mutate_cell_velocities) {
     Config c;
     c.dimensions = dims;
-    c.flux_calculator = (flux_calc == 1) ? FluxCalc::efm : FluxCalc::ausmdv;
+    c.flux_calculator = (flux_calc == 2) ? FluxCalc::adaptive_efm_ausmdv
+                        : (flux_calc == 1) ? FluxCalc::efm : FluxCalc::ausmdv;
     c.cfl_value = cfl;
     c.dt_init = dt_init;
     c.dt_max = dt_max;
@@ -66,6 +67,17 @@ void* orc_sim_create(void* gas, int dims, int flux_calc, double cfl, double dt_i
     c.mutate_cell_velocities = mutate_cell_velocities != 0;
     Sim* s = new Sim{new Simulation(c), static_cast<GasModel*>(gas)};
     return s;
+}
+// Shock-detector options (globalconfig.d:1061-1131); call before adding blocks.
+void orc_sim_set_shock_detector(void* h, double comp_tol, double shear_tol, int strict, int damped_outflow,
+                                int frozen, int freeze_step) {
+    Config& c = static_cast<Sim*>(h)->sim->cfg;
+    c.compression_tolerance = comp_tol;
+    c.shear_tolerance = shear_tol;
+    c.strict_shock_detector = strict != 0;
+    c.damped_outflow = damped_outflow != 0;
+    c.frozen_shock_detector = frozen != 0;
+    c.shock_detector_freeze_step = freeze_step;
 }
 void orc_sim_destroy(void* h) {
     Sim* s = static_cast<Sim*>(h);
@@ -148,7 +160,7 @@ int orc_block_set_initial(void* h, int b, const double* p, const double* T, cons
 }
 
 // field: 0 rho, 1 vx, 2 vy, 3 vz, 4 u, 5 p, 6 T, 7 a, 8 volume, 9 iLength, 10 jLength,
-// 11 kLength, 100+j: U[0][j], 200+j: dUdt[0][j], 300+isp: massf[isp]
+// 11 kLength, 12 S (shock indicator), 100+j: U[0][j], 200+j: dUdt[0][j], 300+isp: massf[isp]
 int orc_block_get(void* h, int b, int field, double* out) {
     Sim* s = static_cast<Sim*>(h);
     try {
@@ -172,6 +184,7 @@ int orc_block_get(void* h, int b, int field, double* out) {
                 case 9: v = c->iLength; break;
                 case 10: v = c->jLength; break;
                 case 11: v = c->kLength; break;
+                case 12: v = c->fs.S; break;
                 default: g_err = "bad field"; return -2;
             }
             out[ci] = v;
@@ -181,6 +194,31 @@ int orc_block_get(void* h, int b, int field, double* out) {
         g_err = e.what();
         return -1;
     }
+}
+
+// PJ detector on one face between two cells: vel*, n, t1, t2 are 3-vectors.
+double orc_pj_detector(const double* velL, double aL, const double* velR, double aR, const double* n,
+                       const double* t1, const double* t2, double comp_tol, double shear_tol) {
+    FVCell L, R;
+    L.fs.vel = Vec3(velL[0], velL[1], velL[2]);
+    L.fs.gas.a = aL;
+    R.fs.vel = Vec3(velR[0], velR[1], velR[2]);
+    R.fs.gas.a = aR;
+    FVInterface f;
+    f.n = Vec3(n[0], n[1], n[2]);
+    f.t1 = Vec3(t1[0], t1[1], t1[2]);
+    f.t2 = Vec3(t2[0], t2[1], t2[2]);
+    f.left_cells.push_back(&L);
+    f.right_cells.push_back(&R);
+    return PJ_shock_detector(f, comp_tol, shear_tol);
+}
+
+// Shock indicator of every face (fs.S), in face order.
+int orc_block_face_S(void* h, int b, double* out) {
+    Sim* s = static_cast<Sim*>(h);
+    SFluidBlock* B = s->sim->blocks.at(b);
+    for (size_t n = 0; n < B->faces.size(); ++n) out[n] = B->faces[n]->S;
+    return (int)B->faces.size();
 }
 
 // Face geometry of face index range: out rows of [nx,ny,nz,t1x,t1y,t1z,t2x,t2y,t2z,area]
