@@ -379,6 +379,48 @@ def bench_batched(nblk, steps, warmup, flux="ausmdv", log=None):
     }
 
 
+def bench_adaptive(nic, njc, nkc, steps, warmup=2, log=None):
+    """The headline Mach-6 box with the adaptive EFM/AUSMDV calculator: shock detection
+    (one launch over all cells in stage 1) plus the face kernels that choose EFM on marked
+    faces.  Returns a sub-record with the throughput, the share of marked cells and the
+    detection kernel's roofline (algorithmic bytes per cell: the cell's velocity and sound
+    speed, the n/t1/t2 frames of its three low-side faces and the 1-byte indicator)."""
+    import numpy as np
+    from gdtk_amd import Simulation, cases
+    case = cases.mach6_box(nic, njc, nkc, seed=1234)
+    sim = Simulation(3, flux="adaptive")
+    sim.add_block(case)
+    if warmup:
+        sim.run(warmup)
+    t0 = time.perf_counter()
+    _, st = sim.run(steps)
+    el = time.perf_counter() - t0
+    sim.profile(True)
+    sim.run(min(5, steps))
+    ks = sim.kernel_stats()
+    sim.profile(False)
+    marked = float(np.mean(sim.get(0, "S")))
+    kd = ks["shock_detect"]
+    avg_ms = kd["ms"] / max(kd["launches"], 1)
+    bpc = 4 * 8 + 3 * 9 * 8 + 1
+    achieved = bpc * kd["items_per_launch"] / (avg_ms * 1e-3) / 1e9
+    sim.close()
+    if log:
+        log(f"adaptive: {case.ncells * steps / el:.3e} cell-updates/s, marked {marked:.3f}")
+    return {
+        "value": case.ncells * steps / el, "unit": "cell-updates/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
+        "marked_cell_fraction": marked,
+        "config": {"workload": f"Mach-6 3-D box {nic}x{njc}x{nkc}, ideal air, l2r2+van Albada, "
+                               "adaptive_efm_ausmdv (PJ detector, strict), pc RK2", "cells": case.ncells,
+                   "final_dt": st["dt"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_shock_detect_batch<3>", "avg_launch_ms": avg_ms,
+                     "bytes_per_cell": bpc, "items_per_launch": kd["items_per_launch"]},
+        "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1)} for k, v in ks.items()
+                    if v["launches"]},
+    }
+
+
 def _cell_fields(sim, nblocks, names):
     import numpy as np
     return {n: np.concatenate([sim.get(q, n) for q in range(nblocks)]) for n in names}
@@ -514,6 +556,9 @@ def main():
     ap.add_argument("--coupled-steps", type=int, default=5000,
                     help="steps of the coupled Mach-10 cylinder runs (0 skips them)")
     ap.add_argument("--coupled-only", action="store_true", help="measure only the coupled cylinder")
+    ap.add_argument("--adaptive-steps", type=int, default=20,
+                    help="steps of the adaptive EFM/AUSMDV run on the headline box (0 skips it)")
+    ap.add_argument("--adaptive-only", action="store_true", help="measure only the adaptive flux calculator")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -544,6 +589,9 @@ def main():
         print(json.dumps(bench_batched(nblk, args.batched_steps, 2, args.flux, log), default=float), flush=True)
         return
     nic, njc, nkc = (int(x) for x in args.size.split(","))
+    if args.adaptive_only:
+        print(json.dumps(bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log), default=float), flush=True)
+        return
     from gdtk_amd import Simulation, cases
     case = cases.mach6_box(nic, njc, nkc, seed=1234)
     sim = Simulation(3, flux=args.flux)
@@ -607,6 +655,8 @@ def main():
     sim.close()
     if nblk != (0,):
         out["batched"] = bench_batched(nblk, args.batched_steps, 2, args.flux, log)
+    if args.adaptive_steps > 0:
+        out["adaptive"] = bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log)
     if args.chem_cells > 0:
         out["chem"] = chem_legs()
     if args.coupled_steps > 0:
