@@ -421,6 +421,37 @@ def bench_adaptive(nic, njc, nkc, steps, warmup=2, log=None):
     }
 
 
+def bench_flow_io(nic, njc, nkc, text_size=(64, 64, 64), log=None):
+    """Flow-solution files of the Mach-6 box: write_solution and the restart read + cold start
+    of one block, raw binary at the headline size and gzip text on a smaller box (the text
+    path formats every value with %.18e on the host threads and compresses with zlib).
+    Wall-clock times around the library calls, file on the local disk."""
+    import os
+    import tempfile
+    from gdtk_amd import Simulation, cases
+    out = {}
+    for fmt, size in (("rawbinary", (nic, njc, nkc)), ("gziptext", text_size)):
+        case = cases.mach6_box(*size, seed=1234)
+        sim = Simulation(3, flux="ausmdv")
+        sim.add_block(case)
+        sim.run(2)
+        with tempfile.TemporaryDirectory() as d:
+            fn = os.path.join(d, "b0000.flow")
+            t0 = time.perf_counter()
+            sim.write_flow(0, fn, fmt)
+            tw = time.perf_counter() - t0
+            nbytes = os.path.getsize(fn)
+            t0 = time.perf_counter()
+            sim.read_flow(0, fn, fmt)
+            tr = time.perf_counter() - t0
+        sim.close()
+        out[fmt] = {"cells": case.ncells, "file_bytes": nbytes, "write_s": tw, "read_s": tr,
+                    "write_cells_per_s": case.ncells / tw, "read_cells_per_s": case.ncells / tr}
+        if log:
+            log(f"flow io {fmt}: {case.ncells} cells, write {tw:.2f} s, read {tr:.2f} s, {nbytes / 1e6:.0f} MB")
+    return out
+
+
 def _cell_fields(sim, nblocks, names):
     import numpy as np
     return {n: np.concatenate([sim.get(q, n) for q in range(nblocks)]) for n in names}
@@ -559,6 +590,8 @@ def main():
     ap.add_argument("--adaptive-steps", type=int, default=20,
                     help="steps of the adaptive EFM/AUSMDV run on the headline box (0 skips it)")
     ap.add_argument("--adaptive-only", action="store_true", help="measure only the adaptive flux calculator")
+    ap.add_argument("--no-flow-io", action="store_true", help="skip the flow-file write/read measurement")
+    ap.add_argument("--flow-io-only", action="store_true", help="measure only the flow-file write/read")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -589,6 +622,9 @@ def main():
         print(json.dumps(bench_batched(nblk, args.batched_steps, 2, args.flux, log), default=float), flush=True)
         return
     nic, njc, nkc = (int(x) for x in args.size.split(","))
+    if args.flow_io_only:
+        print(json.dumps(bench_flow_io(nic, njc, nkc, log=log), default=float), flush=True)
+        return
     if args.adaptive_only:
         print(json.dumps(bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log), default=float), flush=True)
         return
@@ -657,6 +693,8 @@ def main():
         out["batched"] = bench_batched(nblk, args.batched_steps, 2, args.flux, log)
     if args.adaptive_steps > 0:
         out["adaptive"] = bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log)
+    if not args.no_flow_io:
+        out["flow_io"] = bench_flow_io(nic, njc, nkc, log=log)
     if args.chem_cells > 0:
         out["chem"] = chem_legs()
     if args.coupled_steps > 0:

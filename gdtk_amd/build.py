@@ -58,7 +58,7 @@ def build(verbose: bool = False) -> str:
         msg = "\n".join(f"--- {s}\n{e}" for s, e in failed)
         raise RuntimeError(f"HIP build failed:\n{msg}")
     tmp = LIB + ".tmp"
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz", "-pthread"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

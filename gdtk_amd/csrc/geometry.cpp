@@ -99,7 +99,7 @@ double pyramid_volume(const V3& p0, const V3& p1, const V3& p2, const V3& p3, co
     v += tet_volume(p3, p0, pmB, p4);
     return v;
 }
-void hex(const V3* p, double& volume, double& iLen, double& jLen, double& kLen) {
+void hex(const V3* p, double& volume, double& iLen, double& jLen, double& kLen, V3& centroid) {
     const V3 &p0 = p[0], &p1 = p[1], &p2 = p[2], &p3 = p[3], &p4 = p[4], &p5 = p[5], &p6 = p[6], &p7 = p[7];
     V3 c{0.125 * (p0.x + p1.x + p2.x + p3.x + p4.x + p5.x + p6.x + p7.x),
          0.125 * (p0.y + p1.y + p2.y + p3.y + p4.y + p5.y + p6.y + p7.y),
@@ -125,6 +125,7 @@ void hex(const V3* p, double& volume, double& iLen, double& jLen, double& kLen) 
     volume += pyramid_volume(p7, p6, p5, p4, c);
     volume += pyramid_volume(p0, p1, p2, p3, c);
     if ((volume < 0.0 && std::fabs(volume) < 1.0e-12) || (volume >= 0.0 && volume < 1.0e-20)) volume = 0.0;
+    centroid = c;
 }
 }  // namespace
 
@@ -139,6 +140,7 @@ void compute_block_geometry(int dims, int nic, int njc, int nkc, const double* v
     out.iLen.assign(ncell, 0.0);
     out.jLen.assign(ncell, 0.0);
     out.kLen.assign(ncell, 0.0);
+    out.pos.assign(3 * ncell, 0.0);
     out.nf[0][0] = niv; out.nf[0][1] = njc; out.nf[0][2] = nkc;
     out.nf[1][0] = nic; out.nf[1][1] = njv; out.nf[1][2] = nkc;
     out.nf[2][0] = nic; out.nf[2][1] = njc; out.nf[2][2] = (dims == 3) ? nkv : 0;
@@ -161,6 +163,13 @@ void compute_block_geometry(int dims, int nic, int njc, int nkc, const double* v
                 double area = 0.5 * ((xB + xA) * (yB - yA) + (xC + xB) * (yC - yB) + (xD + xC) * (yD - yC) +
                                      (xA + xD) * (yA - yD));
                 if (area < 0.0) throw std::runtime_error("Negative cell volume");
+                // centroid of the quadrilateral (properties.d:177-237)
+                const double cx = 1.0 / (area * 6.0) *
+                                  ((yB - yA) * (xA * xA + xA * xB + xB * xB) + (yC - yB) * (xB * xB + xB * xC + xC * xC) +
+                                   (yD - yC) * (xC * xC + xC * xD + xD * xD) + (yA - yD) * (xD * xD + xD * xA + xA * xA));
+                const double cy = -1.0 / (area * 6.0) *
+                                  ((xB - xA) * (yA * yA + yA * yB + yB * yB) + (xC - xB) * (yB * yB + yB * yC + yC * yC) +
+                                   (xD - xC) * (yC * yC + yC * yD + yD * yD) + (xA - xD) * (yD * yD + yD * yA + yA * yA));
                 double xN = 0.5 * (xC + xB), yN = 0.5 * (yC + yB);
                 double xS = 0.5 * (xD + xA), yS = 0.5 * (yD + yA);
                 double xE = 0.5 * (xA + xB), yE = 0.5 * (yA + yB);
@@ -172,6 +181,8 @@ void compute_block_geometry(int dims, int nic, int njc, int nkc, const double* v
                 out.iLen[c] = std::sqrt(dx * dx + dy * dy);
                 out.kLen[c] = 0.0;
                 out.volume[c] = area;
+                out.pos[c] = cx;
+                out.pos[c + ncell] = cy;
             }
         for (int j = 0; j < njc; ++j)
             for (int i = 0; i < niv; ++i) { face2d(V(i, j, 0), V(i, j + 1, 0), g); put(0, i, j, 0, g); }
@@ -185,9 +196,11 @@ void compute_block_geometry(int dims, int nic, int njc, int nkc, const double* v
                                V(i, j, k + 1), V(i + 1, j, k + 1), V(i + 1, j + 1, k + 1), V(i, j + 1, k + 1)};
                     size_t c = ((size_t)k * njc + j) * nic + i;
                     double vol, iL, jL, kL;
-                    hex(p, vol, iL, jL, kL);
+                    V3 cen;
+                    hex(p, vol, iL, jL, kL, cen);
                     if (vol <= 0.0) throw std::runtime_error("Invalid cell volume");
                     out.volume[c] = vol; out.iLen[c] = iL; out.jLen[c] = jL; out.kLen[c] = kL;
+                    out.pos[c] = cen.x; out.pos[c + ncell] = cen.y; out.pos[c + 2 * ncell] = cen.z;
                 }
         for (int k = 0; k < nkc; ++k)
             for (int j = 0; j < njc; ++j)

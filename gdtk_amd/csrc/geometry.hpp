@@ -7,6 +7,8 @@ namespace gdtk {
 struct BlockGeometry {
     // Active cells in the reference order (k*njc + j)*nic + i.
     std::vector<double> volume, iLen, jLen, kLen;
+    // Cell centroids (pos[0] of the reference's cells): x, y, z planes of ncell values.
+    std::vector<double> pos;
     // Per index direction: 10 planes (n.xyz, t1.xyz, t2.xyz, area) over the face array
     // of extents nf[d][0] x nf[d][1] x nf[d][2].
     std::vector<double> face[3];

@@ -72,6 +72,12 @@ hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flu
                               double eps, const CellLimits& lim, const IdealGasDev& gas, int grid, int tx,
                               hipStream_t s);
 hipError_t launch_clear_u32(unsigned* p, int n, hipStream_t s);
+// Flow-solution records of one block (flow_io.hip): pack the flow state (fs_in) into nv
+// doubles per active cell, and unpack records into the flow-state planes fs.
+hipError_t launch_pack_flow(const BlockDev& b, const double* pos, const double* mu, const double* kc, const double* dtc,
+                            double* rec, int nv, hipStream_t s);
+hipError_t launch_unpack_flow(const BlockDev& b, const double* rec, double* fs, double* mu, double* kc, int nv,
+                              hipStream_t s);
 hipError_t launch_set_f64(double* p, double v, hipStream_t s);
 
 }  // namespace gdtk

@@ -25,6 +25,7 @@
 #include "host_common.hpp"
 #include "chem_device.hpp"
 #include "chem_host.hpp"
+#include "flow_io.hpp"
 #include "launchers.hpp"
 
 namespace gdtk {
@@ -52,6 +53,10 @@ struct Block {
     BlockDev dev{};
     DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], dtpart, dtout, fs_save;
     DevArr<unsigned char> shock;  // padded cells (adaptive flux calculator)
+    // Flow-file fields the update does not touch: cell centroids (3 planes of active cells)
+    // and the gas transport coefficients mu, k carried through an inviscid run (allocated
+    // when set or read from a file; written as 0 otherwise).
+    DevArr<double> pos, mu, kc;
     int* counters = nullptr;
     ~Block() {
         if (counters) (void)hipFree(counters);
@@ -586,6 +591,8 @@ struct Sim {
                         setlen(pad(i, j, -1 - n), cidx(i, j, n));
                         setlen(pad(i, j, nkc + n), cidx(i, j, nkc - 1 - n));
                     }
+        b->pos.alloc(geo.pos.size());
+        HIPCHK(hipMemcpy(b->pos.p, geo.pos.data(), geo.pos.size() * sizeof(double), hipMemcpyHostToDevice));
         b->len.alloc(3 * np);
         b->vol.alloc(np);
         HIPCHK(hipMemcpy(b->len.p, hlen.data(), hlen.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -679,8 +686,19 @@ struct Sim {
         refresh_dev(b);
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipMemcpy(b.fs[cur].p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemsetAsync(b.counters, 0, 4 * sizeof(int), stream));
         if (b.shock.p) HIPCHK(hipMemsetAsync(b.shock.p, 0, b.shock.n, stream));
+        cold_start(b);
+    }
+
+    // The cold start of init_simulation (simcore.d:318-336) on a block whose flow-state planes
+    // hold the initial state: U[0] = encode(fs), fs = decode(U[0]), dt_chem = -1.
+    void cold_start(Block& b) {
+        HIPCHK(hipMemsetAsync(b.counters, 0, 4 * sizeof(int), stream));
+        if (dtc.p) {
+            int64_t off = 0;
+            for (int q = 0; q < b.id; ++q) off += blocks[q]->ncell();
+            HIPCHK(launch_fill_f64(dtc.p + off, b.ncell(), -1.0, stream));
+        }
         if (nsp > 1) {
             HIPCHK(launch_encode_species(b.dev, b.U[u0].p, stream));
             HIPCHK(launch_decode_species(b.dev, tpg_dev.p, b.U[u0].p, b.fs[cur].p, lim.low_T, stream));
@@ -692,6 +710,81 @@ struct Sim {
         HIPCHK(hipMemcpyAsync(cnt, b.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         if (cnt[0] > 0) throw Error(GDTK_ERR_BAD_CELL, "bad cell while initializing flow");
+    }
+
+    // Species names of the flow file's mass-fraction variables: given, or "air" (the
+    // ideal-air model's species) for a single-species gas.
+    std::vector<std::string> species_list(const char* names) const {
+        std::vector<std::string> v = split_names(names);
+        if (v.empty() && nsp == 1) v.push_back("air");
+        if ((int)v.size() != nsp) throw Error(GDTK_ERR_ARG, "one species name per species is needed");
+        return v;
+    }
+
+    // write_solution (fluidblockio_old.d:388-447) of one block: the current flow state packed
+    // into cell records on the device, copied to the host and written.
+    void write_flow(int bid, const char* fn, int format, const char* label, double sim_time, const char* species) {
+        Block& b = blk(bid);
+        const std::vector<std::string> vars = flow_variable_list(nsp, species_list(species));
+        const int nv = (int)vars.size();
+        const int64_t n = b.ncell();
+        DevArr<double> rec;
+        rec.alloc((size_t)n * nv);
+        int64_t off = 0;
+        for (int q = 0; q < bid; ++q) off += blocks[q]->ncell();
+        refresh_dev(b);
+        HIPCHK(launch_pack_flow(b.dev, b.pos.p, b.mu.p, b.kc.p, dtc.p ? dtc.p + off : nullptr, rec.p, nv, stream));
+        std::vector<double> h((size_t)n * nv);
+        HIPCHK(hipMemcpyAsync(h.data(), rec.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        try {
+            write_flow_file(fn, format, label ? label : "", sim_time, cfg.dimensions, b.nic, b.njc,
+                            cfg.dimensions == 3 ? b.nkc : 1, vars, h.data(), n);
+        } catch (std::exception& e) {
+            throw Error(GDTK_ERR_ARG, e.what());
+        }
+    }
+
+    // read_legacy_solution (fluidblockio_old.d:156-273) of one block followed by the cold
+    // start (simcore.d:299-336): the file's rho, vel, p, a, u, T, mass fractions, S, mu and k
+    // become the cell states, then U[0] = encode(fs), fs = decode(U[0]) and dt_chem = -1.
+    // Returns the file's sim_time.
+    double read_flow(int bid, const char* fn, int format, const char* species) {
+        Block& b = blk(bid);
+        const std::vector<std::string> vars = flow_variable_list(nsp, species_list(species));
+        const int nv = (int)vars.size();
+        std::vector<double> h;
+        double t = 0.0;
+        try {
+            t = read_flow_file(fn, format, cfg.dimensions, b.nic, b.njc, b.nkc, vars, h);
+        } catch (std::exception& e) {
+            throw Error(GDTK_ERR_ARG, e.what());
+        }
+        const int64_t n = b.ncell();
+        for (int64_t c = 0; c < n; ++c) {
+            const double* r = &h[(size_t)c * nv];
+            if (!(r[4] > 0.0) || !(r[nv - 1] > 0.0)) throw Error(GDTK_ERR_ARG, "flow file: cell with rho or T <= 0");
+        }
+        DevArr<double> rec;
+        rec.alloc(h.size());
+        if (!b.mu.p) b.mu.alloc((size_t)n);
+        if (!b.kc.p) b.kc.alloc((size_t)n);
+        refresh_dev(b);
+        HIPCHK(hipMemcpy(rec.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(launch_unpack_flow(b.dev, rec.p, b.fs[cur].p, b.mu.p, b.kc.p, nv, stream));
+        cold_start(b);
+        return t;
+    }
+
+    // Per-cell transport coefficients written to the flow files (fs.gas.mu, k; an inviscid
+    // run carries them unchanged from the initial state).
+    void set_transport(int bid, const double* mu_h, const double* k_h) {
+        Block& b = blk(bid);
+        const size_t n = (size_t)b.ncell();
+        if (!b.mu.p) b.mu.alloc(n);
+        if (!b.kc.p) b.kc.alloc(n);
+        HIPCHK(hipMemcpy(b.mu.p, mu_h, n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.kc.p, k_h, n * sizeof(double), hipMemcpyHostToDevice));
     }
 
     void get_field(int bid, int field, double* out) {
@@ -1244,6 +1337,94 @@ int gdtk_hip_sim_set_dt(int sim, double dt_global, int step) {
         Sim& s = get_sim(sim);
         s.dt_global = dt_global;
         s.step = step;
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_sim_get_state(int sim, double* state) {
+    return guarded([&] {
+        if (!state) throw Error(GDTK_ERR_ARG, "null argument");
+        get_sim(sim).state(state);
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_sim_set_time(int sim, double time) {
+    return guarded([&] {
+        get_sim(sim).time = time;
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_block_write_flow(int sim, int block, const char* filename, int format, const char* label,
+                              double sim_time, const char* species_names) {
+    return guarded([&] {
+        if (!filename) throw Error(GDTK_ERR_ARG, "null file name");
+        if (format != GDTK_FLOW_GZIPTEXT && format != GDTK_FLOW_RAWBINARY) throw Error(GDTK_ERR_ARG, "bad flow format");
+        get_sim(sim).write_flow(block, filename, format, label, sim_time, species_names);
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_block_read_flow(int sim, int block, const char* filename, int format, const char* species_names,
+                             double* sim_time) {
+    return guarded([&] {
+        if (!filename) throw Error(GDTK_ERR_ARG, "null file name");
+        if (format != GDTK_FLOW_GZIPTEXT && format != GDTK_FLOW_RAWBINARY) throw Error(GDTK_ERR_ARG, "bad flow format");
+        const double t = get_sim(sim).read_flow(block, filename, format, species_names);
+        if (sim_time) *sim_time = t;
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_block_set_transport(int sim, int block, const double* mu, const double* k) {
+    return guarded([&] {
+        if (!mu || !k) throw Error(GDTK_ERR_ARG, "null argument");
+        get_sim(sim).set_transport(block, mu, k);
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_flow_nvars(int nsp) {
+    return guarded([&] {
+        if (nsp < 1 || nsp > GDTK_MAX_SPECIES) throw Error(GDTK_ERR_ARG, "bad species count");
+        return 17 + nsp + (nsp > 1 ? 1 : 0);
+    });
+}
+
+int gdtk_hip_flow_write_records(const char* filename, int format, const char* label, double sim_time, int dims,
+                                int nic, int njc, int nkc, int nsp, const char* species_names, const double* records) {
+    return guarded([&] {
+        if (!filename || !records || nsp < 1 || nsp > GDTK_MAX_SPECIES || nic < 1 || njc < 1 || nkc < 1)
+            throw Error(GDTK_ERR_ARG, "bad argument");
+        std::vector<std::string> names = split_names(species_names);
+        if (names.empty() && nsp == 1) names.push_back("air");
+        try {
+            write_flow_file(filename, format, label ? label : "", sim_time, dims, nic, njc, nkc,
+                            flow_variable_list(nsp, names), records, (int64_t)nic * njc * nkc);
+        } catch (std::exception& e) {
+            throw Error(GDTK_ERR_ARG, e.what());
+        }
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_flow_read_records(const char* filename, int format, int dims, int nic, int njc, int nkc, int nsp,
+                               const char* species_names, double* records, double* sim_time) {
+    return guarded([&] {
+        if (!filename || !records || nsp < 1 || nsp > GDTK_MAX_SPECIES || nic < 1 || njc < 1 || nkc < 1)
+            throw Error(GDTK_ERR_ARG, "bad argument");
+        std::vector<std::string> names = split_names(species_names);
+        if (names.empty() && nsp == 1) names.push_back("air");
+        std::vector<double> rec;
+        double t = 0.0;
+        try {
+            t = read_flow_file(filename, format, dims, nic, njc, nkc, flow_variable_list(nsp, names), rec);
+        } catch (std::exception& e) {
+            throw Error(GDTK_ERR_ARG, e.what());
+        }
+        std::copy(rec.begin(), rec.end(), records);
+        if (sim_time) *sim_time = t;
         return GDTK_OK;
     });
 }

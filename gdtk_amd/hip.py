@@ -38,7 +38,9 @@ EXPORTS = [
     "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
     "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error", "gdtk_hip_exchange_source",
     "gdtk_hip_sim_set_gas_tpg", "gdtk_hip_block_set_initial_massf", "gdtk_hip_sim_set_chemistry",
-    "gdtk_hip_sim_chem_stats", "gdtk_hip_sim_set_shock_detector",
+    "gdtk_hip_sim_chem_stats", "gdtk_hip_sim_set_shock_detector", "gdtk_hip_block_write_flow",
+    "gdtk_hip_block_read_flow", "gdtk_hip_block_set_transport", "gdtk_hip_sim_set_time", "gdtk_hip_sim_get_state", "gdtk_hip_flow_nvars",
+    "gdtk_hip_flow_write_records", "gdtk_hip_flow_read_records",
     "gdtk_hip_chem_create", "gdtk_hip_chem_destroy", "gdtk_hip_chem_step", "gdtk_hip_chem_step_device",
     "gdtk_hip_chem_stats", "gdtk_hip_chem_substeps", "gdtk_hip_chem_set_tuning", "gdtk_hip_chem_autotune",
     "gdtk_hip_tpg_update",
@@ -119,6 +121,17 @@ def lib():
         L.gdtk_hip_sim_set_chemistry.argtypes = [C.c_int, C.c_int, C.c_int]
         L.gdtk_hip_sim_chem_stats.argtypes = [C.c_int, dp, C.c_int]
         L.gdtk_hip_sim_set_shock_detector.argtypes = [C.c_int, C.POINTER(ShockDetector)]
+        L.gdtk_hip_block_write_flow.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_double,
+                                                C.c_char_p]
+        L.gdtk_hip_block_read_flow.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_char_p, dp]
+        L.gdtk_hip_block_set_transport.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.gdtk_hip_sim_set_time.argtypes = [C.c_int, C.c_double]
+        L.gdtk_hip_sim_get_state.argtypes = [C.c_int, dp]
+        L.gdtk_hip_flow_nvars.argtypes = [C.c_int]
+        L.gdtk_hip_flow_write_records.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_int,
+                                                  C.c_int, C.c_int, C.c_int, C.c_char_p, dp]
+        L.gdtk_hip_flow_read_records.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                 C.c_char_p, dp, dp]
         _lib = L
     return _lib
 
@@ -246,6 +259,11 @@ class Simulation:
             _check(rc)
         return rc, self._state(st)
 
+    def state(self):
+        st = np.zeros(6)
+        _check(lib().gdtk_hip_sim_get_state(self.h, st.ctypes.data_as(dp)))
+        return self._state(st)
+
     def step_fixed_dt(self, dt):
         st = np.zeros(6)
         _check(lib().gdtk_hip_sim_step_fixed_dt(self.h, dt, st.ctypes.data_as(dp)))
@@ -265,6 +283,31 @@ class Simulation:
         served by a gdtk_amd.surrogate.Surrogate built on it."""
         self._chem = (chem, surrogate)
         _check(lib().gdtk_hip_sim_set_chemistry(self.h, chem.h, surrogate.h if surrogate is not None else -1))
+
+    def write_flow(self, b, filename, fmt="gziptext", label="", sim_time=None, species=None):
+        """write_solution of block b: the reference's legacy flow file ("gziptext" or "rawbinary").
+        species: the species names of the mass-fraction variables (default "air" for the ideal
+        gas); sim_time defaults to the simulation's time."""
+        if sim_time is None:
+            sim_time = self.state()["time"]
+        _check(lib().gdtk_hip_block_write_flow(self.h, b, os.fsencode(filename), FLOW_FORMATS[fmt],
+                                               label.encode(), float(sim_time), _names(species)))
+
+    def read_flow(self, b, filename, fmt="gziptext", species=None):
+        """read_legacy_solution + cold start of block b from a flow file; returns its sim_time."""
+        t = C.c_double(0.0)
+        _check(lib().gdtk_hip_block_read_flow(self.h, b, os.fsencode(filename), FLOW_FORMATS[fmt], _names(species),
+                                              C.byref(t)))
+        return t.value
+
+    def set_transport(self, b, mu, k):
+        """Per-cell mu and k of block b, carried to the flow files."""
+        m, mp = _d(mu)
+        kk, kp = _d(k)
+        _check(lib().gdtk_hip_block_set_transport(self.h, b, mp, kp))
+
+    def set_time(self, t):
+        _check(lib().gdtk_hip_sim_set_time(self.h, float(t)))
 
     def chem_stats(self):
         out = np.zeros(3)
@@ -308,6 +351,39 @@ class Simulation:
                       tile_x=int(out[q + 3])) for q in range(4, n, 4)]
         return dict(best="fused" if out[0] == 1 else "face", kchunk=int(out[1]), ms=out[2], tile_x=int(out[3]),
                     candidates=cands)
+
+
+FLOW_FORMATS = {"gziptext": 0, "rawbinary": 1}
+
+
+def _names(species):
+    if species is None:
+        return None
+    if isinstance(species, str):
+        return species.encode()
+    return " ".join(species).encode()
+
+
+def flow_nvars(nsp):
+    """Doubles per cell record of a flow file for a gas of nsp species."""
+    return _check_n(lib().gdtk_hip_flow_nvars(nsp))
+
+
+def write_flow_records(filename, records, dims, nic, njc, nkc, fmt="gziptext", label="", sim_time=0.0, species=None):
+    """The library's flow-file writer on host records (ncell, nvars); needs no GPU."""
+    r, rp = _d(records)
+    nsp = r.shape[1] - 17 - (1 if r.shape[1] > 18 else 0)
+    _check(lib().gdtk_hip_flow_write_records(os.fsencode(filename), FLOW_FORMATS[fmt], label.encode(), float(sim_time),
+                                             dims, nic, njc, nkc, nsp, _names(species), rp))
+
+
+def read_flow_records(filename, dims, nic, njc, nkc, nsp=1, fmt="gziptext", species=None):
+    """The library's flow-file reader: (records (ncell, nvars), sim_time); needs no GPU."""
+    out = np.zeros((nic * njc * nkc, flow_nvars(nsp)))
+    t = C.c_double(0.0)
+    _check(lib().gdtk_hip_flow_read_records(os.fsencode(filename), FLOW_FORMATS[fmt], dims, nic, njc, nkc, nsp,
+                                            _names(species), out.ctypes.data_as(dp), C.byref(t)))
+    return out, t.value
 
 
 def exchange_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc):

@@ -141,6 +141,52 @@ int gdtk_hip_sim_chem_stats(int sim, double* stats, int nstats);
 /* Copies one field of all active cells to host memory (GDTK_FIELD_*). */
 int gdtk_hip_block_get_field(int sim, int block, int field, double* out);
 
+/* ---------------------------------------------------------------------------------
+ * Flow-solution files (the legacy structured-block format, config.flow_format "gziptext"
+ * or "rawbinary").  A cell record holds, in this order: pos.x pos.y pos.z volume rho
+ * vel.x vel.y vel.z p a mu k mu_t k_t S, massf[i]-<species> for each species (1.0 for a
+ * single-species gas), dt_chem (multi-species gas only), u, T -- the variable list of
+ * build_flow_variable_list (src/eilmer/fluidblockio_old.d:107-151) for an inviscid,
+ * laminar, single-temperature gas without MHD, radiation or local time stepping. */
+#define GDTK_FLOW_GZIPTEXT 0
+#define GDTK_FLOW_RAWBINARY 1
+
+/* Replaces write_solution for an SFluidBlock (fluidblockio_old.d:388-447; called per block by
+ * write_solution_files, simcore_io.d:54-136): writes the block's current flow state with
+ * the given label and sim_time.  species_names: whitespace-separated species names for the
+ * mass-fraction variables (NULL: "air" for the ideal gas). */
+int gdtk_hip_block_write_flow(int sim, int block, const char* filename, int format, const char* label,
+                              double sim_time, const char* species_names);
+
+/* Replaces read_legacy_solution for an SFluidBlock (fluidblockio_old.d:156-273) followed by the
+ * cold start of init_simulation (simcore.d:299-336): the file's rho, vel, p, a, u, T, mass
+ * fractions, S, mu and k become the cell states (the file may order its variables freely;
+ * position and volume are not taken), then U[0] = encode(fs), fs = decode(U[0]) and the
+ * cells' dt_chem = -1.  Writes the file's sim_time.  A size mismatch or a missing variable
+ * fails with GDTK_ERR_ARG, as the reference throws. */
+int gdtk_hip_block_read_flow(int sim, int block, const char* filename, int format, const char* species_names,
+                             double* sim_time);
+
+/* The gas transport coefficients mu and k of a block's cells (an inviscid run carries them
+ * unchanged from the initial FlowState, luaflowstate.d:490); written to the flow files
+ * (0 when never set or read). */
+int gdtk_hip_block_set_transport(int sim, int block, const double* mu, const double* k);
+
+/* Sets SimState.time (a restart takes it from the flow files, simcore.d:344). */
+int gdtk_hip_sim_set_time(int sim, double time);
+/* The SimState as gdtk_hip_sim_run reports it: {time, dt_global, step, cfl_max, dt_allow,
+ * attempts_of_last_step}. */
+int gdtk_hip_sim_get_state(int sim, double* state);
+
+/* Host-only halves of the flow files (no GPU): the number of doubles in a cell record, and
+ * the writer and reader on host records (nic*njc*nkc records in the reference's cell order;
+ * nkc = 1 in 2-D). */
+int gdtk_hip_flow_nvars(int nsp);
+int gdtk_hip_flow_write_records(const char* filename, int format, const char* label, double sim_time, int dims,
+                                int nic, int njc, int nkc, int nsp, const char* species_names, const double* records);
+int gdtk_hip_flow_read_records(const char* filename, int format, int dims, int nic, int njc, int nkc, int nsp,
+                               const char* species_names, double* records, double* sim_time);
+
 /* Shock-detector options of GlobalConfig (src/eilmer/globalconfig.d:1061-1131), read when the
  * flux calculator is GDTK_FLUX_ADAPTIVE_EFM_AUSMDV.  Detection (detect_shocks,
  * simcore_gasdynamic_step.d:3195-3224) runs in stage 1 of every step after the ghost-cell

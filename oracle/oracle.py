@@ -41,6 +41,8 @@ def lib():
         L.orc_sim_set_shock_detector.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                                  C.c_int]
         L.orc_block_face_S.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_block_set_fs.argtypes = [C.c_void_p, C.c_int] + [dp] * 10
+        L.orc_sim_set_time.argtypes = [C.c_void_p, C.c_double]
         L.orc_pj_detector.restype = C.c_double
         L.orc_pj_detector.argtypes = [dp, C.c_double, dp, C.c_double, dp, dp, dp, C.c_double, C.c_double]
         L.orc_ffc_source.argtypes = [C.c_int] * 10 + [ip]
@@ -96,7 +98,8 @@ class IdealGas:
         return dict(rho=st[0], p=st[1], T=st[2], u=st[3], a=st[4])
 
 
-FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11, S=12)
+FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11, S=12, pos_x=13, pos_y=14,
+              pos_z=15, dt_chem=16)
 
 
 FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2}
@@ -191,6 +194,15 @@ class Sim:
         lib().orc_block_faces(self.h, b, out.ctypes.data_as(dp))
         return out
 
+    def set_fs(self, b, rho, vx, vy, vz, p, a, u, T, massf=None, S=None):
+        """Cold start of block b from flow-file cell values (no EOS call before the encode)."""
+        arrs = [_d(x) for x in (rho, vx, vy, vz, p, a, u, T)]
+        mf = _d(np.asarray(massf).reshape(len(rho), -1)) if massf is not None else None
+        sv = _d(S) if S is not None else None
+        rc = lib().orc_block_set_fs(self.h, b, *[x[1] for x in arrs], mf[1] if mf else None, sv[1] if sv else None)
+        if rc != 0:
+            raise RuntimeError(err())
+
     def face_S(self, b):
         """Shock indicator of every face of block b (face order ifi, ifj, ifk)."""
         out = np.empty(self.faces(b).shape[0])
@@ -236,6 +248,9 @@ class Sim:
 
     def set_dt(self, dt_global, step):
         lib().orc_sim_set_dt(self.h, dt_global, step)
+
+    def set_time(self, t):
+        lib().orc_sim_set_time(self.h, t)
 
 
 def pj_detector(velL, aL, velR, aR, n, t1, t2, comp_tol=-0.30, shear_tol=0.20):

@@ -159,8 +159,40 @@ int orc_block_set_initial(void* h, int b, const double* p, const double* T, cons
     }
 }
 
+// The cold start from a flow file's cell values (fluidblockio_old.d:1693-1800 then
+// simcore.d:318-336): rho, vel, p, a, u, T, massf (cell-major, may be null for nsp == 1) and
+// S set directly, then encode, decode and dt_chem = -1.
+int orc_block_set_fs(void* h, int b, const double* rho, const double* vx, const double* vy, const double* vz,
+                     const double* p, const double* a, const double* u, const double* T, const double* massf,
+                     const double* S) {
+    Sim* s = static_cast<Sim*>(h);
+    try {
+        SFluidBlock* B = s->sim->blocks.at(b);
+        const int nsp = s->gm->n_species;
+        for (size_t ci = 0; ci < B->cells.size(); ++ci) {
+            FlowState fs = B->cells[ci]->fs;
+            fs.gas.rho = rho[ci];
+            fs.vel = Vec3(vx[ci], vy[ci], vz[ci]);
+            fs.gas.p = p[ci];
+            fs.gas.a = a[ci];
+            fs.gas.u = u[ci];
+            fs.gas.T = T[ci];
+            fs.gas.massf.assign(nsp, 1.0);
+            if (massf)
+                for (int i = 0; i < nsp; ++i) fs.gas.massf[i] = massf[ci * nsp + i];
+            fs.S = S ? S[ci] : 0.0;
+            B->set_flowstate((int)ci, fs);
+            B->cells[ci]->dt_chem = -1.0;
+        }
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 // field: 0 rho, 1 vx, 2 vy, 3 vz, 4 u, 5 p, 6 T, 7 a, 8 volume, 9 iLength, 10 jLength,
-// 11 kLength, 12 S (shock indicator), 100+j: U[0][j], 200+j: dUdt[0][j], 300+isp: massf[isp]
+// 11 kLength, 12 S (shock indicator), 13-15 pos (centroid), 16 dt_chem, 100+j: U[0][j], 200+j: dUdt[0][j], 300+isp: massf[isp]
 int orc_block_get(void* h, int b, int field, double* out) {
     Sim* s = static_cast<Sim*>(h);
     try {
@@ -185,6 +217,10 @@ int orc_block_get(void* h, int b, int field, double* out) {
                 case 10: v = c->jLength; break;
                 case 11: v = c->kLength; break;
                 case 12: v = c->fs.S; break;
+                case 13: v = c->pos.x; break;
+                case 14: v = c->pos.y; break;
+                case 15: v = c->pos.z; break;
+                case 16: v = c->dt_chem; break;
                 default: g_err = "bad field"; return -2;
             }
             out[ci] = v;
@@ -309,6 +345,8 @@ void orc_sim_set_dt(void* h, double dt_global, int step) {
     s->sim->st.dt_global = dt_global;
     s->sim->st.step = step;
 }
+
+void orc_sim_set_time(void* h, double time) { static_cast<Sim*>(h)->sim->st.time = time; }
 
 // Unit-level flux calculators: L, R = [rho, p, T, u, a, velx, vely, velz] in the local frame.
 int orc_flux(void* gas, int flux_calc, int dims, const double* L, const double* R, const double* massfL,
