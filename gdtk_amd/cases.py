@@ -432,6 +432,161 @@ def billig_shock_x(y, R, mach):
     return R + delta - rc * cot2 * (np.sqrt(1.0 + (y * np.tan(theta) / rc) ** 2) - 1.0)
 
 
+def _bezier_spline_path(points):
+    """The reference's Spline through points (src/geom/gpath/polyline.d:52-108): natural
+    spline weight points from d[i-1] + 4 d[i] + d[i+1] = 6 p[i] (solved exactly here; the
+    reference iterates Gauss-Seidel to 1e-10), one cubic Bezier per interval with control
+    points (2 d[i] + d[i+1])/3 and (d[i] + 2 d[i+1])/3, joined as a Polyline whose parameter
+    breakpoints are the segments' cumulative lengths (polyline.d:142-163, 281-290), each
+    length the sum of 100 chords (path.d:80-101)."""
+    P = np.asarray(points, dtype=float)
+    m = len(P) - 1
+    A = np.zeros((m + 1, m + 1))
+    rhs = np.zeros_like(P)
+    A[0, 0] = A[m, m] = 1.0
+    rhs[0], rhs[m] = P[0], P[m]
+    for i in range(1, m):
+        A[i, i - 1], A[i, i], A[i, i + 1] = 1.0, 4.0, 1.0
+        rhs[i] = 6.0 * P[i]
+    D = np.linalg.solve(A, rhs)
+    segs = [(P[i], (2.0 * D[i] + D[i + 1]) / 3.0, (D[i] + 2.0 * D[i + 1]) / 3.0, P[i + 1]) for i in range(m)]
+
+    def bez(seg, t):
+        t = np.asarray(t, dtype=float)[:, None]
+        b0, b1, b2, b3 = seg
+        return (1 - t) ** 3 * b0 + 3 * (1 - t) ** 2 * t * b1 + 3 * (1 - t) * t ** 2 * b2 + t ** 3 * b3
+
+    def chord_length(f):
+        q = f(np.linspace(0.0, 1.0, 101))
+        return float(np.sum(np.hypot(*np.diff(q, axis=0).T)))
+
+    L = np.cumsum([chord_length(lambda t, sg=sg: bez(sg, t)) for sg in segs])
+    tv = L / L[-1]
+
+    def path(t):
+        t = np.atleast_1d(np.asarray(t, dtype=float))
+        out = np.empty((len(t), 2))
+        for k, tk in enumerate(t):
+            i = int(np.searchsorted(tv, tk, side="left"))
+            i = min(i, m - 1)
+            tl = tk / tv[0] if i == 0 else (tk - tv[i - 1]) / (tv[i] - tv[i - 1])
+            out[k] = bez(segs[i], [tl])[0]
+        return out
+    return path
+
+
+def _arc_length_parameterized(path, N=100):
+    """ArcLengthParameterizedPath (src/geom/gpath/modifiedpath.d:55-148): arc lengths of N
+    chords of the underlying path, then a reverse linear look-up of the parameter."""
+    q = path(np.linspace(0.0, 1.0, N + 1))
+    arc = np.concatenate([[0.0], np.cumsum(np.hypot(*np.diff(q, axis=0).T))])
+
+    def f(t):
+        t = np.atleast_1d(np.asarray(t, dtype=float))
+        u = np.empty_like(t)
+        dt = 1.0 / N
+        for k, tk in enumerate(t):
+            if tk <= 0.0:
+                u[k] = 0.0
+                continue
+            if tk >= 1.0:
+                u[k] = 1.0
+                continue
+            Lt = tk * arc[-1]
+            i = N
+            while Lt < arc[i] and i > 0:
+                i -= 1
+            frac = (Lt - arc[i]) / (arc[i + 1] - arc[i])
+            u[k] = (1.0 - frac) * dt * i + frac * dt * (i + 1)
+        return path(u)
+    return f
+
+
+def roberts(t, end0, end1, beta):
+    """RobertsFunction (src/geom/misc/univariatefunctions.d:110-182)."""
+    t = np.asarray(t, dtype=float)
+    if (not end0 and not end1) or beta <= 1.0:
+        return t
+    alpha = 0.5 if (end0 and end1) else 0.0
+    reverse = end0 and not end1
+    x = 1.0 - t if reverse else t
+    lam = ((beta + 1.0) / (beta - 1.0)) ** ((x - alpha) / (1.0 - alpha))
+    xb = ((beta + 2.0 * alpha) * lam - beta + 2.0 * alpha) / ((2.0 * alpha + 1.0) * (1.0 + lam))
+    return 1.0 - xb if reverse else xb
+
+
+def _nasa9_cp_over_R(th, T):
+    """Cp/R of one NASA-9 segment set at T away from a blending range."""
+    k = 0
+    while k + 1 < len(th["coeffs"]) and T > th["T_breaks"][k + 1]:
+        k += 1
+    a = th["coeffs"][k]
+    return a[0] / T ** 2 + a[1] / T + a[2] + a[3] * T + a[4] * T ** 2 + a[5] * T ** 3 + a[6] * T ** 4
+
+
+def n90_blocks(niv=61, njv=41, njb=4):
+    """The reference's cylinder in dissociating nitrogen
+    (examples/eilmer/2D/cylinder-dlr-n90/cpu-chem/n90.lua): N2/N thermally perfect gas,
+    inflow p = 500 Pa, T = 700 K, u = 5000 m/s of N2 onto a cylinder of radius 0.045 m
+    centred at x = 0.045; the gas initially at p = 5 Pa, T = 300 K.  The grid is
+    billig_patch.make_patch{Minf, R, xc = R, scale = 0.95} (src/lib/billig_patch.lua:11-120,
+    billig.lua): a Coons patch between the scaled Billig shock shape (a spline through nine
+    sample points, arc-length parameterised), the body arc and two lines, niv x njv vertices
+    with Roberts clustering (end0, beta = 1.1) on the west and east edges
+    (src/geom/grid/sgrid.d:596-617); split into njb blocks along j.  West: supersonic
+    inflow; north of the last block: simple outflow (flux); east (the body) and the axis:
+    slip walls.  Returns (blocks, gas record, info with Minf and the probe point)."""
+    from gdtk_amd import chem, data
+    d = data.load("tpg-nitrogen-2sp")
+    rec = chem.gas_record(d)
+    p_inf, T_inf, u_inf = 500.0, 700.0, 5000.0
+    Rg = R_UNIVERSAL / d["M"][0]
+    cp = Rg * _nasa9_cp_over_R(d["thermo"][0], T_inf)
+    a_inf = np.sqrt(cp / (cp - Rg) * Rg * T_inf)
+    Minf = u_inf / a_inf
+    R, xc, scale = 0.045, 0.045, 0.95
+    ys = np.array([0.0, 0.2, 0.4, 0.6, 1.0, 1.4, 1.6, 2.0, 2.37]) * R
+    xs = billig_shock_x(ys, R, Minf)
+    pts = np.stack([-scale * xs + xc, scale * ys], axis=1)
+    shock = _arc_length_parameterized(_bezier_spline_path(pts))
+    a_pt, b_pt = np.array([xc - R, 0.0]), np.array([xc, R])
+
+    def body(t):
+        th = 0.5 * np.pi * np.asarray(t, dtype=float)
+        return np.stack([xc - R * np.cos(th), R * np.sin(th)], axis=1)
+    r = np.linspace(0.0, 1.0, niv)
+    sj = roberts(np.linspace(0.0, 1.0, njv), True, False, 1.1)
+    grid = coons_patch(_line(pts[0], a_pt), _line(pts[-1], b_pt), shock, body, r, sj)
+    nj_block = (njv - 1) // njb
+    ni = niv - 1
+    Y = np.array([1.0, 0.0])
+    blocks = []
+    for bj in range(njb):
+        sub = grid[bj * nj_block:(bj + 1) * nj_block + 1]
+        v = np.zeros((sub.shape[0] * sub.shape[1], 3))
+        v[:, 0] = sub[..., 0].reshape(-1)
+        v[:, 1] = sub[..., 1].reshape(-1)
+        n = ni * nj_block
+        b = BlockCase(2, ni, nj_block, 1, v, np.full(n, 5.0), np.full(n, 300.0), np.zeros((n, 3)),
+                      [BC_INFLOW_SUPERSONIC, BC_SLIP_WALL, BC_SLIP_WALL, BC_SLIP_WALL, BC_SLIP_WALL, BC_SLIP_WALL])
+        b.massf = np.tile(Y, (n, 1))
+        b.inflow = np.zeros((6, 5))
+        b.inflow[WEST] = [p_inf, T_inf, u_inf, 0.0, 0.0]
+        b.inflow_massf = np.tile(Y, (6, 1))
+        if bj > 0:
+            b.bc[SOUTH] = BC_EXCHANGE
+            b.other_block[SOUTH] = bj - 1
+            b.other_face[SOUTH] = NORTH
+        if bj < njb - 1:
+            b.bc[NORTH] = BC_EXCHANGE
+            b.other_block[NORTH] = bj + 1
+            b.other_face[NORTH] = SOUTH
+        else:
+            b.bc[NORTH] = BC_OUTFLOW_SIMPLE_FLUX
+        blocks.append(b)
+    return blocks, rec, dict(Minf=Minf, a_inf=a_inf, probe=(-0.010, 0.0), max_time=100.0e-6, dt_init=5.0e-9)
+
+
 def cylinder_blocks(ni=256, nj_block=128, njb=8, R=0.01, mach=10.0, p_inf=500.0, T_inf=300.0, margin=1.15,
                     beta_i=1.0):
     """The coupled 2-D Mach-10 cylinder: 5-species air at p = 500 Pa, T = 300 K, Mach 10, on a

@@ -234,7 +234,7 @@ __device__ __forceinline__ void face_flux_one(const BlockDev& b, const IdealGasD
                     R1 = load_sc(fs, np, cR1);
         FallbackFS fb{fs, np, cL0, cR0};
         // Adaptive calculator: EFM where the face is shocked (strict rule), else AUSMDV.
-        const bool shocked = (FC == 2) && ((b.shock[cL0] | b.shock[cR0]) != 0);
+        const bool shocked = flux_is_adaptive(FC) && ((b.shock[cL0] | b.shock[cR0]) != 0);
         face_flux_l2r2<DIM, FC>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cR0 + FS_A * np], fb, len[cL1],
                                 len[cL0], len[cR0], len[cR1], eps, F, shocked);
     }
@@ -469,11 +469,15 @@ hipError_t launch_face_flux_batch(const BlockDev* tab, const int64_t* start, int
     dim3 grid(nblk(total, 256)), blk(256);
 #define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas)
     if (dims == 3) {
-        if (flux_calc == 2) GDTK_FACE(3, 2);
+        if (flux_calc == 4) GDTK_FACE(3, 4);
+        else if (flux_calc == 3) GDTK_FACE(3, 3);
+        else if (flux_calc == 2) GDTK_FACE(3, 2);
         else if (flux_calc == 1) GDTK_FACE(3, 1);
         else GDTK_FACE(3, 0);
     } else {
-        if (flux_calc == 2) GDTK_FACE(2, 2);
+        if (flux_calc == 4) GDTK_FACE(2, 4);
+        else if (flux_calc == 3) GDTK_FACE(2, 3);
+        else if (flux_calc == 2) GDTK_FACE(2, 2);
         else if (flux_calc == 1) GDTK_FACE(2, 1);
         else GDTK_FACE(2, 0);
     }

@@ -108,11 +108,15 @@ static void face_species(const BlockDev* tab, const int64_t* start, int nb, int6
                          double eps, int d, const ChemDev* cd, hipStream_t s) {
     dim3 g(nblk256(total)), blk(256);
     if (dims == 3) {
-        if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<3, 2, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<3, 4, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<3, 3, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<3, 2, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
         else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<3, 1, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
         else hipLaunchKernelGGL((k_face_flux_species<3, 0, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
     } else {
-        if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<2, 2, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<2, 4, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<2, 3, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<2, 2, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
         else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<2, 1, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
         else hipLaunchKernelGGL((k_face_flux_species<2, 0, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
     }

@@ -168,9 +168,11 @@ __device__ __forceinline__ bool face_flux_species(const BlockDev& b, const ChemD
 #pragma unroll
         for (int j = 0; j < C::n; ++j) F[j] = 0.0;
         // FC 2: adaptive_efm_ausmdv (fluxcalc.d:1315-1332), EFM where the face is shocked.
-        const bool shocked = (FC == 2) && ((b.shock[cL0] | b.shock[cR0]) != 0);
-        if (FC == 1 || shocked)
+        const bool shocked = flux_is_adaptive(FC) && ((b.shock[cL0] | b.shock[cR0]) != 0);
+        if (FC == 1 || (FC == 2 && shocked))
             efmflx<DIM, NSP>(Ls, Rs, tpg_Cv<NSP>(cd, Ls.massf, Ls.T), tpg_Cv<NSP>(cd, Rs.massf, Rs.T), F);
+        else if (FC == 3 || (FC == 4 && shocked))
+            hanel<DIM, NSP>(Ls, Rs, F);
         else
             ausmdv<DIM, NSP>(Ls, Rs, F);
         // The grid-velocity terms of fluxcalc.d:159-171 add signed zeros (see face_flux_l2r2).

@@ -298,6 +298,10 @@ struct StencilCell {
 // EOS-failure fallback.
 // Where the face EOS call fails, the reference copies the whole cell state (onedinterp.d:45-68);
 // p and T of the cells L0/R0 are then read from the flow-state planes.
+// Flux-calculator codes: 0 AUSMDV, 1 EFM, 2 adaptive EFM/AUSMDV, 3 Hanel, 4 adaptive
+// Hanel/AUSMDV.  The adaptive ones read the faces' shock indicators.
+__host__ __device__ constexpr bool flux_is_adaptive(int fc) { return fc == 2 || fc == 4; }
+
 struct FallbackFS {
     const double* fs;
     int64_t np, cL0, cR0;
@@ -337,9 +341,10 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
     Ls.vel = to_local(to_global(Ls.vel, g), g);
     Rs.vel = to_local(to_global(Rs.vel, g), g);
     for (int j = 0; j < C::n; ++j) F[j] = 0.0;
-    // FC 2: adaptive_efm_ausmdv (fluxcalc.d:1315-1332) with the face's indicator 0 or 1, so
-    // one calculator runs with factor 1.
+    // FC 2, 4: adaptive_efm_ausmdv, adaptive_hanel_ausmdv (fluxcalc.d:1315-1348) with the
+    // face's indicator 0 or 1, so one calculator runs with factor 1.
     if (FC == 1 || (FC == 2 && shocked)) efmflx<DIM>(Ls, Rs, gas.Cv, gas.Cv, F);
+    else if (FC == 3 || (FC == 4 && shocked)) hanel<DIM>(Ls, Rs, F);
     else ausmdv<DIM>(Ls, Rs, F);
     // fluxcalc.d:159-171 add grid-velocity terms, here with gvel = 0: each adds a signed
     // zero to a flux component.  Every component starts at +0 and is only ever changed by
@@ -355,6 +360,51 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
 
 // boundary_flux_effect.d:586-645, the simple-outflow flux from the interior cell (massf:
 // its mass fractions, read for NSP > 1).
+// Hanel, Schwane and Seider's flux-vector splitting, fluxcalc.d:1028-1128 (factor 1, so
+// each component is the expression itself): van Leer velocity and pressure splitting with
+// each side's own sound speed.
+template <int DIM, int NSP = 1>
+__device__ __forceinline__ void hanel(const FaceStateT<NSP>& L, const FaceStateT<NSP>& R, double* F) {
+    using C = CQ<DIM, NSP>;
+    const double rL = L.rho, pL = L.p;
+    const double pLrL = pL / rL;
+    const double uL = L.vel.x, vL = L.vel.y, wL = L.vel.z;
+    const double eL = L.u, aL = L.a;
+    const double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    const double HL = eL + pLrL + keL;
+    const double rR = R.rho, pR = R.p;
+    const double pRrR = pR / rR;
+    const double uR = R.vel.x, vR = R.vel.y, wR = R.vel.z;
+    const double eR = R.u, aR = R.a;
+    const double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    const double HR = eR + pRrR + keR;
+    double pLplus, uLplus;
+    if (fabs(uL) <= aL) {
+        uLplus = 1.0 / (4.0 * aL) * (uL + aL) * (uL + aL);
+        pLplus = pL * uLplus * (1.0 / aL * (2.0 - uL / aL));
+    } else {
+        uLplus = 0.5 * (uL + fabs(uL));
+        pLplus = pL * uLplus * (1.0 / uL);
+    }
+    double pRminus, uRminus;
+    if (fabs(uR) <= aR) {
+        uRminus = -1.0 / (4.0 * aR) * (uR - aR) * (uR - aR);
+        pRminus = pR * uRminus * (1.0 / aR * (-2.0 - uR / aR));
+    } else {
+        uRminus = 0.5 * (uR - fabs(uR));
+        pRminus = pR * uRminus * (1.0 / uR);
+    }
+    const double p_half = pLplus + pRminus;
+    F[C::mass] += uLplus * rL + uRminus * rR;
+    F[C::xMom] += uLplus * rL * uL + uRminus * rR * uR + p_half;
+    F[C::yMom] += uLplus * rL * vL + uRminus * rR * vR;
+    if (DIM == 3) F[C::zMom] += uLplus * rL * wL + uRminus * rR * wR;
+    F[C::totEnergy] += uLplus * rL * HL + uRminus * rR * HR;
+    if (NSP > 1)
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) F[C::species + i] += uLplus * rL * L.massf[i] + uRminus * rR * R.massf[i];
+}
+
 template <int DIM, int NSP = 1>
 __device__ __forceinline__ void outflow_flux(const FaceGeom& g, int outsign, double rho, double u, double p, Vec3d vel,
                                              double* F, const double* massf = nullptr) {

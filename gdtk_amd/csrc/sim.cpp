@@ -217,7 +217,7 @@ struct Sim {
         }
     }
 
-    bool adaptive() const { return cfg.flux_calculator == GDTK_FLUX_ADAPTIVE_EFM_AUSMDV; }
+    bool adaptive() const { return flux_is_adaptive(cfg.flux_calculator); }
 
     // Whether this step's stage 1 detects shocks (simcore_gasdynamic_step.d:1033-1035).
     bool detect_now() const {
@@ -822,10 +822,10 @@ struct Sim {
     }
 
     // The fused stage kernel serves the single-species gas with the AUSMDV or EFM
-    // calculator; a multi-species gas and the adaptive calculator run the face-per-thread
-    // kernels.
+    // calculator; a multi-species gas and the Hanel and adaptive calculators run the
+    // face-per-thread kernels.
     bool use_fused() const {
-        if (nsp > 1 || adaptive()) return false;
+        if (nsp > 1 || cfg.flux_calculator > GDTK_FLUX_EFM) return false;
         return variant >= 0 ? variant == 1 : cfg.use_fused != 0;
     }
 
@@ -900,7 +900,7 @@ struct Sim {
         };
         result.push_back(0); result.push_back(0); result.push_back(best); result.push_back(0);
         for (int tx : txs) {
-            if (nsp > 1 || adaptive()) break;  // the fused kernel: single-species gas, AUSMDV or EFM
+            if (nsp > 1 || cfg.flux_calculator > GDTK_FLUX_EFM) break;  // fused: one species, AUSMDV or EFM
             if (cfg.dimensions == 3) {
                 for (int kc : kcs) add(1, kc, tx, time_variant(1, kc, tx));
             } else {
@@ -1198,8 +1198,7 @@ int gdtk_hip_sim_create(const gdtk_sim_config* cfg, int* sim) {
     return guarded([&] {
         if (!cfg || !sim) throw Error(GDTK_ERR_ARG, "null argument");
         if (cfg->dimensions != 2 && cfg->dimensions != 3) throw Error(GDTK_ERR_ARG, "dimensions must be 2 or 3");
-        if (cfg->flux_calculator != GDTK_FLUX_AUSMDV && cfg->flux_calculator != GDTK_FLUX_EFM &&
-            cfg->flux_calculator != GDTK_FLUX_ADAPTIVE_EFM_AUSMDV)
+        if (cfg->flux_calculator < GDTK_FLUX_AUSMDV || cfg->flux_calculator > GDTK_FLUX_ADAPTIVE_HANEL_AUSMDV)
             throw Error(GDTK_ERR_UNSUPPORTED, "flux calculator not implemented");
         if (cfg->interpolation_order != 2) throw Error(GDTK_ERR_UNSUPPORTED, "only interpolation_order 2 is implemented");
         if (!(cfg->mMass > 0.0) || !(cfg->gamma > 1.0)) throw Error(GDTK_ERR_ARG, "bad gas parameters");

@@ -3,6 +3,8 @@ Lua data files (run once where the reference tree is mounted; the JSON is commit
 
   tpg-air-5sp.json      thermally perfect 5-species air, species N2, O2, NO, N, O
                         (src/gas/sample-data/therm-perf-5-species-air.lua)
+  tpg-nitrogen-2sp.json N2, N of the same curves (the n90 cylinder's nitrogen-2sp.inp)
+  nitrogen-2sp-2r.json  the n90 cylinder's 2-reaction N2 dissociation mechanism
   park-air-5sp-1T.json  one-temperature Park 5-reaction mechanism authored from
                         examples/lmr/3D/cylinder-larc/rm-park-5sp-2T.inp: the 'Park'
                         rate constants (T^s Tv^(1-s), identical to T when Tv = T) become
@@ -96,12 +98,60 @@ def park_1T(gas):
             "reactions": reactions}
 
 
+def arrhenius_si(t, ncoeff):
+    """An 'Arrhenius' rate of a Lua Reaction table in SI units (prep-chem's conversion,
+    src/kinetics/reaction.lua:26-45, 486-491, written with "%.12e")."""
+    conv = 1.0e-6 ** (ncoeff - 1)
+    return {"model": "Arrhenius", "A": float("%.12e" % (t["A"] * conv)), "n": float(t["n"]), "C": float(t["C"]),
+            "A_cgs": t["A"]}
+
+
+def nitrogen_2sp_2r(species):
+    """examples/eilmer/2D/cylinder-dlr-n90/cpu-chem/nitrogen-2sp-2r.lua: N2 dissociation by
+    N2 and by N, both directions given as Arrhenius rates (elementary reactions, no
+    anonymous collider)."""
+    _, calls = parse_file(os.path.join(REF, "examples/eilmer/2D/cylinder-dlr-n90/cpu-chem/nitrogen-2sp-2r.lua"))
+    reactions = []
+    for name, t in calls:
+        assert name == "Reaction"
+        eq = t[1]
+        lhs, rhs = eq.split("<=>")
+        reac, colL = parse_side(lhs, species)
+        prod, colR = parse_side(rhs, species)
+        assert not colL and not colR
+        reactions.append({"equation": eq, "label": "", "type": "elementary",
+                          "reactants": [[i, c] for i, c in sorted(reac.items())],
+                          "products": [[i, c] for i, c in sorted(prod.items())],
+                          "frc": arrhenius_si(t["fr"], sum(reac.values())),
+                          "brc": arrhenius_si(t["br"], sum(prod.values()))})
+    return {"species": species, "source": "examples/eilmer/2D/cylinder-dlr-n90/cpu-chem/nitrogen-2sp-2r.lua",
+            "config": {"tempLimits": {"lower": 300.0, "upper": 50000.0},
+                       "odeStep": {"method": "alpha-qss", "eps1": 1.0e-3, "eps2": 5.0e-4, "delta": 1.0e-10,
+                                   "maxIters": 10},
+                       "tightTempCoupling": False, "maxSubcycles": 10000, "maxAttempts": 4},
+            "reactions": reactions}
+
+
+def subset_gas(gas, species, source):
+    idx = [gas["species"].index(s) for s in species]
+    return {"model": gas["model"], "source": source, "species": list(species), "M": [gas["M"][i] for i in idx],
+            "thermo": [gas["thermo"][i] for i in idx]}
+
+
 def main():
     gas = gas_from_db(os.path.join(REF, "src/gas/sample-data/therm-perf-5-species-air.lua"))
     with open(os.path.join(HERE, "tpg-air-5sp.json"), "w") as f:
         json.dump(gas, f, indent=1)
     with open(os.path.join(HERE, "park-air-5sp-1T.json"), "w") as f:
         json.dump(park_1T(gas), f, indent=1)
+    # nitrogen-2sp.inp (N2, N) takes the species database's CEA curves, which are the ones
+    # of the 5-species sample gas (same database, same prep-gas output).
+    n2 = subset_gas(gas, ["N2", "N"], "examples/eilmer/2D/cylinder-dlr-n90/cpu-chem/nitrogen-2sp.inp "
+                                      "(CEA curves as in src/gas/sample-data/therm-perf-5-species-air.lua)")
+    with open(os.path.join(HERE, "tpg-nitrogen-2sp.json"), "w") as f:
+        json.dump(n2, f, indent=1)
+    with open(os.path.join(HERE, "nitrogen-2sp-2r.json"), "w") as f:
+        json.dump(nitrogen_2sp_2r(n2["species"]), f, indent=1)
 
 
 if __name__ == "__main__":

@@ -25,7 +25,8 @@ ERR_NAMES = {-1: "GDTK_ERR_ARG", -2: "GDTK_ERR_HIP", -3: "GDTK_ERR_BAD_CELL", -4
 
 FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11, S=12)
 # Flux calculators (src/eilmer/globalconfig.d:270-333 names; "adaptive" is the reference's alias).
-FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2}
+FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2, "hanel": 3,
+                    "adaptive_hanel_ausmdv": 4}
 # Shock-detector options and defaults (src/eilmer/globalconfig.d:1061-1131).
 SHOCK_DETECTOR_DEFAULTS = dict(compression_tolerance=-0.30, shear_tolerance=0.20, strict_shock_detector=True,
                                damped_outflow=False, frozen_shock_detector=False,

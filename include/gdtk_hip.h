@@ -37,6 +37,11 @@ extern "C" {
 /* adaptive_efm_ausmdv (fluxcalc.d:1315-1332, "adaptive"): EFM on faces the shock detector
  * marks, AUSMDV elsewhere; turns shock detection on (globalconfig.d:2682-2690). */
 #define GDTK_FLUX_ADAPTIVE_EFM_AUSMDV 2
+/* hanel (fluxcalc.d:1028-1128), and adaptive_hanel_ausmdv (fluxcalc.d:1335-1348, the
+ * reference's default calculator, globalconfig.d:1031): Hanel on marked faces, AUSMDV
+ * elsewhere, with shock detection on. */
+#define GDTK_FLUX_HANEL 3
+#define GDTK_FLUX_ADAPTIVE_HANEL_AUSMDV 4
 
 /* Boundary conditions (src/eilmer/bc.lua:757, 1320, 1580, 1601, 1650). */
 #define GDTK_BC_SLIP_WALL 0
@@ -188,7 +193,7 @@ int gdtk_hip_flow_read_records(const char* filename, int format, int dims, int n
                                const char* species_names, double* records, double* sim_time);
 
 /* Shock-detector options of GlobalConfig (src/eilmer/globalconfig.d:1061-1131), read when the
- * flux calculator is GDTK_FLUX_ADAPTIVE_EFM_AUSMDV.  Detection (detect_shocks,
+ * flux calculator is adaptive (GDTK_FLUX_ADAPTIVE_EFM_AUSMDV, GDTK_FLUX_ADAPTIVE_HANEL_AUSMDV).  Detection (detect_shocks,
  * simcore_gasdynamic_step.d:3195-3224) runs in stage 1 of every step after the ghost-cell
  * fill, with the PJ detector (shockdetectors.d:21-113), no smoothing and the strict rule
  * (fluidblock.d:582-605). */

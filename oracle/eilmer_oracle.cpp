@@ -213,6 +213,57 @@ void exxef(double sn, double& exx, double& ef) {
     ef = std::copysign(ef1, sn);
 }
 
+
+// fluxcalc.d:1028-1128 (Hanel, Schwane and Seider's flux-vector splitting)
+void hanel(const FlowState& Lft, const FlowState& Rght, std::vector<double>& F, const CQI& cqi, const GasModel& gm,
+           double factor) {
+    double rL = Lft.gas.rho;
+    double pL = Lft.gas.p;
+    double pLrL = pL / rL;
+    double uL = Lft.vel.x;
+    double vL = Lft.vel.y;
+    double wL = Lft.vel.z;
+    double eL = gm.internal_energy(Lft.gas);
+    double aL = Lft.gas.a;
+    double keL = 0.5 * (uL * uL + vL * vL + wL * wL);
+    double HL = eL + pLrL + keL;
+    double rR = Rght.gas.rho;
+    double pR = Rght.gas.p;
+    double pRrR = pR / rR;
+    double uR = Rght.vel.x;
+    double vR = Rght.vel.y;
+    double wR = Rght.vel.z;
+    double eR = gm.internal_energy(Rght.gas);
+    double aR = Rght.gas.a;
+    double keR = 0.5 * (uR * uR + vR * vR + wR * wR);
+    double HR = eR + pRrR + keR;
+    double pLplus, uLplus;
+    if (std::fabs(uL) <= aL) {
+        uLplus = 1.0 / (4.0 * aL) * (uL + aL) * (uL + aL);
+        pLplus = pL * uLplus * (1.0 / aL * (2.0 - uL / aL));
+    } else {
+        uLplus = 0.5 * (uL + std::fabs(uL));
+        pLplus = pL * uLplus * (1.0 / uL);
+    }
+    double pRminus, uRminus;
+    if (std::fabs(uR) <= aR) {
+        uRminus = -1.0 / (4.0 * aR) * (uR - aR) * (uR - aR);
+        pRminus = pR * uRminus * (1.0 / aR * (-2.0 - uR / aR));
+    } else {
+        uRminus = 0.5 * (uR - std::fabs(uR));
+        pRminus = pR * uRminus * (1.0 / uR);
+    }
+    double p_half = pLplus + pRminus;
+    F[cqi.mass] += factor * (uLplus * rL + uRminus * rR);
+    F[cqi.xMom] += factor * (uLplus * rL * uL + uRminus * rR * uR + p_half);
+    F[cqi.yMom] += factor * (uLplus * rL * vL + uRminus * rR * vR);
+    if (cqi.threeD) F[cqi.zMom] += factor * (uLplus * rL * wL + uRminus * rR * wR);
+    F[cqi.totEnergy] += factor * (uLplus * rL * HL + uRminus * rR * HR);
+    if (cqi.n_species > 1)
+        for (int i = 0; i < cqi.n_species; ++i)
+            F[cqi.species + i] += factor * (uLplus * rL * Lft.gas.massf[i] + uRminus * rR * Rght.gas.massf[i]);
+}
+
 // fluxcalc.d:1131-1277
 void efmflx(const FlowState& Lft, const FlowState& Rght, std::vector<double>& F, const CQI& cqi,
             const GasModel& gm, double factor) {
@@ -941,6 +992,14 @@ void SFluidBlock::compute_interface_flux(FlowState& L, FlowState& R, FVInterface
         // fluxcalc.d:1315-1332: EFM weighted by the face's shock indicator, AUSMDV by the rest.
         double alpha = f->S;
         if (alpha > 0.0) efmflx(L, R, f->F, cqi, *gmodel, alpha);
+        if (alpha < 1.0) ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0 - alpha);
+        break;
+    }
+    case FluxCalc::hanel: hanel(L, R, f->F, cqi, *gmodel); break;
+    case FluxCalc::adaptive_hanel_ausmdv: {
+        // fluxcalc.d:1335-1348
+        double alpha = f->S;
+        if (alpha > 0.0) hanel(L, R, f->F, cqi, *gmodel, alpha);
         if (alpha < 1.0) ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0 - alpha);
         break;
     }

@@ -114,7 +114,7 @@ public:
     double R(const GasState&) const override { return Rgas; }
 };
 
-enum class FluxCalc { ausmdv = 0, efm = 1, adaptive_efm_ausmdv = 2 };
+enum class FluxCalc { ausmdv = 0, efm = 1, adaptive_efm_ausmdv = 2, hanel = 3, adaptive_hanel_ausmdv = 4 };
 enum class BCKind { slip_wall = 0, inflow_supersonic = 1, outflow_simple_flux = 2,
                     outflow_simple_extrapolate = 3, exchange = 4 };
 
@@ -301,6 +301,8 @@ void ausmdv(const FlowState& L, const FlowState& R, std::vector<double>& F, cons
             const GasModel& gm, bool entropy_fix, double factor = 1.0);
 void efmflx(const FlowState& L, const FlowState& R, std::vector<double>& F, const CQI& cqi,
             const GasModel& gm, double factor = 1.0);
+void hanel(const FlowState& L, const FlowState& R, std::vector<double>& F, const CQI& cqi, const GasModel& gm,
+           double factor = 1.0);
 void exxef(double sn, double& exx, double& ef);
 // shockdetectors.d:21-113
 double PJ_shock_detector(const FVInterface& f, double comp_tol, double shear_tol);
@@ -332,7 +334,9 @@ public:
     explicit Simulation(const Config& c) : cfg(c) {
         st.dt_global = cfg.dt_init;
         // configCheckPoint1, globalconfig.d:2682-2690
-        if (cfg.flux_calculator == FluxCalc::adaptive_efm_ausmdv) cfg.do_shock_detect = true;
+        if (cfg.flux_calculator == FluxCalc::adaptive_efm_ausmdv ||
+            cfg.flux_calculator == FluxCalc::adaptive_hanel_ausmdv)
+            cfg.do_shock_detect = true;
     }
     ~Simulation();
     void determine_time_step_size();

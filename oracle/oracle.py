@@ -102,7 +102,8 @@ FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, 
               pos_z=15, dt_chem=16)
 
 
-FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2}
+FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2, "hanel": 3,
+                    "adaptive_hanel_ausmdv": 4}
 # globalconfig.d:1061-1131
 SHOCK_DETECTOR_DEFAULTS = dict(compression_tolerance=-0.30, shear_tolerance=0.20, strict_shock_detector=True,
                                damped_outflow=False, frozen_shock_detector=False,
@@ -269,7 +270,7 @@ def flux(gas, kind, dims, L, R, massfL=None, massfR=None):
         ml, mL = _d(massfL)
     if massfR is not None:
         mr, mR = _d(massfR)
-    n = lib().orc_flux(gas.h, {"ausmdv": 0, "efm": 1}[kind], dims, ap, bp, mL, mR, F.ctypes.data_as(dp))
+    n = lib().orc_flux(gas.h, {"ausmdv": 0, "efm": 1, "hanel": 3}[kind], dims, ap, bp, mL, mR, F.ctypes.data_as(dp))
     return F[:n].copy()
 
 

@@ -58,8 +58,7 @@ void* orc_sim_create(void* gas, int dims, int flux_calc, double cfl, double dt_i
                      int mutate_cell_velocities) {
     Config c;
     c.dimensions = dims;
-    c.flux_calculator = (flux_calc == 2) ? FluxCalc::adaptive_efm_ausmdv
-                        : (flux_calc == 1) ? FluxCalc::efm : FluxCalc::ausmdv;
+    c.flux_calculator = static_cast<FluxCalc>((flux_calc >= 0 && flux_calc <= 4) ? flux_calc : 0);
     c.cfl_value = cfl;
     c.dt_init = dt_init;
     c.dt_max = dt_max;
@@ -363,6 +362,7 @@ int orc_flux(void* gas, int flux_calc, int dims, const double* L, const double* 
     if (massfR) for (int i = 0; i < gm->n_species; ++i) r.gas.massf[i] = massfR[i];
     std::vector<double> Fv(cqi.n, 0.0);
     if (flux_calc == 1) efmflx(l, r, Fv, cqi, *gm);
+    else if (flux_calc == 3) hanel(l, r, Fv, cqi, *gm);
     else ausmdv(l, r, Fv, cqi, *gm, true);
     for (int j = 0; j < cqi.n; ++j) F[j] = Fv[j];
     return cqi.n;

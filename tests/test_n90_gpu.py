@@ -1,0 +1,77 @@
+"""The reference's own integration test of the reacting multi-species path, on the GPU:
+examples/eilmer/2D/cylinder-dlr-n90/cpu-chem (n90.lua, n90-test.rb) -- nitrogen at Mach
+9.4 on a cylinder, N2/N thermally perfect gas with the 2-reaction dissociation mechanism,
+operator-split chemistry, the reference's default flux calculator adaptive_hanel_ausmdv,
+4 blocks joined by exchange boundaries, from an impulsive start to 100 us.
+
+n90-test.rb:35 expects 4428 +/- 30 steps; :43-49 expects, at the probe (-0.010, 0, 0) at
+100 us, rho 1.9090e-2 (1 %), vel.x 579.7 (10 %), p 5.3735e4 (1 %), massf[N] 1.749e-2 (15 %)
+and T 9320.5 K (1 %).  The grid is the restatement of billig_patch in
+gdtk_amd.cases.n90_blocks.  A short run is compared with the oracle at the 1e-12 bar first.
+"""
+import numpy as np
+import pytest
+
+from gdtk_amd import cases
+from tests.parity import TOL, field_errors, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(flux="adaptive_hanel_ausmdv"):
+    from gdtk_amd import Simulation, data
+    from gdtk_amd.chem import Chemistry, mech_record
+    blocks, rec, info = cases.n90_blocks()
+    mech = mech_record(data.load("nitrogen-2sp-2r"))
+    g = Simulation(2, flux=flux, tpg=rec, dt_init=info["dt_init"])
+    for b in blocks:
+        g.add_block(b)
+    g.set_chemistry(Chemistry(rec, mech))
+    return g, blocks, rec, mech, info
+
+
+def test_n90_start_against_oracle():
+    from oracle import oracle as O
+    g, blocks, rec, mech, info = _setup()
+    gas = O.TPG(rec)
+    o = O.Sim(gas, 2, "adaptive_hanel_ausmdv", dt_init=info["dt_init"], mutate_cell_velocities=False)
+    for b in blocks:
+        o.add_block(b)
+    o.set_chemistry(O.Chem(rec, mech))
+    _, sg = g.run(25)
+    _, so = o.run(25)
+    assert sg["dt"] == pytest.approx(so["dt"], rel=1e-12)
+    worst = {}
+    marked = 0
+    for q in range(len(blocks)):
+        e = field_errors(lambda k: g.get(q, k), lambda k: o.get(q, k), ("rho", "vx", "vy", "p", "T"))
+        e["YN"] = rel_err(g.get(q, 301), o.get(q, 301), np.ones(blocks[q].ncells))
+        for k, v in e.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+        assert np.array_equal(g.get(q, "S"), o.get(q, "S"))
+        marked += int(o.get(q, "S").sum())
+    print("n90 start, 25 steps: marked", marked, {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    assert all(v <= TOL for v in worst.values()), worst
+
+
+def test_n90_reference_run():
+    g, blocks, rec, mech, info = _setup()
+    n, st = g.run(40000, info["max_time"])
+    print("n90: steps", st["step"], "time", st["time"], flush=True)
+    # the probe: the cell whose centre is nearest to (-0.010, 0)
+    best = (np.inf, None, None)
+    for q, b in enumerate(blocks):
+        xc = cases.cell_centres(b)
+        d = np.hypot(xc[:, 0] - info["probe"][0], xc[:, 1] - info["probe"][1])
+        i = int(np.argmin(d))
+        if d[i] < best[0]:
+            best = (d[i], q, i)
+    _, q, i = best
+    got = {"rho": g.get(q, "rho")[i], "vel.x": g.get(q, "vx")[i], "p": g.get(q, "p")[i],
+           "massf_N": g.get(q, 301)[i], "T": g.get(q, "T")[i]}
+    refs = {"rho": 1.9090e-02, "vel.x": 5.797e+02, "p": 5.3735e+04, "massf_N": 1.749e-02, "T": 9.3205e+03}
+    tols = {"rho": 0.01, "vel.x": 0.1, "p": 0.01, "massf_N": 0.15, "T": 0.01}
+    print("n90 probe", {k: f"{v:.5g}" for k, v in got.items()}, flush=True)
+    assert abs(st["step"] - 4428) < 30, st["step"]
+    for k in refs:
+        assert abs(got[k] - refs[k]) / refs[k] < tols[k], (k, got[k], refs[k])

@@ -98,7 +98,35 @@ def test_efm_consistency(oracle_lib, vx):
     assert np.all(np.abs(F - E) <= 1e-6 * scale)
 
 
-@pytest.mark.parametrize("kind", ["ausmdv", "efm"])
+@pytest.mark.parametrize("vx", [-800.0, -340.0, -120.0, 0.0, 35.0, 250.0, 900.0])
+def test_hanel_consistency(oracle_lib, vx):
+    """Hanel's splitting sums to the Euler flux for equal states: (u+a)^2/4a - (u-a)^2/4a = u
+    and the pressure split p(u+a)^2(2a-u)/4a^3 + p(u-a)^2(2a+u)/4a^3 = p, subsonic or not."""
+    O = oracle_lib
+    g = O.IdealGas()
+    s = _state(0.9, 8.0e4, (vx, 12.0, -7.0))
+    F = O.flux(g, "hanel", 3, s, s)
+    E = _euler_flux(s)
+    scale = np.abs(E) + s[1]
+    assert np.all(np.abs(F - E) <= 1e-13 * scale), (F, E)
+
+
+def test_hanel_split_known_answer(oracle_lib):
+    """Subsonic left state alone (right state with zero density and pressure): the mass flux
+    is rho (u+a)^2/(4a) and the normal momentum flux adds p (u+a)^2 (2a-u)/(4a^3)."""
+    O = oracle_lib
+    g = O.IdealGas()
+    L = _state(1.2, 1.0e5, (100.0, 0.0, 0.0))
+    R = [1.0e-300, 1.0e-300, 300.0, 2.0e5, 347.0, 0.0, 0.0, 0.0]
+    F = O.flux(g, "hanel", 3, L, R)
+    rho, p, a, u = L[0], L[1], L[4], L[5]
+    up = (u + a) ** 2 / (4 * a)
+    # the right state's split pressure is -p_R (0-a)^2/(4a) (-2)/a = p_R/2
+    assert F[0] == pytest.approx(rho * up, rel=1e-14)
+    assert F[1] == pytest.approx(rho * up * u + p * up * (2 - u / a) / a + 0.5e-300, rel=1e-14)
+
+
+@pytest.mark.parametrize("kind", ["ausmdv", "efm", "hanel"])
 def test_supersonic_upwinding(oracle_lib, kind):
     O = oracle_lib
     g = O.IdealGas()
@@ -106,14 +134,14 @@ def test_supersonic_upwinding(oracle_lib, kind):
     R = _state(1.3, 9.0e4, (2500.0, -4.0, 3.0))
     F = O.flux(g, kind, 3, L, R)
     E = _euler_flux(L)
-    if kind == "ausmdv":
+    if kind in ("ausmdv", "hanel"):
         assert np.allclose(F, E, rtol=1e-12, atol=1e-9)
     else:
         # EFM keeps an exp(-25)-sized contribution of the far side (EXLIM, fluxcalc.d:1296).
         assert np.allclose(F, E, rtol=1e-6, atol=1e-9 * np.max(np.abs(E)))
 
 
-@pytest.mark.parametrize("kind", ["ausmdv", "efm"])
+@pytest.mark.parametrize("kind", ["ausmdv", "efm", "hanel"])
 def test_mirror_symmetry(oracle_lib, kind):
     """Swapping sides and reversing normal velocities reverses the mass/energy flux."""
     O = oracle_lib
