@@ -44,7 +44,13 @@ def test_n90_start_against_oracle():
     worst = {}
     marked = 0
     for q in range(len(blocks)):
-        e = field_errors(lambda k: g.get(q, k), lambda k: o.get(q, k), ("rho", "vx", "vy", "p", "T"))
+        e = field_errors(lambda k: g.get(q, k), lambda k: o.get(q, k), ("rho", "p", "T"))
+        # The gas ahead of the shock is at rest: its velocities are rounding noise of the
+        # curvilinear fluxes, so velocity errors are taken relative to |v| + a, the scale
+        # on which the flow moves.
+        scale = np.hypot(o.get(q, "vx"), o.get(q, "vy")) + o.get(q, "a")
+        for k in ("vx", "vy"):
+            e[k] = rel_err(g.get(q, k), o.get(q, k), scale)
         e["YN"] = rel_err(g.get(q, 301), o.get(q, 301), np.ones(blocks[q].ncells))
         for k, v in e.items():
             worst[k] = max(worst.get(k, 0.0), v)
