@@ -250,3 +250,35 @@ def test_device_entry_point_large_batch_sampled():
              massf=np.ascontiguousarray(g["massf"].reshape(5, n)[:, idx]).reshape(-1),
              dt_chem=dto.cpu().numpy()[idx], status=st.cpu().numpy()[idx])
     _compare(g, o, "device 2^18 sampled", ch.substeps(n)[idx])
+
+
+def test_reactor_n2_reference_and_oracle():
+    """The reference's N2 recombination reactor (examples/eilmer/2D/reactor-n2,
+    reactor-test.rb:40-55): 200 operator-split increments of 1e-6 s of the batched reactor
+    on 256 copies of the cell end at p = 1.494e5 Pa, T = 6381.7 K, Y = (0.8760, 0.1240)
+    within 1e-3, and equal the oracle's increments at the 1e-12 bar after every step."""
+    from oracle import oracle as O
+    gas = gas_record(gdata.load("tpg-nitrogen-2sp"))
+    mech = mech_record(gdata.load("nitrogen-2sp-2r"))
+    ch = Chemistry(gas, mech)
+    n = 256
+    s = O.TPG(gas).update("pT", np.array([0.8, 0.2]), p=1.0e5, T=4000.0)
+    rho = np.full(n, s["rho"])
+    T, u, p = np.full(n, 4000.0), np.full(n, s["u"]), np.full(n, 1.0e5)
+    Y = np.concatenate([np.full(n, 0.8), np.full(n, 0.2)])   # species-major
+    dtc = np.full(n, -1.0)
+    oT, ou, op, oY, odtc = T.copy(), u.copy(), p.copy(), Y.copy(), dtc.copy()
+    worst = 0.0
+    for _ in range(200):
+        g = ch.step(rho, T, u, p, Y, dtc, 1.0e-6)
+        o = O.chem_step_batch(gas, mech, rho, oT, ou, op, oY, odtc, 1.0e-6)
+        assert np.all(g["status"] == 0) and o["failed"] == 0
+        T, u, p, Y, dtc = g["T"], g["u"], g["p"], g["massf"], g["dt_chem"]
+        oT, ou, op, oY, odtc = o["T"], o["u"], o["p"], o["massf"], o["dt_chem"]
+        worst = max(worst, rel_err(T, oT), rel_err(p, op), rel_err(Y, oY, np.ones_like(Y)))
+    print("reactor n2: worst per-step parity", worst, "final", p[0], T[0], Y[0], Y[n], flush=True)
+    assert worst <= TOL
+    refs = {"p": 1.494e5, "T": 6381.7, "Y_N2": 0.8760, "Y_N": 0.1240}
+    got = {"p": p[0], "T": T[0], "Y_N2": Y[0], "Y_N": Y[n]}
+    for k in refs:
+        assert abs(got[k] - refs[k]) / refs[k] < 1.0e-3, (k, got[k], refs[k])

@@ -275,3 +275,33 @@ def test_hi_equilibrium_constant_textbook(oracle_lib):
     dg = 2 * _nasa9_g_molar(gj["thermo"][2], T) - _nasa9_g_molar(gj["thermo"][0], T) \
         - _nasa9_g_molar(gj["thermo"][1], T)
     assert _isclose(math.exp(-dg / (R_U * T)), K, 1.0e-10), (K, math.exp(-dg / (R_U * T)))
+
+
+def reactor_n2_initial(n=1):
+    """examples/eilmer/2D/reactor-n2/reactor.lua: N2/N at p = 1e5 Pa, T = 4000 K, Y = (0.8, 0.2)
+    in a closed, no-flow cell; the mechanism nitrogen-2sp-2r.lua (elementary reactions with
+    Arrhenius rates both ways)."""
+    return 1.0e5, 4000.0, np.array([0.8, 0.2])
+
+
+def test_reactor_n2_reference(oracle_lib):
+    """The reference's reactor-test.rb:40-55: after 2e-4 s the cell holds p = 1.494e5 Pa,
+    T = 6381.7 K, Y_N2 = 0.8760, Y_N = 0.1240, each within 1e-3.  The reference integrates
+    the chemistry with its integral (implicit) update; here the operator-split increment runs
+    in 200 steps of 1e-6 s (the step reactor.lua:34-35 recommends for the split update) --
+    the state relaxes to the same recombination equilibrium."""
+    O = oracle_lib
+    gas = gas_record(gdata.load("tpg-nitrogen-2sp"))
+    mech = mech_record(gdata.load("nitrogen-2sp-2r"))
+    p0, T0, Y0 = reactor_n2_initial()
+    s = O.TPG(gas).update("pT", Y0, p=p0, T=T0)
+    rho = np.array([s["rho"]])
+    T, u, p, Y, dtc = np.array([T0]), np.array([s["u"]]), np.array([p0]), Y0.reshape(1, 2).copy(), np.array([-1.0])
+    for _ in range(200):
+        r = O.chem_step_batch(gas, mech, rho, T, u, p, Y, dtc, 1.0e-6)
+        assert r["failed"] == 0
+        T, u, p, Y, dtc = r["T"], r["u"], r["p"], r["massf"], r["dt_chem"]
+    refs = {"p": 1.494e5, "T": 6381.7, "Y_N2": 0.8760, "Y_N": 0.1240}
+    got = {"p": p[0], "T": T[0], "Y_N2": Y[0, 0], "Y_N": Y[0, 1]}
+    for k in refs:
+        assert abs(got[k] - refs[k]) / refs[k] < 1.0e-3, (k, got[k], refs[k])
