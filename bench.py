@@ -452,6 +452,55 @@ def bench_flow_io(nic, njc, nkc, text_size=(64, 64, 64), log=None):
     return out
 
 
+def bench_n90(cpu_steps=300, log=None):
+    """The reference's own reacting integration case, examples/eilmer/2D/cylinder-dlr-n90
+    (N2/N, 2 reactions, adaptive_hanel_ausmdv, 4 blocks of 60x10, 100 us): time to solution
+    on the GPU, the step count and probe state the reference's n90-test.rb checks, and the
+    oracle's steps/s on one core over its first cpu_steps steps (the CPU leg of the same run)."""
+    import numpy as np
+    from gdtk_amd import Simulation, cases, data
+    from gdtk_amd.chem import Chemistry, mech_record
+    blocks, rec, info = cases.n90_blocks()
+    mech = mech_record(data.load("nitrogen-2sp-2r"))
+    sim = Simulation(2, flux="adaptive_hanel_ausmdv", tpg=rec, dt_init=info["dt_init"])
+    for b in blocks:
+        sim.add_block(b)
+    sim.set_chemistry(Chemistry(rec, mech))
+    t0 = time.perf_counter()
+    _, st = sim.run(40000, info["max_time"])
+    el = time.perf_counter() - t0
+    best = (np.inf, 0, 0)
+    for q, b in enumerate(blocks):
+        xc = cases.cell_centres(b)
+        d = np.hypot(xc[:, 0] - info["probe"][0], xc[:, 1] - info["probe"][1])
+        i = int(np.argmin(d))
+        if d[i] < best[0]:
+            best = (d[i], q, i)
+    _, q, i = best
+    probe = {"rho": sim.get(q, "rho")[i], "vel.x": sim.get(q, "vx")[i], "p": sim.get(q, "p")[i],
+             "massf_N": sim.get(q, 301)[i], "T": sim.get(q, "T")[i]}
+    ncell = sum(b.ncells for b in blocks)
+    sim.close()
+    out = {"steps": st["step"], "reference_steps": "4428 +/- 30", "wall_s": el, "steps_per_s": st["step"] / el,
+           "cell_updates_per_s": ncell * st["step"] / el, "probe": probe,
+           "reference_probe": {"rho": 1.9090e-02, "vel.x": 5.797e+02, "p": 5.3735e+04, "massf_N": 1.749e-02,
+                               "T": 9.3205e+03}}
+    if cpu_steps:
+        from oracle import oracle as O
+        o = O.Sim(O.TPG(rec), 2, "adaptive_hanel_ausmdv", dt_init=info["dt_init"])
+        for b in blocks:
+            o.add_block(b)
+        o.set_chemistry(O.Chem(rec, mech))
+        t0 = time.perf_counter()
+        o.run(cpu_steps)
+        cel = time.perf_counter() - t0
+        out["cpu_baseline"] = {"steps_per_s": cpu_steps / cel, "cores": 1, "kind": "port",
+                               "sample": f"first {cpu_steps} steps of the same run, oracle on one core"}
+    if log:
+        log(f"n90: {st['step']} steps in {el:.2f} s on the GPU")
+    return out
+
+
 def _cell_fields(sim, nblocks, names):
     import numpy as np
     return {n: np.concatenate([sim.get(q, n) for q in range(nblocks)]) for n in names}
@@ -591,6 +640,8 @@ def main():
                     help="steps of the adaptive EFM/AUSMDV run on the headline box (0 skips it)")
     ap.add_argument("--adaptive-only", action="store_true", help="measure only the adaptive flux calculator")
     ap.add_argument("--no-flow-io", action="store_true", help="skip the flow-file write/read measurement")
+    ap.add_argument("--no-n90", action="store_true", help="skip the reference's n90 reacting cylinder case")
+    ap.add_argument("--n90-only", action="store_true", help="run only the n90 case")
     ap.add_argument("--flow-io-only", action="store_true", help="measure only the flow-file write/read")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -622,6 +673,9 @@ def main():
         print(json.dumps(bench_batched(nblk, args.batched_steps, 2, args.flux, log), default=float), flush=True)
         return
     nic, njc, nkc = (int(x) for x in args.size.split(","))
+    if args.n90_only:
+        print(json.dumps(bench_n90(0 if args.no_cpu_baseline else 300, log=log), default=float), flush=True)
+        return
     if args.flow_io_only:
         print(json.dumps(bench_flow_io(nic, njc, nkc, log=log), default=float), flush=True)
         return
@@ -695,6 +749,8 @@ def main():
         out["adaptive"] = bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log)
     if not args.no_flow_io:
         out["flow_io"] = bench_flow_io(nic, njc, nkc, log=log)
+    if not args.no_n90:
+        out["n90"] = bench_n90(0 if args.no_cpu_baseline else 300, log=log)
     if args.chem_cells > 0:
         out["chem"] = chem_legs()
     if args.coupled_steps > 0:
