@@ -23,22 +23,22 @@ pytestmark = pytest.mark.gpu
 FS_NAMES = ["rho", "vx", "vy", "vz", "u", "p", "T", "a"]
 
 
-def _pair(blocks, dims, rec=None, graph=True, **kw):
+def _pair(blocks, dims, rec=None, graph=True, flux="adaptive", **kw):
     from gdtk_amd import Simulation
     from oracle import oracle as O
-    g = Simulation(dims, flux="adaptive", tpg=rec, **kw)
+    g = Simulation(dims, flux=flux, tpg=rec, **kw)
     for b in blocks:
         g.add_block(b)
     g.set_graph(graph)
     gas = O.TPG(rec) if rec is not None else O.IdealGas()
-    o = O.Sim(gas, dims, "adaptive", mutate_cell_velocities=False, **kw)
+    o = O.Sim(gas, dims, flux, mutate_cell_velocities=False, **kw)
     for b in blocks:
         o.add_block(b)
     o._gas = gas
     return g, o
 
 
-def _check(g, o, nblocks, label, names=FS_NAMES, tol=TOL):
+def _check(g, o, nblocks, label, names=FS_NAMES, tol=TOL, shock=True):
     worst = {}
     marked = 0
     for q in range(nblocks):
@@ -46,6 +46,8 @@ def _check(g, o, nblocks, label, names=FS_NAMES, tol=TOL):
                             g.dims)
         for k, v in errs.items():
             worst[k] = max(worst.get(k, 0.0), v)
+        if not shock:
+            continue
         Sg, So = g.get(q, "S"), o.get(q, "S")
         assert np.array_equal(Sg, So), f"{label}: block {q}: shock indicators differ in {np.sum(Sg != So)} cells"
         marked += int(So.sum())
@@ -81,6 +83,22 @@ def test_wedge_adaptive():
     g.run(12)
     o.run(12)
     assert _check(g, o, 1, "wedge adaptive") > 0
+
+
+@pytest.mark.parametrize("flux", ["hanel", "adaptive_hanel_ausmdv"])
+def test_hanel_calculators(flux):
+    """Hanel's splitting alone and as the shock-side calculator of the reference's default
+    adaptive_hanel_ausmdv, on the Sod tube (2-D) and the Mach-6 box (3-D)."""
+    case = _sod_y(120, 40)
+    g, o = _pair([case], 2, flux=flux)
+    g.run(30)
+    o.run(30)
+    _check(g, o, 1, f"sod2d {flux}", shock=flux != "hanel")
+    case3 = cases.mach6_box(24, 16, 12, seed=11)
+    g, o = _pair([case3], 3, flux=flux)
+    g.run(5)
+    o.run(5)
+    _check(g, o, 1, f"mach6 3d {flux}", shock=flux != "hanel")
 
 
 def test_mach6_box_3d_adaptive():
