@@ -114,6 +114,29 @@ struct SideDev {
     int64_t start;
 };
 
+// The faces of every direction of every block in one launch: per direction d, the first
+// launch index of each block's faces (multiples of the 256-thread workgroup) and the
+// direction's launch size; the directions follow each other in the launch.
+struct FaceBatch {
+    const int64_t* start[3];
+    int64_t total[3];
+};
+
+// Direction of the workgroup starting at launch index w0, which becomes the index within
+// that direction.
+__host__ __device__ inline int face_batch_dir(const FaceBatch& fb, int64_t& w0) {
+    int d = 0;
+    if (w0 >= fb.total[0]) {
+        w0 -= fb.total[0];
+        d = 1;
+        if (w0 >= fb.total[1]) {
+            w0 -= fb.total[1];
+            d = 2;
+        }
+    }
+    return d;
+}
+
 // Batched stage launch over every block: per-block device records for this stage (their
 // buffers already point at this stage's input and output copies), the prefix sums of the
 // blocks' tile counts, and the per-XCD tile queues (8 counters, TILE_QSTRIDE apart).

@@ -241,14 +241,15 @@ __device__ __forceinline__ void face_flux_one(const BlockDev& b, const IdealGasD
     for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
 }
 
-// Faces of direction d of every block in one launch.  Each block's faces start at a
-// multiple of the workgroup size in the launch (start[q], q = 0..nblocks), so every
-// workgroup serves one block and reads that block's record with scalar loads.
+// Faces of every direction of every block in one launch.  Each block's faces of a direction
+// start at a multiple of the workgroup size (FaceBatch), so every workgroup serves one
+// direction of one block and reads that block's record with scalar loads.
 template <int DIM, int FC>
-__global__ __launch_bounds__(256, 4) void k_face_flux_batch(const BlockDev* __restrict__ tab,
-                                                         const int64_t* __restrict__ start, int nblocks, int d,
-                                                         double eps, IdealGasDev gas) {
-    const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+__global__ __launch_bounds__(256, 4) void k_face_flux_batch(const BlockDev* __restrict__ tab, FaceBatch fb,
+                                                         int nblocks, double eps, IdealGasDev gas) {
+    int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    const int d = __builtin_amdgcn_readfirstlane(face_batch_dir(fb, w0));
+    const int64_t* __restrict__ start = fb.start[d];
     int lo = 0, hi = nblocks - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -463,11 +464,12 @@ __global__ void k_decode(BlockDev b, IdealGasDev gas, double* U, double* fs, dou
 
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-hipError_t launch_face_flux_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
-                                 int flux_calc, double eps, int d, const IdealGasDev& gas, hipStream_t s) {
+hipError_t launch_face_flux_batch(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
+                                 double eps, const IdealGasDev& gas, hipStream_t s) {
+    const int64_t total = fb.total[0] + fb.total[1] + fb.total[2];
     if (total <= 0) return hipSuccess;
     dim3 grid(nblk(total, 256)), blk(256);
-#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F>), grid, blk, 0, s, tab, start, nblocks, d, eps, gas)
+#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F>), grid, blk, 0, s, tab, fb, nblocks, eps, gas)
     if (dims == 3) {
         if (flux_calc == 4) GDTK_FACE(3, 4);
         else if (flux_calc == 3) GDTK_FACE(3, 3);

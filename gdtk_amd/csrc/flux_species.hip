@@ -10,10 +10,12 @@
 namespace gdtk {
 
 template <int DIM, int FC, int NSP>
-__global__ __launch_bounds__(256) void k_face_flux_species(const BlockDev* __restrict__ tab,
-                                                           const int64_t* __restrict__ start, int nblocks, int d,
-                                                           double eps, const ChemDev* __restrict__ cdp) {
-    const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+__global__ __launch_bounds__(256) void k_face_flux_species(const BlockDev* __restrict__ tab, FaceBatch fb,
+                                                           int nblocks, double eps,
+                                                           const ChemDev* __restrict__ cdp) {
+    int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    const int d = __builtin_amdgcn_readfirstlane(face_batch_dir(fb, w0));
+    const int64_t* __restrict__ start = fb.start[d];
     int lo = 0, hi = nblocks - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -104,21 +106,21 @@ static inline unsigned nblk256(int64_t n) { return (unsigned)((n + 255) / 256); 
 bool species_supported(int nsp) { return nsp == 2 || nsp == 5; }
 
 template <int NSP>
-static void face_species(const BlockDev* tab, const int64_t* start, int nb, int64_t total, int dims, int fc,
-                         double eps, int d, const ChemDev* cd, hipStream_t s) {
+static void face_species(const BlockDev* tab, const FaceBatch& fb, int nb, int64_t total, int dims, int fc,
+                         double eps, const ChemDev* cd, hipStream_t s) {
     dim3 g(nblk256(total)), blk(256);
     if (dims == 3) {
-        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<3, 4, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<3, 3, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<3, 2, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<3, 1, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else hipLaunchKernelGGL((k_face_flux_species<3, 0, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<3, 4, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<3, 3, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<3, 2, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<3, 1, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else hipLaunchKernelGGL((k_face_flux_species<3, 0, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
     } else {
-        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<2, 4, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<2, 3, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<2, 2, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<2, 1, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
-        else hipLaunchKernelGGL((k_face_flux_species<2, 0, NSP>), g, blk, 0, s, tab, start, nb, d, eps, cd);
+        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<2, 4, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<2, 3, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<2, 2, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<2, 1, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        else hipLaunchKernelGGL((k_face_flux_species<2, 0, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
     }
 }
 
@@ -135,11 +137,12 @@ static void cell_species(const BlockDev* tab, const int64_t* start, int nb, int6
     }
 }
 
-hipError_t launch_face_flux_species(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
-                                   int flux_calc, double eps, int d, int nsp, const ChemDev* cd, hipStream_t s) {
+hipError_t launch_face_flux_species(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
+                                   double eps, int nsp, const ChemDev* cd, hipStream_t s) {
+    const int64_t total = fb.total[0] + fb.total[1] + fb.total[2];
     if (total <= 0) return hipSuccess;
-    if (nsp == 2) face_species<2>(tab, start, nblocks, total, dims, flux_calc, eps, d, cd, s);
-    else if (nsp == 5) face_species<5>(tab, start, nblocks, total, dims, flux_calc, eps, d, cd, s);
+    if (nsp == 2) face_species<2>(tab, fb, nblocks, total, dims, flux_calc, eps, cd, s);
+    else if (nsp == 5) face_species<5>(tab, fb, nblocks, total, dims, flux_calc, eps, cd, s);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

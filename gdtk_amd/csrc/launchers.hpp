@@ -21,11 +21,11 @@ hipError_t launch_dt(const BlockDev& b, double dt_current, double cfl, double* p
 hipError_t launch_encode(const BlockDev& b, double* U, hipStream_t s);
 hipError_t launch_decode(const BlockDev& b, const IdealGasDev& gas, double* U, double* fs, double low_T,
                          hipStream_t s);
-// Face-per-thread kernels over all blocks: faces of direction d, then the cell update (dt
-// read from device memory).  start[q] = first launch index of block q, a multiple of 256;
-// total = launch size.
-hipError_t launch_face_flux_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
-                                 int flux_calc, double eps, int d, const IdealGasDev& gas, hipStream_t s);
+// Face-per-thread kernels over all blocks: the faces of every direction in one launch
+// (FaceBatch), then the cell update (dt read from device memory; start[q] = first launch
+// index of block q, a multiple of 256; total = launch size).
+hipError_t launch_face_flux_batch(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
+                                 double eps, const IdealGasDev& gas, hipStream_t s);
 hipError_t launch_cell_update_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
                                    int stage, const double* dtp, const CellLimits& lim, const IdealGasDev& gas,
                                    hipStream_t s);
@@ -33,8 +33,8 @@ hipError_t launch_cell_update_batch(const BlockDev* tab, const int64_t* start, i
 // face and cell-update launches, and one block's encode/decode at start-up.
 struct ChemDev;
 bool species_supported(int nsp);
-hipError_t launch_face_flux_species(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
-                                   int flux_calc, double eps, int d, int nsp, const ChemDev* cd, hipStream_t s);
+hipError_t launch_face_flux_species(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
+                                   double eps, int nsp, const ChemDev* cd, hipStream_t s);
 hipError_t launch_cell_update_species(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
                                      int stage, const double* dtp, const CellLimits& lim, int nsp, const ChemDev* cd,
                                      hipStream_t s);

@@ -425,14 +425,19 @@ struct Sim {
             });
         if (!use_fused()) {
             const int nd = (cfg.dimensions == 3) ? 3 : 2;
-            for (int d = 0; d < nd; ++d)
-                timed(0, (double)nfaces_total(d), [&] {
-                    if (nsp > 1)
-                        return launch_face_flux_species(t, fstart[d].p, nb, ftotal[d], cfg.dimensions,
-                                                        cfg.flux_calculator, eps, d, nsp, tpg_dev.p, stream);
-                    return launch_face_flux_batch(t, fstart[d].p, nb, ftotal[d], cfg.dimensions, cfg.flux_calculator,
-                                                  eps, d, gas, stream);
-                });
+            FaceBatch fb{};
+            int64_t nfaces = 0;
+            for (int d = 0; d < 3; ++d) {
+                fb.start[d] = fstart[d].p;
+                fb.total[d] = (d < nd) ? ftotal[d] : 0;
+                if (d < nd) nfaces += nfaces_total(d);
+            }
+            timed(0, (double)nfaces, [&] {
+                if (nsp > 1)
+                    return launch_face_flux_species(t, fb, nb, cfg.dimensions, cfg.flux_calculator, eps, nsp,
+                                                    tpg_dev.p, stream);
+                return launch_face_flux_batch(t, fb, nb, cfg.dimensions, cfg.flux_calculator, eps, gas, stream);
+            });
             timed(1, nc, [&] {
                 if (nsp > 1)
                     return launch_cell_update_species(t, cstart.p, nb, ctotal, cfg.dimensions, stage, dt_dev.p, lim,
