@@ -116,6 +116,86 @@ struct RawFile {
     }
 };
 
+// One gzip member written from text pieces compressed on several threads: each piece is
+// an independent raw deflate stream primed with the last 32 KiB of the text before it and
+// ended with a sync flush (the last one with the final block), so the pieces concatenate
+// into one deflate stream; the CRC-32 of the whole text comes from the pieces' CRCs
+// (crc32_combine).  Any gzip reader, zlib's included, reads it as the single-stream file
+// the reference writes.
+class ParallelGzip {
+public:
+    explicit ParallelGzip(RawFile& f) : f_(f) {
+        const unsigned char header[10] = {0x1f, 0x8b, 8, 0, 0, 0, 0, 0, 0, 3};
+        f_.write(header, sizeof header);
+    }
+    // Compresses the pieces in parallel and writes them in order; `last` ends the stream.
+    void add_parallel(const std::vector<std::string>& pieces, bool last) {
+        const size_t n = pieces.size();
+        std::vector<std::string> out(n);
+        std::vector<unsigned long> crc(n);
+        std::vector<std::string> dict(n);
+        for (size_t t = 0; t < n; ++t) {
+            dict[t] = tail_;
+            if (!pieces[t].empty()) tail_ = tail_of(tail_, pieces[t]);
+        }
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < n; ++t)
+            th.emplace_back([&, t] {
+                out[t] = deflate_piece(pieces[t], dict[t], last && t + 1 == n);
+                crc[t] = crc32(0L, reinterpret_cast<const Bytef*>(pieces[t].data()), (uInt)pieces[t].size());
+            });
+        for (auto& x : th) x.join();
+        for (size_t t = 0; t < n; ++t) {
+            f_.write(out[t].data(), out[t].size());
+            crc_ = crc32_combine(crc_, crc[t], (z_off_t)pieces[t].size());
+            size_ += pieces[t].size();
+        }
+        finished_ = last;
+    }
+    // One piece that does not end the stream (the header).
+    void add_text(const std::string& text) { add_parallel(std::vector<std::string>{text}, false); }
+    void finish() {
+        if (!finished_) add_parallel(std::vector<std::string>{std::string()}, true);
+        unsigned char trailer[8];
+        for (int i = 0; i < 4; ++i) trailer[i] = (unsigned char)((crc_ >> (8 * i)) & 0xff);
+        for (int i = 0; i < 4; ++i) trailer[4 + i] = (unsigned char)((size_ >> (8 * i)) & 0xff);
+        f_.write(trailer, sizeof trailer);
+    }
+
+private:
+    RawFile& f_;
+    std::string tail_;
+    unsigned long crc_ = crc32(0L, Z_NULL, 0);
+    uint64_t size_ = 0;
+    bool finished_ = false;
+
+    static std::string tail_of(const std::string& before, const std::string& text) {
+        const size_t W = 32768;
+        if (text.size() >= W) return text.substr(text.size() - W);
+        std::string t = before + text;
+        return t.size() > W ? t.substr(t.size() - W) : t;
+    }
+    static std::string deflate_piece(const std::string& in, const std::string& dict, bool last) {
+        z_stream z{};
+        if (deflateInit2(&z, Z_DEFAULT_COMPRESSION, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK)
+            throw std::runtime_error("deflateInit2 failed");
+        if (!dict.empty())
+            deflateSetDictionary(&z, reinterpret_cast<const Bytef*>(dict.data()), (uInt)dict.size());
+        std::string out;
+        out.resize(deflateBound(&z, (uLong)in.size()) + 64);
+        z.next_in = reinterpret_cast<Bytef*>(const_cast<char*>(in.data()));
+        z.avail_in = (uInt)in.size();
+        z.next_out = reinterpret_cast<Bytef*>(&out[0]);
+        z.avail_out = (uInt)out.size();
+        const int rc = deflate(&z, last ? Z_FINISH : Z_SYNC_FLUSH);
+        const bool ok = last ? (rc == Z_STREAM_END) : (rc == Z_OK && z.avail_in == 0);
+        out.resize(out.size() - z.avail_out);
+        deflateEnd(&z);
+        if (!ok) throw std::runtime_error("deflate failed");
+        return out;
+    }
+};
+
 int hw_threads() {
     const unsigned n = std::thread::hardware_concurrency();
     return (int)std::max(1u, std::min(n, 16u));
@@ -163,7 +243,6 @@ void write_flow_file(const std::string& fn, int format, const std::string& label
         return;
     }
     if (format != FLOW_GZIPTEXT) throw std::runtime_error("unknown flow format");
-    GzFile f(fn, "wb");
     char buf[128];
     std::string h = "structured_grid_flow 1.0\n";
     h += "label: " + label + "\n";
@@ -176,10 +255,12 @@ void write_flow_file(const std::string& fn, int format, const std::string& label
     h += "nicell: " + std::to_string(nic) + "\n";
     h += "njcell: " + std::to_string(njc) + "\n";
     h += "nkcell: " + std::to_string(nkc) + "\n";
-    f.write(h);
-    // Format chunks of cells on the host threads, write them in order.
+    RawFile f(fn, "wb");
+    ParallelGzip gz(f);
+    gz.add_text(h);
+    // Format and compress chunks of cells on the host threads, write them in order.
     const int nt = hw_threads();
-    const int64_t chunk = 65536;
+    const int64_t chunk = 32768;
     std::vector<std::string> parts(nt);
     for (int64_t base = 0; base < ncell; base += chunk * nt) {
         std::vector<std::thread> th;
@@ -188,8 +269,9 @@ void write_flow_file(const std::string& fn, int format, const std::string& label
             th.emplace_back([&, t, c0, c1] { format_records(rec, c0, c1, nv, parts[t]); });
         }
         for (auto& x : th) x.join();
-        for (int t = 0; t < nt; ++t) f.write(parts[t]);
+        gz.add_parallel(parts, base + chunk * nt >= ncell);
     }
+    gz.finish();
     f.close();
 }
 
@@ -306,17 +388,47 @@ double read_flow_file(const std::string& fn, int format, int dims, int nic, int 
     const int njc_read = std::atoi(after_prefix(next(), "njcell:").c_str());
     const int nkc_read = std::atoi(after_prefix(next(), "nkcell:").c_str());
     check_size(dims_read, nic_read, njc_read, nkc_read, dims, nic, njc, nkc);
-    std::vector<double> row((size_t)n);
-    for (int64_t c = 0; c < ncell; ++c) {
-        const char* p = next().c_str();
-        for (int j = 0; j < n; ++j) {
-            char* e = nullptr;
-            row[(size_t)j] = std::strtod(p, &e);
-            if (e == p) throw std::runtime_error("flow file: cell " + std::to_string(c) + " has too few values");
-            p = e;
-        }
-        for (int v = 0; v < nv; ++v) rec[(size_t)c * nv + v] = row[(size_t)col[v]];
+    // The cell lines: decompressed in one piece, then parsed on the host threads.
+    std::string body;
+    {
+        std::vector<char> buf(1 << 22);
+        int got;
+        while ((got = gzread(f.f, buf.data(), (unsigned)buf.size())) > 0) body.append(buf.data(), (size_t)got);
+        if (got < 0) throw std::runtime_error("flow file: gzip read failed");
     }
+    std::vector<size_t> line_start;
+    line_start.reserve((size_t)ncell + 1);
+    for (size_t pos = 0; pos < body.size() && (int64_t)line_start.size() < ncell;) {
+        line_start.push_back(pos);
+        const size_t e = body.find('\n', pos);
+        pos = (e == std::string::npos) ? body.size() : e + 1;
+    }
+    if ((int64_t)line_start.size() < ncell) throw std::runtime_error("flow file ends early");
+    const int nt = hw_threads();
+    std::vector<std::string> errs((size_t)nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            std::vector<double> row((size_t)n);
+            const int64_t c0 = ncell * t / nt, c1 = ncell * (t + 1) / nt;
+            for (int64_t c = c0; c < c1 && errs[(size_t)t].empty(); ++c) {
+                const char* p = body.c_str() + line_start[(size_t)c];
+                for (int j = 0; j < n; ++j) {
+                    while (*p == ' ' || *p == '\t') ++p;
+                    char* e = nullptr;
+                    if (*p != '\n' && *p != '\r' && *p != '\0') row[(size_t)j] = std::strtod(p, &e);
+                    if (!e || e == p) {
+                        errs[(size_t)t] = "flow file: cell " + std::to_string(c) + " has too few values";
+                        break;
+                    }
+                    p = e;
+                }
+                for (int v = 0; v < nv; ++v) rec[(size_t)c * nv + v] = row[(size_t)col[v]];
+            }
+        });
+    for (auto& x : th) x.join();
+    for (const auto& e : errs)
+        if (!e.empty()) throw std::runtime_error(e);
     return sim_time;
 }
 

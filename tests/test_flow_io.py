@@ -117,3 +117,10 @@ def test_reader_errors(tmp_path):
         hip.read_flow_records(fn, 3, nic, njc, nkc, 1, "gziptext")
     with pytest.raises(hip.HipError):
         hip.read_flow_records(os.path.join(str(tmp_path), "absent.gz"), 2, nic, njc, nkc, 1, "gziptext")
+    # a cell line one value short
+    txt = FF.text(rec, names, "x", 0.0, 2, nic, njc, nkc).split("\n")
+    txt[9 + 5] = " ".join(txt[9 + 5].split(" ")[:-1])
+    with gzip.open(fn, "wt", newline="") as f:
+        f.write("\n".join(txt))
+    with pytest.raises(hip.HipError, match="cell 5 has too few values"):
+        hip.read_flow_records(fn, 2, nic, njc, nkc, 1, "gziptext")
