@@ -132,13 +132,14 @@ def test_turned_blocks_adaptive():
     _check(g, o, len(blocks), "turned blocks adaptive")
 
 
-def test_species_box_adaptive():
+@pytest.mark.parametrize("flux", ["adaptive", "adaptive_hanel_ausmdv", "hanel"])
+def test_species_box_adaptive(flux):
     case, rec = cases.species_box(48, 24, nsp=5)
-    g, o = _pair([case], 2, rec=rec)
+    g, o = _pair([case], 2, rec=rec, flux=flux)
     g.run(10)
     o.run(10)
     names = [n for n in FS_NAMES if n != "u"]
-    _check(g, o, 1, "species box adaptive", names=names)
+    _check(g, o, 1, f"species box {flux}", names=names, shock=flux != "hanel")
 
 
 def test_converging_flow_every_face_efm():
