@@ -381,10 +381,11 @@ def bench_batched(nblk, steps, warmup, flux="ausmdv", log=None):
 
 def bench_adaptive(nic, njc, nkc, steps, warmup=2, log=None):
     """The headline Mach-6 box with the adaptive EFM/AUSMDV calculator: shock detection
-    (one launch over all cells in stage 1) plus the face kernels that choose EFM on marked
-    faces.  Returns a sub-record with the throughput, the share of marked cells and the
-    detection kernel's roofline (algorithmic bytes per cell: the cell's velocity and sound
-    speed, the n/t1/t2 frames of its three low-side faces and the 1-byte indicator)."""
+    (in stage 1, a launch over all faces for the detector's decisions, then one over all
+    cells for the indicators) plus the face kernels that choose EFM on marked faces.
+    Returns a sub-record with the throughput, the share of marked cells and the detection's
+    roofline over its two launches (algorithmic bytes per cell: the cell's velocity and
+    sound speed, the n/t1/t2 frames of its three low-side faces and the 1-byte indicator)."""
     import numpy as np
     from gdtk_amd import Simulation, cases
     case = cases.mach6_box(nic, njc, nkc, seed=1234)
@@ -414,7 +415,7 @@ def bench_adaptive(nic, njc, nkc, steps, warmup=2, log=None):
                                "adaptive_efm_ausmdv (PJ detector, strict), pc RK2", "cells": case.ncells,
                    "final_dt": st["dt"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_shock_detect_batch<3>", "avg_launch_ms": avg_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "kernel": "k_shock_faces_batch<3> + k_shock_cells_batch<3>", "avg_launch_ms": avg_ms,
                      "bytes_per_cell": bpc, "items_per_launch": kd["items_per_launch"]},
         "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1)} for k, v in ks.items()
                     if v["launches"]},

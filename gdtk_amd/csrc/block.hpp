@@ -59,6 +59,9 @@ struct BlockDev {
     // interior cells hold the last detection, ghost cells the copy made by the ghost fill
     // of the stage that detected.
     unsigned char* shock;
+    // The PJ detector's decision per face of each direction (face-array order; adaptive
+    // flux calculator only): written and read by the detection launches alone.
+    unsigned char* fshock[3];
 };
 
 __host__ __device__ inline int64_t pidx(const BlockDev& b, int i, int j, int k) {

@@ -145,16 +145,54 @@ __device__ __forceinline__ bool pj_shock(const double* fs, int64_t np, int64_t c
     return (shear < shear_tol) && (comp < comp_tol);
 }
 
-// Shock detection of every interior cell of every block (detect_shock_points then
-// shock_faces_to_cells, fluidblock.d:478-520, 568-580): a cell is shocked if the PJ
-// detector fires on any of its faces, or (damped_outflow) one of its faces lies on an
-// outflow_simple_extrapolate boundary.  The strict rule (fluidblock.d:582-605) is applied
-// where the flux kernels read the indicators: a face is shocked if either adjacent cell is
-// (a face the detector marks has its interior cells marked too).
+// Shock detection (detect_shock_points then shock_faces_to_cells, fluidblock.d:478-520,
+// 568-580) in two launches.  Pass 1, one thread per face of every direction of every block
+// (FaceBatch order, as the face-flux launch): the PJ detector's decision for the face, or 1
+// for a face on an outflow_simple_extrapolate boundary when damped_outflow is set.  Each
+// face's frame and its two cells are read once, where a cell-per-thread detection reads
+// every interior face twice from workgroups that run on different XCDs.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_shock_detect_batch(const BlockDev* __restrict__ tab,
-                                                            const int64_t* __restrict__ start, int nblocks,
-                                                            double comp_tol, double shear_tol, int damped_outflow) {
+__global__ __launch_bounds__(256) void k_shock_faces_batch(const BlockDev* __restrict__ tab, FaceBatch fb,
+                                                           int nblocks, double comp_tol, double shear_tol,
+                                                           int damped_outflow) {
+    int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+    const int d = __builtin_amdgcn_readfirstlane(face_batch_dir(fb, w0));
+    const int64_t* __restrict__ start = fb.start[d];
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= w0) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const BlockDev& b = tab[q];
+    const int64_t f = w0 - start[q] + threadIdx.x;
+    if (f >= b.nface[d]) return;
+    const int fi = (int)(f % b.nf[d][0]);
+    const int64_t rest = f / b.nf[d][0];
+    const int fj = (int)(rest % b.nf[d][1]);
+    const int fk = (int)(rest / b.nf[d][1]);
+    const int pos = (d == 0) ? fi : (d == 1) ? fj : fk;
+    const int nd = (d == 0) ? b.nic : (d == 1) ? b.njc : b.nkc;
+    const int side = (pos == 0) ? 2 * d : (pos == nd) ? 2 * d + 1 : -1;
+    bool shocked;
+    if (damped_outflow && side >= 0 && b.bc_kind[side] == BC_OUTFLOW_SIMPLE_EXTRAPOLATE) {
+        shocked = true;
+    } else {
+        const int64_t cl = pidx(b, fi - (d == 0), fj - (d == 1), fk - (d == 2));
+        const int64_t cr = pidx(b, fi, fj, fk);
+        shocked = pj_shock(b.fs_in, b.npad, cl, cr, load_face(b.fgeom[d], b.nface[d], f), comp_tol, shear_tol);
+    }
+    b.fshock[d][f] = shocked ? 1 : 0;
+}
+
+// Pass 2, one thread per interior cell: the cell is shocked if any of its faces is.  The
+// strict rule (fluidblock.d:582-605) is applied where the flux kernels read the indicators:
+// a face is shocked if either adjacent cell is (a face the detector marks has its interior
+// cells marked too).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_shock_cells_batch(const BlockDev* __restrict__ tab,
+                                                           const int64_t* __restrict__ start, int nblocks) {
     const int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
     int lo = 0, hi = nblocks - 1;
     while (lo < hi) {
@@ -170,30 +208,10 @@ __global__ __launch_bounds__(256) void k_shock_detect_batch(const BlockDev* __re
     const int64_t rest = t / b.nic;
     const int j = (int)(rest % b.njc);
     const int k = (int)(rest / b.njc);
-    const int64_t c = pidx(b, i, j, k);
-    const int64_t np = b.npad;
-    const double* fs = b.fs_in;
-    bool shocked = false;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const int pos = (d == 0) ? i : (d == 1) ? j : k;
-        const int nd = (d == 0) ? b.nic : (d == 1) ? b.njc : b.nkc;
-#pragma unroll
-        for (int hiside = 0; hiside < 2; ++hiside) {
-            const int side = 2 * d + hiside;
-            if (damped_outflow && (pos == (hiside ? nd - 1 : 0)) && b.bc_kind[side] == BC_OUTFLOW_SIMPLE_EXTRAPOLATE) {
-                shocked = true;
-                continue;
-            }
-            const int fi = i + (d == 0 ? hiside : 0), fj = j + (d == 1 ? hiside : 0), fk = k + (d == 2 ? hiside : 0);
-            const FaceGeom g = load_face(b.fgeom[d], b.nface[d], face_index(b, d, fi, fj, fk));
-            const int64_t other = pidx(b, i + (d == 0 ? 2 * hiside - 1 : 0), j + (d == 1 ? 2 * hiside - 1 : 0),
-                                       k + (d == 2 ? 2 * hiside - 1 : 0));
-            shocked |= hiside ? pj_shock(fs, np, c, other, g, comp_tol, shear_tol)
-                              : pj_shock(fs, np, other, c, g, comp_tol, shear_tol);
-        }
-    }
-    b.shock[c] = shocked ? 1 : 0;
+    unsigned s = b.fshock[0][face_index(b, 0, i, j, k)] | b.fshock[0][face_index(b, 0, i + 1, j, k)] |
+                 b.fshock[1][face_index(b, 1, i, j, k)] | b.fshock[1][face_index(b, 1, i, j + 1, k)];
+    if (DIM == 3) s |= b.fshock[2][face_index(b, 2, i, j, k)] | b.fshock[2][face_index(b, 2, i, j, k + 1)];
+    b.shock[pidx(b, i, j, k)] = s ? 1 : 0;
 }
 
 // One thread per face of index direction d: reconstruction + flux calculator, or the
@@ -487,16 +505,21 @@ hipError_t launch_face_flux_batch(const BlockDev* tab, const FaceBatch& fb, int 
     return hipGetLastError();
 }
 
-hipError_t launch_shock_detect(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
-                               double comp_tol, double shear_tol, int damped_outflow, hipStream_t s) {
-    if (total <= 0) return hipSuccess;
-    dim3 grid(nblk(total, 256)), blk(256);
-    if (dims == 3)
-        hipLaunchKernelGGL((k_shock_detect_batch<3>), grid, blk, 0, s, tab, start, nblocks, comp_tol, shear_tol,
+hipError_t launch_shock_detect(const BlockDev* tab, const FaceBatch& fb, const int64_t* cstart, int nblocks,
+                               int64_t ctotal, int dims, double comp_tol, double shear_tol, int damped_outflow,
+                               hipStream_t s) {
+    const int64_t ftotal = fb.total[0] + fb.total[1] + fb.total[2];
+    if (ctotal <= 0 || ftotal <= 0) return hipSuccess;
+    dim3 fgrid(nblk(ftotal, 256)), cgrid(nblk(ctotal, 256)), blk(256);
+    if (dims == 3) {
+        hipLaunchKernelGGL((k_shock_faces_batch<3>), fgrid, blk, 0, s, tab, fb, nblocks, comp_tol, shear_tol,
                            damped_outflow);
-    else
-        hipLaunchKernelGGL((k_shock_detect_batch<2>), grid, blk, 0, s, tab, start, nblocks, comp_tol, shear_tol,
+        hipLaunchKernelGGL((k_shock_cells_batch<3>), cgrid, blk, 0, s, tab, cstart, nblocks);
+    } else {
+        hipLaunchKernelGGL((k_shock_faces_batch<2>), fgrid, blk, 0, s, tab, fb, nblocks, comp_tol, shear_tol,
                            damped_outflow);
+        hipLaunchKernelGGL((k_shock_cells_batch<2>), cgrid, blk, 0, s, tab, cstart, nblocks);
+    }
     return hipGetLastError();
 }
 

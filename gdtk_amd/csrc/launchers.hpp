@@ -59,8 +59,9 @@ hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
 // the adaptive flux kernels), launched like the cell update.
 hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsides, int64_t total, int mode,
                               hipStream_t s);
-hipError_t launch_shock_detect(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
-                               double comp_tol, double shear_tol, int damped_outflow, hipStream_t s);
+hipError_t launch_shock_detect(const BlockDev* tab, const FaceBatch& fb, const int64_t* cstart, int nblocks,
+                               int64_t ctotal, int dims, double comp_tol, double shear_tol, int damped_outflow,
+                               hipStream_t s);
 // Fused LDS-tiled stage kernel over all blocks (fused_stage.hip): tiles of one block for
 // k chunks of kc planes, the persistent grid size for the device, and the launch (dt is
 // read from device memory).

@@ -53,6 +53,7 @@ struct Block {
     BlockDev dev{};
     DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], dtpart, dtout, fs_save;
     DevArr<unsigned char> shock;  // padded cells (adaptive flux calculator)
+    DevArr<unsigned char> fshock;  // faces of the three directions, one after another
     // Flow-file fields the update does not touch: cell centroids (3 planes of active cells)
     // and the gas transport coefficients mu, k carried through an inviscid run (allocated
     // when set or read from a file; written as 0 otherwise).
@@ -406,8 +407,8 @@ struct Sim {
     }
 
     // Batched stage over all blocks: ghost cells (exchange + BCs), then the fused stage
-    // kernel, or the face-per-thread kernels (one launch per direction, then the cell
-    // update); dt is read from dt_dev.  Counters of the stage must be clear.
+    // kernel, or the face-per-thread kernels (one launch for the faces of every direction,
+    // then the cell update); dt is read from dt_dev.  Counters of the stage must be clear.
     void launch_batched_stage(int stage, int c, int u) {
         const BlockDev* t = tab(stage, c, u);
         const double nc = (double)total_cells();
@@ -416,22 +417,22 @@ struct Sim {
         // Stage 1 of a step that detects shocks: the ghost fill also copies the indicators,
         // then detect_shocks (simcore_gasdynamic_step.d:1030-1037).
         const bool detect = (stage == 1) && detect_now();
+        const int nd = (cfg.dimensions == 3) ? 3 : 2;
+        FaceBatch fb{};
+        int64_t nfaces = 0;
+        for (int d = 0; d < 3; ++d) {
+            fb.start[d] = fstart[d].p;
+            fb.total[d] = (d < nd) ? ftotal[d] : 0;
+            if (d < nd) nfaces += nfaces_total(d);
+        }
         timed(3, (double)nghost,
               [&] { return launch_ghost_batch(t, sides.p, nsides, nghost, detect ? 2 : 0, stream); });
         if (detect)
             timed(4, nc, [&] {
-                return launch_shock_detect(t, cstart.p, nb, ctotal, cfg.dimensions, sdet.compression_tolerance,
+                return launch_shock_detect(t, fb, cstart.p, nb, ctotal, cfg.dimensions, sdet.compression_tolerance,
                                            sdet.shear_tolerance, sdet.damped_outflow, stream);
             });
         if (!use_fused()) {
-            const int nd = (cfg.dimensions == 3) ? 3 : 2;
-            FaceBatch fb{};
-            int64_t nfaces = 0;
-            for (int d = 0; d < 3; ++d) {
-                fb.start[d] = fstart[d].p;
-                fb.total[d] = (d < nd) ? ftotal[d] : 0;
-                if (d < nd) nfaces += nfaces_total(d);
-            }
             timed(0, (double)nfaces, [&] {
                 if (nsp > 1)
                     return launch_face_flux_species(t, fb, nb, cfg.dimensions, cfg.flux_calculator, eps, nsp,
@@ -638,9 +639,16 @@ struct Sim {
         HIPCHK(hipMemset(b->counters, 0, 4 * sizeof(int)));
         d.counters = b->counters;
         d.shock = nullptr;
+        for (int q = 0; q < 3; ++q) d.fshock[q] = nullptr;
         if (adaptive()) {
             b->shock.alloc(np);  // zero: the initial flow state's S (flowstate.d:93)
             d.shock = b->shock.p;
+            b->fshock.alloc((size_t)(d.nface[0] + d.nface[1] + d.nface[2]));
+            int64_t off = 0;
+            for (int q = 0; q < 3; ++q) {
+                d.fshock[q] = b->fshock.p + off;
+                off += d.nface[q];
+            }
         }
         b->dtpart.alloc(2 * (size_t)dt_partial_count(d) + 2);
         b->dtout.alloc(4);
