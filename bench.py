@@ -422,6 +422,36 @@ def bench_adaptive(nic, njc, nkc, steps, warmup=2, log=None):
     }
 
 
+def bench_lmr(nic, njc, nkc, steps, warmup=2, log=None):
+    """The headline Mach-6 box with Eilmer 5 (lmr) numerics: the face kernel computes the
+    dimensionalised van Albada epsilon per reconstructed scalar and AUSMDV's smooth-maximum
+    sound speed.  Returns the throughput and the face kernel's average launch."""
+    from gdtk_amd import Simulation, cases
+    case = cases.mach6_box(nic, njc, nkc, seed=1234)
+    sim = Simulation(3, flux="ausmdv", numerics="lmr")
+    sim.add_block(case)
+    if warmup:
+        sim.run(warmup)
+    t0 = time.perf_counter()
+    _, st = sim.run(steps)
+    el = time.perf_counter() - t0
+    sim.profile(True)
+    sim.run(min(5, steps))
+    ks = sim.kernel_stats()
+    sim.profile(False)
+    sim.close()
+    if log:
+        log(f"lmr numerics: {case.ncells * steps / el:.3e} cell-updates/s")
+    return {
+        "value": case.ncells * steps / el, "unit": "cell-updates/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
+        "config": {"workload": f"Mach-6 3-D box {nic}x{njc}x{nkc}, ideal air, l2r2+van Albada with Eilmer 5 "
+                               "epsilon scaling, ausmdv with smooth-maximum sound speed, pc RK2",
+                   "cells": case.ncells, "final_dt": st["dt"]},
+        "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1)} for k, v in ks.items()
+                    if v["launches"]},
+    }
+
+
 def bench_flow_io(nic, njc, nkc, text_size=(64, 64, 64), log=None):
     """Flow-solution files of the Mach-6 box: write_solution and the restart read + cold start
     of one block, raw binary at the headline size and gzip text on a smaller box (the text
@@ -640,6 +670,9 @@ def main():
     ap.add_argument("--adaptive-steps", type=int, default=20,
                     help="steps of the adaptive EFM/AUSMDV run on the headline box (0 skips it)")
     ap.add_argument("--adaptive-only", action="store_true", help="measure only the adaptive flux calculator")
+    ap.add_argument("--lmr-steps", type=int, default=20,
+                    help="steps of the Eilmer 5 (lmr) numerics run on the headline box (0 skips it)")
+    ap.add_argument("--lmr-only", action="store_true", help="measure only the Eilmer 5 (lmr) numerics")
     ap.add_argument("--no-flow-io", action="store_true", help="skip the flow-file write/read measurement")
     ap.add_argument("--no-n90", action="store_true", help="skip the reference's n90 reacting cylinder case")
     ap.add_argument("--n90-only", action="store_true", help="run only the n90 case")
@@ -679,6 +712,9 @@ def main():
         return
     if args.flow_io_only:
         print(json.dumps(bench_flow_io(nic, njc, nkc, log=log), default=float), flush=True)
+        return
+    if args.lmr_only:
+        print(json.dumps(bench_lmr(nic, njc, nkc, args.lmr_steps, log=log), default=float), flush=True)
         return
     if args.adaptive_only:
         print(json.dumps(bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log), default=float), flush=True)
@@ -748,6 +784,8 @@ def main():
         out["batched"] = bench_batched(nblk, args.batched_steps, 2, args.flux, log)
     if args.adaptive_steps > 0:
         out["adaptive"] = bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log)
+    if args.lmr_steps > 0:
+        out["lmr"] = bench_lmr(nic, njc, nkc, args.lmr_steps, log=log)
     if not args.no_flow_io:
         out["flow_io"] = bench_flow_io(nic, njc, nkc, log=log)
     if not args.no_n90:

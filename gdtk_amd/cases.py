@@ -111,6 +111,20 @@ def sod_3d_tube(nic=100) -> BlockCase:
     return case
 
 
+def rippled_box(nic=40, njc=12, ripple=1.0e-9, seed=5) -> BlockCase:
+    """A 2-D box of air at 1e5 Pa, 300 K moving at about 50 m/s, with pressure ripples of
+    relative size ripple, on a perturbed grid with slip walls.  Differences of u between
+    neighbouring cells are then comparable to the van Albada smoothing of Eilmer 5
+    (epsilon scaled by |u| and the cell size), so the reconstruction depends on it."""
+    c = sod_2d(nic, njc)
+    xc = cell_centres(c)
+    c.p = np.full(c.ncells, 1.0e5) * (1.0 + ripple * np.sin(20.0 * xc[:, 0]) * np.cos(9.0 * xc[:, 1]))
+    c.T = np.full(c.ncells, 300.0)
+    c.vel = np.zeros((c.ncells, 3))
+    c.vel[:, 0] = 50.0 + 5.0 * np.sin(7.0 * xc[:, 1])
+    return perturb_vertices(c, amp=0.1, seed=seed)
+
+
 def mach6_box(nic=256, njc=128, nkc=128, seed=1234, dims=3) -> BlockCase:
     """Flux-path bandwidth stress: Mach-6 supersonic box, unit cell spacing.
 

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void k_shock_cells_batch(const BlockDev* __res
 
 // One thread per face of index direction d: reconstruction + flux calculator, or the
 // simple-outflow boundary flux effect on faces whose BC computes the flux itself.
-template <int DIM, int FC>
+template <int DIM, int FC, bool LMR>
 __device__ __forceinline__ void face_flux_one(const BlockDev& b, const IdealGasDev& gas, int d, double eps,
                                               int64_t f) {
     using C = CQ<DIM>;
@@ -253,16 +253,17 @@ __device__ __forceinline__ void face_flux_one(const BlockDev& b, const IdealGasD
         FallbackFS fb{fs, np, cL0, cR0};
         // Adaptive calculator: EFM where the face is shocked (strict rule), else AUSMDV.
         const bool shocked = flux_is_adaptive(FC) && ((b.shock[cL0] | b.shock[cR0]) != 0);
-        face_flux_l2r2<DIM, FC>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cR0 + FS_A * np], fb, len[cL1],
-                                len[cL0], len[cR0], len[cR1], eps, F, shocked);
+        face_flux_l2r2<DIM, FC, LMR>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cR0 + FS_A * np], fb,
+                                     len[cL1], len[cL0], len[cR0], len[cR1], eps, F, shocked);
     }
     for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
 }
 
 // Faces of every direction of every block in one launch.  Each block's faces of a direction
 // start at a multiple of the workgroup size (FaceBatch), so every workgroup serves one
-// direction of one block and reads that block's record with scalar loads.
-template <int DIM, int FC>
+// direction of one block and reads that block's record with scalar loads.  LMR selects
+// Eilmer 5's limiter smoothing and AUSMDV sound speed.
+template <int DIM, int FC, bool LMR>
 __global__ __launch_bounds__(256, 4) void k_face_flux_batch(const BlockDev* __restrict__ tab, FaceBatch fb,
                                                          int nblocks, double eps, IdealGasDev gas) {
     int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(256, 4) void k_face_flux_batch(const BlockDev* __re
     const BlockDev& b = tab[q];
     const int64_t f = w0 - start[q] + threadIdx.x;
     if (f >= b.nface[d]) return;
-    face_flux_one<DIM, FC>(b, gas, d, eps, f);
+    face_flux_one<DIM, FC, LMR>(b, gas, d, eps, f);
 }
 
 // Per interior cell: time derivatives (fvcell.d:824-854), the pc stage update
@@ -483,11 +484,29 @@ __global__ void k_decode(BlockDev b, IdealGasDev gas, double* U, double* fs, dou
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 hipError_t launch_face_flux_batch(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
-                                 double eps, const IdealGasDev& gas, hipStream_t s) {
+                                 int numerics, double eps, const IdealGasDev& gas, hipStream_t s) {
     const int64_t total = fb.total[0] + fb.total[1] + fb.total[2];
     if (total <= 0) return hipSuccess;
     dim3 grid(nblk(total, 256)), blk(256);
-#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F>), grid, blk, 0, s, tab, fb, nblocks, eps, gas)
+#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F, true>), grid, blk, 0, s, tab, fb, nblocks, eps, gas)
+    if (numerics == 1) {
+        if (dims == 3) {
+            if (flux_calc == 4) GDTK_FACE(3, 4);
+            else if (flux_calc == 3) GDTK_FACE(3, 3);
+            else if (flux_calc == 2) GDTK_FACE(3, 2);
+            else if (flux_calc == 1) GDTK_FACE(3, 1);
+            else GDTK_FACE(3, 0);
+        } else {
+            if (flux_calc == 4) GDTK_FACE(2, 4);
+            else if (flux_calc == 3) GDTK_FACE(2, 3);
+            else if (flux_calc == 2) GDTK_FACE(2, 2);
+            else if (flux_calc == 1) GDTK_FACE(2, 1);
+            else GDTK_FACE(2, 0);
+        }
+        return hipGetLastError();
+    }
+#undef GDTK_FACE
+#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F, false>), grid, blk, 0, s, tab, fb, nblocks, eps, gas)
     if (dims == 3) {
         if (flux_calc == 4) GDTK_FACE(3, 4);
         else if (flux_calc == 3) GDTK_FACE(3, 3);

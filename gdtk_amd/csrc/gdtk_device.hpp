@@ -83,12 +83,20 @@ struct L2R2 {
         twoR0R1 = (2.0 * lR0 + lenR1);
     }
     // onedinterp.d:357-384 with the defaults apply_limiter = extrema_clipping = true, beta = 1.
-    __device__ __forceinline__ void scalar(double qL1, double qL0, double qR0, double qR1, double eps, double& qL,
+    // LMR: Eilmer 5's limiter smoothing, epsilon_van_albada scaled by max(1e-12, |qL0|, |qR0|)
+    // and by 2 / (lenR0 + lenL0) (lmr/onedinterp.d:137-169).
+    template <bool LMR = false>
+    __device__ __forceinline__ void scalar(double qL1, double qL0, double qR0, double qR1, double eps_va, double& qL,
                                            double& qR) const {
         const double beta = 1.0;
         double delLminus = (qL0 - qL1) * tL01;
         double del = (qR0 - qL0) * tR0L0;
         double delRplus = (qR1 - qR0) * tR1R0;
+        double eps = eps_va;
+        if (LMR) {
+            const double qq = fmax(fmax(1e-12, fabs(qL0)), fmax(1e-12, fabs(qR0)));
+            eps = qq * eps_va * tR0L0;
+        }
         double sL = (delLminus * del + fabs(delLminus * del) + eps) / (delLminus * delLminus + del * del + eps);
         double sR = (del * delRplus + fabs(del * delRplus) + eps) / (del * del + delRplus * delRplus + eps);
         qL = qL0 + beta * sL * aL0 * (del * twoL0L1 + delLminus * lenR0);
@@ -118,8 +126,9 @@ struct CQ {
     static constexpr int n = ((DIM == 3) ? 5 : 4) + ((NSP > 1) ? NSP : 0);
 };
 
-// fluxcalc.d:474-647, factor == 1, accumulating into F (which starts at zero).
-template <int DIM, int NSP = 1>
+// fluxcalc.d:474-647, factor == 1, accumulating into F (which starts at zero).  LMR: Eilmer
+// 5's common sound speed, a smooth maximum of aL and aR (lmr/fluxcalc.d ausmdv).
+template <int DIM, int NSP = 1, bool LMR = false>
 __device__ __forceinline__ void ausmdv(const FaceStateT<NSP>& L, const FaceStateT<NSP>& R, double* F) {
     using C = CQ<DIM, NSP>;
     const double factor = 1.0;
@@ -139,7 +148,15 @@ __device__ __forceinline__ void ausmdv(const FaceStateT<NSP>& L, const FaceState
     double HR = eR + pRrR + keR;
     double alphaL = 2.0 * pLrL / (pLrL + pRrR);
     double alphaR = 2.0 * pRrR / (pLrL + pRrR);
-    double am = fmax(aL, aR);
+    double am;
+    if (LMR) {
+        const double da = aL - aR;
+        const double scale = 0.5 * (aL + aR);
+        const double eps = 1e-6 * scale + 1e-12;
+        am = 0.5 * (aL + aR) + 0.5 * sqrt(da * da + eps * eps);
+    } else {
+        am = fmax(aL, aR);
+    }
     double ML = uL / am;
     double MR = uR / am;
     double pLplus, uLplus;
@@ -307,7 +324,7 @@ struct FallbackFS {
     int64_t np, cL0, cR0;
 };
 
-template <int DIM, int FC>
+template <int DIM, int FC, bool LMR = false>
 __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const FaceGeom& g, const StencilCell& L1,
                                                const StencilCell& L0, const StencilCell& R0, const StencilCell& R1,
                                                double aL0, double aR0, const FallbackFS& fb, double lenL1,
@@ -318,11 +335,11 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
     L2R2 c;
     c.prepare(lenL1, lenL0, lenR0, lenR1);
     FaceState Ls, Rs;
-    c.scalar(vL1.x, vL0.x, vR0.x, vR1.x, eps, Ls.vel.x, Rs.vel.x);
-    c.scalar(vL1.y, vL0.y, vR0.y, vR1.y, eps, Ls.vel.y, Rs.vel.y);
-    c.scalar(vL1.z, vL0.z, vR0.z, vR1.z, eps, Ls.vel.z, Rs.vel.z);
-    c.scalar(L1.rho, L0.rho, R0.rho, R1.rho, eps, Ls.rho, Rs.rho);
-    c.scalar(L1.u, L0.u, R0.u, R1.u, eps, Ls.u, Rs.u);
+    c.scalar<LMR>(vL1.x, vL0.x, vR0.x, vR1.x, eps, Ls.vel.x, Rs.vel.x);
+    c.scalar<LMR>(vL1.y, vL0.y, vR0.y, vR1.y, eps, Ls.vel.y, Rs.vel.y);
+    c.scalar<LMR>(vL1.z, vL0.z, vR0.z, vR1.z, eps, Ls.vel.z, Rs.vel.z);
+    c.scalar<LMR>(L1.rho, L0.rho, R0.rho, R1.rho, eps, Ls.rho, Rs.rho);
+    c.scalar<LMR>(L1.u, L0.u, R0.u, R1.u, eps, Ls.u, Rs.u);
     Ls.a = aL0;
     Rs.a = aR0;
     if (!gas.update_rhou(Ls.rho, Ls.u, Ls.T, Ls.p)) {
@@ -345,7 +362,7 @@ __device__ __forceinline__ void face_flux_l2r2(const IdealGasDev& gas, const Fac
     // face's indicator 0 or 1, so one calculator runs with factor 1.
     if (FC == 1 || (FC == 2 && shocked)) efmflx<DIM>(Ls, Rs, gas.Cv, gas.Cv, F);
     else if (FC == 3 || (FC == 4 && shocked)) hanel<DIM>(Ls, Rs, F);
-    else ausmdv<DIM>(Ls, Rs, F);
+    else ausmdv<DIM, 1, LMR>(Ls, Rs, F);
     // fluxcalc.d:159-171 add grid-velocity terms, here with gvel = 0: each adds a signed
     // zero to a flux component.  Every component starts at +0 and is only ever changed by
     // x + y or x - y, so it is never -0 (a sum is -0 only if both terms are -0), and

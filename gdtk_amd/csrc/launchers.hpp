@@ -23,9 +23,10 @@ hipError_t launch_decode(const BlockDev& b, const IdealGasDev& gas, double* U, d
                          hipStream_t s);
 // Face-per-thread kernels over all blocks: the faces of every direction in one launch
 // (FaceBatch), then the cell update (dt read from device memory; start[q] = first launch
-// index of block q, a multiple of 256; total = launch size).
+// index of block q, a multiple of 256; total = launch size).  numerics: 0 Eilmer 4, 1 Eilmer 5
+// (lmr) limiter smoothing and AUSMDV sound speed.
 hipError_t launch_face_flux_batch(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
-                                 double eps, const IdealGasDev& gas, hipStream_t s);
+                                 int numerics, double eps, const IdealGasDev& gas, hipStream_t s);
 hipError_t launch_cell_update_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
                                    int stage, const double* dtp, const CellLimits& lim, const IdealGasDev& gas,
                                    hipStream_t s);

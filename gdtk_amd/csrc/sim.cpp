@@ -134,6 +134,7 @@ struct Sim {
     double prof_items[NKIND] = {0, 0, 0, 0, 0};
     // Shock-detector options (globalconfig.d:1061-1131) of the adaptive flux calculator.
     gdtk_shock_detector sdet{-0.30, 0.20, 1, 0, 0, 1000000000};
+    int numerics = GDTK_NUMERICS_EILMER4;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Stage variant chosen by the autotuner: -1 not tuned (use cfg.use_fused), 0 face-per-
     // thread, 1 fused.
@@ -226,10 +227,10 @@ struct Sim {
     }
 
     // What a captured step depends on besides the buffers: the kernel variant, the fused
-    // kernel's launch shape and whether stage 1 detects shocks.
+    // kernel's launch shape, whether stage 1 detects shocks and the numerics.
     long long launch_signature() const {
         return (use_fused() ? 1000000LL + 1000LL * kchunk + tile_x + 100000000LL * wg_per_cu : 0LL) +
-               (detect_now() ? 10000000000LL : 0LL);
+               (detect_now() ? 10000000000LL : 0LL) + 100000000000LL * numerics;
     }
 
     void build_starts() {
@@ -437,7 +438,8 @@ struct Sim {
                 if (nsp > 1)
                     return launch_face_flux_species(t, fb, nb, cfg.dimensions, cfg.flux_calculator, eps, nsp,
                                                     tpg_dev.p, stream);
-                return launch_face_flux_batch(t, fb, nb, cfg.dimensions, cfg.flux_calculator, eps, gas, stream);
+                return launch_face_flux_batch(t, fb, nb, cfg.dimensions, cfg.flux_calculator, numerics, eps, gas,
+                                              stream);
             });
             timed(1, nc, [&] {
                 if (nsp > 1)
@@ -835,10 +837,13 @@ struct Sim {
     }
 
     // The fused stage kernel serves the single-species gas with the AUSMDV or EFM
-    // calculator; a multi-species gas and the Hanel and adaptive calculators run the
-    // face-per-thread kernels.
+    // calculator and Eilmer 4 numerics; a multi-species gas, the Hanel and adaptive
+    // calculators and Eilmer 5 numerics run the face-per-thread kernels.
+    bool fusable() const {
+        return nsp == 1 && cfg.flux_calculator <= GDTK_FLUX_EFM && numerics == GDTK_NUMERICS_EILMER4;
+    }
     bool use_fused() const {
-        if (nsp > 1 || cfg.flux_calculator > GDTK_FLUX_EFM) return false;
+        if (!fusable()) return false;
         return variant >= 0 ? variant == 1 : cfg.use_fused != 0;
     }
 
@@ -913,7 +918,7 @@ struct Sim {
         };
         result.push_back(0); result.push_back(0); result.push_back(best); result.push_back(0);
         for (int tx : txs) {
-            if (nsp > 1 || cfg.flux_calculator > GDTK_FLUX_EFM) break;  // fused: one species, AUSMDV or EFM
+            if (!fusable()) break;  // fused: one species, AUSMDV or EFM, Eilmer 4 numerics
             if (cfg.dimensions == 3) {
                 for (int kc : kcs) add(1, kc, tx, time_variant(1, kc, tx));
             } else {
@@ -1294,7 +1299,10 @@ int gdtk_hip_sim_chem_stats(int sim, double* stats, int nstats) {
 int gdtk_hip_sim_set_gas_tpg(int sim, const gdtk_tpg_gas* gas) {
     return guarded([&] {
         if (!gas) throw Error(GDTK_ERR_ARG, "null argument");
-        get_sim(sim).set_gas_tpg(*gas);
+        Sim& s = get_sim(sim);
+        if (s.numerics == GDTK_NUMERICS_LMR && gas->nsp > 1)
+            throw Error(GDTK_ERR_UNSUPPORTED, "Eilmer 5 numerics are implemented for a single-species gas");
+        s.set_gas_tpg(*gas);
         return GDTK_OK;
     });
 }
@@ -1464,6 +1472,20 @@ int gdtk_hip_sim_set_shock_detector(int sim, const gdtk_shock_detector* opts) {
             throw Error(GDTK_ERR_UNSUPPORTED, "only the strict shock detector is implemented");
         s.sdet = *opts;
         s.drop_graphs();  // the tolerances are launch arguments of the captured steps
+        return GDTK_OK;
+    });
+}
+
+int gdtk_hip_sim_set_numerics(int sim, int numerics) {
+    return guarded([&] {
+        Sim& s = get_sim(sim);
+        if (numerics != GDTK_NUMERICS_EILMER4 && numerics != GDTK_NUMERICS_LMR)
+            throw Error(GDTK_ERR_ARG, "unknown numerics");
+        // lmr reconstructs mass fractions rather than species densities (lmr/onedinterp.d:268-286)
+        if (numerics == GDTK_NUMERICS_LMR && s.nsp > 1)
+            throw Error(GDTK_ERR_UNSUPPORTED, "Eilmer 5 numerics are implemented for a single-species gas");
+        s.numerics = numerics;
+        s.drop_graphs();
         return GDTK_OK;
     });
 }

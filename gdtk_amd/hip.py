@@ -27,6 +27,9 @@ FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, 
 # Flux calculators (src/eilmer/globalconfig.d:270-333 names; "adaptive" is the reference's alias).
 FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2, "hanel": 3,
                     "adaptive_hanel_ausmdv": 4}
+# Numerics of the flux path: Eilmer 4 (src/eilmer) or Eilmer 5 (src/lmr: dimensionalised
+# van Albada epsilon, smooth-maximum AUSMDV sound speed).
+NUMERICS = {"eilmer4": 0, "lmr": 1}
 # Shock-detector options and defaults (src/eilmer/globalconfig.d:1061-1131).
 SHOCK_DETECTOR_DEFAULTS = dict(compression_tolerance=-0.30, shear_tolerance=0.20, strict_shock_detector=True,
                                damped_outflow=False, frozen_shock_detector=False,
@@ -39,7 +42,7 @@ EXPORTS = [
     "gdtk_hip_sim_step_fixed_dt", "gdtk_hip_sim_set_dt", "gdtk_hip_sim_profile", "gdtk_hip_sim_kernel_stats",
     "gdtk_hip_sim_set_tuning", "gdtk_hip_sim_autotune", "gdtk_hip_last_error", "gdtk_hip_exchange_source",
     "gdtk_hip_sim_set_gas_tpg", "gdtk_hip_block_set_initial_massf", "gdtk_hip_sim_set_chemistry",
-    "gdtk_hip_sim_chem_stats", "gdtk_hip_sim_set_shock_detector", "gdtk_hip_block_write_flow",
+    "gdtk_hip_sim_chem_stats", "gdtk_hip_sim_set_shock_detector", "gdtk_hip_sim_set_numerics", "gdtk_hip_block_write_flow",
     "gdtk_hip_block_read_flow", "gdtk_hip_block_set_transport", "gdtk_hip_sim_set_time", "gdtk_hip_sim_get_state", "gdtk_hip_flow_nvars",
     "gdtk_hip_flow_write_records", "gdtk_hip_flow_read_records",
     "gdtk_hip_chem_create", "gdtk_hip_chem_destroy", "gdtk_hip_chem_step", "gdtk_hip_chem_step_device",
@@ -122,6 +125,7 @@ def lib():
         L.gdtk_hip_sim_set_chemistry.argtypes = [C.c_int, C.c_int, C.c_int]
         L.gdtk_hip_sim_chem_stats.argtypes = [C.c_int, dp, C.c_int]
         L.gdtk_hip_sim_set_shock_detector.argtypes = [C.c_int, C.POINTER(ShockDetector)]
+        L.gdtk_hip_sim_set_numerics.argtypes = [C.c_int, C.c_int]
         L.gdtk_hip_block_write_flow.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_double,
                                                 C.c_char_p]
         L.gdtk_hip_block_read_flow.argtypes = [C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_char_p, dp]
@@ -159,11 +163,12 @@ class Simulation:
 
     def __init__(self, dims, flux="ausmdv", cfl=0.5, dt_init=1.0e-3, dt_max=1.0e-3, cfl_count=10,
                  max_attempts_for_step=3, max_invalid_cells=0, mMass=0.02896, gamma=1.4, fused=True, tpg=None,
-                 shock_detector=None):
+                 shock_detector=None, numerics="eilmer4"):
         """flux: "ausmdv", "efm" or "adaptive_efm_ausmdv" ("adaptive": EFM on faces the shock
         detector marks, AUSMDV elsewhere).  tpg: a gdtk_amd.chem.TPGGas record for a
         multi-species thermally perfect gas (replaces the ideal gas given by mMass and gamma).
-        shock_detector: dict of SHOCK_DETECTOR_DEFAULTS keys to override."""
+        shock_detector: dict of SHOCK_DETECTOR_DEFAULTS keys to override.  numerics:
+        "eilmer4" or "lmr" (NUMERICS)."""
         self.dims = dims
         self.nsp = int(tpg.nsp) if tpg is not None else 1
         self.ncq = (5 if dims == 3 else 4) + (self.nsp if self.nsp > 1 else 0)
@@ -178,6 +183,14 @@ class Simulation:
             _check(lib().gdtk_hip_sim_set_gas_tpg(self.h, C.byref(tpg)))
         if shock_detector:
             self.set_shock_detector(**shock_detector)
+        if numerics != "eilmer4":
+            self.set_numerics(numerics)
+
+    def set_numerics(self, numerics):
+        """"eilmer4" or "lmr" (Eilmer 5's limiter smoothing and AUSMDV sound speed)."""
+        if numerics not in NUMERICS:
+            raise ValueError(f"unknown numerics {numerics!r}")
+        _check(lib().gdtk_hip_sim_set_numerics(self.h, NUMERICS[numerics]))
 
     def set_shock_detector(self, **opts):
         """Shock-detector options of the adaptive flux calculator (SHOCK_DETECTOR_DEFAULTS keys)."""

@@ -210,6 +210,19 @@ typedef struct gdtk_shock_detector {
  * Call before the first stage; defaults as commented above. */
 int gdtk_hip_sim_set_shock_detector(int sim, const gdtk_shock_detector* opts);
 
+/* Numerics of the flux path.  GDTK_NUMERICS_EILMER4 (the default) follows src/eilmer.
+ * GDTK_NUMERICS_LMR binds the same path for Eilmer 5 (src/lmr), whose structured-grid l2r2
+ * (lmr/onedinterp.d:137-169, 212-380) scales epsilon_van_albada by
+ * max(1e-12, |qL0|, |qR0|) * 2 / (lenR0 + lenL0), and whose AUSMDV (lmr/fluxcalc.d, ausmdv)
+ * takes the common sound speed as the smooth maximum
+ * 0.5 (aL + aR) + 0.5 sqrt((aL - aR)^2 + eps^2), eps = 1e-6 * 0.5 (aL + aR) + 1e-12.
+ * Implemented for a single-species gas (GDTK_ERR_UNSUPPORTED otherwise: lmr reconstructs
+ * mass fractions, Eilmer 4 species densities).  The stage then runs the face-per-thread
+ * kernels. */
+#define GDTK_NUMERICS_EILMER4 0
+#define GDTK_NUMERICS_LMR 1
+int gdtk_hip_sim_set_numerics(int sim, int numerics);
+
 /* One stage of the explicit update over all blocks (simcore_gasdynamic_step.d:946-1426):
  * exchange, ghost-cell effects, convective_flux_phase0 (sfluidblock.d:2188), boundary flux
  * effects, time_derivatives, the pc stage update and decode_conserved.  Returns

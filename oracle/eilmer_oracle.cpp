@@ -100,7 +100,7 @@ void scale_mass_fractions(std::vector<double>& massf, double tolerance, double a
 
 // fluxcalc.d:474-647
 void ausmdv(const FlowState& Lft, const FlowState& Rght, std::vector<double>& F, const CQI& cqi,
-            const GasModel& gm, bool entropy_fix, double factor) {
+            const GasModel& gm, bool entropy_fix, double factor, bool lmr) {
     double rL = Lft.gas.rho;
     double pL = Lft.gas.p;
     double pLrL = pL / rL;
@@ -124,6 +124,13 @@ void ausmdv(const FlowState& Lft, const FlowState& Rght, std::vector<double>& F,
     double alphaL = 2.0 * pLrL / (pLrL + pRrR);
     double alphaR = 2.0 * pRrR / (pLrL + pRrR);
     double am = std::fmax(aL, aR);
+    if (lmr) {
+        // lmr/fluxcalc.d, ausmdv: the smooth maximum of aL and aR (Biswas et al., CVPR 2021).
+        double da = aL - aR;
+        double scale = 0.5 * (aL + aR);
+        double eps = 1e-6 * scale + 1e-12;
+        am = 0.5 * (aL + aR) + 0.5 * std::sqrt(da * da + eps * eps);
+    }
     double ML = uL / am;
     double MR = uR / am;
     double pLplus, uLplus;
@@ -877,11 +884,18 @@ void L2R2::prepare(double lenL1, double lenL0, double lenR0, double lenR1) {
 
 // onedinterp.d:357-384 (beta == 1: apply_heuristic_pressure_based_limiting is off by default)
 void L2R2::scalar(double qL1, double qL0, double qR0, double qR1, double& qL, double& qR, bool apply_limiter,
-                  bool extrema_clipping, double epsilon_van_albada) const {
+                  bool extrema_clipping, double epsilon_van_albada, bool lmr) const {
     const double beta = 1.0;
     double delLminus = (qL0 - qL1) * two_over_lenL0_plus_lenL1;
     double del = (qR0 - qL0) * two_over_lenR0_plus_lenL0;
     double delRplus = (qR1 - qR0) * two_over_lenR1_plus_lenR0;
+    if (lmr) {
+        // lmr/onedinterp.d:147-151: epsilon dimensionalised by the larger cell value.
+        double qqL = std::fmax(1e-12, std::fabs(qL0));
+        double qqR = std::fmax(1e-12, std::fabs(qR0));
+        double qq = std::fmax(qqL, qqR);
+        epsilon_van_albada = qq * epsilon_van_albada * two_over_lenR0_plus_lenL0;
+    }
     double sL = 1.0, sR = 1.0;
     if (apply_limiter) {
         sL = (delLminus * del + std::fabs(delLminus * del) + epsilon_van_albada) /
@@ -986,13 +1000,13 @@ void SFluidBlock::compute_interface_flux(FlowState& L, FlowState& R, FVInterface
     L.vel.to_local(f->n, f->t1, f->t2);
     R.vel.to_local(f->n, f->t1, f->t2);
     switch (cfg.flux_calculator) {
-    case FluxCalc::ausmdv: ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix); break;
+    case FluxCalc::ausmdv: ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0, cfg.lmr); break;
     case FluxCalc::efm: efmflx(L, R, f->F, cqi, *gmodel); break;
     case FluxCalc::adaptive_efm_ausmdv: {
         // fluxcalc.d:1315-1332: EFM weighted by the face's shock indicator, AUSMDV by the rest.
         double alpha = f->S;
         if (alpha > 0.0) efmflx(L, R, f->F, cqi, *gmodel, alpha);
-        if (alpha < 1.0) ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0 - alpha);
+        if (alpha < 1.0) ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0 - alpha, cfg.lmr);
         break;
     }
     case FluxCalc::hanel: hanel(L, R, f->F, cqi, *gmodel); break;
@@ -1000,7 +1014,7 @@ void SFluidBlock::compute_interface_flux(FlowState& L, FlowState& R, FVInterface
         // fluxcalc.d:1335-1348
         double alpha = f->S;
         if (alpha > 0.0) hanel(L, R, f->F, cqi, *gmodel, alpha);
-        if (alpha < 1.0) ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0 - alpha);
+        if (alpha < 1.0) ausmdv(L, R, f->F, cqi, *gmodel, cfg.apply_entropy_fix, 1.0 - alpha, cfg.lmr);
         break;
     }
     }

@@ -126,6 +126,9 @@ struct Config {
     bool apply_limiter = true;               // globalconfig.d:1077
     bool extrema_clipping = true;            // globalconfig.d:1078
     double epsilon_van_albada = 1.0e-12;     // globalconfig.d:1079
+    // Eilmer 5 (src/lmr) numerics on this path: the dimensionalised van Albada epsilon
+    // (lmr/onedinterp.d:147-151) and AUSMDV's smooth-maximum sound speed (lmr/fluxcalc.d).
+    bool lmr = false;
     bool interpolate_in_local_frame = true;  // globalconfig.d:1081
     bool apply_entropy_fix = true;           // globalconfig.d:1088
     bool allow_reconstruction_for_species = true;   // globalconfig.d:1073
@@ -162,7 +165,7 @@ struct L2R2 {
         two_over_lenR1_plus_lenR0, two_lenL0_plus_lenL1, two_lenR0_plus_lenR1;
     void prepare(double lenL1, double lenL0, double lenR0, double lenR1);
     void scalar(double qL1, double qL0, double qR0, double qR1, double& qL, double& qR, bool apply_limiter,
-                bool extrema_clipping, double epsilon_van_albada) const;
+                bool extrema_clipping, double epsilon_van_albada, bool lmr = false) const;
 };
 
 // src/eilmer/fvcell.d (fields used on this path)
@@ -286,7 +289,8 @@ public:
 private:
     L2R2 l2r2;
     void interp_l2r2_scalar(double qL1, double qL0, double qR0, double qR1, double& qL, double& qR) {
-        l2r2.scalar(qL1, qL0, qR0, qR1, qL, qR, cfg.apply_limiter, cfg.extrema_clipping, cfg.epsilon_van_albada);
+        l2r2.scalar(qL1, qL0, qR0, qR1, qL, qR, cfg.apply_limiter, cfg.extrema_clipping, cfg.epsilon_van_albada,
+                    cfg.lmr);
     }
     FVCell* new_ghost();
 };
@@ -298,7 +302,7 @@ void ffc_source_index(int dims, int dface, int sface, int orient, int a, int b, 
 
 // Pure functions exposed for unit tests (same code the block uses).
 void ausmdv(const FlowState& L, const FlowState& R, std::vector<double>& F, const CQI& cqi,
-            const GasModel& gm, bool entropy_fix, double factor = 1.0);
+            const GasModel& gm, bool entropy_fix, double factor = 1.0, bool lmr = false);
 void efmflx(const FlowState& L, const FlowState& R, std::vector<double>& F, const CQI& cqi,
             const GasModel& gm, double factor = 1.0);
 void hanel(const FlowState& L, const FlowState& R, std::vector<double>& F, const CQI& cqi, const GasModel& gm,

@@ -56,9 +56,10 @@ def lib():
         L.orc_sim_stage.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_block_dt.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, dp]
         L.orc_sim_set_dt.argtypes = [C.c_void_p, C.c_double, C.c_int]
-        L.orc_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        L.orc_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp, dp, C.c_int]
+        L.orc_sim_set_numerics.argtypes = [C.c_void_p, C.c_int]
         L.orc_exxef.argtypes = [C.c_double, dp, dp]
-        L.orc_l2r2_scalar.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double, dp]
+        L.orc_l2r2_scalar.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double, dp, C.c_int]
         _lib = L
     return _lib
 
@@ -114,11 +115,12 @@ class Sim:
     """Whole-simulation oracle: blocks, BCs, the pc explicit update and the dt controller."""
 
     def __init__(self, gas, dims, flux="ausmdv", cfl=0.5, dt_init=1.0e-3, dt_max=1.0e-3, cfl_count=10,
-                 mutate_cell_velocities=True, shock_detector=None):
+                 mutate_cell_velocities=True, shock_detector=None, numerics="eilmer4"):
         """flux: "ausmdv", "efm" or "adaptive_efm_ausmdv" (EFM at detected shocks);
         shock_detector: dict of the detector options compression_tolerance (-0.30),
         shear_tolerance (0.20), strict_shock_detector (True), damped_outflow (False),
-        frozen_shock_detector (False), shock_detector_freeze_step (1e9)."""
+        frozen_shock_detector (False), shock_detector_freeze_step (1e9); numerics: "eilmer4"
+        or "lmr" (Eilmer 5's limiter smoothing and AUSMDV sound speed)."""
         self.gas = gas
         self.dims = dims
         fc = FLUX_CALCULATORS[flux]
@@ -129,6 +131,9 @@ class Sim:
         lib().orc_sim_set_shock_detector(self.h, sd["compression_tolerance"], sd["shear_tolerance"],
                                          int(sd["strict_shock_detector"]), int(sd["damped_outflow"]),
                                          int(sd["frozen_shock_detector"]), int(sd["shock_detector_freeze_step"]))
+        if numerics not in ("eilmer4", "lmr"):
+            raise ValueError(f"unknown numerics {numerics!r}")
+        lib().orc_sim_set_numerics(self.h, 1 if numerics == "lmr" else 0)
         self.blocks = []
 
     def close(self):
@@ -260,8 +265,9 @@ def pj_detector(velL, aL, velR, aR, n, t1, t2, comp_tol=-0.30, shear_tol=0.20):
     return lib().orc_pj_detector(v[0][1], aL, v[1][1], aR, v[2][1], v[3][1], v[4][1], comp_tol, shear_tol)
 
 
-def flux(gas, kind, dims, L, R, massfL=None, massfR=None):
-    """Unit flux: L, R = [rho, p, T, u, a, vx, vy, vz] in the face-local frame."""
+def flux(gas, kind, dims, L, R, massfL=None, massfR=None, lmr=False):
+    """Unit flux: L, R = [rho, p, T, u, a, vx, vy, vz] in the face-local frame; lmr: Eilmer
+    5's AUSMDV."""
     a, ap = _d(L)
     b, bp = _d(R)
     F = np.zeros(5 + max(gas.nsp, 1) + 1)
@@ -270,7 +276,8 @@ def flux(gas, kind, dims, L, R, massfL=None, massfR=None):
         ml, mL = _d(massfL)
     if massfR is not None:
         mr, mR = _d(massfR)
-    n = lib().orc_flux(gas.h, {"ausmdv": 0, "efm": 1, "hanel": 3}[kind], dims, ap, bp, mL, mR, F.ctypes.data_as(dp))
+    n = lib().orc_flux(gas.h, {"ausmdv": 0, "efm": 1, "hanel": 3}[kind], dims, ap, bp, mL, mR, F.ctypes.data_as(dp),
+                       1 if lmr else 0)
     return F[:n].copy()
 
 
@@ -280,12 +287,12 @@ def exxef(sn):
     return e1.value, e2.value
 
 
-def l2r2_scalar(lens, q, apply_limiter=True, extrema_clipping=True, eps=1.0e-12):
+def l2r2_scalar(lens, q, apply_limiter=True, extrema_clipping=True, eps=1.0e-12, lmr=False):
     l, lp = _d(lens)
     qq, qp = _d(q)
     out = np.zeros(2)
     lib().orc_l2r2_scalar(lp, qp, 1 if apply_limiter else 0, 1 if extrema_clipping else 0, eps,
-                          out.ctypes.data_as(dp))
+                          out.ctypes.data_as(dp), 1 if lmr else 0)
     return out[0], out[1]
 
 

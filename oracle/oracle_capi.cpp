@@ -78,6 +78,8 @@ void orc_sim_set_shock_detector(void* h, double comp_tol, double shear_tol, int 
     c.frozen_shock_detector = frozen != 0;
     c.shock_detector_freeze_step = freeze_step;
 }
+// Eilmer 5 (lmr) numerics on the flux path (Config::lmr).
+void orc_sim_set_numerics(void* h, int lmr) { static_cast<Sim*>(h)->sim->cfg.lmr = lmr != 0; }
 void orc_sim_destroy(void* h) {
     Sim* s = static_cast<Sim*>(h);
     delete s->sim;
@@ -349,7 +351,7 @@ void orc_sim_set_time(void* h, double time) { static_cast<Sim*>(h)->sim->st.time
 
 // Unit-level flux calculators: L, R = [rho, p, T, u, a, velx, vely, velz] in the local frame.
 int orc_flux(void* gas, int flux_calc, int dims, const double* L, const double* R, const double* massfL,
-             const double* massfR, double* F) {
+             const double* massfR, double* F, int lmr) {
     GasModel* gm = static_cast<GasModel*>(gas);
     CQI cqi;
     cqi.init(dims, gm->n_species);
@@ -363,7 +365,7 @@ int orc_flux(void* gas, int flux_calc, int dims, const double* L, const double* 
     std::vector<double> Fv(cqi.n, 0.0);
     if (flux_calc == 1) efmflx(l, r, Fv, cqi, *gm);
     else if (flux_calc == 3) hanel(l, r, Fv, cqi, *gm);
-    else ausmdv(l, r, Fv, cqi, *gm, true);
+    else ausmdv(l, r, Fv, cqi, *gm, true, 1.0, lmr != 0);
     for (int j = 0; j < cqi.n; ++j) F[j] = Fv[j];
     return cqi.n;
 }
@@ -372,10 +374,10 @@ void orc_exxef(double sn, double* exx, double* ef) { exxef(sn, *exx, *ef); }
 
 // l2r2 scalar reconstruction through the oracle's own L2R2 (same code the block uses).
 void orc_l2r2_scalar(const double* len4, const double* q4, int apply_limiter, int extrema_clipping, double eps,
-                     double* qLqR) {
+                     double* qLqR, int lmr) {
     L2R2 c;
     c.prepare(len4[0], len4[1], len4[2], len4[3]);
-    c.scalar(q4[0], q4[1], q4[2], q4[3], qLqR[0], qLqR[1], apply_limiter != 0, extrema_clipping != 0, eps);
+    c.scalar(q4[0], q4[1], q4[2], q4[3], qLqR[0], qLqR[1], apply_limiter != 0, extrema_clipping != 0, eps, lmr != 0);
 }
 
 }  // extern "C"

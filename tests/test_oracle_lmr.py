@@ -1,0 +1,127 @@
+"""The oracle's Eilmer 5 (src/lmr) numerics on the structured explicit path.
+
+Two functions differ from Eilmer 4 there:
+  - interp_l2r2_scalar (lmr/onedinterp.d:137-169) scales epsilon_van_albada by
+    max(1e-12, |qL0|, |qR0|) * 2 / (lenR0 + lenL0);
+  - ausmdv (lmr/fluxcalc.d) takes the common sound speed as the smooth maximum
+    0.5 (aL + aR) + 0.5 sqrt((aL - aR)^2 + eps^2), eps = 1e-6 * 0.5 (aL + aR) + 1e-12.
+
+Pins:
+  - limiter known answers worked by hand for cases where one of the two smoothing terms
+    decides the result (del = 0, so s = eps / (delLminus^2 + eps)), chosen so that each part
+    of the scaling (the cell value, the floor 1e-12, the length factor 2 / (lenR0 + lenL0)
+    rather than 2 / (lenL0 + lenL1)) changes the answer if it is dropped or misplaced;
+  - AUSMDV mass flux for gas at rest with aL = aR and pL != pR, where the split velocities
+    give ru_half = am (rL - rR) / 4 exactly, so the flux ratio lmr / Eilmer 4 is
+    am_smooth / a = 1 + 0.5e-6 + 0.5e-12 / a; with aL = 2 aR the smooth maximum equals
+    max(aL, aR) to 1e-12;
+  - the reference's own lmr integration test of this path,
+    examples/lmr/3D/sod-shock-tube/sg/test_sod_sg.py: 75 +/- 5 steps to 0.6 ms and the
+    probes at x = 0.78 and 0.6 within |v - ref| / (|ref| + 1) < 1 %.
+"""
+import numpy as np
+import pytest
+
+from gdtk_amd import cases
+
+R_AIR = 8.31446261815324 / 0.02896
+GAMMA = 1.4
+
+
+def test_limiter_known_answers(oracle_lib):
+    O = oracle_lib
+    q = [3.0, 4.0, 4.0, 5.0]          # delLminus = delRplus = 1 (unit lengths), del = 0
+    ones = [1.0] * 4                  # aL0 = aR0 = 1/8
+    # Eilmer 4: s = 0.25 / (1 + 0.25) = 0.2, qL = 4 + 0.2 / 8, qR = 4 - 0.2 / 8
+    qL, qR = O.l2r2_scalar(ones, q, extrema_clipping=False, eps=0.25)
+    assert qL == pytest.approx(4.025, rel=1e-15) and qR == pytest.approx(3.975, rel=1e-15)
+    # lmr: eps' = 4 * 0.25 * 1 = 1, s = 1 / 2, qL = 4 + 1 / 16
+    qL, qR = O.l2r2_scalar(ones, q, extrema_clipping=False, eps=0.25, lmr=True)
+    assert qL == 4.0625 and qR == 3.9375
+    # cell lengths 2: differences halve (delLminus = 1/2), eps' = 4 * 0.25 * 0.5 = 1/2,
+    # s = 0.5 / (0.25 + 0.5) = 2/3, aL0 = 1/8, qL = 4 + (2/3)(1/8)(0.5 * 2) = 4 + 1/12
+    qL, qR = O.l2r2_scalar([2.0] * 4, q, extrema_clipping=False, eps=0.25, lmr=True)
+    assert qL == pytest.approx(4.0 + 1.0 / 12.0, rel=1e-15)
+    assert qR == pytest.approx(4.0 - 1.0 / 12.0, rel=1e-15)
+    # lenL1 = 3: 2 / (lenL0 + lenL1) = 1/2 but 2 / (lenR0 + lenL0) = 1, so eps' = 1,
+    # delLminus = 1/2, s = 1 / 1.25 = 0.8, aL0 = 1/12, qL = 4 + 0.8 (1/12)(0.5) = 4 + 1/30
+    qL, _ = O.l2r2_scalar([3.0, 1.0, 1.0, 1.0], q, extrema_clipping=False, eps=0.25, lmr=True)
+    assert qL == pytest.approx(4.0 + 1.0 / 30.0, rel=1e-15)
+    # the floor: qL0 = qR0 = 0 gives eps' = 1e-12 * 0.25, with delLminus = 1e-6:
+    # s = 2.5e-13 / (1e-12 + 2.5e-13) = 0.2 and qL = 0.2 * (1/8) * 1e-6 (Eilmer 4: s ~ 1)
+    qL, _ = O.l2r2_scalar(ones, [-1.0e-6, 0.0, 0.0, 0.0], extrema_clipping=False, eps=0.25, lmr=True)
+    assert qL == pytest.approx(0.2 * 0.125 * 1.0e-6, rel=1e-12)
+    qL4, _ = O.l2r2_scalar(ones, [-1.0e-6, 0.0, 0.0, 0.0], extrema_clipping=False, eps=0.25)
+    assert qL4 == pytest.approx(0.125 * 1.0e-6, rel=1e-9)
+    # the defaults (eps 1e-12, clipping on) leave a steep profile unchanged by the scaling
+    qa = O.l2r2_scalar(ones, [1.0, 2.0, 4.0, 5.0])
+    qb = O.l2r2_scalar(ones, [1.0, 2.0, 4.0, 5.0], lmr=True)
+    assert qa == pytest.approx(qb, rel=1e-12)
+
+
+def _state(p, T, vx=0.0):
+    rho = p / (R_AIR * T)
+    cv = R_AIR / (GAMMA - 1.0)
+    return [rho, p, T, cv * T, np.sqrt(GAMMA * R_AIR * T), vx, 0.0, 0.0]
+
+
+def test_ausmdv_smooth_sound_speed(oracle_lib):
+    O = oracle_lib
+    g = O.IdealGas()
+    L, R = _state(2.0e5, 300.0), _state(1.0e5, 300.0)
+    a = L[4]
+    f4 = O.flux(g, "ausmdv", 2, L, R)
+    f5 = O.flux(g, "ausmdv", 2, L, R, lmr=True)
+    # gas at rest, alphaL = alphaR = 1: ru_half = am (rL - rR) / 4
+    assert f4[0] == pytest.approx(0.25 * a * (L[0] - R[0]), rel=1e-14)
+    assert f5[0] / f4[0] == pytest.approx(1.0 + 0.5e-6 + 0.5e-12 / a, rel=1e-14)
+    assert f5[0] / f4[0] - 1.0 > 4.9e-7      # the smoothing term is present
+    # the pressure flux (pL+ + pR- = (pL + pR) / 2) does not depend on am
+    assert f5[1] == pytest.approx(f4[1], rel=1e-15)
+    assert f4[1] == pytest.approx(0.5 * (L[1] + R[1]), rel=1e-15)
+    # aL = 2 aR: the smooth maximum is max(aL, aR) to about 1e-12
+    L2, R2 = _state(1.0e5, 1200.0, 50.0), _state(1.0e5, 300.0, 50.0)
+    assert np.allclose(O.flux(g, "ausmdv", 3, L2, R2, lmr=True), O.flux(g, "ausmdv", 3, L2, R2), rtol=1e-11,
+                       atol=0.0)
+    # consistency F(U, U) = physical flux holds for any common sound speed
+    S = _state(8.0e4, 500.0, 120.0)
+    F = O.flux(g, "ausmdv", 2, S, S, lmr=True)
+    rho, p, u = S[0], S[1], S[5]
+    E = S[3] + 0.5 * u * u
+    assert F[0] == pytest.approx(rho * u, rel=1e-14)
+    assert F[1] == pytest.approx(rho * u * u + p, rel=1e-14)
+    assert F[3] == pytest.approx(rho * u * E + p * u, rel=1e-14)
+
+
+def test_lmr_sod_tube_reference_probes(oracle_lib):
+    """examples/lmr/3D/sod-shock-tube/sg (job.lua, test_sod_sg.py:54-86)."""
+    O = oracle_lib
+    case = cases.sod_3d_tube()
+    s = O.Sim(O.IdealGas(), 3, "ausmdv", numerics="lmr")
+    s.add_block(case)
+    _, st = s.run(600, 0.6e-3)
+    assert abs(st["step"] - 75) < 5
+    assert abs(st["time"] - 0.6e-3) / 0.6e-3 < 0.01
+    xc = cases.cell_centres(case)[:, 0]
+    vals = {k: s.get(0, k) for k in ("rho", "p", "T", "vx")}
+    for x, ref in ((0.78, dict(rho=0.2647, p=30.2e3, T=398.0, vx=293.0)),
+                   (0.6, dict(rho=0.4271, p=30.2e3, T=247.0, vx=293.0))):
+        i = int(np.argmin(np.abs(xc - x)))
+        for k, v in ref.items():
+            assert abs(vals[k][i] - v) / (abs(v) + 1.0) < 0.01, (x, k, vals[k][i])
+
+
+def test_lmr_numerics_change_a_smooth_field(oracle_lib):
+    """On a flow with pressure ripples of 1e-9 relative, the dimensionalised epsilon is
+    comparable to the squared differences, so the two numerics give measurably different
+    states (the parity tests of the library rely on this to tell them apart)."""
+    O = oracle_lib
+    case = cases.rippled_box()
+    out = {}
+    for num in ("eilmer4", "lmr"):
+        s = O.Sim(O.IdealGas(), 2, "ausmdv", numerics=num)
+        s.add_block(case)
+        s.run(4)
+        out[num] = s.get(0, "p")
+    d = np.max(np.abs(out["lmr"] - out["eilmer4"]) / out["eilmer4"])
+    assert d > 1.0e-11, d
