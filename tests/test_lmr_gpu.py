@@ -2,10 +2,11 @@
 
 The library's lmr mode (gdtk_hip_sim_set_numerics) scales the van Albada epsilon by the
 cell values and the cell size (lmr/onedinterp.d:147-151) and takes AUSMDV's common sound
-speed as a smooth maximum (lmr/fluxcalc.d).  The rippled box (pressure ripples of 1e-9)
-makes the reconstruction depend on the epsilon scaling: there the two numerics differ by
-about 1e-8 relative in p after a few steps, so agreement with the oracle's lmr mode at the
-1e-12 bar, and disagreement with its Eilmer 4 mode, shows which numerics ran.  The 3-D
+speed as a smooth maximum (lmr/fluxcalc.d); EFM and Hanel take only the scaled limiter.
+The rippled box (pressure ripples of 1e-9) makes the reconstruction depend on the epsilon
+scaling: there the two numerics differ by about 5e-9 relative in p after a few steps, so
+agreement with the oracle's lmr mode at the 1e-12 bar, and disagreement with its Eilmer 4
+mode, shows which numerics ran.  The 3-D
 Sod tube is the reference's lmr integration test (examples/lmr/3D/sod-shock-tube/sg,
 test_sod_sg.py: 75 +/- 5 steps, probes within 1 %).
 """
@@ -34,7 +35,7 @@ def _oracle(case, flux, numerics):
     return o
 
 
-@pytest.mark.parametrize("flux", ["ausmdv", "efm", "adaptive"])
+@pytest.mark.parametrize("flux", ["ausmdv", "efm", "adaptive", "hanel", "adaptive_hanel_ausmdv"])
 def test_rippled_box_matches_lmr_oracle(flux):
     case = cases.rippled_box()
     g = _gpu(case, flux, "lmr")
