@@ -44,9 +44,8 @@ def air_cells(n, seed=0, T_lo=1500.0, T_hi=12000.0, p_lo=1.0e3, p_hi=1.0e6):
     R = 8.31451 / np.array(gd["M"])
     Rmix = Y @ R
     rho = p / (Rmix * T)
-    u = Y @ np.stack([_cea_h(gd["thermo"][i], T) - R[i] * T for i in range(5)], axis=0) \
-        if False else np.einsum("ns,sn->n", Y, np.stack([_cea_h(gd["thermo"][i], T, R[i]) - R[i] * T
-                                                         for i in range(5)], axis=0))
+    e_sp = np.stack([_cea_h(gd["thermo"][i], T, R[i]) - R[i] * T for i in range(5)], axis=0)
+    u = np.einsum("ns,sn->n", Y, e_sp)
     return dict(rho=rho, T=T, p=p, u=u, massf=np.ascontiguousarray(Y.T).reshape(-1))
 
 
