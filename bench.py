@@ -1,12 +1,20 @@
-"""Headline benchmark: FP64 cell-updates/s of the structured-block explicit update.
+"""Headline benchmark: FP64 cell-updates/s of the structured-block explicit update with
+operator-split chemistry (flux + chem).
 
-Workload (one GPU): the flux-path bandwidth stress case, a single 3-D block of
-256x128x128 cells of ideal air, Mach-6 free stream (p = 2e3 Pa, T = 220 K) with seeded
-+/-5 % Fourier perturbations of density and temperature, supersonic inflow on the west
-face and simple outflow elsewhere, MUSCL (l2r2) + van Albada, AUSMDV, predictor-corrector
-(2 stages) with the reference's CFL controller.  A step is one full explicit update of
-all cells (dt control, ghost cells, two flux/update stages); a cell-update is one cell
-advanced one step.  Inputs are resident in HBM before timing starts.
+Headline workload (one GPU): the coupled 2-D Mach-10 cylinder, R = 1 cm, 5-species air
+(N2, O2, NO, N, O; Park 1-T mechanism) at p = 500 Pa, T = 300 K, on 8 blocks of 256x128
+cells joined by exchange boundaries; MUSCL (l2r2) + van Albada, AUSMDV, predictor-corrector
+(2 stages) with the reference's CFL controller, then the alpha-QSS chemistry step of every
+cell (operator split).  A step is one flux step plus one chemistry step of all cells; a
+cell-update is one cell advanced one such step.  The flow is started from the free stream
+and spun up (set-up, untimed) until the bow shock and its reacting shock layer stand off
+the body; inputs are resident in HBM before timing starts.
+
+Sub-legs (compact records, summarised last in the JSON line): the Sod tube (200x100, 500
+steps), the Mach-6 box (256x128x128, AUSMDV and EFM), 64 uneven blocks, 4 Mi chemistry
+cells with the alpha-QSS integrator and the learned surrogate, the cylinder's surrogate
+run against the integrator run, adaptive flux, Eilmer 5 numerics, flow files and the
+reference's n90 case.
 
 Prints one JSON line (see the repository's benchmark contract).
 """
@@ -54,14 +62,14 @@ def face_bytes_per_face(dims=3):
     return 8.0 * (10 + ncq + 7)
 
 
-def cpu_baseline(seconds_target=15.0):
+def cpu_baseline(seconds_target=15.0, flux="ausmdv"):
     """The oracle (C++ restatement of the D path, one thread as in the reference's one-block
     mode) timed on a bounded sample: a 256x128x8 slab of the same Mach-6 box."""
     from gdtk_amd import cases
     from oracle import oracle as O
     case = cases.mach6_box(256, 128, 8, seed=1234)
     g = O.IdealGas()
-    s = O.Sim(g, 3, "ausmdv")
+    s = O.Sim(g, 3, flux)
     s.add_block(case)
     s.run(1)  # first step (dt set-up) untimed
     steps = 0
@@ -72,20 +80,45 @@ def cpu_baseline(seconds_target=15.0):
         el = time.perf_counter() - t0
         if el > seconds_target or steps >= 50:
             break
+    s.close()
     return dict(value=case.ncells * steps / el, unit="cell-updates/s", cores=1, kind="port",
-                sample=f"oracle, 1 thread, 256x128x8 slab of the Mach-6 box, {steps} steps in {el:.1f} s")
+                sample=f"oracle, 1 thread, {flux}, 256x128x8 slab of the Mach-6 box, {steps} steps in {el:.1f} s")
+
+
+def source_digest(names):
+    """A digest of the library sources that define a kernel, so that a committed counter
+    summary is used only for the kernel build it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for n in names:
+        with open(os.path.join(ROOT, "gdtk_amd", "csrc", n), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+# Sources of each kernel whose HBM traffic is taken from profiles/traffic.json.
+KERNEL_SOURCES = {
+    "k_face_flux_batch": ("flux_kernels.hip", "gdtk_device.hpp", "kernel_common.hpp", "block.hpp"),
+    "k_face_flux_species": ("flux_species.hip", "flux_species.hpp", "chem_device.hpp", "gdtk_device.hpp",
+                            "kernel_common.hpp", "block.hpp"),
+}
 
 
 def measured_traffic(kernel, items):
-    """HBM bytes per launch of the kernel from the committed counter summary, if one exists
-    for this kernel and workload (profiles/*_traffic.json, written from rocprofv3 --pmc
-    FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel of known size)."""
-    path = os.path.join(ROOT, "profiles", "flux_mach6_256x128x128_traffic.json")
+    """HBM bytes per launch of the kernel from the committed counter summary
+    (profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a copy
+    kernel of known size), if a record exists for this kernel, this launch size and the
+    current sources of the kernel; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    base = kernel.split("<")[0]
     try:
         with open(path) as f:
-            d = json.load(f)
-        if d.get("kernel") == kernel and abs(d.get("items_per_launch", 0) - items) < 0.5:
-            return d["hbm_bytes_per_launch"]
+            recs = json.load(f)
+        digest = source_digest(KERNEL_SOURCES[base])
+        for d in recs:
+            if d.get("kernel") == kernel and abs(d.get("items_per_launch", 0) - items) < 0.5 \
+                    and d.get("source_digest") == digest:
+                return d["hbm_bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         pass
     return None
@@ -240,6 +273,7 @@ def bench_chem(ncells, steps, warmup, with_cpu=True, seed=2024, dt=1.0e-7):
 
 MLP_FLOPS_PER_CELL = 2 * (8 * 256 + 256 * 256 + 256 * 256 + 256 * 5)  # 268 800 per network
 BF16_PEAK_TFLOPS = 2500.0
+DY_BOUND, T_BOUND, SPEEDUP_TARGET = 1.0e-3, 5.0e-3, 5.0
 
 
 def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, tols=(2.0e-4, 5.0e-4, 2.0e-3), log=None):
@@ -309,10 +343,18 @@ def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, tols=(
             log(f"surrogate tol {tol:g}: {sweep[-1]['value']:.3e} cells/s, fallback "
                 f"{sweep[-1]['fallback_fraction']:.3f}, max dY {sweep[-1]['max_abs_dY']:.2e}")
         sur.close()
-    pick = next((r for r in sweep if r["max_abs_dY"] <= 1.0e-3), sweep[-1])
+    # The target operating point: max |dY| <= 1e-3 and T error <= 0.5 % over the cells that
+    # both complete.  Where no tolerance reaches it, value is None and the nearest point (the
+    # smallest max |dY|) is reported beside it.
+    meets = [r for r in sweep if r["max_abs_dY"] <= DY_BOUND and r["max_rel_T_err"] <= T_BOUND]
+    nearest = min(sweep, key=lambda r: r["max_abs_dY"])
+    pick = meets[0] if meets else nearest
     return {
         "metric": "chem cells/s (surrogate + alpha-QSS fallback)",
-        "value": pick["value"],
+        "value": pick["value"] if meets else None,
+        "meets_target": bool(meets),
+        "target": {"max_abs_dY": DY_BOUND, "max_rel_T_err": T_BOUND},
+        "nearest": nearest,
         "unit": "cells/s",
         "ms_per_step": pick["ms_per_step"],
         "steps": steps,
@@ -328,15 +370,15 @@ def bench_surrogate(ncells, steps, seed=2024, dt=1.0e-7, n_train=1 << 22, tols=(
     }
 
 
-def bench_batched(nblk, steps, warmup, flux="ausmdv", log=None):
-    """Batched uneven blocks: nblk[0] x nblk[1] x nblk[2] blocks with edges drawn from 16..192
-    cells (seeded), joined by exchange boundaries, each with its own Mach 2-10 free stream;
-    every stage is one ghost-cell launch and one persistent stage launch over all blocks,
-    each step one captured graph.  Returns a sub-record with its own throughput and the
-    stage kernel's roofline."""
+def bench_batched(nblocks, steps, warmup, flux="ausmdv", log=None):
+    """Batched uneven blocks: nblocks blocks with per-block edges drawn from 16..192 cells
+    (seeded; cases.batched_ragged), in pairs joined by exchange boundaries, each with its
+    own Mach 2-10 free stream; every stage is one ghost-cell launch and one face and one cell
+    launch (or one persistent stage launch) over all blocks, each step one captured graph.
+    Returns a sub-record with its own throughput and the dominant kernel's roofline."""
     from gdtk_amd import Simulation, cases
     t0 = time.perf_counter()
-    blocks = cases.batched_uneven(nblk=nblk)
+    blocks = cases.batched_ragged(nblocks)
     ncells = sum(b.ncells for b in blocks)
     sim = Simulation(3, flux=flux)
     for q, b in enumerate(blocks):
@@ -365,8 +407,8 @@ def bench_batched(nblk, steps, warmup, flux="ausmdv", log=None):
     sim.close()
     return {
         "value": ncells * steps / el, "unit": "cell-updates/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
-        "config": {"workload": f"batched uneven blocks {nblk[0]}x{nblk[1]}x{nblk[2]}, edges 16-192, exchange, "
-                               f"Mach 2-10 per block, {flux}, pc RK2",
+        "config": {"workload": f"batched uneven blocks: {nblocks} blocks, per-block edges 16-192, pairs joined "
+                               f"by exchange, Mach 2-10 per block, {flux}, pc RK2",
                    "blocks": len(blocks), "cells": ncells, "stage_kernels": tune["best"], "kchunk": tune["kchunk"],
                    "tile_x": tune.get("tile_x"),
                    "graph": True, "final_dt": st["dt"]},
@@ -573,8 +615,10 @@ def bench_coupled(steps, snapshot_every=250, tols=(5.0e-4, 2.0e-3), log=None):
             done += n
             if sample is not None:
                 sample.append(_cell_fields(g, nb, ["rho", "T", "u", "p", 300, 301, 302, 303, 304]))
-            if log:
-                log(f"coupled: {done} steps, t = {st['time']:.3e} s, {cases.stagnation_line(g, blocks)}")
+            if log and done % 1000 == 0:
+                sl = cases.stagnation_line(g, blocks)
+                log(f"cylinder: {done} steps, t = {st['time']:.3e} s, stand-off {sl['standoff'] * 1e3:.3f} mm, "
+                    f"T_stag {sl['T_stag']:.0f} K")
         return el, st
 
     pool = []
@@ -620,7 +664,12 @@ def bench_coupled(steps, snapshot_every=250, tols=(5.0e-4, 2.0e-3), log=None):
             "T_stag_K": sl_s["T_stag"], "T_stag_rel_diff": sl_s["T_stag"] / sl_i["T_stag"] - 1.0,
             "max_abs_dY": dY, "T_rel_err_max": float(Terr.max()), "T_rel_err_p99": float(np.quantile(Terr, 0.99)),
             "final_time": st_s["time"]})
+        r = results[-1]
+        r["meets_target"] = bool(r["chem_speedup_vs_alpha_qss"] >= SPEEDUP_TARGET and r["max_abs_dY"] <= DY_BOUND
+                                 and abs(r["T_stag_rel_diff"]) <= T_BOUND)
     return {
+        "meets_target": any(r["meets_target"] for r in results),
+        "target": {"chem_speedup_vs_alpha_qss": SPEEDUP_TARGET, "max_abs_dY": DY_BOUND, "T_stag_rel_diff": T_BOUND},
         "config": {"workload": "2-D Mach-10 cylinder, R = 1 cm, p = 500 Pa, T = 300 K, 5-species air (Park 1-T), "
                                "8 blocks of 256x128, AUSMDV, pc RK2 + operator-split chemistry",
                    "cells": ncell, "steps": steps, "final_time": st_i["time"]},
@@ -644,34 +693,301 @@ def copy_probe(nbytes=1 << 30):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+def species_face_bytes(dims, nsp):
+    """Algorithmic HBM bytes per face of the multi-species face-flux kernel (FP64): the face's
+    frame and area (10 doubles), F (ncq doubles), and one new stencil cell per face (its
+    stencil cells are shared with the neighbouring faces of the same direction): internal
+    energy, the velocity components, the nsp species densities, the cell length along the
+    direction, and the cell's T (the Newton start) and sound speed."""
+    ncq = (5 if dims == 3 else 4) + nsp
+    return 8.0 * (10 + ncq + 3 + dims + nsp + 1)
+
+
+def coupled_cpu_baseline(g, blocks, rec, mech, dt, seconds_target=12.0):
+    """The oracle's coupled step (flux + chemistry, one thread, as the reference runs one
+    block per thread) on the GPU run's spun-up state: every block's flow state handed over,
+    then whole steps at the GPU's dt until about seconds_target has passed."""
+    from oracle import oracle as O
+    gas = O.TPG(rec)
+    t0 = time.perf_counter()
+    o = O.Sim(gas, 2, "ausmdv", dt_init=1.0e-10, dt_max=1.0e-6, mutate_cell_velocities=False)
+    for b in blocks:
+        o.add_block(b)
+    import numpy as np
+    for q, b in enumerate(blocks):
+        mf = np.stack([g.get(q, 300 + s) for s in range(5)], axis=1)
+        o.set_fs(q, g.get(q, "rho"), g.get(q, "vx"), g.get(q, "vy"), g.get(q, "vz"), g.get(q, "p"), g.get(q, "a"),
+                 g.get(q, "u"), g.get(q, "T"), mf)
+    o.set_chemistry(O.Chem(rec, mech))
+    setup = time.perf_counter() - t0
+    ncell = sum(b.ncells for b in blocks)
+    steps, el = 0, 0.0
+    while el < seconds_target and steps < 20:
+        t1 = time.perf_counter()
+        o.step_fixed_dt(dt)
+        el += time.perf_counter() - t1
+        steps += 1
+    o.close()
+    return dict(value=ncell * steps / el, unit="cell-updates/s", cores=1, kind="port",
+                sample=f"oracle, 1 thread, {steps} whole steps (flux + alpha-QSS chemistry) of all {ncell} cells "
+                       f"from the GPU run's spun-up state at its dt, {el:.1f} s (hand-over {setup:.1f} s untimed)")
+
+
+def bench_coupled_headline(steps, warmup, spinup, with_cpu=True, log=None):
+    """The headline: coupled cylinder flux + chemistry steps with the alpha-QSS integrator,
+    timed after the spin-up; the dominant kernel's roofline from HIP events around its
+    launches on the library's stream."""
+    from gdtk_amd import Simulation, cases
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, mech_record
+    blocks, rec, info = cases.cylinder_blocks()
+    ncell = sum(b.ncells for b in blocks)
+    mech = mech_record(gdata.load("park-air-5sp-1T"))
+    ch = Chemistry(rec, mech)
+    g = Simulation(2, tpg=rec, dt_init=1.0e-10, dt_max=1.0e-6)
+    for b in blocks:
+        g.add_block(b)
+    g.set_chemistry(ch)
+    t0 = time.perf_counter()
+    g.run(spinup)
+    sl = cases.stagnation_line(g, blocks)
+    if log:
+        log(f"headline: spin-up {spinup} steps in {time.perf_counter() - t0:.1f} s, t = {g.state()['time']:.3e} s, "
+            f"stand-off {sl['standoff'] * 1e3:.3f} mm, T_stag {sl['T_stag']:.0f} K")
+    if warmup:
+        g.run(warmup)
+    cs0 = g.chem_stats()
+    t0 = time.perf_counter()
+    _, st = g.run(steps)
+    el = time.perf_counter() - t0
+    cs1 = g.chem_stats()
+    nprof = max(1, min(10, steps))
+    g.profile(True)
+    cs2 = g.chem_stats()
+    g.run(nprof)
+    ks = g.kernel_stats()
+    cs3 = g.chem_stats()
+    g.profile(False)
+    chs = ch.stats()
+    per_step = {k: v["ms"] / nprof for k, v in ks.items() if v["launches"]}
+    per_step["chemistry"] = (cs3["ms"] - cs2["ms"]) / max(cs3["steps"] - cs2["steps"], 1)
+    hot = sum(int((g.get(q, "T") > 2000.0).sum()) for q in range(len(blocks)))
+    face = ks["face_flux"]
+    face_avg = face["ms"] / max(face["launches"], 1)
+    bpf = species_face_bytes(2, 5)
+    face_ach = bpf * face["items_per_launch"] / (face_avg * 1e-3) / 1e9
+    kname = "k_face_flux_species<2,0,5>"
+    roof_face = {"bound": "hbm", "achieved": face_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": face_ach / HBM_PEAK_GBS, "traffic": measured_traffic(kname, face["items_per_launch"]),
+                 "kernel": kname, "avg_launch_ms": face_avg, "bytes_per_face": bpf,
+                 "items_per_launch": face["items_per_launch"]}
+    per_att, per_it = alpha_qss_flops(mech, 5)
+    chem_flops = per_att * chs["attempts_total"] + per_it * chs["iters_total"]
+    chem_ach = chem_flops / max(chs["ms_integrate"] * 1e-3, 1e-12) / 1e12
+    roof_chem = {"bound": "fp64", "achieved": chem_ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": chem_ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_chem_integrate<5,5,0>",
+                 "avg_launch_ms": chs["ms_integrate"], "flops_per_launch": chem_flops}
+    # the flux face launches (two per step) against the whole chemistry step
+    dominant = "face_flux" if 2 * face_avg >= per_step["chemistry"] else "chemistry"
+    out = {
+        "metric": "cell-updates/s (flux+chem) FP64",
+        "value": ncell * steps / el,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "coupled 2-D Mach-10 cylinder, R = 1 cm, p = 500 Pa, T = 300 K, 5-species air "
+                               "(Park 1-T), 8 blocks of 256x128, l2r2+van Albada, AUSMDV, pc RK2 + operator-split "
+                               "alpha-QSS chemistry", "cells": ncell, "spinup_steps": spinup,
+                   "time_at_start": g.state()["time"] - 0.0, "final_dt": st["dt"],
+                   "cells_above_2000K": hot, "chem_ms_per_step": (cs1["ms"] - cs0["ms"]) / max(
+                       cs1["steps"] - cs0["steps"], 1)},
+        "roofline": roof_face if dominant == "face_flux" else roof_chem,
+        "roofline_other": roof_chem if dominant == "face_flux" else roof_face,
+        "kernels_ms_per_step": per_step,
+    }
+    if with_cpu:
+        try:
+            out["cpu_baseline"] = coupled_cpu_baseline(g, blocks, rec, mech, st["dt"])
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"error": str(e)}
+    g.close()
+    return out
+
+
+def bench_box(nic, njc, nkc, flux, steps, warmup, with_cpu=True, cpu_seconds=15.0, log=None):
+    """The Mach-6 box (flux-path bandwidth stress, ideal air) with one flux calculator: the
+    autotuner's stage kernels, K timed steps, the face kernel's roofline by HIP events."""
+    from gdtk_amd import Simulation, cases
+    case = cases.mach6_box(nic, njc, nkc, seed=1234)
+    sim = Simulation(3, flux=flux)
+    sim.add_block(case)
+    tune = sim.autotune()
+    if warmup:
+        sim.run(warmup)
+    t0 = time.perf_counter()
+    _, st = sim.run(steps)
+    el = time.perf_counter() - t0
+    sim.profile(True)
+    sim.run(min(10, steps))
+    ks = sim.kernel_stats()
+    sim.profile(False)
+    dom = max(("face_flux", "fused_stage"), key=lambda k: ks[k]["ms"])
+    kd = ks[dom]
+    avg_ms = kd["ms"] / max(kd["launches"], 1)
+    fc = 1 if flux == "efm" else 0
+    if dom == "face_flux":
+        bpu, unit_name, kname = face_bytes_per_face(3), "face", "k_face_flux_batch<3,%d,false>" % fc
+    else:
+        bpu, unit_name, kname = stage_bytes_per_cell(3), "cell", "k_fused_batch<3,%d,*>" % fc
+    achieved = bpu * kd["items_per_launch"] / (avg_ms * 1e-3) / 1e9
+    out = {
+        "value": case.ncells * steps / el, "unit": "cell-updates/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * el / steps,
+        "config": {"workload": f"Mach-6 3-D box {nic}x{njc}x{nkc}, ideal air, l2r2+van Albada, {flux}, pc RK2",
+                   "cells": case.ncells, "flux": flux, "stage_kernels": tune["best"], "final_dt": st["dt"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, kd["items_per_launch"]),
+                     "kernel": kname, "avg_launch_ms": avg_ms, "bytes_per_" + unit_name: bpu,
+                     "items_per_launch": kd["items_per_launch"]},
+        "whole_step_hbm_frac": stage_bytes_per_cell(3) * 2 * case.ncells / (1e-3 * el / steps) / 1e9 / HBM_PEAK_GBS,
+        "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1)} for k, v in ks.items()
+                    if v["launches"]},
+    }
+    sim.close()
+    if with_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(cpu_seconds, flux)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"error": str(e)}
+    if log:
+        log(f"box {flux}: {out['value']:.3e} cell-updates/s, face kernel {avg_ms * 1e3:.0f} us "
+            f"({out['roofline']['frac']:.3f} of HBM peak)")
+    return out
+
+
+def bench_sod(steps=500, cpu_seconds=10.0, log=None):
+    """The plain method, small: the 2-D Sod tube of 200x100 cells, AUSMDV, CFL 0.5, 500 steps
+    from the initial split on the GPU; the oracle on one thread (the reference's mode for one
+    block) over a bounded number of steps of the same run."""
+    from gdtk_amd import Simulation, cases
+    from oracle import oracle as O
+    case = cases.sod_2d(200, 100)
+    sim = Simulation(2, flux="ausmdv")
+    sim.add_block(case)
+    t0 = time.perf_counter()
+    _, st = sim.run(steps)
+    el = time.perf_counter() - t0
+    sim.close()
+    o = O.Sim(O.IdealGas(), 2, "ausmdv")
+    o.add_block(case)
+    n, t1 = 0, time.perf_counter()
+    while n < steps and time.perf_counter() - t1 < cpu_seconds:
+        o.run(10)
+        n += 10
+    cel = time.perf_counter() - t1
+    o.close()
+    out = {"value": case.ncells * steps / el, "unit": "cell-updates/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
+           "config": {"workload": "2-D Sod tube 200x100, ideal air, l2r2+van Albada, ausmdv, pc RK2, CFL 0.5, "
+                                  "from the initial split", "cells": case.ncells, "final_time": st["time"],
+                      "final_dt": st["dt"]},
+           "cpu_baseline": {"value": case.ncells * n / cel, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                            "sample": f"oracle, 1 thread, the first {n} of the {steps} steps, {cel:.1f} s"}}
+    if log:
+        log(f"sod: {out['value']:.3e} cell-updates/s on the GPU (latency-bound grid), oracle "
+            f"{out['cpu_baseline']['value']:.3e}")
+    return out
+
+
+def _batched_slab_rate(args):
+    """Worker of the batched CPU baseline: the oracle on one thread over an 8-layer slab of
+    one block of the batched case, for about `seconds`."""
+    nblocks, q, seconds = args
+    sys.path.insert(0, ROOT)
+    from gdtk_amd import cases
+    from oracle import oracle as O
+    b = cases.batched_ragged(nblocks, only=q)[0]
+    slab = cases.k_slab(b, min(8, b.nkc))
+    o = O.Sim(O.IdealGas(), 3, "ausmdv")
+    o.add_block(slab)
+    o.run(1)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds and n < 200:
+        o.run(1)
+        n += 1
+    el = time.perf_counter() - t0
+    o.close()
+    return slab.ncells * n / el
+
+
+def batched_cpu_baseline(nblocks, seconds=10.0):
+    """The oracle over the batched blocks in the reference's mode (one thread per block,
+    threads = min(host cores, blocks)): as many worker processes, each timing an 8-layer slab
+    of its own block; the rates add up."""
+    import multiprocessing as mp
+    nproc = max(1, min(16, os.cpu_count() or 1, nblocks))
+    qs = [int(q) for q in range(0, nblocks, max(1, nblocks // nproc))][:nproc]
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        rates = pool.map(_batched_slab_rate, [(nblocks, q, seconds) for q in qs])
+    return dict(value=float(sum(rates)), unit="cell-updates/s", cores=nproc, kind="port",
+                sample=f"oracle, {nproc} processes of one thread, each an 8-layer k-slab of one block "
+                       f"(blocks {qs[0]}..{qs[-1]} step {max(1, nblocks // nproc)}), about {seconds:.0f} s each")
+
+
+def _leg_summary(r):
+    """Compact figures of a sub-leg record."""
+    out = {}
+    for k in ("value", "unit", "ms_per_step"):
+        if k in r:
+            out[k] = r[k]
+    if "roofline" in r:
+        out["roofline"] = {k: r["roofline"][k] for k in ("kernel", "bound", "frac") if k in r["roofline"]}
+    cb = r.get("cpu_baseline")
+    if isinstance(cb, dict) and "value" in cb:
+        out["cpu_baseline"] = {"value": cb["value"], "cores": cb["cores"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--flux", default="ausmdv", choices=["ausmdv", "efm"])
-    ap.add_argument("--size", default="256,128,128")
+    ap.add_argument("--spinup", type=int, default=1500,
+                    help="untimed steps of the headline cylinder from the free stream (set-up)")
+    ap.add_argument("--size", default="256,128,128", help="cells of the Mach-6 box sub-legs")
+    ap.add_argument("--box-steps", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a device copy of known size (HBM counter calibration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="measure only the headline coupled cylinder")
+    ap.add_argument("--box-only", action="store_true", help="measure only the Mach-6 box (AUSMDV and EFM)")
+    ap.add_argument("--flux", default="ausmdv", choices=["ausmdv", "efm"], help="flux calculator of --box-only")
+    ap.add_argument("--no-sod", action="store_true", help="skip the Sod tube")
     ap.add_argument("--chem-cells", type=int, default=1 << 22,
                     help="cells of the chemistry workload (0 skips the chemistry measurement)")
     ap.add_argument("--chem-steps", type=int, default=5)
     ap.add_argument("--chem-only", action="store_true", help="measure only the chemistry workload")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
     ap.add_argument("--surrogate-train", type=int, default=1 << 22, help="integrator samples to train on")
-    ap.add_argument("--batched", default="4,4,4",
-                    help="blocks per direction of the batched uneven-blocks workload ('0' skips it)")
+    ap.add_argument("--batched", type=int, default=64,
+                    help="blocks of the batched uneven-blocks workload (0 skips it)")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-only", action="store_true", help="measure only the batched uneven-blocks workload")
     ap.add_argument("--coupled-steps", type=int, default=5000,
-                    help="steps of the coupled Mach-10 cylinder runs (0 skips them)")
-    ap.add_argument("--coupled-only", action="store_true", help="measure only the coupled cylinder")
+                    help="steps of the cylinder's integrator and surrogate runs (0 skips them)")
+    ap.add_argument("--coupled-only", action="store_true", help="measure only the cylinder's surrogate comparison")
     ap.add_argument("--adaptive-steps", type=int, default=20,
-                    help="steps of the adaptive EFM/AUSMDV run on the headline box (0 skips it)")
+                    help="steps of the adaptive EFM/AUSMDV run on the Mach-6 box (0 skips it)")
     ap.add_argument("--adaptive-only", action="store_true", help="measure only the adaptive flux calculator")
     ap.add_argument("--lmr-steps", type=int, default=20,
-                    help="steps of the Eilmer 5 (lmr) numerics run on the headline box (0 skips it)")
+                    help="steps of the Eilmer 5 (lmr) numerics run on the Mach-6 box (0 skips it)")
     ap.add_argument("--lmr-only", action="store_true", help="measure only the Eilmer 5 (lmr) numerics")
     ap.add_argument("--no-flow-io", action="store_true", help="skip the flow-file write/read measurement")
     ap.add_argument("--no-n90", action="store_true", help="skip the reference's n90 reacting cylinder case")
@@ -685,116 +1001,94 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
+
     def log(msg):
         print("bench:", msg, file=sys.stderr, flush=True)
 
+    def emit(rec):
+        print(json.dumps(rec, default=float), flush=True)
+
+    with_cpu = not args.no_cpu_baseline
+    nic, njc, nkc = (int(x) for x in args.size.split(","))
+    nblk = args.batched
+
     def chem_legs():
-        res = bench_chem(args.chem_cells, args.chem_steps, 1, not args.no_cpu_baseline)
+        res = bench_chem(args.chem_cells, args.chem_steps, 1, with_cpu)
         if not args.no_surrogate:
             sres = bench_surrogate(args.chem_cells, args.chem_steps, n_train=args.surrogate_train, log=log)
-            sres["speedup_vs_alpha_qss"] = sres["value"] / res["value"]
+            sres["speedup_vs_alpha_qss"] = (sres["value"] / res["value"]) if sres["value"] else None
             res["surrogate"] = sres
         return res
 
     if args.chem_only:
-        print(json.dumps(chem_legs(), default=float), flush=True)
-        return
+        return emit(chem_legs())
     if args.coupled_only:
-        print(json.dumps(bench_coupled(args.coupled_steps, log=log), default=float), flush=True)
-        return
-    nblk = tuple(int(x) for x in args.batched.split(","))
+        return emit(bench_coupled(args.coupled_steps, log=log))
     if args.batched_only:
-        print(json.dumps(bench_batched(nblk, args.batched_steps, 2, args.flux, log), default=float), flush=True)
-        return
-    nic, njc, nkc = (int(x) for x in args.size.split(","))
+        r = bench_batched(nblk, args.batched_steps, 2, "ausmdv", log)
+        if with_cpu:
+            r["cpu_baseline"] = batched_cpu_baseline(nblk)
+        return emit(r)
     if args.n90_only:
-        print(json.dumps(bench_n90(0 if args.no_cpu_baseline else 300, log=log), default=float), flush=True)
-        return
+        return emit(bench_n90(300 if with_cpu else 0, log=log))
     if args.flow_io_only:
-        print(json.dumps(bench_flow_io(nic, njc, nkc, log=log), default=float), flush=True)
-        return
+        return emit(bench_flow_io(nic, njc, nkc, log=log))
     if args.lmr_only:
-        print(json.dumps(bench_lmr(nic, njc, nkc, args.lmr_steps, log=log), default=float), flush=True)
-        return
+        return emit(bench_lmr(nic, njc, nkc, args.lmr_steps, log=log))
     if args.adaptive_only:
-        print(json.dumps(bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log), default=float), flush=True)
-        return
-    from gdtk_amd import Simulation, cases
-    case = cases.mach6_box(nic, njc, nkc, seed=1234)
-    sim = Simulation(3, flux=args.flux)
-    sim.add_block(case)
-    print(f"bench: {nic}x{njc}x{nkc} = {case.ncells} cells, flux={args.flux}, warmup {args.warmup}", file=sys.stderr,
-          flush=True)
-    # The autotuner (part of set-up, not timed) picks the stage kernels for this block.
-    tune = sim.autotune()
-    print("bench: autotune", tune, file=sys.stderr, flush=True)
-    if args.warmup:
-        sim.run(args.warmup)
-    # Timed region: K whole steps; run() synchronises on the device before returning.
-    t0 = time.perf_counter()
-    n, st = sim.run(args.steps)
-    el = time.perf_counter() - t0
-    ms_per_step = 1e3 * el / args.steps
-    value = case.ncells * args.steps / el
-    # Dominant kernel: timed with HIP events around each of its launches on the library's
-    # stream (the stream every kernel of the library runs on).
-    sim.profile(True)
-    sim.run(min(10, args.steps))
-    ks = sim.kernel_stats()
-    sim.profile(False)
-    dom = max(("face_flux", "fused_stage"), key=lambda k: ks[k]["ms"])
-    kd = ks[dom]
-    avg_ms = kd["ms"] / max(kd["launches"], 1)
-    if dom == "face_flux":
-        bpu, unit_name, kname = face_bytes_per_face(3), "face", "k_face_flux_batch<3,%d>" % (1 if args.flux == "efm" else 0)
-    else:
-        bpu, unit_name, kname = stage_bytes_per_cell(3), "cell", "k_fused_batch<3,%d,*>" % (1 if args.flux == "efm" else 0)
-    achieved = bpu * kd["items_per_launch"] / (avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic(kname, kd["items_per_launch"])
-    out = {
-        "metric": "cell-updates/s (flux+chem) FP64",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"Mach-6 3-D box {nic}x{njc}x{nkc}, ideal air, l2r2+van Albada, {args.flux}, pc RK2",
-                   "cells": case.ncells, "flux": args.flux, "stage_kernels": tune["best"],
-                   "kchunk": tune["kchunk"], "tile_x": tune.get("tile_x"), "final_dt": st["dt"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
-                     "avg_launch_ms": avg_ms, "bytes_per_" + unit_name: bpu,
-                     "items_per_launch": kd["items_per_launch"]},
-        "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / max(v["launches"], 1)} for k, v in ks.items()
-                    if v["launches"]},
-    }
-    if not args.no_cpu_baseline:
-        try:
-            out["cpu_baseline"] = cpu_baseline()
-        except Exception as e:  # the baseline is reported, never required
-            out["cpu_baseline"] = {"error": str(e)}
-    sim.close()
-    if nblk != (0,):
-        out["batched"] = bench_batched(nblk, args.batched_steps, 2, args.flux, log)
-    if args.adaptive_steps > 0:
-        out["adaptive"] = bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log)
-    if args.lmr_steps > 0:
-        out["lmr"] = bench_lmr(nic, njc, nkc, args.lmr_steps, log=log)
-    if not args.no_flow_io:
-        out["flow_io"] = bench_flow_io(nic, njc, nkc, log=log)
-    if not args.no_n90:
-        out["n90"] = bench_n90(0 if args.no_cpu_baseline else 300, log=log)
+        return emit(bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log))
+    if args.box_only:
+        return emit(bench_box(nic, njc, nkc, args.flux, args.steps, args.warmup, with_cpu, log=log))
+
+    out = bench_coupled_headline(args.steps, args.warmup, args.spinup, with_cpu, log=log)
+    log(f"headline: {out['value']:.4e} cell-updates/s, {out['ms_per_step']:.3f} ms/step, "
+        f"dominant {out['roofline']['kernel']} at {out['roofline']['frac']:.3f} of peak")
+    if args.headline_only:
+        return emit(out)
+    legs = {}
+    if not args.no_sod:
+        legs["sod_200x100"] = bench_sod(log=log)
+    legs["box_ausmdv"] = bench_box(nic, njc, nkc, "ausmdv", args.box_steps, 2, with_cpu, log=log)
+    legs["box_efm"] = bench_box(nic, njc, nkc, "efm", args.box_steps, 2, with_cpu, cpu_seconds=8.0, log=log)
+    if nblk > 0:
+        r = bench_batched(nblk, args.batched_steps, 2, "ausmdv", log)
+        if with_cpu:
+            try:
+                r["cpu_baseline"] = batched_cpu_baseline(nblk)
+            except Exception as e:  # the baseline is reported, never required
+                r["cpu_baseline"] = {"error": str(e)}
+        legs["batched"] = r
     if args.chem_cells > 0:
-        out["chem"] = chem_legs()
+        legs["chem"] = chem_legs()
     if args.coupled_steps > 0:
-        out["coupled"] = bench_coupled(args.coupled_steps, log=log)
-    print(json.dumps(out, default=float), flush=True)
+        legs["cylinder_surrogate"] = bench_coupled(args.coupled_steps, log=log)
+    if args.adaptive_steps > 0:
+        legs["adaptive"] = bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log)
+    if args.lmr_steps > 0:
+        legs["lmr"] = bench_lmr(nic, njc, nkc, args.lmr_steps, log=log)
+    if not args.no_flow_io:
+        legs["flow_io"] = bench_flow_io(nic, njc, nkc, log=log)
+    if not args.no_n90:
+        legs["n90"] = bench_n90(300 if with_cpu else 0, log=log)
+    out["legs"] = legs
+    # compact per-leg figures last, so that they survive a truncated head of the line
+    summ = {k: _leg_summary(v) for k, v in legs.items()}
+    if "chem" in legs and "surrogate" in legs["chem"]:
+        su = legs["chem"]["surrogate"]
+        summ["chem_surrogate"] = {"value": su["value"], "meets_target": su["meets_target"],
+                                  "speedup_vs_alpha_qss": legs["chem"]["surrogate"]["speedup_vs_alpha_qss"],
+                                  "nearest": {k: su["nearest"][k] for k in ("disagree_tol", "max_abs_dY",
+                                                                              "max_rel_T_err", "value")}}
+    if "cylinder_surrogate" in legs:
+        cyl = legs["cylinder_surrogate"]
+        summ["cylinder_surrogate"] = {"meets_target": cyl["meets_target"],
+                                      "points": [{k: r[k] for k in ("disagree_tol", "chem_speedup_vs_alpha_qss",
+                                                                    "max_abs_dY", "T_stag_rel_diff")}
+                                                 for r in cyl["surrogate"]]}
+    out["summary"] = {"headline": {"value": out["value"], "ms_per_step": out["ms_per_step"],
+                                   "roofline": {k: out["roofline"][k] for k in ("kernel", "bound", "frac")},
+                                   "cpu_baseline": out.get("cpu_baseline", {}).get("value")}, **summ}
+    emit(out)
 
 
 if __name__ == "__main__":

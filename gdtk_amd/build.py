@@ -32,7 +32,21 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def build(verbose: bool = False) -> str:
+def _up_to_date(obj: str, dep: str) -> bool:
+    """True if obj exists and is newer than every file its dependency list names."""
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return False
+    with open(dep) as f:
+        text = f.read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else []
+    t = os.path.getmtime(obj)
+    return bool(files) and all(os.path.exists(x) and os.path.getmtime(x) < t for x in files)
+
+
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every source and link the library.  incremental=True skips a source whose
+    object is newer than every file the compiler's dependency list for it names (a
+    developer convenience; the default compiles everything)."""
     cc = hipcc()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
@@ -41,8 +55,12 @@ def build(verbose: bool = False) -> str:
     procs = []
     for src in srcs:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        dep = obj[:-2] + ".d"
+        if incremental and _up_to_date(obj, dep):
+            objs.append(obj)
+            continue
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [cc] + HIP_FLAGS + lang + ["-c", src, "-o", obj]
+        cmd = [cc] + HIP_FLAGS + lang + ["-MMD", "-MF", dep, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd, stderr=subprocess.PIPE, text=True), src))
@@ -65,4 +83,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

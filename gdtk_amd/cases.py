@@ -301,6 +301,67 @@ def batched_uneven(nblk=(4, 4, 4), lo=16, hi=192, seed=2024, p0=2.0e3, T0=220.0,
     return out
 
 
+def batched_ragged(nblocks=64, lo=16, hi=192, seed=2024, p0=2.0e3, T0=220.0, mach=(2.0, 10.0), only=None) -> list:
+    """Batched uneven blocks with per-block edge lengths: nblocks 3-D structured blocks on
+    unit cells, in pairs joined by a full-face-copy exchange.  Every block draws its own i
+    length uniformly from lo..hi (seeded); the two blocks of a pair share j and k lengths
+    drawn per pair (a full-face copy needs the same face), so no two pairs need have any
+    edge length in common.
+
+    Each block starts from its own ideal-air free stream, p0 and T0 with a Mach number drawn
+    uniformly from `mach`, flowing along +x.  The first block of a pair has a supersonic
+    inflow of its free stream on its west face and exchanges its east face with the west
+    face of the second; every other face is a simple outflow (flux).  Pairs are laid out
+    side by side in y and z (they do not touch).  only=q builds block q alone (same draws).
+    """
+    assert nblocks % 2 == 0
+    rng = np.random.default_rng(seed)
+    npair = nblocks // 2
+    ni = rng.integers(lo, hi + 1, nblocks)
+    nj = rng.integers(lo, hi + 1, npair)
+    nk = rng.integers(lo, hi + 1, npair)
+    machs = rng.uniform(mach[0], mach[1], nblocks)
+    R = R_UNIVERSAL / IDEAL_AIR_MMASS
+    a0 = np.sqrt(IDEAL_AIR_GAMMA * R * T0)
+    side = int(np.ceil(np.sqrt(npair)))
+    out = []
+    for q in range(nblocks) if only is None else (only,):
+        pr, second = q // 2, q % 2
+        nic, njc, nkc = int(ni[q]), int(nj[pr]), int(nk[pr])
+        x0 = float(ni[q - 1]) if second else 0.0
+        v = box_vertices(nic, njc, nkc, float(nic), float(njc), float(nkc), dims=3,
+                         x0=x0, y0=float((pr % side) * (hi + 1)), z0=float((pr // side) * (hi + 1)))
+        n = nic * njc * nkc
+        u = machs[q] * a0
+        vel = np.zeros((n, 3))
+        vel[:, 0] = u
+        b = BlockCase(3, nic, njc, nkc, v, np.full(n, p0), np.full(n, T0), vel, [BC_OUTFLOW_SIMPLE_FLUX] * 6)
+        if second:
+            b.bc[WEST] = BC_EXCHANGE
+            b.other_block[WEST], b.other_face[WEST] = q - 1, EAST
+        else:
+            b.bc[WEST] = BC_INFLOW_SUPERSONIC
+            b.inflow[WEST] = [p0, T0, u, 0.0, 0.0]
+            b.bc[EAST] = BC_EXCHANGE
+            b.other_block[EAST], b.other_face[EAST] = q + 1, WEST
+        out.append(b)
+    return out
+
+
+def k_slab(case: BlockCase, nk: int) -> BlockCase:
+    """The first nk cell layers of a 3-D block as a block of its own: the block's boundary
+    conditions, with its top face taking the block's top condition and every exchange side
+    turned into a simple outflow (flux)."""
+    sub = split_block(case, (0, case.nic), (0, case.njc), (0, nk, case.nkc) if nk < case.nkc else (0, case.nkc))[0]
+    sub.bc[TOP] = case.bc[TOP]
+    sub.inflow[TOP] = case.inflow[TOP]
+    for f in range(6):
+        if sub.bc[f] == BC_EXCHANGE:
+            sub.bc[f] = BC_OUTFLOW_SIMPLE_FLUX
+            sub.other_block[f], sub.other_face[f] = -1, -1
+    return sub
+
+
 # ---------------------------------------------------------------------------------------
 # Multi-species thermally perfect gas cases
 

@@ -135,7 +135,8 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
     cd.T_lower = m.T_lower;
     cd.T_upper = m.T_upper;
     if (m.method != GDTK_ODE_ALPHA_QSS && m.method != GDTK_ODE_RKF) throw Error(GDTK_ERR_ARG, "bad ODE method");
-    if (m.tightTempCoupling) throw Error(GDTK_ERR_UNSUPPORTED, "tightTempCoupling is not implemented");
+    if (m.tightTempCoupling && m.method != GDTK_ODE_ALPHA_QSS)
+        throw Error(GDTK_ERR_UNSUPPORTED, "tightTempCoupling is implemented for the alpha-QSS step only");
     if (m.maxIters < 1 || m.maxSubcycles < 1 || m.maxAttempts < 1) throw Error(GDTK_ERR_ARG, "bad step control");
     cd.method = m.method;
     cd.eps1 = m.eps1;
@@ -146,6 +147,7 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
     cd.maxSubcycles = m.maxSubcycles;
     cd.maxAttempts = m.maxAttempts;
     cd.T_frozen = m.T_frozen;
+    cd.tight = m.tightTempCoupling ? 1 : 0;
 }
 
 template <class T>
@@ -267,8 +269,8 @@ int gdtk_hip_chem_substeps(int chem, int n, int* substeps) {
         HIPCHK(hipEventSynchronize(c.ev[3]));
         // Cells that were not integrated (frozen, invalid) report 0 substeps.
         std::vector<int> f((size_t)n);
-        HIPCHK(hipMemcpy(f.data(), c.flag.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(substeps, c.nsub.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(f.data(), c.last_w.flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(substeps, c.last_w.nsub, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i)
             if (f[i] != CF_INTEGRATE) substeps[i] = 0;
         return GDTK_OK;

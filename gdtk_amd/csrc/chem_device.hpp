@@ -61,6 +61,7 @@ struct ChemDev {
     double errTol;
     int maxSubcycles, maxAttempts;
     double T_frozen;
+    int tight;  // tightTempCoupling: T(u) and the rate constants after every subcycle
 };
 
 constexpr double CH_R_UNIVERSAL = 8.31451;               // gas/physical_constants.d

@@ -21,15 +21,16 @@ struct Chem {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     ChemTuning tuning;
-    DevArr<double> kfkb, conc0, h0, hest, conc;
+    DevArr<double> kfkb, conc0, h0, hest, conc, Tend, uend;
     DevArr<int> flag, st, nsub;
     DevArr<unsigned> queue;
     DevArr<unsigned long long> stats;
     // Staging of the host-array entry points.
     DevArr<double> s_rho, s_T, s_u, s_p, s_massf, s_dtc, s_a;
     DevArr<int> s_status;
-    // Last call: cells, kernel milliseconds (setup, integrate, finish), event stream.
+    // Last call: cells, its work buffers (the handle's own or a caller's), event stream.
     int last_n = 0;
+    ChemWork last_w{};
     hipStream_t last_stream = nullptr;
     bool have_events = false;
 
@@ -52,26 +53,68 @@ struct Chem {
         nsub.reserve(N);
         queue.reserve(8 * 32);
         stats.reserve(4);
-        return ChemWork{kfkb.p, conc0.p, h0.p, hest.p, conc.p, flag.p, st.p, nsub.p, queue.p, stats.p};
+        Tend.reserve(host.tight ? N : 1);
+        uend.reserve(host.tight ? N : 1);
+        return ChemWork{kfkb.p, conc0.p, h0.p, hest.p, conc.p, flag.p, st.p, nsub.p, queue.p, stats.p, Tend.p, uend.p};
     }
 
-    // The three kernels on device arrays; outputs may alias inputs.
+    // Bytes of the work buffers of one call over n cells when they are laid out in one
+    // region (carve_work), each buffer starting on a 256-byte boundary.
+    size_t work_bytes(int n) const {
+        const size_t N = (size_t)std::max(n, 1);
+        auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        return up(4 * sizeof(unsigned long long)) + up(8 * 32 * sizeof(unsigned)) + 3 * up(N * sizeof(int)) +
+               (host.tight ? 4 : 2) * up(N * sizeof(double)) + 2 * up((size_t)nsp * N * sizeof(double)) +
+               up(2 * (size_t)std::max(nr, 1) * N * sizeof(double));
+    }
+
+    // The work buffers of a call over n cells carved out of one device region of at least
+    // work_bytes(n) bytes (256-byte aligned).
+    ChemWork carve_work(int n, void* region) const {
+        const size_t N = (size_t)std::max(n, 1);
+        auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        char* q = static_cast<char*>(region);
+        auto take = [&](size_t b) {
+            char* r = q;
+            q += up(b);
+            return r;
+        };
+        ChemWork w;
+        w.stats = reinterpret_cast<unsigned long long*>(take(4 * sizeof(unsigned long long)));
+        w.queue = reinterpret_cast<unsigned*>(take(8 * 32 * sizeof(unsigned)));
+        w.flag = reinterpret_cast<int*>(take(N * sizeof(int)));
+        w.st = reinterpret_cast<int*>(take(N * sizeof(int)));
+        w.nsub = reinterpret_cast<int*>(take(N * sizeof(int)));
+        w.h0 = reinterpret_cast<double*>(take(N * sizeof(double)));
+        w.hest = reinterpret_cast<double*>(take(N * sizeof(double)));
+        w.conc0 = reinterpret_cast<double*>(take((size_t)nsp * N * sizeof(double)));
+        w.conc = reinterpret_cast<double*>(take((size_t)nsp * N * sizeof(double)));
+        w.kfkb = reinterpret_cast<double*>(take(2 * (size_t)std::max(nr, 1) * N * sizeof(double)));
+        w.Tend = host.tight ? reinterpret_cast<double*>(take(N * sizeof(double))) : nullptr;
+        w.uend = host.tight ? reinterpret_cast<double*>(take(N * sizeof(double))) : nullptr;
+        return w;
+    }
+
+    // The three kernels on device arrays; outputs may alias inputs.  The work buffers are the
+    // handle's own (grown as needed) unless the caller brings them (wk).
     void step_device(int n, double dt, const double* rho, const double* T, const double* u, const double* p,
                      const double* massf, const double* dtc, double* T_out, double* u_out, double* p_out,
-                     double* massf_out, double* dtc_out, int* status, hipStream_t s) {
+                     double* massf_out, double* dtc_out, int* status, hipStream_t s, const ChemWork* wk = nullptr) {
         if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
         if (!(dt > 0.0)) throw Error(GDTK_ERR_ARG, "dt must be positive");
-        ChemWork w = work(n);
+        ChemWork w = wk ? *wk : work(n);
         HIPCHK(hipEventRecord(ev[0], s));
         HIPCHK(launch_chem_clear(w, s));
         HIPCHK(launch_chem_setup(dev, nsp, nr, n, dt, rho, T, massf, dtc, w, s));
         HIPCHK(hipEventRecord(ev[1], s));
-        HIPCHK(launch_chem_integrate(dev, nsp, nr, host.method, n, dt, rho, dtc, dtc_out, w, tuning, num_cu, s));
+        HIPCHK(launch_chem_integrate(dev, nsp, nr, host.method, host.tight, n, dt, rho, T, u, dtc, dtc_out, w, tuning,
+                                     num_cu, s));
         HIPCHK(hipEventRecord(ev[2], s));
         HIPCHK(launch_chem_finish(dev, nsp, n, rho, T, u, p, massf, dtc, T_out, u_out, p_out, massf_out, dtc_out,
                                   status, w, s));
         HIPCHK(hipEventRecord(ev[3], s));
         last_n = n;
+        last_w = w;
         last_stream = s;
         have_events = true;
     }
@@ -82,7 +125,7 @@ struct Chem {
         if (!have_events) throw Error(GDTK_ERR_ARG, "no chemistry step has run");
         HIPCHK(hipEventSynchronize(ev[3]));
         unsigned long long h[4] = {0, 0, 0, 0};
-        HIPCHK(hipMemcpy(h, stats.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h, last_w.stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         float ms[3] = {0, 0, 0};
         for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
         const double v[8] = {(double)h[0], (double)h[1], (double)h[2], (double)h[3],

@@ -46,13 +46,13 @@ hipError_t launch_chem_setup(const ChemDev* cd, int nsp, int nr, int n, double d
     return run_nsp(nsp, c);
 }
 
-hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int method, int n, double dt, const double* rho,
-                                 const double* dtc_in, double* dtc_out, const ChemWork& w, const ChemTuning& tu,
-                                 int num_cu, hipStream_t s) {
+hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int method, int tight, int n, double dt,
+                                 const double* rho, const double* T, const double* u, const double* dtc_in,
+                                 double* dtc_out, const ChemWork& w, const ChemTuning& tu, int num_cu, hipStream_t s) {
     ChemCall c{};
     c.phase = CHEM_PHASE_INTEGRATE;
-    c.cd = cd; c.nr = nr; c.method = method; c.n = n; c.dt = dt;
-    c.rho = rho; c.dtc_in = dtc_in; c.dtc_out = dtc_out;
+    c.cd = cd; c.nr = nr; c.method = method; c.tight = tight; c.n = n; c.dt = dt;
+    c.rho = rho; c.T = T; c.u = u; c.dtc_in = dtc_in; c.dtc_out = dtc_out;
     c.w = w; c.tu = tu; c.num_cu = num_cu; c.s = s;
     return run_nsp(nsp, c);
 }

@@ -75,22 +75,28 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_setup(const ChemDev* __rest
     w.flag[i] = CF_INTEGRATE;
 }
 
-// Per-lane state of one cell inside the subcycle controller.
+// Per-lane state of one cell inside the subcycle controller; T and u are the gas state's
+// temperature and internal energy, which only the tight temperature coupling changes.
 template <int NSP, int NRM>
 struct CellRun {
     int cell = -1;
     int pass, cycle, attempt, nsub, iters;
-    double rho, t, h, dtSave, dtSug;
+    double rho, t, h, dtSave, dtSug, T, u;
     double kf[NRM], kb[NRM], c0[NSP];
 };
 
-template <int NSP, int NRM>
+template <int NSP, int NRM, bool TIGHT>
 __device__ __forceinline__ void load_cell(const ChemDev& cd, int n, int i, const double* __restrict__ rho,
+                                          const double* __restrict__ T, const double* __restrict__ u,
                                           const double* __restrict__ dtc_in, const ChemWork& w,
                                           CellRun<NSP, NRM>& c) {
     const int nr = cd.nr;
     c.cell = i;
     c.rho = rho[i];
+    if (TIGHT) {
+        c.T = T[i];
+        c.u = u[i];
+    }
 #pragma unroll
     for (int ir = 0; ir < NRM; ++ir) {
         if (ir >= nr) continue;
@@ -110,9 +116,17 @@ __device__ __forceinline__ void load_cell(const ChemDev& cd, int n, int i, const
     c.iters = 0;
 }
 
-template <int NSP, int NRM, int METHOD>
+// TIGHT: tightTempCoupling (chemistry_update.d:265-269): after every successful subcycle the
+// mass fractions of the new concentrations, T from u by Newton (started at the gas state's
+// T, which keeps the result of the previous update), and the rate constants at that T.  A
+// failed T(u) iteration there raises GasModelException in the reference, which the
+// thermochemical_increment of fvcell.d:894-905 does not catch: the cell fails without the
+// retry.
+template <int NSP, int NRM, int METHOD, bool TIGHT>
 __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __restrict__ cdp, int n, double dt,
                                                                const double* __restrict__ rho,
+                                                               const double* __restrict__ T_in,
+                                                               const double* __restrict__ u_in,
                                                                const double* __restrict__ dtc_in,
                                                                double* __restrict__ dtc_out, ChemWork w,
                                                                int persistent, int chunk, int refill_period) {
@@ -131,7 +145,7 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
         exhausted = false;
     } else {
         const int g = blockIdx.x * CHEM_BLOCK + threadIdx.x;
-        if (g < n && w.flag[g] == CF_INTEGRATE) load_cell<NSP, NRM>(cd, n, g, rho, dtc_in, w, c);
+        if (g < n && w.flag[g] == CF_INTEGRATE) load_cell<NSP, NRM, TIGHT>(cd, n, g, rho, T_in, u_in, dtc_in, w, c);
     }
 
     for (int trip = 0;; ++trip) {
@@ -170,7 +184,8 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
                     const int rank = __popcll(need & (lane_bit - 1ull));
                     if ((need & lane_bit) && rank < take) {
                         const int i = w_next + rank;
-                        if (w.flag[i] == CF_INTEGRATE) load_cell<NSP, NRM>(cd, n, i, rho, dtc_in, w, c);
+                        if (w.flag[i] == CF_INTEGRATE)
+                            load_cell<NSP, NRM, TIGHT>(cd, n, i, rho, T_in, u_in, dtc_in, w, c);
                     }
                     w_next += take;
                     need = __ballot(c.cell < 0);
@@ -207,7 +222,18 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
             const double hMax = c.h * (1.0 + DT_INCREASE_PERCENT / 100.0);
             if (c.dtSug > c.h) c.h = (hMax < c.dtSug) ? hMax : c.dtSug;
             ++c.nsub;
-            if (c.t >= dt) {
+            bool fatal = false;
+            if (TIGHT) {
+                double mf[NSP];
+                (void)conc_to_massf<NSP>(cd, c.c0, c.rho, mf);
+                if (tpg_update_temperature<NSP>(cd, mf, c.T, c.u)) eval_rate_constants<NSP, NRM>(cd, c.T, c.kf, c.kb);
+                else fatal = true;
+            }
+            if (fatal) {
+                c.pass = 1;  // no retry
+                c.dtSug = -1.0;
+                outcome = -1;
+            } else if (c.t >= dt) {
                 if (c.cycle == 0) c.dtSave = c.dtSug;
                 outcome = 1;
             } else {
@@ -229,6 +255,18 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
             const int i = c.cell;
 #pragma unroll
             for (int s = 0; s < NSP; ++s) c.c0[s] = w.conc0[(size_t)s * n + i];
+            if (TIGHT) {
+                // the gas state and rate constants of the first attempt's start
+                c.T = T_in[i];
+                c.u = u_in[i];
+                const int nr = cd.nr;
+#pragma unroll
+                for (int ir = 0; ir < NRM; ++ir) {
+                    if (ir >= nr) continue;
+                    c.kf[ir] = w.kfkb[(size_t)ir * n + i];
+                    c.kb[ir] = w.kfkb[(size_t)(nr + ir) * n + i];
+                }
+            }
             c.pass = 1;
             c.dtSug = -1.0;
             c.h = w.hest[i];
@@ -245,6 +283,10 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
                 for (int s = 0; s < NSP; ++s) w.conc[(size_t)s * n + i] = c.c0[s];
                 dtc_out[i] = c.dtSave;
                 w.st[i] = 0;
+                if (TIGHT) {
+                    w.Tend[i] = c.T;
+                    w.uend[i] = c.u;
+                }
             } else {
                 dtc_out[i] = c.dtSug;
                 w.st[i] = -1;
@@ -305,8 +347,13 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_finish(const ChemDev* __res
         const double massfTotal = conc_to_massf<NSP>(cd, conc, r, mfn);
 #pragma unroll
         for (int s = 0; s < NSP; ++s) mfn[s] /= massfTotal;
-        // update_thermo_from_rhou (therm_perf_gas.d:120-135): T from u by Newton, then p
+        // update_thermo_from_rhou (therm_perf_gas.d:120-135): T from u by Newton, then p.  With
+        // the tight coupling the gas state's T and u are those of the last subcycle's update.
         double Tn = Ti, un = ui;
+        if (cd.tight) {
+            Tn = w.Tend[i];
+            un = w.uend[i];
+        }
         if (tpg_update_temperature<NSP>(cd, mfn, Tn, un)) {
             const double Rm = tpg_Rmix<NSP>(cd, mfn);
             pi = r * Rm * Tn;
@@ -377,16 +424,17 @@ struct SetupL {
 };
 
 struct IntegrateL {
-    const ChemDev* cd; int n; double dt; const double* rho; const double* dtc_in; double* dtc_out; ChemWork w;
-    ChemTuning tu; int num_cu; int method; hipStream_t s;
-    template <int NSP, int NRM, int METHOD>
+    const ChemDev* cd; int n; double dt; const double* rho; const double *T, *u; const double* dtc_in;
+    double* dtc_out; ChemWork w; ChemTuning tu; int num_cu; int method; int tight; hipStream_t s;
+    template <int NSP, int NRM, int METHOD, bool TIGHT>
     hipError_t launch() {
         unsigned grid = grid_for(n);
         if (tu.persistent) {
             int per_cu = tu.blocks_per_cu;
             if (per_cu <= 0) {
                 hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void*>(&k_chem_integrate<NSP, NRM, METHOD>), CHEM_BLOCK, 0);
+                    &per_cu, reinterpret_cast<const void*>(&k_chem_integrate<NSP, NRM, METHOD, TIGHT>), CHEM_BLOCK,
+                    0);
                 if (e != hipSuccess) return e;
                 if (per_cu < 1) per_cu = 1;
             }
@@ -395,13 +443,14 @@ struct IntegrateL {
             if (grid < 8) grid = 8;  // one workgroup per XCD range at least
         }
         const int chunk = tu.chunk > 0 ? tu.chunk : 64;
-        hipLaunchKernelGGL((k_chem_integrate<NSP, NRM, METHOD>), dim3(grid), dim3(CHEM_BLOCK), 0, s, cd, n, dt, rho,
-                           dtc_in, dtc_out, w, tu.persistent, chunk, tu.refill_period);
+        hipLaunchKernelGGL((k_chem_integrate<NSP, NRM, METHOD, TIGHT>), dim3(grid), dim3(CHEM_BLOCK), 0, s, cd, n, dt,
+                           rho, T, u, dtc_in, dtc_out, w, tu.persistent, chunk, tu.refill_period);
         return hipGetLastError();
     }
     template <int NSP, int NRM>
     hipError_t run() {
-        return method == 1 ? launch<NSP, NRM, 1>() : launch<NSP, NRM, 0>();
+        if (method == 1) return launch<NSP, NRM, 1, false>();  // RKF with tight coupling is refused at set-up
+        return tight ? launch<NSP, NRM, 0, true>() : launch<NSP, NRM, 0, false>();
     }
 };
 
@@ -438,7 +487,7 @@ hipError_t chem_run(const ChemCall& c) {
         return l.template run<NSP, 32>();
     }
     case CHEM_PHASE_INTEGRATE: {
-        IntegrateL l{c.cd, c.n, c.dt, c.rho, c.dtc_in, c.dtc_out, c.w, c.tu, c.num_cu, c.method, c.s};
+        IntegrateL l{c.cd, c.n, c.dt, c.rho, c.T, c.u, c.dtc_in, c.dtc_out, c.w, c.tu, c.num_cu, c.method, c.tight, c.s};
         if (NSP == 5 && c.nr == 5) return l.template run<NSP, 5>();
         if (c.nr <= 8) return l.template run<NSP, 8>();
         return l.template run<NSP, 32>();

@@ -24,6 +24,8 @@ struct ChemWork {
     int* nsub;          // [n]        substeps per cell
     unsigned* queue;    // [8*32]     per-XCD next-cell counters, 128 B apart
     unsigned long long* stats;  // [4] total substeps, corrector iterations, max substeps, step attempts
+    double* Tend;       // [n]        tight temperature coupling: T and u as the last subcycle's
+    double* uend;       // [n]        update left them (the start of the final T(u) update)
 };
 
 // Launch shape of the integration kernel.
@@ -40,7 +42,7 @@ enum ChemPhase : int { CHEM_PHASE_SETUP = 0, CHEM_PHASE_INTEGRATE = 1, CHEM_PHAS
 struct ChemCall {
     int phase;
     const ChemDev* cd;
-    int nr, n, mode, method;
+    int nr, n, mode, method, tight;
     double dt;
     const double *rho, *T, *u, *p, *massf, *dtc_in;
     double *T_out, *u_out, *p_out, *massf_out, *dtc_out;
@@ -70,9 +72,9 @@ hipError_t launch_chem_clear(const ChemWork& w, hipStream_t s);
 hipError_t launch_chem_setup(const ChemDev* cd, int nsp, int nr, int n, double dt, const double* rho,
                              const double* T, const double* massf, const double* dtc, const ChemWork& w,
                              hipStream_t s);
-hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int method, int n, double dt, const double* rho,
-                                 const double* dtc_in, double* dtc_out, const ChemWork& w, const ChemTuning& tu,
-                                 int num_cu, hipStream_t s);
+hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int method, int tight, int n, double dt,
+                                 const double* rho, const double* T, const double* u, const double* dtc_in,
+                                 double* dtc_out, const ChemWork& w, const ChemTuning& tu, int num_cu, hipStream_t s);
 hipError_t launch_chem_finish(const ChemDev* cd, int nsp, int n, const double* rho, const double* T,
                               const double* u, const double* p, const double* massf, const double* dtc_in,
                               double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,

@@ -1,15 +1,17 @@
 // flux_kernels.hip -- HIP kernels of the structured-block explicit update (gfx950).
 //
-// Face-per-thread path: ghost fill, per-direction face flux into face arrays, then a
-// cell kernel that integrates the face fluxes, applies the predictor-corrector stage
-// update and decodes the new flow state.  The fused LDS-tiled stage kernel lives in
-// fused_stage.hip and shares the device physics in gdtk_device.hpp.
+// Face-per-thread path: ghost fill, the face flux of every direction into face arrays
+// (flux_face.hpp, instantiated in flux_face_*.hip), then a cell kernel that integrates the
+// face fluxes, applies the predictor-corrector stage update and decodes the new flow state.
+// The fused LDS-tiled stage kernel lives in fused_stage.hip and shares the device physics in
+// gdtk_device.hpp.
 #include <hip/hip_runtime.h>
 
 #include "block.hpp"
 #include "gdtk_device.hpp"
 #include "kernel_common.hpp"
 #include "launchers.hpp"
+#include "flux_face.hpp"
 
 namespace gdtk {
 
@@ -212,74 +214,6 @@ __global__ __launch_bounds__(256) void k_shock_cells_batch(const BlockDev* __res
                  b.fshock[1][face_index(b, 1, i, j, k)] | b.fshock[1][face_index(b, 1, i, j + 1, k)];
     if (DIM == 3) s |= b.fshock[2][face_index(b, 2, i, j, k)] | b.fshock[2][face_index(b, 2, i, j, k + 1)];
     b.shock[pidx(b, i, j, k)] = s ? 1 : 0;
-}
-
-// One thread per face of index direction d: reconstruction + flux calculator, or the
-// simple-outflow boundary flux effect on faces whose BC computes the flux itself.
-template <int DIM, int FC, bool LMR>
-__device__ __forceinline__ void face_flux_one(const BlockDev& b, const IdealGasDev& gas, int d, double eps,
-                                              int64_t f) {
-    using C = CQ<DIM>;
-    int fi = (int)(f % b.nf[d][0]);
-    int64_t rest = f / b.nf[d][0];
-    int fj = (int)(rest % b.nf[d][1]);
-    int fk = (int)(rest / b.nf[d][1]);
-    const double* fs = b.fs_in;
-    const int64_t np = b.npad;
-    int pos = (d == 0) ? fi : (d == 1) ? fj : fk;
-    int nd = (d == 0) ? b.nic : (d == 1) ? b.njc : b.nkc;
-    auto at = [&](int s) -> int64_t {
-        if (d == 0) return pidx(b, s, fj, fk);
-        if (d == 1) return pidx(b, fi, s, fk);
-        return pidx(b, fi, fj, s);
-    };
-    FaceGeom g = load_face(b.fgeom[d], b.nface[d], f);
-    double F[C::n];
-    int side = -1;
-    if (pos == 0) side = 2 * d;
-    else if (pos == nd) side = 2 * d + 1;
-    double* out = b.flux[d];
-    const int64_t nfc = b.nface[d];
-    if (side >= 0 && b.bc_kind[side] == BC_OUTFLOW_SIMPLE_FLUX) {
-        int outsign = (side % 2 == 0) ? -1 : 1;
-        int64_t c = at(side % 2 == 0 ? 0 : nd - 1);
-        Vec3d v{fs[c + FS_VX * np], fs[c + FS_VY * np], fs[c + FS_VZ * np]};
-        outflow_flux<DIM>(g, outsign, fs[c + FS_RHO * np], fs[c + FS_U * np], fs[c + FS_P * np], v, F);
-    } else {
-        int64_t cL1 = at(pos - 2), cL0 = at(pos - 1), cR0 = at(pos), cR1 = at(pos + 1);
-        const double* len = b.len + d * np;
-        StencilCell L1 = load_sc(fs, np, cL1), L0 = load_sc(fs, np, cL0), R0 = load_sc(fs, np, cR0),
-                    R1 = load_sc(fs, np, cR1);
-        FallbackFS fb{fs, np, cL0, cR0};
-        // Adaptive calculator: EFM where the face is shocked (strict rule), else AUSMDV.
-        const bool shocked = flux_is_adaptive(FC) && ((b.shock[cL0] | b.shock[cR0]) != 0);
-        face_flux_l2r2<DIM, FC, LMR>(gas, g, L1, L0, R0, R1, fs[cL0 + FS_A * np], fs[cR0 + FS_A * np], fb,
-                                     len[cL1], len[cL0], len[cR0], len[cR1], eps, F, shocked);
-    }
-    for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
-}
-
-// Faces of every direction of every block in one launch.  Each block's faces of a direction
-// start at a multiple of the workgroup size (FaceBatch), so every workgroup serves one
-// direction of one block and reads that block's record with scalar loads.  LMR selects
-// Eilmer 5's limiter smoothing and AUSMDV sound speed.
-template <int DIM, int FC, bool LMR>
-__global__ __launch_bounds__(256, 4) void k_face_flux_batch(const BlockDev* __restrict__ tab, FaceBatch fb,
-                                                         int nblocks, double eps, IdealGasDev gas) {
-    int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
-    const int d = __builtin_amdgcn_readfirstlane(face_batch_dir(fb, w0));
-    const int64_t* __restrict__ start = fb.start[d];
-    int lo = 0, hi = nblocks - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= w0) lo = mid;
-        else hi = mid - 1;
-    }
-    const int q = __builtin_amdgcn_readfirstlane(lo);
-    const BlockDev& b = tab[q];
-    const int64_t f = w0 - start[q] + threadIdx.x;
-    if (f >= b.nface[d]) return;
-    face_flux_one<DIM, FC, LMR>(b, gas, d, eps, f);
 }
 
 // Per interior cell: time derivatives (fvcell.d:824-854), the pc stage update
@@ -487,41 +421,12 @@ hipError_t launch_face_flux_batch(const BlockDev* tab, const FaceBatch& fb, int 
                                  int numerics, double eps, const IdealGasDev& gas, hipStream_t s) {
     const int64_t total = fb.total[0] + fb.total[1] + fb.total[2];
     if (total <= 0) return hipSuccess;
-    dim3 grid(nblk(total, 256)), blk(256);
-#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F, true>), grid, blk, 0, s, tab, fb, nblocks, eps, gas)
-    if (numerics == 1) {
-        if (dims == 3) {
-            if (flux_calc == 4) GDTK_FACE(3, 4);
-            else if (flux_calc == 3) GDTK_FACE(3, 3);
-            else if (flux_calc == 2) GDTK_FACE(3, 2);
-            else if (flux_calc == 1) GDTK_FACE(3, 1);
-            else GDTK_FACE(3, 0);
-        } else {
-            if (flux_calc == 4) GDTK_FACE(2, 4);
-            else if (flux_calc == 3) GDTK_FACE(2, 3);
-            else if (flux_calc == 2) GDTK_FACE(2, 2);
-            else if (flux_calc == 1) GDTK_FACE(2, 1);
-            else GDTK_FACE(2, 0);
-        }
-        return hipGetLastError();
-    }
-#undef GDTK_FACE
-#define GDTK_FACE(D, F) hipLaunchKernelGGL((k_face_flux_batch<D, F, false>), grid, blk, 0, s, tab, fb, nblocks, eps, gas)
-    if (dims == 3) {
-        if (flux_calc == 4) GDTK_FACE(3, 4);
-        else if (flux_calc == 3) GDTK_FACE(3, 3);
-        else if (flux_calc == 2) GDTK_FACE(3, 2);
-        else if (flux_calc == 1) GDTK_FACE(3, 1);
-        else GDTK_FACE(3, 0);
-    } else {
-        if (flux_calc == 4) GDTK_FACE(2, 4);
-        else if (flux_calc == 3) GDTK_FACE(2, 3);
-        else if (flux_calc == 2) GDTK_FACE(2, 2);
-        else if (flux_calc == 1) GDTK_FACE(2, 1);
-        else GDTK_FACE(2, 0);
-    }
-#undef GDTK_FACE
-    return hipGetLastError();
+    const bool lmr = numerics == 1;
+    if (dims == 3)
+        return lmr ? launch_face_flux_3d_lmr(tab, fb, nblocks, flux_calc, eps, gas, total, s)
+                   : launch_face_flux_3d_e4(tab, fb, nblocks, flux_calc, eps, gas, total, s);
+    return lmr ? launch_face_flux_2d_lmr(tab, fb, nblocks, flux_calc, eps, gas, total, s)
+               : launch_face_flux_2d_e4(tab, fb, nblocks, flux_calc, eps, gas, total, s);
 }
 
 hipError_t launch_shock_detect(const BlockDev* tab, const FaceBatch& fb, const int64_t* cstart, int nblocks,

@@ -200,4 +200,21 @@ int gdtk_hip_surrogate_stats(int h, double* stats, int nstats) {
     });
 }
 
+int gdtk_hip_surrogate_project(int nsp, int n_elem, const double* elem, int n, const double* Y0, const double* d0,
+                               double* d, int* ok, void* stream) {
+    return guarded([&] {
+        if (nsp < 1 || nsp > CH_MAX_SP) throw Error(GDTK_ERR_ARG, "nsp must be 1..8");
+        if (n_elem < 1 || n_elem > SURR_MAX_ELEM) throw Error(GDTK_ERR_ARG, "n_elem must be 1..4");
+        if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
+        if (!elem || (n > 0 && (!Y0 || !d0 || !d || !ok))) throw Error(GDTK_ERR_ARG, "null array");
+        SurrCfg cfg{};
+        cfg.n_elem = n_elem;
+        for (int e = 0; e < n_elem; ++e)
+            for (int k = 0; k < nsp; ++k) cfg.elem[e][k] = elem[e * nsp + k];
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        HIPCHK(launch_surr_project(nsp, n, Y0, d0, cfg, d, ok, s));
+        return GDTK_OK;
+    });
+}
+
 }  // extern "C"

@@ -225,6 +225,26 @@ __global__ __launch_bounds__(256) void k_surr_finish(int n, const double* __rest
     status[i] = 0;
 }
 
+// The projection alone, one thread per cell, species-major arrays: d = the projected change
+// of d0 at Y0 and ok = 1, or ok = 0 and d = 0 where the projection is infeasible.
+template <int NSP>
+__global__ __launch_bounds__(256) void k_surr_project(int n, const double* __restrict__ Y0,
+                                                      const double* __restrict__ d0, SurrCfg cfg,
+                                                      double* __restrict__ d, int* __restrict__ ok) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double y[NSP], dd[NSP], r[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        y[s] = Y0[(size_t)s * n + i];
+        dd[s] = d0[(size_t)s * n + i];
+    }
+    const bool good = project<NSP>(cfg, y, dd, r);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) d[(size_t)s * n + i] = good ? r[s] : 0.0;
+    ok[i] = good ? 1 : 0;
+}
+
 // Gather / scatter of the queued cells: compact arrays of m cells <-> the full arrays.
 template <int NSP>
 __global__ void k_surr_gather(int m, int n, const int* __restrict__ idx, const double* __restrict__ rho,
@@ -339,6 +359,16 @@ hipError_t launch_surr_scatter(int nsp, int m, int n, const int* idx, const doub
 #define F(N)                                                                                                         \
     hipLaunchKernelGGL((k_surr_scatter<N>), dim3(g256(m)), dim3(256), 0, s, m, n, idx, cT, cu, cp, cmassf, cdtc, cst, \
                        T_out, u_out, p_out, massf_out, dtc_out, status);                                             \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_project(int nsp, int n, const double* Y0, const double* d0, const SurrCfg& cfg, double* d,
+                               int* ok, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define F(N)                                                                                                   \
+    hipLaunchKernelGGL((k_surr_project<N>), dim3(g256(n)), dim3(256), 0, s, n, Y0, d0, cfg, d, ok);           \
     return hipGetLastError();
     GDTK_SURR_NSP(F)
 #undef F

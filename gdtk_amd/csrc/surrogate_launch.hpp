@@ -35,5 +35,7 @@ hipError_t launch_surr_scatter(int nsp, int m, int n, const int* idx, const doub
                                const double* cp, const double* cmassf, const double* cdtc, const int* cst,
                                double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
                                int* status, hipStream_t s);
+hipError_t launch_surr_project(int nsp, int n, const double* Y0, const double* d0, const SurrCfg& cfg, double* d,
+                               int* ok, hipStream_t s);
 
 }  // namespace gdtk

@@ -59,6 +59,7 @@ def _lib():
         L.gdtk_hip_surrogate_step_device.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.c_double] + \
             [C.c_void_p] * 8
         L.gdtk_hip_surrogate_stats.argtypes = [C.c_int, _dp, C.c_int]
+        L.gdtk_hip_surrogate_project.argtypes = [C.c_int, C.c_int, _dp, C.c_int] + [C.c_void_p] * 5
         _bound = True
     return L
 
@@ -179,6 +180,14 @@ class Surrogate:
         _check(_lib().gdtk_hip_surrogate_stats(self.h, v.ctypes.data_as(_dp), 6))
         return dict(cells=int(v[0]), fallback=int(v[1]), ms_network=v[2], ms_project=v[3], ms_integrator=v[4],
                     max_disagreement=v[5])
+
+
+def project_device(E, n, Y0, d0, d, ok, stream=None):
+    """The element-conservation projection alone (gdtk_hip_surrogate_project): E [n_elem][nsp]
+    on the host; Y0, d0, d (float64 [nsp][n]) and ok (int32 [n]) are device pointers."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    _check(_lib().gdtk_hip_surrogate_project(E.shape[1], E.shape[0], E.ctypes.data_as(_dp), n, Y0, d0, d, ok,
+                                             stream))
 
 
 def integrator_samples(chem, cells, dt, device):

@@ -322,6 +322,8 @@ typedef struct gdtk_mechanism {
     double eps1, eps2, delta;                   /* alpha-QSS */
     int maxIters;
     double errTol;                              /* RKF */
+    /* tightTempCoupling (chemistry_update.d:265-269): T(u) and the rate constants after every
+     * subcycle; with the alpha-QSS step only (GDTK_ERR_UNSUPPORTED with RKF) */
     int tightTempCoupling, maxSubcycles, maxAttempts;
     double T_frozen;                            /* globalconfig T_frozen (300 K) */
 } gdtk_mechanism;
@@ -454,6 +456,13 @@ int gdtk_hip_surrogate_step_device(int surrogate, int n, const double* rho, cons
 /* {cells, cells sent to the integrator, ms features + networks, ms projection + T(u), ms
  * integrator, max network disagreement} of the last step. */
 int gdtk_hip_surrogate_stats(int surrogate, double* stats, int nstats);
+/* The element-conservation projection of the surrogate step on its own, for device arrays
+ * Y0, d0, d [nsp][n] (species-major) and ok [n], on `stream`: d is the change nearest d0 (least
+ * squares) with E d = 0 and Y0 + d >= 0, where E = elem [n_elem][nsp] (host, row-major; element
+ * mass per unit species mass); ok[i] = 0 and d = 0 where the free species cannot carry the
+ * element balance.  No counterpart in the reference. */
+int gdtk_hip_surrogate_project(int nsp, int n_elem, const double* elem, int n, const double* Y0, const double* d0,
+                               double* d, int* ok, void* stream);
 
 /* Message of the last failing call on this thread. */
 const char* gdtk_hip_last_error(void);

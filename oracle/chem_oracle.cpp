@@ -620,24 +620,25 @@ bool ChemistryUpdate::update(GasState& Q, double tInterval, double& dtSuggest, C
     return true;
 }
 
-// fvcell.d:858-972 (single temperature, no ignition zones, species written back by the caller)
+// fvcell.d:858-972 (single temperature, no ignition zones, species written back by the caller).
+// The reference retries only on ThermochemicalReactorUpdateException (here: update() returning
+// false).  A GasModelException raised inside the update (a bad density in massf2conc, a NaN
+// temperature in the CEA curves, a failed T(u) iteration of the tight temperature coupling)
+// is not caught there and ends the run; here the cell fails, without the retry.
 int thermochemical_increment(ChemistryUpdate& chem, const ThermallyPerfectGas& gm, GasState& Q, double dt,
                              double& dt_chem, double T_frozen, ChemStats* stats) {
     if (Q.T <= T_frozen) return 0;
     bool ok = false;
     try {
         ok = chem.update(Q, dt, dt_chem, stats);
-    } catch (GasModelException&) {
-        ok = false;
-    }
-    if (!ok) {
-        dt_chem = -1.0;
-        try {
+        if (!ok) {
+            dt_chem = -1.0;
             ok = chem.update(Q, dt, dt_chem, stats);
-        } catch (GasModelException&) {
-            ok = false;
+            if (!ok) return -1;
         }
-        if (!ok) return -1;
+    } catch (GasModelException&) {
+        dt_chem = -1.0;
+        return -1;
     }
     try {
         gm.update_thermo_from_rhou(Q);

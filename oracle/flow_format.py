@@ -98,3 +98,31 @@ def read_gziptext(path, want_names, dims, nic, njc, nkc):
     ncell = nic * njc * nkc
     rows = np.array([[float(x) for x in lines[9 + c].split()] for c in range(ncell)])
     return rows[:, cols], sim_time, label
+
+
+def read_rawbinary(path, want_names, ncell):
+    """(records in the order of want_names, sim_time) of a rawbinary file
+    (read_legacy_solution's raw binary branch for an SFluidBlock, fluidblockio_old.d:172-213)."""
+    data = open(path, "rb").read()
+    p = len(b"structured_grid_flow 1.0")
+    if data[:p] != b"structured_grid_flow 1.0":
+        raise ValueError("not a structured_grid_flow 1.0 file")
+    lab = struct.unpack_from("<i", data, p)[0]
+    p += 4 + lab
+    sim_time = struct.unpack_from("<d", data, p)[0]
+    p += 8
+    nvar = struct.unpack_from("<i", data, p)[0]
+    p += 4
+    names = []
+    for _ in range(nvar):
+        ln = struct.unpack_from("<i", data, p)[0]
+        names.append(data[p + 4:p + 4 + ln].decode())
+        p += 4 + ln
+    p += 16  # dims, nicell, njcell, nkcell
+    rows = np.frombuffer(data[p:], "<f8").reshape(ncell, nvar)
+    cols = []
+    for n in want_names:
+        if n not in names:
+            raise ValueError("Could not find variable: " + n)
+        cols.append(names.index(n))
+    return rows[:, cols], sim_time

@@ -305,3 +305,42 @@ def test_reactor_n2_reference(oracle_lib):
     got = {"p": p[0], "T": T[0], "Y_N2": Y[0, 0], "Y_N": Y[0, 1]}
     for k in refs:
         assert abs(got[k] - refs[k]) / refs[k] < 1.0e-3, (k, got[k], refs[k])
+
+
+def test_tight_coupling_reaches_mass_action_at_final_temperature(oracle_lib):
+    """tightTempCoupling (chemistry_update.d:265-269): after every subcycle T is re-solved from
+    the unchanged internal energy and the rate constants are re-evaluated there.  O2 + M <=>
+    O + O + M in a closed adiabatic reactor at fixed density, from pure O2 at 4500 K, run
+    to equilibrium (thermochemical_increment over 1 s, so with the final T(u) update): the
+    dissociation cools the gas, and with the coupling the final composition obeys the law of
+    mass action at the final temperature, x_O^2 / x_O2 * p / P_atm = exp(-(2 g_O - g_O2) /
+    (R T_final)), with Gibbs energies from the NASA-9 polynomials evaluated here.  Without it
+    the rate constants stay at 4500 K and the reactor dissociates towards the 4500 K
+    equilibrium, which the gas's energy cannot pay for: the increment fails."""
+    O = oracle_lib
+    gd, md = gdata.load("tpg-air-5sp"), dict(gdata.load("park-air-5sp-1T"))
+    md["reactions"] = [r for r in md["reactions"] if r["equation"].startswith("O2 + M")]
+    M = np.array(gd["M"])
+    T0 = 4500.0
+    Y = np.array([0.0, 1.0, 0.0, 0.0, 0.0])
+    rho = 2.0e3 / (R_U / M[1] * T0)
+    st = O.TPG(gas_record(gd)).update("rhoT", Y, rho=rho, T=T0)
+    g_o2 = lambda T: _nasa9_g_molar(gd["thermo"][1], T)
+    g_o = lambda T: _nasa9_g_molar(gd["thermo"][4], T)
+    kp = lambda T: math.exp(-(2.0 * g_o(T) - g_o2(T)) / (R_U * T))
+    res = {}
+    for tight in (True, False):
+        m = dict(md)
+        m["config"] = dict(md["config"], tightTempCoupling=tight)
+        res[tight] = O.chem_step_batch(gas_record(gd), mech_record(m), [rho], [T0], [st["u"]], [st["p"]], Y,
+                                       [-1.0], 1.0)
+    r = res[True]
+    assert r["status"][0] == 0
+    conc = r["massf"] * rho / M
+    x_o, x_o2 = conc[4] / (conc[1] + conc[4]), conc[1] / (conc[1] + conc[4])
+    Tf = float(r["T"][0])
+    p_eq = (conc[1] + conc[4]) * R_U * Tf
+    lhs = x_o ** 2 / x_o2 * p_eq / 101.325e3
+    assert 0.05 < x_o < 0.95 and Tf < T0 - 1000.0, (x_o, Tf)
+    assert _isclose(kp(Tf), lhs, 2.0e-3), (kp(Tf), lhs)
+    assert res[False]["status"][0] == -1

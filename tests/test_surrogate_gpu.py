@@ -191,3 +191,63 @@ def test_surrogate_step_conservation_and_fallback():
     assert np.array_equal(st.cpu().numpy()[fb], ref_st.cpu().numpy()[fb])
     err = np.max(np.abs(Y[:, net_cells] - refY[:, net_cells]))
     print("max |dY| error of network cells (two epochs of training)", err, flush=True)
+
+
+def test_device_projection_matches_oracle():
+    """The device's element-conservation projection (the one k_surr_apply runs on every
+    network answer) against the oracle's active-set projection, which
+    tests/test_surrogate_oracle.py pins to a general constrained least-squares solve, on the
+    same (Y0, d0): clean cases, cases where one or more species hit zero (the active set
+    grows), and infeasible ones (both refuse): an element that no species carries leaves
+    E E^T singular."""
+    import torch
+
+    from gdtk_amd import data as gdata
+    from gdtk_amd.surrogate import air_element_matrix, project_device
+    species = ["N2", "O2", "NO", "N", "O"]
+    E = air_element_matrix(species, gdata.load("tpg-air-5sp")["M"])
+    rng = np.random.default_rng(11)
+    n = 4096
+    Y0 = rng.dirichlet(np.ones(5) * 0.6, size=n)
+    tiny = rng.integers(0, 5, n)
+    Y0[np.arange(n), tiny] *= 10.0 ** rng.uniform(-12, -2, n)
+    Y0 /= Y0.sum(axis=1, keepdims=True)
+    d0 = rng.normal(0.0, 1.0, (n, 5)) * (10.0 ** rng.uniform(-6, -0.5, (n, 1)))
+    # a quarter of the cases: a pure O2 -> 2 O or N2 -> 2 N shift (already conserving)
+    q = n // 4
+    amt = rng.uniform(-0.2, 0.2, q)
+    d0[:q] = 0.0
+    d0[:q, 1], d0[:q, 4] = -amt, amt
+    dev = torch.device("cuda:0")
+    tY = torch.tensor(np.ascontiguousarray(Y0.T), dtype=torch.float64, device=dev)
+    td0 = torch.tensor(np.ascontiguousarray(d0.T), dtype=torch.float64, device=dev)
+    td = torch.empty_like(tY)
+    tok = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def run(Em):
+        project_device(Em, n, tY.data_ptr(), td0.data_ptr(), td.data_ptr(), tok.data_ptr(),
+                       torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        return td.cpu().numpy().T, tok.cpu().numpy()
+
+    # a third element that no species carries: every cell is infeasible for both
+    E3 = np.vstack([E, np.zeros((1, 5))])
+    d_gpu, ok = run(E3)
+    assert not ok.any() and not d_gpu.any()
+    assert all(MO.project(Y0[i], d0[i], E3) is None for i in range(0, n, 64))
+    d_gpu, ok = run(E)
+    grew = worst = 0
+    for i in range(n):
+        ref = MO.project(Y0[i], d0[i], E)
+        if ref is None:
+            assert ok[i] == 0, i
+            assert np.all(d_gpu[i] == 0.0)
+            continue
+        assert ok[i] == 1, i
+        err = float(np.max(np.abs(d_gpu[i] - ref)))
+        worst = max(worst, err)
+        assert err <= 1e-13, (i, Y0[i], d0[i], d_gpu[i], ref)
+        grew += int(np.any(Y0[i] + ref == 0.0))
+    print(f"projection: {n} cells, {grew} with a species driven to zero, {int((ok == 0).sum())} infeasible, "
+          f"max |d_gpu - d_oracle| {worst:.2e}", flush=True)
+    assert grew > 100
