@@ -354,6 +354,23 @@ int gdtk_hip_chem_step_device(int chem, int n, const double* rho, const double* 
                               const double* massf, const double* dt_chem, double dt, double* T_out, double* u_out,
                               double* p_out, double* massf_out, double* dt_chem_out, int* status, void* stream);
 
+/* gdtk_hip_chem_step_device with the same arguments and results, but allocating nothing: the
+ * step's work buffers (rate constants, concentrations, step sizes, flags, queue counters,
+ * statistics) live in a fixed device array of GDTK_CHEM_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  n above gdtk_hip_chem_noalloc_max_cells (which depends on the
+ * mechanism's species and reaction counts) is refused with GDTK_ERR_ARG.  All calls on one
+ * GPU share its array: calls that overlap on one GPU race, and gdtk_hip_chem_stats and
+ * gdtk_hip_chem_substeps read the array as the last call on that GPU left it. */
+#define GDTK_CHEM_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int gdtk_hip_chem_step_device_noalloc(int chem, int n, const double* rho, const double* T, const double* u,
+                                      const double* p, const double* massf, const double* dt_chem, double dt,
+                                      double* T_out, double* u_out, double* p_out, double* massf_out,
+                                      double* dt_chem_out, int* status, void* stream);
+/* The largest cell count of one gdtk_hip_chem_step_device_noalloc call for this handle. */
+int gdtk_hip_chem_noalloc_max_cells(int chem);
+
 /* Statistics of the last step (waits for it): {total substeps, total corrector
  * iterations, max substeps of a cell, total ODE step attempts, ms setup kernel, ms
  * integration kernel, ms finish kernel, cells}; returns the number of values written
