@@ -120,16 +120,39 @@ __host__ __device__ inline double cea_Cp(const CEADev& c, double T) {
     return c.R * Cp_on_R;
 }
 
-// cea_thermo_curves.d:71-102
-__host__ __device__ inline double cea_h(const CEADev& c, double T) {
+// cea_thermo_curves.d:71-102, with log(T) given (it is the same for every species)
+__host__ __device__ inline double cea_h(const CEADev& c, double T, double logT) {
     if (T < c.T_low) return c.h_low - c.Cp_low * (c.T_low - T);
     if (T > c.T_high) return c.h_high + c.Cp_high * (T - c.T_high);
     double a[9];
     cea_coeffs(c, T, a);
-    double logT = log(T);
     double h_on_RT = -a[0] / T + a[1] * logT + a[2] * T + a[3] * T * T / 2.0;
     h_on_RT += a[4] * T * T * T / 3.0 + a[5] * T * T * T * T / 4.0 + a[6] * T * T * T * T * T / 5.0 + a[7];
     return c.R * h_on_RT;
+}
+__host__ __device__ inline double cea_h(const CEADev& c, double T) { return cea_h(c, T, log(T)); }
+
+// cea_h and cea_Cp at one temperature from one coefficient selection (the same values as
+// the two separate calls).
+__host__ __device__ inline void cea_h_Cp(const CEADev& c, double T, double logT, double& h, double& Cp) {
+    if (T < c.T_low) {
+        h = c.h_low - c.Cp_low * (c.T_low - T);
+        Cp = c.Cp_low;
+        return;
+    }
+    if (T > c.T_high) {
+        h = c.h_high + c.Cp_high * (T - c.T_high);
+        Cp = c.Cp_high;
+        return;
+    }
+    double a[9];
+    cea_coeffs(c, T, a);
+    double Cp_on_R = a[0] / (T * T) + a[1] / T + a[2] + a[3] * T;
+    Cp_on_R += a[4] * T * T + a[5] * T * T * T + a[6] * T * T * T * T;
+    Cp = c.R * Cp_on_R;
+    double h_on_RT = -a[0] / T + a[1] * logT + a[2] * T + a[3] * T * T / 2.0;
+    h_on_RT += a[4] * T * T * T / 3.0 + a[5] * T * T * T * T / 4.0 + a[6] * T * T * T * T * T / 5.0 + a[7];
+    h = c.R * h_on_RT;
 }
 
 // cea_thermo_curves.d:103-130
@@ -147,10 +170,28 @@ __host__ __device__ inline double cea_s(const CEADev& c, double T) {
 // therm_perf_gas_mix_eos.d:61-67 with gas_model.d:410-419 (mass_average)
 template <int NSP>
 __device__ __forceinline__ double tpg_energy(const ChemDev& cd, const double* massf, double T) {
+    const double logT = log(T);
     double u = 0.0;
 #pragma unroll
-    for (int i = 0; i < NSP; ++i) u += massf[i] * (cea_h(cd.cea[i], T) - cd.R[i] * T);
+    for (int i = 0; i < NSP; ++i) u += massf[i] * (cea_h(cd.cea[i], T, logT) - cd.R[i] * T);
     return u;
+}
+// tpg_energy and tpg_Cv at one temperature (the Newton iteration of T(u) evaluates both at
+// each iterate), sharing each species' coefficient selection and log(T).
+template <int NSP>
+__device__ __forceinline__ void tpg_energy_Cv(const ChemDev& cd, const double* massf, double T, double& u,
+                                              double& Cv) {
+    const double logT = log(T);
+    double uu = 0.0, v = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) {
+        double h, Cp;
+        cea_h_Cp(cd.cea[i], T, logT, h, Cp);
+        uu += massf[i] * (h - cd.R[i] * T);
+        v += massf[i] * (Cp - cd.R[i]);
+    }
+    u = uu;
+    Cv = v;
 }
 template <int NSP>
 __device__ __forceinline__ double tpg_Cv(const ChemDev& cd, const double* massf, double T) {
@@ -207,9 +248,10 @@ __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const 
         double dxold = (xMax - xMin);
         double dx = dxold;
         T = rts;
-        u = tpg_energy<NSP>(cd, massf, T);
+        double Cv;
+        tpg_energy_Cv<NSP>(cd, massf, T, u, Cv);
         double f0 = e_tgt - u;
-        double df0 = -1.0 * tpg_Cv<NSP>(cd, massf, rts);
+        double df0 = -1.0 * Cv;
         for (int j = 0; j < 30; ++j) {
             if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
                 dxold = dx;
@@ -225,9 +267,9 @@ __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const 
             }
             if (fabs(dx) < TOL) { T = rts; return true; }
             T = rts;
-            u = tpg_energy<NSP>(cd, massf, T);
+            tpg_energy_Cv<NSP>(cd, massf, T, u, Cv);
             f0 = e_tgt - u;
-            df0 = -1.0 * tpg_Cv<NSP>(cd, massf, rts);
+            df0 = -1.0 * Cv;
             if (f0 < 0.0) xL = rts;
             else xH = rts;
         }

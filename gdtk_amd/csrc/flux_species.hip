@@ -9,11 +9,13 @@
 
 namespace gdtk {
 
+// Two lanes per face (face_flux_species_pair): a workgroup of 256 lanes serves 128 faces of
+// one direction of one block (the FaceBatch ranges start at multiples of 256 faces).
 template <int DIM, int FC, int NSP>
-__global__ __launch_bounds__(256) void k_face_flux_species(const BlockDev* __restrict__ tab, FaceBatch fb,
-                                                           int nblocks, double eps,
-                                                           const ChemDev* __restrict__ cdp) {
-    int64_t w0 = (int64_t)blockIdx.x * blockDim.x;
+__global__ __launch_bounds__(256) void k_face_flux_species_pair(const BlockDev* __restrict__ tab, FaceBatch fb,
+                                                                int nblocks, double eps,
+                                                                const ChemDev* __restrict__ cdp) {
+    int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / 2);
     const int d = __builtin_amdgcn_readfirstlane(face_batch_dir(fb, w0));
     const int64_t* __restrict__ start = fb.start[d];
     int lo = 0, hi = nblocks - 1;
@@ -24,11 +26,10 @@ __global__ __launch_bounds__(256) void k_face_flux_species(const BlockDev* __res
     }
     const int q = __builtin_amdgcn_readfirstlane(lo);
     const BlockDev& b = tab[q];
-    const int64_t f = w0 - start[q] + threadIdx.x;
+    const int64_t f = w0 - start[q] + (threadIdx.x >> 1);
     if (f >= b.nface[d]) return;
-    // A reconstructed state whose mass fractions cannot be scaled fails the stage, like a
-    // cell that fails to decode.
-    if (!face_flux_species<DIM, FC, NSP>(b, *cdp, d, eps, f)) atomicAdd(&b.counters[0], 1);
+    if (!face_flux_species_pair<DIM, FC, NSP>(b, *cdp, d, eps, f, (int)(threadIdx.x & 1)))
+        atomicAdd(&b.counters[0], 1);
 }
 
 template <int DIM, int STAGE, int NSP>
@@ -108,20 +109,22 @@ bool species_supported(int nsp) { return nsp == 2 || nsp == 5; }
 template <int NSP>
 static void face_species(const BlockDev* tab, const FaceBatch& fb, int nb, int64_t total, int dims, int fc,
                          double eps, const ChemDev* cd, hipStream_t s) {
-    dim3 g(nblk256(total)), blk(256);
+    dim3 g(nblk256(2 * total)), blk(256);
+#define GDTK_FS(D, F) hipLaunchKernelGGL((k_face_flux_species_pair<D, F, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd)
     if (dims == 3) {
-        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<3, 4, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<3, 3, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<3, 2, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<3, 1, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else hipLaunchKernelGGL((k_face_flux_species<3, 0, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        if (fc == 4) GDTK_FS(3, 4);
+        else if (fc == 3) GDTK_FS(3, 3);
+        else if (fc == 2) GDTK_FS(3, 2);
+        else if (fc == 1) GDTK_FS(3, 1);
+        else GDTK_FS(3, 0);
     } else {
-        if (fc == 4) hipLaunchKernelGGL((k_face_flux_species<2, 4, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else if (fc == 3) hipLaunchKernelGGL((k_face_flux_species<2, 3, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else if (fc == 2) hipLaunchKernelGGL((k_face_flux_species<2, 2, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else if (fc == 1) hipLaunchKernelGGL((k_face_flux_species<2, 1, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
-        else hipLaunchKernelGGL((k_face_flux_species<2, 0, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd);
+        if (fc == 4) GDTK_FS(2, 4);
+        else if (fc == 3) GDTK_FS(2, 3);
+        else if (fc == 2) GDTK_FS(2, 2);
+        else if (fc == 1) GDTK_FS(2, 1);
+        else GDTK_FS(2, 0);
     }
+#undef GDTK_FS
 }
 
 template <int NSP>

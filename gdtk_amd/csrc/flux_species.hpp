@@ -80,14 +80,23 @@ __device__ __forceinline__ bool tpg_update_rhou(const ChemDev& cd, FaceStateT<NS
     return true;
 }
 
-// Flux of one face of direction d (face-per-thread): onedinterp.d:751-988 with species
+// A double from the other lane of an adjacent lane pair (lanes 2k and 2k + 1).
+__device__ __forceinline__ double pair_swap(double v) { return __shfl_xor(v, 1); }
+
+// Flux of one face of direction d, two lanes per face: onedinterp.d:751-988 with species
 // reconstruction (allow_reconstruction_for_species and scale_species_after_reconstruction
 // true, thermo_interpolator rhou, local frame) and fluxcalc.d:54-184; or the simple-outflow
-// flux of boundary_flux_effect.d:557-645.  Returns false where the reference throws
-// (mass fractions of a reconstructed state far from summing to one).
+// flux of boundary_flux_effect.d:557-645.  The even lane of a pair builds the left state
+// and the odd lane the right one (one side of the l2r2 reconstruction, mass-fraction
+// scaling, the Newton T(u) of update_thermo_from_rhou with its fallback to the cell state),
+// the pair swaps the states, and both lanes evaluate the flux, each storing every other
+// component.  Each side's arithmetic is the reference's for that side, so a face's two
+// Newton iterations run side by side and a lane holds one side's state.  Both lanes of a
+// pair take the same branches up to the swap.  Returns false where the reference throws
+// (either side's mass fractions far from summing to one).
 template <int DIM, int FC, int NSP>
-__device__ __forceinline__ bool face_flux_species(const BlockDev& b, const ChemDev& cd, int d, double eps,
-                                                  int64_t f) {
+__device__ __forceinline__ bool face_flux_species_pair(const BlockDev& b, const ChemDev& cd, int d, double eps,
+                                                       int64_t f, int right) {
     using C = CQ<DIM, NSP>;
     const int fi = (int)(f % b.nf[d][0]);
     const int64_t rest = f / b.nf[d][0];
@@ -107,8 +116,10 @@ __device__ __forceinline__ bool face_flux_species(const BlockDev& b, const ChemD
     int side = -1;
     if (pos == 0) side = 2 * d;
     else if (pos == nd) side = 2 * d + 1;
-    bool ok = true;
+    double* out = b.flux[d];
+    const int64_t nfc = b.nface[d];
     if (side >= 0 && b.bc_kind[side] == BC_OUTFLOW_SIMPLE_FLUX) {
+        if (right) return true;
         const int outsign = (side % 2 == 0) ? -1 : 1;
         const int64_t c = at(side % 2 == 0 ? 0 : nd - 1);
         const Vec3d v{fs[c + FS_VX * np], fs[c + FS_VY * np], fs[c + FS_VZ * np]};
@@ -116,77 +127,95 @@ __device__ __forceinline__ bool face_flux_species(const BlockDev& b, const ChemD
 #pragma unroll
         for (int i = 0; i < NSP; ++i) mf[i] = fs[c + (FS_BASE_PLANES + i) * np];
         outflow_flux<DIM, NSP>(g, outsign, fs[c + FS_RHO * np], fs[c + FS_U * np], fs[c + FS_P * np], v, F, mf);
-    } else {
-        const int64_t cL1 = at(pos - 2), cL0 = at(pos - 1), cR0 = at(pos), cR1 = at(pos + 1);
-        const double* len = b.len + d * np;
-        const SpeciesCell<NSP> L1 = load_species_cell<NSP>(fs, np, cL1), L0 = load_species_cell<NSP>(fs, np, cL0),
-                               R0 = load_species_cell<NSP>(fs, np, cR0), R1 = load_species_cell<NSP>(fs, np, cR1);
-        const Vec3d vL1 = to_local(L1.vel, g), vL0 = to_local(L0.vel, g), vR0 = to_local(R0.vel, g),
-                    vR1 = to_local(R1.vel, g);
-        L2R2 c;
-        c.prepare(len[cL1], len[cL0], len[cR0], len[cR1]);
-        FaceStateT<NSP> Ls, Rs;
-        // Lft/Rght start as copies of the cells L0/R0 (onedinterp.d:127-128): T is the
-        // Newton start of the EOS call and a is never updated.
-        Ls.T = fs[cL0 + FS_T * np];
-        Rs.T = fs[cR0 + FS_T * np];
-        Ls.a = fs[cL0 + FS_A * np];
-        Rs.a = fs[cR0 + FS_A * np];
-        c.scalar(vL1.x, vL0.x, vR0.x, vR1.x, eps, Ls.vel.x, Rs.vel.x);
-        c.scalar(vL1.y, vL0.y, vR0.y, vR1.y, eps, Ls.vel.y, Rs.vel.y);
-        c.scalar(vL1.z, vL0.z, vR0.z, vR1.z, eps, Ls.vel.z, Rs.vel.z);
-        double rsL[NSP], rsR[NSP];
 #pragma unroll
-        for (int i = 0; i < NSP; ++i) c.scalar(L1.rho_s[i], L0.rho_s[i], R0.rho_s[i], R1.rho_s[i], eps, rsL[i], rsR[i]);
-        double rho_L = 0.0, rho_R = 0.0;
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) {
-            rho_L += rsL[i];
-            rho_R += rsR[i];
-        }
-        Ls.rho = rho_L;
-        Rs.rho = rho_R;
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) {
-            Ls.massf[i] = rsL[i] / Ls.rho;
-            Rs.massf[i] = rsR[i] / Rs.rho;
-        }
-        ok = scale_mass_fractions<NSP>(Ls.massf) && scale_mass_fractions<NSP>(Rs.massf);
-        c.scalar(L1.u, L0.u, R0.u, R1.u, eps, Ls.u, Rs.u);
-        // The EOS calls of the two sides share one copy of the Newton code (a loop that is
-        // not unrolled, with the side's state selected in and out).
-#pragma clang loop unroll(disable)
-        for (int side = 0; side < 2; ++side) {
-            FaceStateT<NSP> st = side ? Rs : Ls;
-            if (!tpg_update_rhou<NSP>(cd, st))
-                copy_cell_state<NSP>(fs, np, side ? cR0 : cL0, side ? vR0 : vL0, st);
-            if (side) Rs = st;
-            else Ls = st;
-        }
-        Ls.vel = to_local(to_global(Ls.vel, g), g);
-        Rs.vel = to_local(to_global(Rs.vel, g), g);
-#pragma unroll
-        for (int j = 0; j < C::n; ++j) F[j] = 0.0;
-        // FC 2: adaptive_efm_ausmdv (fluxcalc.d:1315-1332), EFM where the face is shocked.
-        const bool shocked = flux_is_adaptive(FC) && ((b.shock[cL0] | b.shock[cR0]) != 0);
-        if (FC == 1 || (FC == 2 && shocked))
-            efmflx<DIM, NSP>(Ls, Rs, tpg_Cv<NSP>(cd, Ls.massf, Ls.T), tpg_Cv<NSP>(cd, Rs.massf, Rs.T), F);
-        else if (FC == 3 || (FC == 4 && shocked))
-            hanel<DIM, NSP>(Ls, Rs, F);
-        else
-            ausmdv<DIM, NSP>(Ls, Rs, F);
-        // The grid-velocity terms of fluxcalc.d:159-171 add signed zeros (see face_flux_l2r2).
-        const double fz = (DIM == 3) ? F[C::zMom] : 0.0;
-        const Vec3d m = to_global(Vec3d{F[C::xMom], F[C::yMom], fz}, g);
-        F[C::xMom] = m.x;
-        F[C::yMom] = m.y;
-        if (DIM == 3) F[C::zMom] = m.z;
+        for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
+        return true;
     }
-    double* out = b.flux[d];
-    const int64_t nfc = b.nface[d];
+    // This lane's three stencil cells: L1, L0, R0 (left) or L0, R0, R1 (right); c1 is the
+    // lane's own cell, the one the face state starts as a copy of (onedinterp.d:127-128).
+    const int64_t cL0 = at(pos - 1), cR0 = at(pos);
+    const int64_t c0 = right ? cL0 : at(pos - 2);
+    const int64_t c1 = right ? cR0 : cL0;
+    const int64_t c2 = right ? at(pos + 1) : cR0;
+    const double* len = b.len + d * np;
+    const SpeciesCell<NSP> S0 = load_species_cell<NSP>(fs, np, c0), S1 = load_species_cell<NSP>(fs, np, c1),
+                           S2 = load_species_cell<NSP>(fs, np, c2);
+    const Vec3d v0 = to_local(S0.vel, g), v1 = to_local(S1.vel, g), v2 = to_local(S2.vel, g);
+    const double l0 = len[c0], l1 = len[c1], l2 = len[c2];
+    FaceStateT<NSP> st;
+    st.T = fs[c1 + FS_T * np];
+    st.a = fs[c1 + FS_A * np];
+    double rs[NSP];
+    bool ok;
+    if (right) {
+        L2R2Side<true> c;
+        c.prepare(0.0, l0, l1, l2);
+        st.vel.x = c.value(v0.x, v1.x, v2.x, eps);
+        st.vel.y = c.value(v0.y, v1.y, v2.y, eps);
+        st.vel.z = c.value(v0.z, v1.z, v2.z, eps);
 #pragma unroll
-    for (int j = 0; j < C::n; ++j) out[f + j * nfc] = F[j];
-    return ok;
+        for (int i = 0; i < NSP; ++i) rs[i] = c.value(S0.rho_s[i], S1.rho_s[i], S2.rho_s[i], eps);
+        double rho = 0.0;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) rho += rs[i];
+        st.rho = rho;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) st.massf[i] = rs[i] / st.rho;
+        ok = scale_mass_fractions<NSP>(st.massf);
+        st.u = c.value(S0.u, S1.u, S2.u, eps);
+    } else {
+        L2R2Side<false> c;
+        c.prepare(l0, l1, l2, 0.0);
+        st.vel.x = c.value(v0.x, v1.x, v2.x, eps);
+        st.vel.y = c.value(v0.y, v1.y, v2.y, eps);
+        st.vel.z = c.value(v0.z, v1.z, v2.z, eps);
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) rs[i] = c.value(S0.rho_s[i], S1.rho_s[i], S2.rho_s[i], eps);
+        double rho = 0.0;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) rho += rs[i];
+        st.rho = rho;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) st.massf[i] = rs[i] / st.rho;
+        ok = scale_mass_fractions<NSP>(st.massf);
+        st.u = c.value(S0.u, S1.u, S2.u, eps);
+    }
+    if (!tpg_update_rhou<NSP>(cd, st)) copy_cell_state<NSP>(fs, np, c1, v1, st);
+    st.vel = to_local(to_global(st.vel, g), g);
+    // EFM's specific heats, each at its own side.
+    const bool shocked = flux_is_adaptive(FC) && ((b.shock[cL0] | b.shock[cR0]) != 0);
+    const bool efm = FC == 1 || (FC == 2 && shocked);
+    const double Cv = efm ? tpg_Cv<NSP>(cd, st.massf, st.T) : 0.0;
+    // swap the two sides' states
+    FaceStateT<NSP> o;
+    o.rho = pair_swap(st.rho);
+    o.p = pair_swap(st.p);
+    o.T = pair_swap(st.T);
+    o.u = pair_swap(st.u);
+    o.a = pair_swap(st.a);
+    o.vel.x = pair_swap(st.vel.x);
+    o.vel.y = pair_swap(st.vel.y);
+    o.vel.z = pair_swap(st.vel.z);
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) o.massf[i] = pair_swap(st.massf[i]);
+    const double Cv_o = pair_swap(Cv);
+    const bool ok_o = pair_swap(ok ? 1.0 : 0.0) != 0.0;
+    const FaceStateT<NSP>& Ls = right ? o : st;
+    const FaceStateT<NSP>& Rs = right ? st : o;
+#pragma unroll
+    for (int j = 0; j < C::n; ++j) F[j] = 0.0;
+    if (efm) efmflx<DIM, NSP>(Ls, Rs, right ? Cv_o : Cv, right ? Cv : Cv_o, F);
+    else if (FC == 3 || (FC == 4 && shocked)) hanel<DIM, NSP>(Ls, Rs, F);
+    else ausmdv<DIM, NSP>(Ls, Rs, F);
+    const double fz = (DIM == 3) ? F[C::zMom] : 0.0;
+    const Vec3d m = to_global(Vec3d{F[C::xMom], F[C::yMom], fz}, g);
+    F[C::xMom] = m.x;
+    F[C::yMom] = m.y;
+    if (DIM == 3) F[C::zMom] = m.z;
+#pragma unroll
+    for (int j = 0; j < C::n; ++j)
+        if ((j & 1) == right) out[f + j * nfc] = F[j];
+    return right ? true : (ok && ok_o);
 }
 
 // decode_conserved for the multi-species gas (fvcell.d:586-821, cqi.mass present):

@@ -106,6 +106,46 @@ struct L2R2 {
     }
 };
 
+// One side of the l2r2 reconstruction: the coefficients and operations of L2R2::prepare and
+// L2R2::scalar that the left value (from L1, L0, R0) or the right value (from L0, R0, R1)
+// needs, in the same order, so that each side equals scalar()'s bit for bit.
+template <bool RIGHT>
+struct L2R2Side {
+    double a, len_other, tO, tD, two;  // left: aL0, lenR0, tL01, tR0L0, twoL0L1
+    __device__ __forceinline__ void prepare(double lenL1, double lL0, double lR0, double lenR1) {
+        if (!RIGHT) {
+            len_other = lR0;
+            a = 0.5 * lL0 / (lenL1 + 2.0 * lL0 + lR0);
+            tO = 2.0 / (lL0 + lenL1);
+            tD = 2.0 / (lR0 + lL0);
+            two = (2.0 * lL0 + lenL1);
+        } else {
+            len_other = lL0;
+            a = 0.5 * lR0 / (lL0 + 2.0 * lR0 + lenR1);
+            tD = 2.0 / (lR0 + lL0);
+            tO = 2.0 / (lenR1 + lR0);
+            two = (2.0 * lR0 + lenR1);
+        }
+    }
+    // left: value(qL1, qL0, qR0); right: value(qL0, qR0, qR1)
+    __device__ __forceinline__ double value(double q0, double q1, double q2, double eps) const {
+        const double beta = 1.0;
+        if (!RIGHT) {
+            const double delLminus = (q1 - q0) * tO;
+            const double del = (q2 - q1) * tD;
+            const double sL = (delLminus * del + fabs(delLminus * del) + eps) / (delLminus * delLminus + del * del + eps);
+            const double qL = q1 + beta * sL * a * (del * two + delLminus * len_other);
+            return clip_to_limits(qL, q1, q2);
+        } else {
+            const double del = (q1 - q0) * tD;
+            const double delRplus = (q2 - q1) * tO;
+            const double sR = (del * delRplus + fabs(del * delRplus) + eps) / (del * del + delRplus * delRplus + eps);
+            const double qR = q1 - beta * sR * a * (delRplus * len_other + del * two);
+            return clip_to_limits(qR, q0, q1);
+        }
+    }
+};
+
 // A reconstructed face state; massf is used for NSP > 1 (multi-species gas).
 template <int NSP>
 struct FaceStateT {
