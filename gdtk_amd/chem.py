@@ -123,7 +123,7 @@ def element_matrix(species):
 # ThermochemicalReactor.opCall (src/kinetics/thermochemical_reactor.d:45-47) applied per
 # cell as FVCell.thermochemical_increment does (src/eilmer/fvcell.d:858-972).
 
-TUNE_PERSISTENT, TUNE_CHUNK, TUNE_REFILL, TUNE_BLOCKS_PER_CU = 1, 2, 3, 4
+TUNE_PERSISTENT, TUNE_CHUNK, TUNE_REFILL, TUNE_BLOCKS_PER_CU, TUNE_HEAVY_RATIO = 1, 2, 3, 4, 5
 TPG_MODES = {"pT": 0, "rhou": 1, "rhoT": 2}
 
 _dp = C.POINTER(C.c_double)
@@ -218,9 +218,9 @@ class Chemistry:
         _check(_lib().gdtk_hip_chem_substeps(self.h, n, out.ctypes.data_as(_ip)))
         return out
 
-    def set_tuning(self, persistent=None, chunk=None, refill=None, blocks_per_cu=None):
+    def set_tuning(self, persistent=None, chunk=None, refill=None, blocks_per_cu=None, heavy_ratio=None):
         for key, val in ((TUNE_PERSISTENT, persistent), (TUNE_CHUNK, chunk), (TUNE_REFILL, refill),
-                         (TUNE_BLOCKS_PER_CU, blocks_per_cu)):
+                         (TUNE_BLOCKS_PER_CU, blocks_per_cu), (TUNE_HEAVY_RATIO, heavy_ratio)):
             if val is not None:
                 _check(_lib().gdtk_hip_chem_set_tuning(self.h, key, int(val)))
 

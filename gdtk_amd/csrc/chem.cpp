@@ -131,6 +131,22 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
             d.part[s] = in;
         }
     }
+    // The rows of the branch-free rate walk, when every reaction has its dense form.
+    cd.all_sorted = 1;
+    for (int ir = 0; ir < m.nr; ++ir) cd.all_sorted &= cd.r[ir].sorted;
+    for (int ir = 0; ir < CH_MAX_R; ++ir) cd.rr[ir] = RateRow{};
+    if (cd.all_sorted) {
+        for (int ir = 0; ir < m.nr; ++ir) {
+            const ReacDev& d = cd.r[ir];
+            RateRow& w = cd.rr[ir];
+            w.flags = d.collider ? 1 : 0;
+            for (int s = 0; s < g.nsp; ++s) {
+                const bool listed = d.collider && ((d.eff_set >> s) & 1u);
+                w.eff[s] = listed ? d.eff_d[s] : 0.0;
+                w.code[s] = d.reac_d[s] | (d.prod_d[s] << 2) | ((d.nu[s] & 15) << 4) | (listed ? 256 : 0);
+            }
+        }
+    }
     if (!(m.T_lower < m.T_upper)) throw Error(GDTK_ERR_ARG, "tempLimits lower must be below upper");
     cd.T_lower = m.T_lower;
     cd.T_upper = m.T_upper;
@@ -272,7 +288,7 @@ int gdtk_hip_chem_substeps(int chem, int n, int* substeps) {
         HIPCHK(hipMemcpy(f.data(), c.last_w.flag, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(substeps, c.last_w.nsub, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i)
-            if (f[i] != CF_INTEGRATE) substeps[i] = 0;
+            if (f[i] != CF_INTEGRATE && f[i] != CF_HEAVY) substeps[i] = 0;
         return GDTK_OK;
     });
 }
@@ -281,7 +297,7 @@ int gdtk_hip_chem_set_tuning(int chem, int key, int value) {
     return guarded([&] {
         Chem& c = get_chem(chem);
         switch (key) {
-        case GDTK_CHEM_TUNE_PERSISTENT: c.tuning.persistent = value ? 1 : 0; break;
+        case GDTK_CHEM_TUNE_PERSISTENT: c.tuning.persistent = (value < 0) ? -1 : (value ? 1 : 0); break;
         case GDTK_CHEM_TUNE_CHUNK:
             if (value < 1 || value > 4096) throw Error(GDTK_ERR_ARG, "chunk must be 1..4096");
             c.tuning.chunk = value;
@@ -293,6 +309,10 @@ int gdtk_hip_chem_set_tuning(int chem, int key, int value) {
         case GDTK_CHEM_TUNE_BLOCKS_PER_CU:
             if (value < 0 || value > 64) throw Error(GDTK_ERR_ARG, "blocks per CU must be 0..64");
             c.tuning.blocks_per_cu = value;
+            break;
+        case GDTK_CHEM_TUNE_HEAVY_RATIO:
+            if (value < 0 || value > 1000000) throw Error(GDTK_ERR_ARG, "heavy ratio must be 0..1e6");
+            c.tuning.heavy_ratio = value;
             break;
         default: throw Error(GDTK_ERR_ARG, "unknown tuning key");
         }
@@ -320,7 +340,7 @@ int gdtk_hip_chem_autotune(int chem, int n, const double* rho, const double* T, 
         int k = 0;
         hipStream_t s = c.stream;
         for (const Cand& cd : cands) {
-            c.tuning = ChemTuning{cd.persistent, cd.chunk, cd.refill, cd.bpc};
+            c.tuning = ChemTuning{cd.persistent, cd.chunk, cd.refill, cd.bpc, saved.heavy_ratio};
             c.step_device(n, dt, rho, T, u, p, massf, dt_chem, To.p, uo.p, po.p, mo.p, dto.p, so.p, s);
             double v[8];
             c.read_stats(v, 8);

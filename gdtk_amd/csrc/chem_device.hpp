@@ -49,11 +49,28 @@ struct ReacDev {
     int part[CH_MAX_SP];
 };
 
+// One reaction of the branch-free rate evaluation, for a mechanism whose lists are all in
+// ascending species order: per species the third-body efficiency (0 where the species is
+// not in the list, or the reaction has no third body) and a code word holding the reactant
+// exponent (bits 0-1, 0 = absent), the product exponent (bits 2-3), the net coefficient
+// as a 4-bit two's complement number (bits 4-7) and the third-body list membership (bit
+// 8).  flags bit 0: third-body reaction.  Rows
+// past the reaction count are all zero: every species' net coefficient is 0 there, so they
+// add nothing.
+struct RateRow {
+    double eff[CH_MAX_SP];
+    int code[CH_MAX_SP];
+    int flags;
+    int pad_[3];
+};
+
 struct ChemDev {
     int nsp, nr;
     double M[CH_MAX_SP], R[CH_MAX_SP];
     CEADev cea[CH_MAX_SP];
     ReacDev r[CH_MAX_R];
+    int all_sorted;  // every reaction in dense form: rr[] holds the mechanism
+    RateRow rr[CH_MAX_R];
     double T_lower, T_upper;
     int method;  // 0 alpha-QSS, 1 RKF
     double eps1, eps2, delta;
@@ -315,7 +332,11 @@ __device__ __forceinline__ void eval_rate_constants(const ChemDev& cd, double Ti
     }
 #pragma unroll
     for (int ir = 0; ir < NRM; ++ir) {
-        if (ir >= cd.nr) continue;
+        if (ir >= cd.nr) {
+            kf[ir] = 0.0;
+            kb[ir] = 0.0;
+            continue;
+        }
         const ReacDev& r = cd.r[ir];
         double Keq = 0.0;
         if (r.f_from_eq || r.b_from_eq) {
@@ -345,65 +366,105 @@ __device__ __forceinline__ void eval_rate_constants(const ChemDev& cd, double Ti
 // reaction_mechanism.d:83-96 (eval_split_rates) with reaction.d's forward/backward rates
 // (collider sum over the efficiency list, then the concentration products in list order)
 // and production/loss per participant in reaction order.
-template <int NSP, int NRM>
+//
+// DENSE (every list of the mechanism in ascending species order, rr[] filled; the host
+// picks the instantiation) takes a branch-free walk over the rows: an absent reactant or
+// product multiplies by exactly 1, a species outside the third-body list or the reaction
+// adds exactly 0 (a selected zero, not a product with zero), and a reaction without third
+// body takes kf as it is, so every result equals the list walk bit for bit while the table
+// words load in bulk instead of one load and branch per entry.
+template <int NSP, int NRM, bool DENSE>
 __device__ __forceinline__ void eval_split_rates(const ChemDev& cd, const double* kf, const double* kb,
                                                  const double* conc, double* q, double* L) {
 #pragma unroll
     for (int s = 0; s < NSP; ++s) { q[s] = 0.0; L[s] = 0.0; }
+    if constexpr (DENSE) {
+        double c2[NSP];
 #pragma unroll
-    for (int ir = 0; ir < NRM; ++ir) {
-        if (ir >= cd.nr) continue;
-        const ReacDev& r = cd.r[ir];
-        double collider = 0.0, wf, wb;
-        if (r.sorted) {
-            if (r.collider) {
+        for (int s = 0; s < NSP; ++s) c2[s] = conc[s] * conc[s];
+        const int nr = cd.nr;
+#pragma unroll
+        for (int ir = 0; ir < NRM; ++ir) {
+            if (ir >= nr) break;
+            const RateRow& rr = cd.rr[ir];
+            int code[NSP];
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) code[s] = rr.code[s];
+            double collider = 0.0;
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) collider += (code[s] & 256) ? rr.eff[s] * conc[s] : 0.0;
+            double wf = (rr.flags & 1) ? kf[ir] * collider : kf[ir];
+            double wb = (rr.flags & 1) ? kb[ir] * collider : kb[ir];
+            // c^e for e in 0..3 as (c or 1) * (c^2 or 1): c * (c * c) for e = 3, as ch_ipow
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) wf *= ((code[s] & 1) ? conc[s] : 1.0) * ((code[s] & 2) ? c2[s] : 1.0);
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) wb *= ((code[s] & 4) ? conc[s] : 1.0) * ((code[s] & 8) ? c2[s] : 1.0);
+#pragma unroll
+            for (int s = 0; s < NSP; ++s) {
+                const int nu = (int)((unsigned)code[s] << 24) >> 28;
+                const double m = (double)(nu < 0 ? -nu : nu);
+                const bool pos = nu > 0;
+                q[s] += (nu != 0) ? m * (pos ? wf : wb) : 0.0;
+                L[s] += (nu != 0) ? m * (pos ? wb : wf) : 0.0;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int ir = 0; ir < NRM; ++ir) {
+            if (ir >= cd.nr) continue;
+            const ReacDev& r = cd.r[ir];
+            double collider = 0.0, wf, wb;
+            if (r.sorted) {
+                if (r.collider) {
+#pragma unroll
+                    for (int s = 0; s < NSP; ++s)
+                        if ((r.eff_set >> s) & 1u) collider += r.eff_d[s] * conc[s];
+                }
+                wf = r.collider ? kf[ir] * collider : kf[ir];
 #pragma unroll
                 for (int s = 0; s < NSP; ++s)
-                    if ((r.eff_set >> s) & 1u) collider += r.eff_d[s] * conc[s];
+                    if (r.reac_d[s]) wf *= ch_ipow(conc[s], r.reac_d[s]);
+                wb = r.collider ? kb[ir] * collider : kb[ir];
+#pragma unroll
+                for (int s = 0; s < NSP; ++s)
+                    if (r.prod_d[s]) wb *= ch_ipow(conc[s], r.prod_d[s]);
+            } else {
+                if (r.collider) {
+#pragma unroll
+                    for (int e = 0; e < NSP; ++e)
+                        if (e < r.neff) collider += r.eff[e] * pick<NSP>(conc, r.eff_sp[e]);
+                }
+                wf = r.collider ? kf[ir] * collider : kf[ir];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    if (k < r.nreac) wf *= ch_ipow(pick<NSP>(conc, r.reac_sp[k]), r.reac_nu[k]);
+                wb = r.collider ? kb[ir] * collider : kb[ir];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    if (k < r.nprod) wb *= ch_ipow(pick<NSP>(conc, r.prod_sp[k]), r.prod_nu[k]);
             }
-            wf = r.collider ? kf[ir] * collider : kf[ir];
 #pragma unroll
-            for (int s = 0; s < NSP; ++s)
-                if (r.reac_d[s]) wf *= ch_ipow(conc[s], r.reac_d[s]);
-            wb = r.collider ? kb[ir] * collider : kb[ir];
-#pragma unroll
-            for (int s = 0; s < NSP; ++s)
-                if (r.prod_d[s]) wb *= ch_ipow(conc[s], r.prod_d[s]);
-        } else {
-            if (r.collider) {
-#pragma unroll
-                for (int e = 0; e < NSP; ++e)
-                    if (e < r.neff) collider += r.eff[e] * pick<NSP>(conc, r.eff_sp[e]);
-            }
-            wf = r.collider ? kf[ir] * collider : kf[ir];
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                if (k < r.nreac) wf *= ch_ipow(pick<NSP>(conc, r.reac_sp[k]), r.reac_nu[k]);
-            wb = r.collider ? kb[ir] * collider : kb[ir];
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                if (k < r.nprod) wb *= ch_ipow(pick<NSP>(conc, r.prod_sp[k]), r.prod_nu[k]);
-        }
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) {
-            const int nu = r.nu[s];
-            if (nu > 0) {
-                q[s] += nu * wf;
-                L[s] += nu * wb;
-            } else if (nu < 0) {
-                q[s] += -nu * wb;
-                L[s] += -nu * wf;
+            for (int s = 0; s < NSP; ++s) {
+                const int nu = r.nu[s];
+                if (nu > 0) {
+                    q[s] += nu * wf;
+                    L[s] += nu * wb;
+                } else if (nu < 0) {
+                    q[s] += -nu * wb;
+                    L[s] += -nu * wf;
+                }
             }
         }
     }
 }
 
 // reaction_mechanism.d:162-203
-template <int NSP, int NRM>
+template <int NSP, int NRM, bool DENSE>
 __device__ __forceinline__ double estimate_step_size(const ChemDev& cd, const double* kf, const double* kb, const double* conc) {
     const double MUCH_LARGER_FACTOR = 10000.0, ZERO_EPS = 1.0e-50, ZERO_TOL = 1.0e-30, EPS1 = 1.0e-3;
     double q[NSP], L[NSP];
-    eval_split_rates<NSP, NRM>(cd, kf, kb, conc, q, L);
+    eval_split_rates<NSP, NRM, DENSE>(cd, kf, kb, conc, q, L);
     double min_dt = 1.0e6, dt;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
@@ -436,7 +497,7 @@ __device__ __forceinline__ double qss_alpha(double p, double h) {
 //   predictor  yp = y0 + h (q0 - p0 y0) / (1 + alpha(p0) h p0)
 //   corrector  yc = y0 + h (q~ - p- y0) / (1 + alpha(pp) h p-),
 //              q~ = alpha(pp) qp + (1 - alpha(pp)) q0,  p- = (p0 + pp) / 2
-template <int NSP, int NRM>
+template <int NSP, int NRM, bool DENSE>
 __device__ __forceinline__ bool alpha_qss_step(const ChemDev& cd, const double* kf, const double* kb, const double* y0,
                                                double h, double* yOut, double& hSuggest, int& iters) {
     const double ZERO_EPS = 1.0e-50;
@@ -451,7 +512,7 @@ __device__ __forceinline__ bool alpha_qss_step(const ChemDev& cd, const double* 
     for (int trip = 0; trip <= cd.maxIters; ++trip) {
         const bool pred = (trip == 0);
         double q[NSP], L[NSP];
-        eval_split_rates<NSP, NRM>(cd, kf, kb, yp, q, L);  // yp = y0 on the predictor trip
+        eval_split_rates<NSP, NRM, DENSE>(cd, kf, kb, yp, q, L);  // yp = y0 on the predictor trip
         bool passes = true;
 #pragma unroll
         for (int s = 0; s < NSP; ++s) {
@@ -503,7 +564,7 @@ __device__ __forceinline__ bool alpha_qss_step(const ChemDev& cd, const double* 
 }
 
 // RKFStep.opCall, chemistry_update.d:385-484 (Cash-Karp tableau); yOut is always written.
-template <int NSP, int NRM>
+template <int NSP, int NRM, bool DENSE>
 __device__ __forceinline__ bool rkf_step(const ChemDev& cd, const double* kf, const double* kb, const double* y0, double h,
                          double* yOut, double& hSuggest) {
     const double a21 = 1. / 5., a31 = 3. / 40., a32 = 9. / 40., a41 = 3. / 10., a42 = -9. / 10., a43 = 6. / 5.,
@@ -513,7 +574,7 @@ __device__ __forceinline__ bool rkf_step(const ChemDev& cd, const double* kf, co
                  b43 = 18575. / 48384., b44 = 13525. / 55296., b45 = 277. / 14336., b46 = 1.0 / 4.0;
     double k1[NSP], k2[NSP], k3[NSP], k4[NSP], k5[NSP], k6[NSP], yT[NSP], q[NSP], L[NSP];
     auto rates = [&](const double* y, double* k) {
-        eval_split_rates<NSP, NRM>(cd, kf, kb, y, q, L);
+        eval_split_rates<NSP, NRM, DENSE>(cd, kf, kb, y, q, L);
 #pragma unroll
         for (int s = 0; s < NSP; ++s) k[s] = q[s] - L[s];
     };

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "chem_device.hpp"
 #include "chem_launch.hpp"
@@ -22,14 +23,14 @@ struct Chem {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     ChemTuning tuning;
     DevArr<double> kfkb, conc0, h0, hest, conc, Tend, uend;
-    DevArr<int> flag, st, nsub;
+    DevArr<int> flag, heavy, st, nsub;
     DevArr<unsigned> queue;
     DevArr<unsigned long long> stats;
     // Staging of the host-array entry points.
     DevArr<double> s_rho, s_T, s_u, s_p, s_massf, s_dtc, s_a;
     DevArr<int> s_status;
     // Last call: cells, its work buffers (the handle's own or a caller's), event stream.
-    int last_n = 0;
+    int last_n = 0, last_waves = 0;
     ChemWork last_w{};
     hipStream_t last_stream = nullptr;
     bool have_events = false;
@@ -49,13 +50,15 @@ struct Chem {
         h0.reserve(N);
         hest.reserve(N);
         flag.reserve(N);
+        heavy.reserve(N);
         st.reserve(N);
         nsub.reserve(N);
-        queue.reserve(8 * 32);
-        stats.reserve(4);
+        queue.reserve(QUEUE_COUNTERS * QUEUE_STRIDE);
+        stats.reserve(4 * (size_t)chem_stat_waves(n));
         Tend.reserve(host.tight ? N : 1);
         uend.reserve(host.tight ? N : 1);
-        return ChemWork{kfkb.p, conc0.p, h0.p, hest.p, conc.p, flag.p, st.p, nsub.p, queue.p, stats.p, Tend.p, uend.p};
+        return ChemWork{kfkb.p, conc0.p, h0.p, hest.p, conc.p, flag.p, heavy.p, st.p, nsub.p, queue.p, stats.p, Tend.p,
+                        uend.p};
     }
 
     // Bytes of the work buffers of one call over n cells when they are laid out in one
@@ -63,7 +66,8 @@ struct Chem {
     size_t work_bytes(int n) const {
         const size_t N = (size_t)std::max(n, 1);
         auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-        return up(4 * sizeof(unsigned long long)) + up(8 * 32 * sizeof(unsigned)) + 3 * up(N * sizeof(int)) +
+        return up(4 * (size_t)chem_stat_waves(n) * sizeof(unsigned long long)) +
+               up(QUEUE_COUNTERS * QUEUE_STRIDE * sizeof(unsigned)) + 4 * up(N * sizeof(int)) +
                (host.tight ? 4 : 2) * up(N * sizeof(double)) + 2 * up((size_t)nsp * N * sizeof(double)) +
                up(2 * (size_t)std::max(nr, 1) * N * sizeof(double));
     }
@@ -80,9 +84,10 @@ struct Chem {
             return r;
         };
         ChemWork w;
-        w.stats = reinterpret_cast<unsigned long long*>(take(4 * sizeof(unsigned long long)));
-        w.queue = reinterpret_cast<unsigned*>(take(8 * 32 * sizeof(unsigned)));
+        w.stats = reinterpret_cast<unsigned long long*>(take(4 * (size_t)chem_stat_waves(n) * sizeof(unsigned long long)));
+        w.queue = reinterpret_cast<unsigned*>(take(QUEUE_COUNTERS * QUEUE_STRIDE * sizeof(unsigned)));
         w.flag = reinterpret_cast<int*>(take(N * sizeof(int)));
+        w.heavy = reinterpret_cast<int*>(take(N * sizeof(int)));
         w.st = reinterpret_cast<int*>(take(N * sizeof(int)));
         w.nsub = reinterpret_cast<int*>(take(N * sizeof(int)));
         w.h0 = reinterpret_cast<double*>(take(N * sizeof(double)));
@@ -96,24 +101,33 @@ struct Chem {
     }
 
     // The three kernels on device arrays; outputs may alias inputs.  The work buffers are the
-    // handle's own (grown as needed) unless the caller brings them (wk).
+    // handle's own (grown as needed) unless the caller brings them (wk).  Cells with
+    // mask[i] != 0 (mask may be null) are neither read nor written: another path serves them.
     void step_device(int n, double dt, const double* rho, const double* T, const double* u, const double* p,
                      const double* massf, const double* dtc, double* T_out, double* u_out, double* p_out,
-                     double* massf_out, double* dtc_out, int* status, hipStream_t s, const ChemWork* wk = nullptr) {
+                     double* massf_out, double* dtc_out, int* status, hipStream_t s, const ChemWork* wk = nullptr,
+                     const int* mask = nullptr) {
         if (n < 0) throw Error(GDTK_ERR_ARG, "negative cell count");
         if (!(dt > 0.0)) throw Error(GDTK_ERR_ARG, "dt must be positive");
         ChemWork w = wk ? *wk : work(n);
+        // Below two cells per lane of a full chip (4 waves per SIMD) the one-lane-per-cell
+        // launch starts every cell at once; the queue pays off on larger batches.
+        ChemTuning tu = tuning;
+        if (tu.persistent < 0) tu.persistent = ((long long)n > 2048LL * std::max(num_cu, 1)) ? 1 : 0;
         HIPCHK(hipEventRecord(ev[0], s));
         HIPCHK(launch_chem_clear(w, s));
-        HIPCHK(launch_chem_setup(dev, nsp, nr, n, dt, rho, T, massf, dtc, w, s));
+        HIPCHK(launch_chem_setup(dev, nsp, nr, host.all_sorted, n, dt, rho, T, massf, dtc, mask,
+                                 tu.persistent ? tu.heavy_ratio : 0, w, s));
         HIPCHK(hipEventRecord(ev[1], s));
-        HIPCHK(launch_chem_integrate(dev, nsp, nr, host.method, host.tight, n, dt, rho, T, u, dtc, dtc_out, w, tuning,
-                                     num_cu, s));
+        int waves = 0;
+        HIPCHK(launch_chem_integrate(dev, nsp, nr, host.all_sorted, host.method, host.tight, n, dt, rho, T, u, dtc,
+                                     dtc_out, w, tu, num_cu, &waves, s));
         HIPCHK(hipEventRecord(ev[2], s));
         HIPCHK(launch_chem_finish(dev, nsp, n, rho, T, u, p, massf, dtc, T_out, u_out, p_out, massf_out, dtc_out,
                                   status, w, s));
         HIPCHK(hipEventRecord(ev[3], s));
         last_n = n;
+        last_waves = waves;
         last_w = w;
         last_stream = s;
         have_events = true;
@@ -125,7 +139,17 @@ struct Chem {
         if (!have_events) throw Error(GDTK_ERR_ARG, "no chemistry step has run");
         HIPCHK(hipEventSynchronize(ev[3]));
         unsigned long long h[4] = {0, 0, 0, 0};
-        HIPCHK(hipMemcpy(h, last_w.stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        if (last_waves > 0) {
+            std::vector<unsigned long long> part(4 * (size_t)last_waves);
+            HIPCHK(hipMemcpy(part.data(), last_w.stats, part.size() * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+            for (int k = 0; k < last_waves; ++k) {
+                h[0] += part[4 * k + 0];
+                h[1] += part[4 * k + 1];
+                h[2] = std::max(h[2], part[4 * k + 2]);
+                h[3] += part[4 * k + 3];
+            }
+        }
         float ms[3] = {0, 0, 0};
         for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
         const double v[8] = {(double)h[0], (double)h[1], (double)h[2], (double)h[3],

@@ -7,16 +7,15 @@
 
 namespace gdtk {
 
-__global__ void k_chem_clear(unsigned* queue, unsigned long long* stats) {
+__global__ void k_chem_clear(unsigned* queue) {
     int t = threadIdx.x;
-    if (t < 8) queue[t * QUEUE_STRIDE] = 0u;
-    if (t < 4) stats[t] = 0ull;
+    if (t < QUEUE_COUNTERS) queue[t * QUEUE_STRIDE] = 0u;
 }
 
 bool chem_supported(int nsp, int nr) { return nsp >= 1 && nsp <= CH_MAX_SP && nr >= 0 && nr <= CH_MAX_R; }
 
 hipError_t launch_chem_clear(const ChemWork& w, hipStream_t s) {
-    hipLaunchKernelGGL(k_chem_clear, dim3(1), dim3(64), 0, s, w.queue, w.stats);
+    hipLaunchKernelGGL(k_chem_clear, dim3(1), dim3(64), 0, s, w.queue);
     return hipGetLastError();
 }
 
@@ -35,25 +34,26 @@ static hipError_t run_nsp(int nsp, const ChemCall& c) {
     }
 }
 
-hipError_t launch_chem_setup(const ChemDev* cd, int nsp, int nr, int n, double dt, const double* rho,
-                             const double* T, const double* massf, const double* dtc, const ChemWork& w,
-                             hipStream_t s) {
+hipError_t launch_chem_setup(const ChemDev* cd, int nsp, int nr, int dense, int n, double dt, const double* rho,
+                             const double* T, const double* massf, const double* dtc, const int* mask,
+                             int heavy_ratio, const ChemWork& w, hipStream_t s) {
     ChemCall c{};
     c.phase = CHEM_PHASE_SETUP;
-    c.cd = cd; c.nr = nr; c.n = n; c.dt = dt;
-    c.rho = rho; c.T = T; c.massf = massf; c.dtc_in = dtc;
+    c.cd = cd; c.nr = nr; c.dense = dense; c.n = n; c.dt = dt;
+    c.rho = rho; c.T = T; c.massf = massf; c.dtc_in = dtc; c.mask = mask; c.tu.heavy_ratio = heavy_ratio;
     c.w = w; c.s = s;
     return run_nsp(nsp, c);
 }
 
-hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int method, int tight, int n, double dt,
+hipError_t launch_chem_integrate(const ChemDev* cd, int nsp, int nr, int dense, int method, int tight, int n, double dt,
                                  const double* rho, const double* T, const double* u, const double* dtc_in,
-                                 double* dtc_out, const ChemWork& w, const ChemTuning& tu, int num_cu, hipStream_t s) {
+                                 double* dtc_out, const ChemWork& w, const ChemTuning& tu, int num_cu, int* waves,
+                                 hipStream_t s) {
     ChemCall c{};
     c.phase = CHEM_PHASE_INTEGRATE;
-    c.cd = cd; c.nr = nr; c.method = method; c.tight = tight; c.n = n; c.dt = dt;
+    c.cd = cd; c.nr = nr; c.dense = dense; c.method = method; c.tight = tight; c.n = n; c.dt = dt;
     c.rho = rho; c.T = T; c.u = u; c.dtc_in = dtc_in; c.dtc_out = dtc_out;
-    c.w = w; c.tu = tu; c.num_cu = num_cu; c.s = s;
+    c.w = w; c.tu = tu; c.num_cu = num_cu; c.waves_out = waves; c.s = s;
     return run_nsp(nsp, c);
 }
 

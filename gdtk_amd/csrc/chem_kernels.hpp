@@ -24,14 +24,20 @@
 
 namespace gdtk {
 
-template <int NSP, int NRM>
+template <int NSP, int NRM, bool DENSE>
 __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_setup(const ChemDev* __restrict__ cdp, int n, double dt,
                                                            const double* __restrict__ rho,
                                                            const double* __restrict__ T,
                                                            const double* __restrict__ massf,
-                                                           const double* __restrict__ dtc, ChemWork w) {
+                                                           const double* __restrict__ dtc,
+                                                           const int* __restrict__ mask, int heavy_ratio,
+                                                           ChemWork w) {
     const int i = blockIdx.x * CHEM_BLOCK + threadIdx.x;
     if (i >= n) return;
+    if (mask && mask[i] != 0) {
+        w.flag[i] = CF_SKIP;
+        return;
+    }
     const ChemDev& cd = *cdp;
     const double Ti = T[i];
     // fvcell.d:870: frozen below T_frozen
@@ -65,14 +71,25 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_setup(const ChemDev* __rest
     // chemistry_update.d:152-158.  The estimate is also the first step of the retry with
     // dt_chem = -1 (fvcell.d:899-905), which starts from the same state.
     const double d = dtc[i];
-    const double est = estimate_step_size<NSP, NRM>(cd, kf, kb, conc);
+    const double est = estimate_step_size<NSP, NRM, DENSE>(cd, kf, kb, conc);
     double h;
     if (d > dt) h = dt;
     else if (d <= 0.0) h = est;
     else h = d;
     w.h0[i] = h;
     w.hest[i] = est;
-    w.flag[i] = CF_INTEGRATE;
+    // Cells whose integration is predicted long go to the heavy list, one atomic per wave.
+    const bool heavy = heavy_ratio > 0 && dt > (double)heavy_ratio * h;
+    w.flag[i] = heavy ? CF_HEAVY : CF_INTEGRATE;
+    const unsigned long long hb = __ballot(heavy);
+    if (hb != 0ull) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)hb) - 1;
+        unsigned base = 0u;
+        if (lane == leader) base = atomicAdd(&w.queue[9 * QUEUE_STRIDE], (unsigned)__popcll(hb));
+        base = __shfl(base, leader);
+        if (heavy) w.heavy[base + (unsigned)__popcll(hb & ((1ull << lane) - 1ull))] = i;
+    }
 }
 
 // Per-lane state of one cell inside the subcycle controller; T and u are the gas state's
@@ -99,9 +116,8 @@ __device__ __forceinline__ void load_cell(const ChemDev& cd, int n, int i, const
     }
 #pragma unroll
     for (int ir = 0; ir < NRM; ++ir) {
-        if (ir >= nr) continue;
-        c.kf[ir] = w.kfkb[(size_t)ir * n + i];
-        c.kb[ir] = w.kfkb[(size_t)(nr + ir) * n + i];
+        c.kf[ir] = (ir < nr) ? w.kfkb[(size_t)ir * n + i] : 0.0;
+        c.kb[ir] = (ir < nr) ? w.kfkb[(size_t)(nr + ir) * n + i] : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < NSP; ++s) c.c0[s] = w.conc0[(size_t)s * n + i];
@@ -122,7 +138,7 @@ __device__ __forceinline__ void load_cell(const ChemDev& cd, int n, int i, const
 // failed T(u) iteration there raises GasModelException in the reference, which the
 // thermochemical_increment of fvcell.d:894-905 does not catch: the cell fails without the
 // retry.
-template <int NSP, int NRM, int METHOD, bool TIGHT>
+template <int NSP, int NRM, int METHOD, bool TIGHT, bool DENSE>
 __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __restrict__ cdp, int n, double dt,
                                                                const double* __restrict__ rho,
                                                                const double* __restrict__ T_in,
@@ -140,12 +156,15 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
 
     bool exhausted = true;
     int w_next = 0, w_end = 0;
+    int from_heavy = 0;      // [w_next, w_end) indexes the heavy list (wave-uniform)
+    unsigned drained = 0u;   // lane 0: ranges seen empty (bits 0-7 per XCD, bit 8 heavy list)
     const int xcd = blockIdx.x & 7;
     if (persistent) {
         exhausted = false;
     } else {
         const int g = blockIdx.x * CHEM_BLOCK + threadIdx.x;
-        if (g < n && w.flag[g] == CF_INTEGRATE) load_cell<NSP, NRM, TIGHT>(cd, n, g, rho, T_in, u_in, dtc_in, w, c);
+        if (g < n && (w.flag[g] == CF_INTEGRATE || w.flag[g] == CF_HEAVY))
+            load_cell<NSP, NRM, TIGHT>(cd, n, g, rho, T_in, u_in, dtc_in, w, c);
     }
 
     for (int trip = 0;; ++trip) {
@@ -156,21 +175,46 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
                 unsigned long long need = __ballot(c.cell < 0);
                 while (need != 0ull) {
                     if (w_next >= w_end) {
-                        int base = -1, end = 0;
+                        int base = -1, end = 0, heavy = 0;
                         if (lane == 0) {
-                            for (int j = 0; j < 8; ++j) {
+                            // The heavy list first, then the own XCD's range, then the others'.
+                            // A list or range seen drained is not probed again; probing one
+                            // costs a load, grabbing from it a contended atomic.
+                            if (!(drained & 256u)) {
+                                const unsigned len = __hip_atomic_load(&w.queue[9 * QUEUE_STRIDE], __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_AGENT);
+                                const unsigned seen = __hip_atomic_load(&w.queue[8 * QUEUE_STRIDE], __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT);
+                                const unsigned got =
+                                    (seen < len) ? atomicAdd(&w.queue[8 * QUEUE_STRIDE], (unsigned)chunk) : len;
+                                if (got < len) {
+                                    base = (int)got;
+                                    end = (int)((got + (unsigned)chunk < len) ? got + (unsigned)chunk : len);
+                                    heavy = 1;
+                                } else {
+                                    drained |= 256u;
+                                }
+                            }
+                            for (int j = 0; j < 8 && base < 0; ++j) {
                                 const int q = (xcd + j) & 7;
+                                if (drained & (1u << q)) continue;
                                 const long long lo = (long long)n * q / 8, hi = (long long)n * (q + 1) / 8;
-                                const unsigned got = atomicAdd(&w.queue[q * QUEUE_STRIDE], (unsigned)chunk);
+                                const unsigned seen = __hip_atomic_load(&w.queue[q * QUEUE_STRIDE], __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT);
+                                const unsigned got = (lo + (long long)seen < hi)
+                                                         ? atomicAdd(&w.queue[q * QUEUE_STRIDE], (unsigned)chunk)
+                                                         : (unsigned)(hi - lo);
                                 if (lo + (long long)got < hi) {
                                     base = (int)(lo + got);
                                     end = (int)((lo + got + chunk < hi) ? lo + got + chunk : hi);
-                                    break;
+                                } else {
+                                    drained |= 1u << q;
                                 }
                             }
                         }
                         base = __shfl(base, 0);
                         end = __shfl(end, 0);
+                        from_heavy = __shfl(heavy, 0);
                         if (base < 0) {
                             exhausted = true;
                             break;
@@ -183,9 +227,14 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
                     const int take = (k < avail) ? k : avail;
                     const int rank = __popcll(need & (lane_bit - 1ull));
                     if ((need & lane_bit) && rank < take) {
-                        const int i = w_next + rank;
-                        if (w.flag[i] == CF_INTEGRATE)
+                        if (from_heavy) {
+                            const int i = w.heavy[w_next + rank];
                             load_cell<NSP, NRM, TIGHT>(cd, n, i, rho, T_in, u_in, dtc_in, w, c);
+                        } else {
+                            const int i = w_next + rank;
+                            if (w.flag[i] == CF_INTEGRATE)
+                                load_cell<NSP, NRM, TIGHT>(cd, n, i, rho, T_in, u_in, dtc_in, w, c);
+                        }
                     }
                     w_next += take;
                     need = __ballot(c.cell < 0);
@@ -204,9 +253,9 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
         double y[NSP];
         bool ok;
         if (METHOD == 1) {
-            ok = rkf_step<NSP, NRM>(cd, c.kf, c.kb, c.c0, c.h, y, c.dtSug);
+            ok = rkf_step<NSP, NRM, DENSE>(cd, c.kf, c.kb, c.c0, c.h, y, c.dtSug);
         } else {
-            ok = alpha_qss_step<NSP, NRM>(cd, c.kf, c.kb, c.c0, c.h, y, c.dtSug, c.iters);
+            ok = alpha_qss_step<NSP, NRM, DENSE>(cd, c.kf, c.kb, c.c0, c.h, y, c.dtSug, c.iters);
         }
         bool passes = true;
         if (ok) {
@@ -298,7 +347,8 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
             c.cell = -1;
         }
     }
-    // Wave totals of the statistics, one atomic per wave.
+    // Wave totals of the statistics, stored per wave (the host reduces them when asked):
+    // thousands of same-address atomics would serialise at the end of every launch.
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         acc_sub += __shfl_xor(acc_sub, off);
@@ -308,10 +358,11 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
         max_sub = (o > max_sub) ? o : max_sub;
     }
     if (lane == 0) {
-        atomicAdd(&w.stats[0], acc_sub);
-        atomicAdd(&w.stats[1], acc_it);
-        atomicMax(&w.stats[2], max_sub);
-        atomicAdd(&w.stats[3], acc_att);
+        const int wave = blockIdx.x * CHEM_WAVES_PER_BLOCK + (threadIdx.x >> 6);
+        w.stats[4 * wave + 0] = acc_sub;
+        w.stats[4 * wave + 1] = acc_it;
+        w.stats[4 * wave + 2] = max_sub;
+        w.stats[4 * wave + 3] = acc_att;
     }
 }
 
@@ -326,6 +377,7 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_finish(const ChemDev* __res
     if (i >= n) return;
     const ChemDev& cd = *cdp;
     const int f = w.flag[i];
+    if (f == CF_SKIP) return;
     double Ti = T[i], ui = u[i], pi = p[i];
     double mf[NSP];
 #pragma unroll
@@ -414,26 +466,27 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_tpg_update(const ChemDev* __rest
 static inline unsigned grid_for(int n) { return (unsigned)((n + CHEM_BLOCK - 1) / CHEM_BLOCK); }
 
 struct SetupL {
-    const ChemDev* cd; int n; double dt; const double *rho, *T, *massf, *dtc; ChemWork w; hipStream_t s;
-    template <int NSP, int NRM>
+    const ChemDev* cd; int n; double dt; const double *rho, *T, *massf, *dtc; const int* mask; int heavy_ratio;
+    ChemWork w; hipStream_t s;
+    template <int NSP, int NRM, bool DENSE>
     hipError_t run() {
-        hipLaunchKernelGGL((k_chem_setup<NSP, NRM>), dim3(grid_for(n)), dim3(CHEM_BLOCK), 0, s, cd, n, dt, rho, T,
-                           massf, dtc, w);
+        hipLaunchKernelGGL((k_chem_setup<NSP, NRM, DENSE>), dim3(grid_for(n)), dim3(CHEM_BLOCK), 0, s, cd, n, dt, rho,
+                           T, massf, dtc, mask, heavy_ratio, w);
         return hipGetLastError();
     }
 };
 
 struct IntegrateL {
     const ChemDev* cd; int n; double dt; const double* rho; const double *T, *u; const double* dtc_in;
-    double* dtc_out; ChemWork w; ChemTuning tu; int num_cu; int method; int tight; hipStream_t s;
-    template <int NSP, int NRM, int METHOD, bool TIGHT>
+    double* dtc_out; ChemWork w; ChemTuning tu; int num_cu; int method; int tight; int* waves_out; hipStream_t s;
+    template <int NSP, int NRM, int METHOD, bool TIGHT, bool DENSE>
     hipError_t launch() {
         unsigned grid = grid_for(n);
         if (tu.persistent) {
             int per_cu = tu.blocks_per_cu;
             if (per_cu <= 0) {
                 hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void*>(&k_chem_integrate<NSP, NRM, METHOD, TIGHT>), CHEM_BLOCK,
+                    &per_cu, reinterpret_cast<const void*>(&k_chem_integrate<NSP, NRM, METHOD, TIGHT, DENSE>), CHEM_BLOCK,
                     0);
                 if (e != hipSuccess) return e;
                 if (per_cu < 1) per_cu = 1;
@@ -442,15 +495,16 @@ struct IntegrateL {
             grid = pgrid < grid ? pgrid : grid;
             if (grid < 8) grid = 8;  // one workgroup per XCD range at least
         }
+        if (waves_out) *waves_out = (int)grid * CHEM_WAVES_PER_BLOCK;
         const int chunk = tu.chunk > 0 ? tu.chunk : 64;
-        hipLaunchKernelGGL((k_chem_integrate<NSP, NRM, METHOD, TIGHT>), dim3(grid), dim3(CHEM_BLOCK), 0, s, cd, n, dt,
-                           rho, T, u, dtc_in, dtc_out, w, tu.persistent, chunk, tu.refill_period);
+        hipLaunchKernelGGL((k_chem_integrate<NSP, NRM, METHOD, TIGHT, DENSE>), dim3(grid), dim3(CHEM_BLOCK), 0, s, cd,
+                           n, dt, rho, T, u, dtc_in, dtc_out, w, tu.persistent, chunk, tu.refill_period);
         return hipGetLastError();
     }
-    template <int NSP, int NRM>
+    template <int NSP, int NRM, bool DENSE>
     hipError_t run() {
-        if (method == 1) return launch<NSP, NRM, 1, false>();  // RKF with tight coupling is refused at set-up
-        return tight ? launch<NSP, NRM, 0, true>() : launch<NSP, NRM, 0, false>();
+        if (method == 1) return launch<NSP, NRM, 1, false, DENSE>();  // RKF with tight coupling is refused at set-up
+        return tight ? launch<NSP, NRM, 0, true, DENSE>() : launch<NSP, NRM, 0, false, DENSE>();
     }
 };
 
@@ -475,22 +529,26 @@ struct TpgL {
     }
 };
 
-// One phase of a chemistry call for NSP species; the reaction bound is 5 for the
-// 5-species, 5-reaction air mechanism, otherwise 8 or 32.
+// One phase of a chemistry call for NSP species.  Mechanisms in dense form get the
+// branch-free rate walk with the reaction bound 5 for the 5-species, 5-reaction air
+// mechanism, otherwise 8 or 32; the others the list walk with the bound 32.
 template <int NSP>
 hipError_t chem_run(const ChemCall& c) {
     switch (c.phase) {
     case CHEM_PHASE_SETUP: {
-        SetupL l{c.cd, c.n, c.dt, c.rho, c.T, c.massf, c.dtc_in, c.w, c.s};
-        if (NSP == 5 && c.nr == 5) return l.template run<NSP, 5>();
-        if (c.nr <= 8) return l.template run<NSP, 8>();
-        return l.template run<NSP, 32>();
+        SetupL l{c.cd, c.n, c.dt, c.rho, c.T, c.massf, c.dtc_in, c.mask, c.tu.heavy_ratio, c.w, c.s};
+        if (!c.dense) return l.template run<NSP, 32, false>();
+        if (NSP == 5 && c.nr == 5) return l.template run<NSP, 5, true>();
+        if (c.nr <= 8) return l.template run<NSP, 8, true>();
+        return l.template run<NSP, 32, true>();
     }
     case CHEM_PHASE_INTEGRATE: {
-        IntegrateL l{c.cd, c.n, c.dt, c.rho, c.T, c.u, c.dtc_in, c.dtc_out, c.w, c.tu, c.num_cu, c.method, c.tight, c.s};
-        if (NSP == 5 && c.nr == 5) return l.template run<NSP, 5>();
-        if (c.nr <= 8) return l.template run<NSP, 8>();
-        return l.template run<NSP, 32>();
+        IntegrateL l{c.cd,    c.n,  c.dt,     c.rho,  c.T,      c.u,        c.dtc_in, c.dtc_out,
+                     c.w,     c.tu, c.num_cu, c.method, c.tight, c.waves_out, c.s};
+        if (!c.dense) return l.template run<NSP, 32, false>();
+        if (NSP == 5 && c.nr == 5) return l.template run<NSP, 5, true>();
+        if (c.nr <= 8) return l.template run<NSP, 8, true>();
+        return l.template run<NSP, 32, true>();
     }
     case CHEM_PHASE_FINISH: {
         FinishL l{c.cd, c.n, c.rho, c.T, c.u, c.p, c.massf, c.dtc_in, c.T_out, c.u_out, c.p_out, c.massf_out,

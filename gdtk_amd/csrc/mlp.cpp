@@ -34,9 +34,9 @@ Mlp& get_mlp(int h) {
 }
 
 // BF16 inference on device SoA arrays (used by the surrogate step as well).
-void mlp_infer(Mlp& m, int n, const float* x, float* y, hipStream_t s) {
+void mlp_infer(Mlp& m, int n, const float* x, float* y, hipStream_t s, const unsigned* count) {
     m.pack(s);
-    HIPCHK(launch_mlp_infer(m.H, m.NH, x, m.n_in, n, m.frag.p, m.bias_cat.p, m.n_out, y, s));
+    HIPCHK(launch_mlp_infer(m.H, m.NH, x, m.n_in, n, m.frag.p, m.bias_cat.p, m.n_out, y, s, count));
 }
 
 }  // namespace gdtk

@@ -179,6 +179,8 @@ struct Mlp {
 
 Mlp& get_mlp(int h);
 // BF16 inference of n cells: x SoA [n_in][n] -> y SoA [n_out][n] (device arrays).
-void mlp_infer(Mlp& m, int n, const float* x, float* y, hipStream_t s);
+// Inference over x, y [n_in/n_out][n]; with count (device memory, may be null) only the
+// first *count cells are evaluated (the arrays keep the stride n).
+void mlp_infer(Mlp& m, int n, const float* x, float* y, hipStream_t s, const unsigned* count = nullptr);
 
 }  // namespace gdtk

@@ -119,10 +119,19 @@ hipError_t launch_mlp_pack(const float* W, int n_out, int n_in, int chain, unsig
 constexpr int INF_WAVES = 4, INF_CB = 2;
 constexpr int INF_CELLS = INF_WAVES * INF_CB * 16;
 
+// count (may be null): the number of leading cells to evaluate, read from device memory (at
+// most n, the arrays' stride); workgroups past it exit at once.
 template <int H, int NH>
 __global__ __launch_bounds__(256, 1) void k_mlp_infer(const float* __restrict__ x, int n_in, int n,
+                                                      const unsigned* __restrict__ count,
                                                       const unsigned short* __restrict__ frag,
                                                       const float* __restrict__ bias, int n_out, float* __restrict__ y) {
+    const int stride = n;
+    if (count) {
+        const unsigned c = *count;
+        n = (c < (unsigned)n) ? (int)c : n;
+        if ((int)(blockIdx.x * INF_CELLS) >= n) return;
+    }
     constexpr int NRB = H / 16, NKS = H / 32;
     constexpr int FRAG_PER_STEP = NRB * 64;                 // 16-byte fragments per k-step
     constexpr int LOADS = FRAG_PER_STEP / 256;               // per thread
@@ -146,7 +155,7 @@ __global__ __launch_bounds__(256, 1) void k_mlp_infer(const float* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = 8 * g + j;
-            xb[j] = (cell < n && k < n_in) ? f2bf(x[(size_t)k * n + cell]) : (unsigned short)0;
+            xb[j] = (cell < n && k < n_in) ? f2bf(x[(size_t)k * stride + cell]) : (unsigned short)0;
         }
         act[cb][0] = __builtin_bit_cast(bf16x8, xb);
     }
@@ -232,20 +241,22 @@ __global__ __launch_bounds__(256, 1) void k_mlp_infer(const float* __restrict__ 
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int row = 4 * g + i;
-                if (row < n_out) y[(size_t)row * n + cell] = o[i] + bl[row];
+                if (row < n_out) y[(size_t)row * stride + cell] = o[i] + bl[row];
             }
         }
     }
 }
 
 hipError_t launch_mlp_infer(int H, int NH, const float* x, int n_in, int n, const unsigned short* frag,
-                            const float* bias, int n_out, float* y, hipStream_t s) {
+                            const float* bias, int n_out, float* y, hipStream_t s, const unsigned* count) {
     if (n <= 0) return hipSuccess;
     const unsigned grid = (unsigned)((n + INF_CELLS - 1) / INF_CELLS);
     if (H == 256 && NH == 3) {
-        hipLaunchKernelGGL((k_mlp_infer<256, 3>), dim3(grid), dim3(256), 0, s, x, n_in, n, frag, bias, n_out, y);
+        hipLaunchKernelGGL((k_mlp_infer<256, 3>), dim3(grid), dim3(256), 0, s, x, n_in, n, count, frag, bias, n_out,
+                           y);
     } else if (H == 64 && NH == 2) {
-        hipLaunchKernelGGL((k_mlp_infer<64, 2>), dim3(grid), dim3(256), 0, s, x, n_in, n, frag, bias, n_out, y);
+        hipLaunchKernelGGL((k_mlp_infer<64, 2>), dim3(grid), dim3(256), 0, s, x, n_in, n, count, frag, bias, n_out,
+                           y);
     } else {
         return hipErrorNotSupported;
     }

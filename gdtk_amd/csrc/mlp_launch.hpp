@@ -9,7 +9,7 @@ namespace gdtk {
 hipError_t launch_mfma_probe(const float* A, const float* B, float* D, hipStream_t s);
 hipError_t launch_mlp_pack(const float* W, int n_out, int n_in, int chain, unsigned short* frag, hipStream_t s);
 hipError_t launch_mlp_infer(int H, int NH, const float* x, int n_in, int n, const unsigned short* frag,
-                            const float* bias, int n_out, float* y, hipStream_t s);
+                            const float* bias, int n_out, float* y, hipStream_t s, const unsigned* count = nullptr);
 hipError_t launch_gemm_f32(int M, int N, int K, const float* A, int sam, int sak, const float* B, int sbk, int sbn,
                            float* C, int ldc, int accumulate, hipStream_t s);
 hipError_t launch_bias_act(int rows, int cols, const float* bias, float* Z, float* A, int act, int bf16,

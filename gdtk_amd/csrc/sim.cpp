@@ -93,8 +93,12 @@ struct Sim {
     DevArr<int64_t> coff;
     DevArr<unsigned> cfail;
     double chem_ms = 0.0;
-    long long chem_calls = 0, surr_queued = 0;
+    long long chem_calls = 0;
     hipEvent_t cev0 = nullptr, cev1 = nullptr;
+    // The failed-cell count of the last chemistry step, copied to pinned host memory behind
+    // the step and checked at the next point where the host waits for the stream anyway.
+    unsigned* cfail_host = nullptr;
+    bool chem_pending = false;
     std::vector<std::unique_ptr<Block>> blocks;
     hipStream_t stream = nullptr;
     // Which flow-state copy is the input of the next stage, and which U copy holds U[0]
@@ -161,6 +165,7 @@ struct Sim {
         drop_graphs();
         if (cev0) (void)hipEventDestroy(cev0);
         if (cev1) (void)hipEventDestroy(cev1);
+        if (cfail_host) (void)hipHostFree(cfail_host);
         blocks.clear();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -805,6 +810,16 @@ struct Sim {
     void get_field(int bid, int field, double* out) {
         Block& b = blk(bid);
         BlockDev& d = b.dev;
+        if (field == GDTK_FIELD_DT_CHEM) {
+            // the cell's dt_chem (fvcell.d:78), kept with the chemistry's cell arrays
+            if (!ncell_all) throw Error(GDTK_ERR_ARG, "no chemistry step has run");
+            check_chem();
+            int64_t off = 0;
+            for (int q = 0; q < bid; ++q) off += blocks[q]->ncell();
+            HIPCHK(hipStreamSynchronize(stream));
+            HIPCHK(hipMemcpy(out, dtc.p + off, (size_t)b.ncell() * sizeof(double), hipMemcpyDeviceToHost));
+            return;
+        }
         const size_t np = (size_t)d.npad;
         const double* src = nullptr;
         if (field >= GDTK_FIELD_DUDT0 && field < GDTK_FIELD_DUDT0 + b.ncq) src = b.dUdt0.p + (field - GDTK_FIELD_DUDT0) * np;
@@ -1026,6 +1041,8 @@ struct Sim {
         if (!cev0) {
             HIPCHK(hipEventCreate(&cev0));
             HIPCHK(hipEventCreate(&cev1));
+            HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&cfail_host), sizeof(unsigned)));
+            *cfail_host = 0;
         }
     }
 
@@ -1062,7 +1079,6 @@ struct Sim {
         if (surr_h >= 0) {
             rc = gdtk_hip_surrogate_step_device(surr_h, (int)n, rho, T, u, p, mf, dtc.p, dt, To, uo, po, mfo, dto,
                                                 cstatus.p, nullptr, stream);
-            if (rc >= 0) surr_queued += rc;
         } else {
             rc = gdtk_hip_chem_step_device(chem_h, (int)n, rho, T, u, p, mf, dtc.p, dt, To, uo, po, mfo, dto,
                                            cstatus.p, stream);
@@ -1071,15 +1087,23 @@ struct Sim {
         HIPCHK(launch_clear_u32(cfail.p, 1, stream));
         HIPCHK(launch_cells_scatter(t, cstart.p, coff.p, nb, ctotal, nsp, n, To, uo, po, mfo, dto, dtc.p, cstatus.p,
                                     cfail.p, stream));
+        HIPCHK(hipMemcpyAsync(cfail_host, cfail.p, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipEventRecord(cev1, stream));
-        unsigned nf = 0;
-        HIPCHK(hipMemcpyAsync(&nf, cfail.p, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
+        chem_pending = true;
+    }
+
+    // The outcome of the last chemistry step: its time, and the reference's fatal
+    // FlowSolverException (fvcell.d:923) if a cell failed.  Called where the host has waited
+    // for the stream anyway (after the next step's flux attempt) and before a call returns.
+    void check_chem() {
+        if (!chem_pending) return;
+        chem_pending = false;
+        HIPCHK(hipEventSynchronize(cev1));
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, cev0, cev1));
         chem_ms += ms;
         ++chem_calls;
-        if (nf > 0) throw Error(GDTK_ERR_STEP_FAILED, "The thermochemical_increment() failed.");
+        if (*cfail_host > 0) throw Error(GDTK_ERR_STEP_FAILED, "The thermochemical_increment() failed.");
     }
 
     // Restart of a failed attempt from U[0]: the flow state is restored by decoding U[0]
@@ -1167,6 +1191,7 @@ struct Sim {
             }
         } while (failed && attempt < cfg.max_attempts_for_step);
         attempts_last = attempt;
+        check_chem();  // the previous step's chemistry, long done: the attempt has waited
         if (failed) throw Error(GDTK_ERR_STEP_FAILED, "Explicit update failed after max attempts; giving up.");
         // swap(U[0], U[end]) (simcore_gasdynamic_step.d:1557-1562).  The two stages leave
         // the flow state in the copy the step started from.
@@ -1186,6 +1211,7 @@ struct Sim {
             if (time >= max_time) break;
             if (taken >= max_steps) break;
         }
+        check_chem();
         return taken;
     }
 
@@ -1289,7 +1315,13 @@ int gdtk_hip_sim_set_chemistry(int sim, int chem, int surrogate) {
 int gdtk_hip_sim_chem_stats(int sim, double* stats, int nstats) {
     return guarded([&] {
         Sim& s = get_sim(sim);
-        const double v[3] = {(double)s.chem_calls, s.chem_ms, (double)s.surr_queued};
+        s.check_chem();
+        double queued = 0.0;
+        if (s.surr_h >= 0 && s.chem_calls > 0) {
+            double sv[2];
+            if (gdtk_hip_surrogate_stats(s.surr_h, sv, 2) == 2) queued = sv[1];
+        }
+        const double v[3] = {(double)s.chem_calls, s.chem_ms, queued};
         int m = std::min(nstats, 3);
         for (int k = 0; k < m; ++k) stats[k] = v[k];
         return m;
@@ -1346,6 +1378,7 @@ int gdtk_hip_sim_step_fixed_dt(int sim, double dt, double* state) {
         sp->gasdynamic_step();
         sp->chemistry_step(sp->dt_global);
         sp->step += 1;
+        sp->check_chem();
         return GDTK_OK;
     });
     if (sp) sp->state(state);

@@ -29,12 +29,10 @@ struct Surrogate {
     SurrCfg cfg{};
     DevArr<float> x, ya, yb;
     DevArr<double> massf_new, disagree;
-    DevArr<int> path, fb_idx;
-    DevArr<unsigned> fb_count;
-    DevArr<double> c_rho, c_T, c_u, c_p, c_massf, c_dtc;
-    DevArr<int> c_st;
+    DevArr<int> path, taken, net_idx;
+    DevArr<unsigned> counts;  // {network cells, cells sent back to the integrator}
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    int last_n = 0, last_m = 0, last_frozen = 0;
+    int last_n = 0;
     bool have = false;
 
     ~Surrogate() {
@@ -42,6 +40,9 @@ struct Surrogate {
             if (e) (void)hipEventDestroy(e);
     }
 
+    // Everything stays on the device: the network cells are compacted by an atomic counter
+    // that the kernels read, and the integrator runs over all cells with the mask of the
+    // cells the networks answered.  Nothing is read back to the host.
     void step(int n, double dt, const double* rho, const double* T, const double* u, const double* p,
               const double* massf, const double* dtc, double* T_out, double* u_out, double* p_out, double* massf_out,
               double* dtc_out, int* status, int* path_out, hipStream_t s) {
@@ -57,62 +58,48 @@ struct Surrogate {
         massf_new.reserve((size_t)nsp * N);
         disagree.reserve(N);
         path.reserve(N);
-        fb_idx.reserve(N);
-        fb_count.reserve(1);
+        taken.reserve(N);
+        net_idx.reserve(N);
+        counts.reserve(2);
         int* pth = path_out ? path_out : path.p;
         HIPCHK(hipEventRecord(ev[0], s));
-        HIPCHK(launch_surr_clear(fb_count.p, s));
-        HIPCHK(launch_surr_features(nsp, n, rho, T, massf, dt, cfg, x.p, s));
-        mlp_infer(ma, n, x.p, ya.p, s);
-        mlp_infer(mb, n, x.p, yb.p, s);
+        HIPCHK(launch_surr_clear(counts.p, 2, s));
+        HIPCHK(launch_surr_route(nsp, n, T, dtc, dt, cfg, pth, taken.p, net_idx.p, counts.p, s));
+        HIPCHK(launch_surr_features(nsp, n, net_idx.p, counts.p, rho, T, massf, dt, cfg, x.p, s));
+        mlp_infer(ma, n, x.p, ya.p, s, counts.p);
+        mlp_infer(mb, n, x.p, yb.p, s, counts.p);
         HIPCHK(hipEventRecord(ev[1], s));
-        HIPCHK(launch_surr_apply(nsp, n, T, x.p, ya.p, yb.p, massf, cfg, massf_new.p, pth, fb_idx.p, fb_count.p,
-                                 disagree.p, s));
-        HIPCHK(launch_surr_finish(nsp, n, rho, T, u, p, massf, dtc, massf_new.p, ch.dev, T_out, u_out, p_out, massf_out,
-                                  dtc_out, status, pth, fb_idx.p, fb_count.p, s));
+        HIPCHK(launch_surr_apply(nsp, n, net_idx.p, counts.p, x.p, ya.p, yb.p, massf, cfg, massf_new.p, pth, taken.p,
+                                 counts.p + 1, disagree.p, s));
+        HIPCHK(launch_surr_finish(nsp, n, net_idx.p, counts.p, rho, T, u, p, massf, dtc, massf_new.p, ch.dev, T_out,
+                                  u_out, p_out, massf_out, dtc_out, status, pth, s));
         HIPCHK(hipEventRecord(ev[2], s));
-        unsigned m = 0;
-        HIPCHK(hipMemcpyAsync(&m, fb_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (m > (unsigned)n) throw Error(GDTK_ERR_HIP, "fallback queue overflow");
-        if (m > 0) {
-            const size_t M = m;
-            c_rho.reserve(M);
-            c_T.reserve(M);
-            c_u.reserve(M);
-            c_p.reserve(M);
-            c_dtc.reserve(M);
-            c_massf.reserve((size_t)nsp * M);
-            c_st.reserve(M);
-            HIPCHK(launch_surr_gather(nsp, (int)m, n, fb_idx.p, rho, T, u, p, massf, dtc, c_rho.p, c_T.p, c_u.p, c_p.p,
-                                      c_massf.p, c_dtc.p, s));
-            ch.step_device((int)m, dt, c_rho.p, c_T.p, c_u.p, c_p.p, c_massf.p, c_dtc.p, c_T.p, c_u.p, c_p.p, c_massf.p,
-                           c_dtc.p, c_st.p, s);
-            HIPCHK(launch_surr_scatter(nsp, (int)m, n, fb_idx.p, c_T.p, c_u.p, c_p.p, c_massf.p, c_dtc.p, c_st.p, T_out,
-                                       u_out, p_out, massf_out, dtc_out, status, s));
-        }
+        ch.step_device(n, dt, rho, T, u, p, massf, dtc, T_out, u_out, p_out, massf_out, dtc_out, status, s, nullptr,
+                       taken.p);
         HIPCHK(hipEventRecord(ev[3], s));
         last_n = n;
-        last_m = (int)m;
         have = true;
     }
 
-    // {cells, cells queued for the integrator, ms features + networks, ms projection + T(u),
-    //  ms integrator (gather, alpha-QSS, scatter), max network disagreement}
+    // {cells, network cells sent back to the integrator, ms routing + features + networks,
+    //  ms projection + T(u), ms integrator, max network disagreement, network cells} of the
+    //  last step (waits for it).
     int stats(double* out, int nout) {
         if (!have) throw Error(GDTK_ERR_ARG, "no surrogate step has run");
         HIPCHK(hipEventSynchronize(ev[3]));
         float ms[3] = {0, 0, 0};
         for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+        unsigned c[2] = {0, 0};
+        HIPCHK(hipMemcpy(c, counts.p, sizeof(c), hipMemcpyDeviceToHost));
         double dmax = 0.0;
-        if (last_n > 0) {
-            std::vector<double> d((size_t)last_n);
+        if (c[0]) {
+            std::vector<double> d((size_t)c[0]);
             HIPCHK(hipMemcpy(d.data(), disagree.p, d.size() * sizeof(double), hipMemcpyDeviceToHost));
             for (double v : d)
                 if (v > dmax) dmax = v;
         }
-        const double v[6] = {(double)last_n, (double)last_m, ms[0], ms[1], ms[2], dmax};
-        const int m = std::min(nout, 6);
+        const double v[7] = {(double)last_n, (double)c[1], ms[0], ms[1], ms[2], dmax, (double)c[0]};
+        const int m = std::min(nout, 7);
         for (int k = 0; k < m; ++k) out[k] = v[k];
         return m;
     }
@@ -160,6 +147,8 @@ int gdtk_hip_surrogate_create(int chem, int net_a, int net_b, const gdtk_surroga
         s->cfg.n_elem = cfg->n_elem;
         for (int e = 0; e < SURR_MAX_ELEM; ++e)
             for (int k = 0; k < CH_MAX_SP; ++k) s->cfg.elem[e][k] = cfg->elem[e][k];
+        s->cfg.route_ratio = cfg->route_ratio;
+        s->cfg.route_T = cfg->route_T;
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
         std::lock_guard<std::mutex> lk(g_surr_mu);
         g_surrs.push_back(std::move(s));
@@ -189,7 +178,7 @@ int gdtk_hip_surrogate_step_device(int h, int n, const double* rho, const double
             throw Error(GDTK_ERR_ARG, "null array");
         s.step(n, dt, rho, T, u, p, massf, dt_chem, T_out, u_out, p_out, massf_out, dt_chem_out, status, path,
                static_cast<hipStream_t>(stream));
-        return s.last_m;
+        return GDTK_OK;
     });
 }
 

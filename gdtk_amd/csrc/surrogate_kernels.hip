@@ -1,35 +1,23 @@
 // surrogate_kernels.hip -- the learned chemistry step around the BF16 network (no
 // counterpart in the reference).
 //
-// Per cell: features (log T, log rho, Y[nsp], log dt, standardised) -> two networks ->
-// mean mass-fraction change -> exact element-conservation projection with an active set that
-// keeps every mass fraction >= 0 -> T, u, p by the gas model's Newton T(u) (as the
-// reference's thermochemical_increment ends, fvcell.d:906-925).  Cells whose features lie
-// outside the trained range, whose two networks disagree, or whose projection cannot be
-// made feasible are queued (compacted by an atomic counter) for the alpha-QSS integrator.
+// Routing: a cell at or below T_frozen is the chemistry step's (left unchanged); of the others,
+// the cells whose integration would be long (the interval spans more than route_ratio of the
+// cell's last suggested substep, or, with no suggestion yet, T above route_T) go to the
+// networks, compacted into a list by an atomic counter, and the rest to the alpha-QSS
+// integrator.  Per network cell: features (log T, log rho, Y[nsp], log dt, standardised) ->
+// two networks -> mean mass-fraction change -> exact element-conservation projection with an
+// active set that keeps every mass fraction >= 0 -> T, u, p by the gas model's Newton T(u)
+// (as the reference's thermochemical_increment ends, fvcell.d:906-925).  A network cell whose
+// features lie outside the trained range, whose two networks disagree, or whose projection
+// cannot be made feasible goes back to the integrator.  The integrator then runs over all
+// cells with a mask of the cells the networks answered, so no count returns to the host.
 #include <hip/hip_runtime.h>
 
 #include "chem_device.hpp"
 #include "surrogate_launch.hpp"
 
 namespace gdtk {
-
-template <int NSP>
-__global__ __launch_bounds__(256) void k_surr_features(int n, const double* __restrict__ rho,
-                                                       const double* __restrict__ T,
-                                                       const double* __restrict__ massf, double dt, SurrCfg cfg,
-                                                       float* __restrict__ x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double f[NSP + 3];
-    f[0] = log(T[i]);
-    f[1] = log(rho[i]);
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) f[2 + s] = massf[(size_t)s * n + i];
-    f[NSP + 2] = log(dt);
-#pragma unroll
-    for (int k = 0; k < NSP + 3; ++k) x[(size_t)k * n + i] = (float)((f[k] - cfg.x_mean[k]) * cfg.x_inv_std[k]);
-}
 
 // Active-set least-squares projection: the change d closest to the network's prediction
 // d0 with E d = 0 (every element's mass conserved; the rows of E sum to the species' unit
@@ -124,82 +112,117 @@ __device__ __forceinline__ bool project(const SurrCfg& cfg, const double* Y0, co
     return false;
 }
 
-// path: 0 surrogate, 1 queued for the integrator, 2 frozen (T <= T_frozen: left unchanged,
-// fvcell.d:870), 3 failed (set by k_surr_finish).
+// path: 0 network, 1 integrator, 2 frozen (T <= T_frozen: the integrator's call leaves it
+// unchanged, fvcell.d:870), 3 failed (network composition without a temperature).  taken:
+// 1 where the networks answer the cell (the integrator's mask).
 template <int NSP>
-__global__ __launch_bounds__(256) void k_surr_apply(int n, const double* __restrict__ T, const float* __restrict__ x,
-                                                    const float* __restrict__ ya, const float* __restrict__ yb,
-                                                    const double* __restrict__ massf, SurrCfg cfg,
-                                                    double* __restrict__ massf_new, int* __restrict__ path,
-                                                    int* __restrict__ fb_idx, unsigned* __restrict__ fb_count,
-                                                    double* __restrict__ disagree) {
+__global__ __launch_bounds__(256) void k_surr_route(int n, const double* __restrict__ T,
+                                                    const double* __restrict__ dtc, double dt, SurrCfg cfg,
+                                                    int* __restrict__ path, int* __restrict__ taken,
+                                                    int* __restrict__ net_idx, unsigned* __restrict__ net_count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (T[i] <= cfg.T_frozen) {
+    const double Ti = T[i];
+    if (Ti <= cfg.T_frozen) {
         path[i] = 2;
-        disagree[i] = 0.0;
+        taken[i] = 0;
         return;
     }
+    const double d = dtc[i];
+    const bool stiff = (cfg.route_ratio <= 0.0) || ((d > 0.0) ? (dt > cfg.route_ratio * d) : (Ti > cfg.route_T));
+    if (!stiff) {
+        path[i] = 1;
+        taken[i] = 0;
+        return;
+    }
+    path[i] = 0;
+    taken[i] = 1;
+    net_idx[atomicAdd(net_count, 1u)] = i;
+}
+
+// Features of the listed cells, x [nsp + 3][n] in list order (entries past the count unused).
+template <int NSP>
+__global__ __launch_bounds__(256) void k_surr_features(int n, const int* __restrict__ net_idx,
+                                                       const unsigned* __restrict__ net_count,
+                                                       const double* __restrict__ rho,
+                                                       const double* __restrict__ T,
+                                                       const double* __restrict__ massf, double dt, SurrCfg cfg,
+                                                       float* __restrict__ x) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int)*net_count) return;
+    const int i = net_idx[j];
+    double f[NSP + 3];
+    f[0] = log(T[i]);
+    f[1] = log(rho[i]);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) f[2 + s] = massf[(size_t)s * n + i];
+    f[NSP + 2] = log(dt);
+#pragma unroll
+    for (int k = 0; k < NSP + 3; ++k) x[(size_t)k * n + j] = (float)((f[k] - cfg.x_mean[k]) * cfg.x_inv_std[k]);
+}
+
+// Network cells (list order j): the trained-range and agreement tests, the projection; a
+// cell that fails one goes back to the integrator (path 1, taken 0).
+template <int NSP>
+__global__ __launch_bounds__(256) void k_surr_apply(int n, const int* __restrict__ net_idx,
+                                                    const unsigned* __restrict__ net_count,
+                                                    const float* __restrict__ x, const float* __restrict__ ya,
+                                                    const float* __restrict__ yb, const double* __restrict__ massf,
+                                                    SurrCfg cfg, double* __restrict__ massf_new, int* __restrict__ path,
+                                                    int* __restrict__ taken, unsigned* __restrict__ fb_count,
+                                                    double* __restrict__ disagree) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int)*net_count) return;
+    const int i = net_idx[j];
     bool fallback = false;
 #pragma unroll
     for (int k = 0; k < NSP + 3; ++k)
-        if (!(fabsf(x[(size_t)k * n + i]) <= (float)cfg.x_abs_max)) fallback = true;
+        if (!(fabsf(x[(size_t)k * n + j]) <= (float)cfg.x_abs_max)) fallback = true;
     double Y0[NSP], d0[NSP], d[NSP];
     double dis = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
         Y0[s] = massf[(size_t)s * n + i];
-        const double a = ya[(size_t)s * n + i], b = yb[(size_t)s * n + i];
+        const double a = ya[(size_t)s * n + j], b = yb[(size_t)s * n + j];
         d0[s] = 0.5 * (a + b) * cfg.y_scale[s];
         dis = fmax(dis, fabs(a - b) * cfg.y_scale[s]);
     }
-    disagree[i] = dis;
+    disagree[j] = dis;
     if (!(dis <= cfg.disagree_tol)) fallback = true;
     if (!fallback && !project<NSP>(cfg, Y0, d0, d)) fallback = true;
-    if (!fallback) {
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) massf_new[(size_t)s * n + i] = Y0[s] + d[s];
+    if (fallback) {
+        path[i] = 1;
+        taken[i] = 0;
+        atomicAdd(fb_count, 1u);
+        return;
     }
-    path[i] = fallback ? 1 : 0;
-    if (fallback) fb_idx[atomicAdd(fb_count, 1u)] = i;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) massf_new[(size_t)s * n + j] = Y0[s] + d[s];
 }
 
-// T, u, p of the surrogate cells by Newton T(u) at the new composition (therm_perf_gas.d).
+// T, u, p of the network cells by Newton T(u) at the new composition (therm_perf_gas.d).
 // A composition that admits no temperature is the reference's failure mode of
 // thermochemical_increment (update_thermo_from_rhou throws after the chemistry, fvcell.d:
 // 906-925): the cell is reported failed (path 3, status -1, state unchanged, dt_chem -1)
 // without running the integrator on it.
 template <int NSP>
-__global__ __launch_bounds__(256) void k_surr_finish(int n, const double* __restrict__ rho, const double* T,
+__global__ __launch_bounds__(256) void k_surr_finish(int n, const int* __restrict__ net_idx,
+                                                     const unsigned* __restrict__ net_count,
+                                                     const double* __restrict__ rho, const double* T,
                                                      const double* u, const double* p_in, const double* massf_in,
-                                                     const double* dtc_in,
-                                                     const double* __restrict__ massf_new, const ChemDev* __restrict__ cd,
-                                                     double* T_out, double* u_out, double* p_out, double* massf_out,
-                                                     double* dtc_out, int* status, int* __restrict__ path,
-                                                     int* __restrict__ fb_idx, unsigned* __restrict__ fb_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int pth = path[i];
-    if (pth == 1) return;
+                                                     const double* dtc_in, const double* __restrict__ massf_new,
+                                                     const ChemDev* __restrict__ cd, double* T_out, double* u_out,
+                                                     double* p_out, double* massf_out, double* dtc_out, int* status,
+                                                     int* __restrict__ path) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int)*net_count) return;
+    const int i = net_idx[j];
+    if (path[i] != 0) return;
     double mf[NSP];
     double Tn = T[i], un = u[i];
     const double dc = dtc_in[i];
-    if (pth == 2) {
-        // frozen: the state is passed through (outputs may be the input arrays)
-        const double pv = p_in[i];
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) mf[s] = massf_in[(size_t)s * n + i];
-        T_out[i] = Tn;
-        u_out[i] = un;
-        p_out[i] = pv;
-        dtc_out[i] = dc;
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) massf_out[(size_t)s * n + i] = mf[s];
-        status[i] = 0;
-        return;
-    }
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) mf[s] = massf_new[(size_t)s * n + i];
+    for (int s = 0; s < NSP; ++s) mf[s] = massf_new[(size_t)s * n + j];
     if (!tpg_update_temperature<NSP>(*cd, mf, Tn, un)) {
         path[i] = 3;
         const double pv = p_in[i];
@@ -245,43 +268,8 @@ __global__ __launch_bounds__(256) void k_surr_project(int n, const double* __res
     ok[i] = good ? 1 : 0;
 }
 
-// Gather / scatter of the queued cells: compact arrays of m cells <-> the full arrays.
-template <int NSP>
-__global__ void k_surr_gather(int m, int n, const int* __restrict__ idx, const double* __restrict__ rho,
-                              const double* __restrict__ T, const double* __restrict__ u, const double* __restrict__ p,
-                              const double* __restrict__ massf, const double* __restrict__ dtc, double* crho, double* cT,
-                              double* cu, double* cp, double* cmassf, double* cdtc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    const int i = idx[k];
-    crho[k] = rho[i];
-    cT[k] = T[i];
-    cu[k] = u[i];
-    cp[k] = p[i];
-    cdtc[k] = dtc[i];
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) cmassf[(size_t)s * m + k] = massf[(size_t)s * n + i];
-}
-
-template <int NSP>
-__global__ void k_surr_scatter(int m, int n, const int* __restrict__ idx, const double* cT, const double* cu,
-                               const double* cp, const double* cmassf, const double* cdtc, const int* cst,
-                               double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
-                               int* status) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    const int i = idx[k];
-    T_out[i] = cT[k];
-    u_out[i] = cu[k];
-    p_out[i] = cp[k];
-    dtc_out[i] = cdtc[k];
-    status[i] = cst[k];
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) massf_out[(size_t)s * n + i] = cmassf[(size_t)s * m + k];
-}
-
-__global__ void k_surr_clear(unsigned* c) {
-    if (threadIdx.x == 0) c[0] = 0u;
+__global__ void k_surr_clear(unsigned* c, int k) {
+    if ((int)threadIdx.x < k) c[threadIdx.x] = 0u;
 }
 
 static inline unsigned g256(int n) { return (unsigned)((n + 255) / 256); }
@@ -299,66 +287,55 @@ static inline unsigned g256(int n) { return (unsigned)((n + 255) / 256); }
     default: return hipErrorInvalidValue; \
     }
 
-hipError_t launch_surr_clear(unsigned* c, hipStream_t s) {
-    hipLaunchKernelGGL(k_surr_clear, dim3(1), dim3(64), 0, s, c);
+hipError_t launch_surr_clear(unsigned* c, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_surr_clear, dim3(1), dim3(64), 0, s, c, k);
     return hipGetLastError();
 }
 
-hipError_t launch_surr_features(int nsp, int n, const double* rho, const double* T, const double* massf, double dt,
-                                const SurrCfg& cfg, float* x, hipStream_t s) {
+hipError_t launch_surr_route(int nsp, int n, const double* T, const double* dtc, double dt, const SurrCfg& cfg,
+                             int* path, int* taken, int* net_idx, unsigned* net_count, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-#define F(N)                                                                                                    \
-    hipLaunchKernelGGL((k_surr_features<N>), dim3(g256(n)), dim3(256), 0, s, n, rho, T, massf, dt, cfg, x); \
-    return hipGetLastError();
-    GDTK_SURR_NSP(F)
-#undef F
-}
-
-hipError_t launch_surr_apply(int nsp, int n, const double* T, const float* x, const float* ya, const float* yb,
-                             const double* massf, const SurrCfg& cfg, double* massf_new, int* path, int* fb_idx,
-                             unsigned* fb_count, double* disagree, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-#define F(N)                                                                                                            \
-    hipLaunchKernelGGL((k_surr_apply<N>), dim3(g256(n)), dim3(256), 0, s, n, T, x, ya, yb, massf, cfg, massf_new, path, \
-                       fb_idx, fb_count, disagree);                                                                     \
-    return hipGetLastError();
-    GDTK_SURR_NSP(F)
-#undef F
-}
-
-hipError_t launch_surr_finish(int nsp, int n, const double* rho, const double* T, const double* u, const double* p,
-                              const double* massf, const double* dtc, const double* massf_new, const ChemDev* cd,
-                              double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
-                              int* status, int* path, int* fb_idx, unsigned* fb_count, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-#define F(N)                                                                                                           \
-    hipLaunchKernelGGL((k_surr_finish<N>), dim3(g256(n)), dim3(256), 0, s, n, rho, T, u, p, massf, dtc, massf_new, cd, \
-                       T_out, u_out, p_out, massf_out, dtc_out, status, path, fb_idx, fb_count);                       \
-    return hipGetLastError();
-    GDTK_SURR_NSP(F)
-#undef F
-}
-
-hipError_t launch_surr_gather(int nsp, int m, int n, const int* idx, const double* rho, const double* T,
-                              const double* u, const double* p, const double* massf, const double* dtc, double* crho,
-                              double* cT, double* cu, double* cp, double* cmassf, double* cdtc, hipStream_t s) {
-    if (m <= 0) return hipSuccess;
-#define F(N)                                                                                                        \
-    hipLaunchKernelGGL((k_surr_gather<N>), dim3(g256(m)), dim3(256), 0, s, m, n, idx, rho, T, u, p, massf, dtc, crho, \
-                       cT, cu, cp, cmassf, cdtc);                                                                   \
-    return hipGetLastError();
-    GDTK_SURR_NSP(F)
-#undef F
-}
-
-hipError_t launch_surr_scatter(int nsp, int m, int n, const int* idx, const double* cT, const double* cu,
-                               const double* cp, const double* cmassf, const double* cdtc, const int* cst,
-                               double* T_out, double* u_out, double* p_out, double* massf_out, double* dtc_out,
-                               int* status, hipStream_t s) {
-    if (m <= 0) return hipSuccess;
 #define F(N)                                                                                                         \
-    hipLaunchKernelGGL((k_surr_scatter<N>), dim3(g256(m)), dim3(256), 0, s, m, n, idx, cT, cu, cp, cmassf, cdtc, cst, \
-                       T_out, u_out, p_out, massf_out, dtc_out, status);                                             \
+    hipLaunchKernelGGL((k_surr_route<N>), dim3(g256(n)), dim3(256), 0, s, n, T, dtc, dt, cfg, path, taken, net_idx, \
+                       net_count);                                                                                   \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_features(int nsp, int n, const int* net_idx, const unsigned* net_count, const double* rho,
+                                const double* T, const double* massf, double dt, const SurrCfg& cfg, float* x,
+                                hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define F(N)                                                                                                         \
+    hipLaunchKernelGGL((k_surr_features<N>), dim3(g256(n)), dim3(256), 0, s, n, net_idx, net_count, rho, T, massf, \
+                       dt, cfg, x);                                                                                  \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_apply(int nsp, int n, const int* net_idx, const unsigned* net_count, const float* x,
+                             const float* ya, const float* yb, const double* massf, const SurrCfg& cfg,
+                             double* massf_new, int* path, int* taken, unsigned* fb_count, double* disagree,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define F(N)                                                                                                          \
+    hipLaunchKernelGGL((k_surr_apply<N>), dim3(g256(n)), dim3(256), 0, s, n, net_idx, net_count, x, ya, yb, massf,  \
+                       cfg, massf_new, path, taken, fb_count, disagree);                                              \
+    return hipGetLastError();
+    GDTK_SURR_NSP(F)
+#undef F
+}
+
+hipError_t launch_surr_finish(int nsp, int n, const int* net_idx, const unsigned* net_count, const double* rho,
+                              const double* T, const double* u, const double* p, const double* massf, const double* dtc,
+                              const double* massf_new, const ChemDev* cd, double* T_out, double* u_out, double* p_out,
+                              double* massf_out, double* dtc_out, int* status, int* path, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define F(N)                                                                                                          \
+    hipLaunchKernelGGL((k_surr_finish<N>), dim3(g256(n)), dim3(256), 0, s, n, net_idx, net_count, rho, T, u, p,     \
+                       massf, dtc, massf_new, cd, T_out, u_out, p_out, massf_out, dtc_out, status, path);             \
     return hipGetLastError();
     GDTK_SURR_NSP(F)
 #undef F

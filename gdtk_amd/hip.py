@@ -23,7 +23,8 @@ GDTK_OK = 0
 ERR_NAMES = {-1: "GDTK_ERR_ARG", -2: "GDTK_ERR_HIP", -3: "GDTK_ERR_BAD_CELL", -4: "GDTK_ERR_TOO_MANY_BAD_CELLS",
              -5: "GDTK_ERR_STEP_FAILED", -6: "GDTK_ERR_UNSUPPORTED"}
 
-FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11, S=12)
+FIELDS = dict(rho=0, vx=1, vy=2, vz=3, u=4, p=5, T=6, a=7, volume=8, iLength=9, jLength=10, kLength=11, S=12,
+              dt_chem=13)
 # Flux calculators (src/eilmer/globalconfig.d:270-333 names; "adaptive" is the reference's alias).
 FLUX_CALCULATORS = {"ausmdv": 0, "efm": 1, "adaptive_efm_ausmdv": 2, "adaptive": 2, "hanel": 3,
                     "adaptive_hanel_ausmdv": 4}

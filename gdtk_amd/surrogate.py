@@ -33,7 +33,8 @@ class MlpDesc(C.Structure):
 class SurrogateCfg(C.Structure):
     _fields_ = [("x_mean", C.c_double * MAX_FEAT), ("x_inv_std", C.c_double * MAX_FEAT),
                 ("y_scale", C.c_double * MAX_SPECIES), ("x_abs_max", C.c_double), ("disagree_tol", C.c_double),
-                ("elem", (C.c_double * MAX_SPECIES) * MAX_ELEM), ("n_elem", C.c_int)]
+                ("elem", (C.c_double * MAX_SPECIES) * MAX_ELEM), ("n_elem", C.c_int),
+                ("route_ratio", C.c_double), ("route_T", C.c_double)]
 
 
 def _lib():
@@ -170,16 +171,18 @@ class Surrogate:
 
     def step_device(self, n, rho, T, u, p, massf, dt_chem, dt, T_out, u_out, p_out, massf_out, dt_chem_out, status,
                     path=None, stream=None):
-        """Device pointers (ints); returns the number of cells sent to the integrator.
-        path (optional int32 array): 0 network, 1 integrator, 2 frozen, 3 failed."""
-        return _check(_lib().gdtk_hip_surrogate_step_device(self.h, n, rho, T, u, p, massf, dt_chem, dt, T_out, u_out,
-                                                            p_out, massf_out, dt_chem_out, status, path, stream))
+        """Device pointers (ints), asynchronous on stream.  path (optional int32 array): 0
+        network, 1 integrator, 2 frozen, 3 failed."""
+        _check(_lib().gdtk_hip_surrogate_step_device(self.h, n, rho, T, u, p, massf, dt_chem, dt, T_out, u_out,
+                                                     p_out, massf_out, dt_chem_out, status, path, stream))
 
     def stats(self):
-        v = np.zeros(6)
-        _check(_lib().gdtk_hip_surrogate_stats(self.h, v.ctypes.data_as(_dp), 6))
+        """Of the last step: cells, network cells sent back to the integrator (fallback), the
+        three phases' milliseconds, the largest network disagreement, network cells."""
+        v = np.zeros(7)
+        _check(_lib().gdtk_hip_surrogate_stats(self.h, v.ctypes.data_as(_dp), 7))
         return dict(cells=int(v[0]), fallback=int(v[1]), ms_network=v[2], ms_project=v[3], ms_integrator=v[4],
-                    max_disagreement=v[5])
+                    max_disagreement=v[5], network_cells=int(v[6]))
 
 
 def project_device(E, n, Y0, d0, d, ok, stream=None):
@@ -278,11 +281,17 @@ def train_surrogate(chem, gas_json, cells, dts, seed=7, epochs=(8, 6, 4), lrs=(2
     return sur, info
 
 
-def with_tolerance(sur, disagree_tol, x_abs_max=None):
-    """A second surrogate over the same networks and chemistry with other thresholds."""
+def with_tolerance(sur, disagree_tol, x_abs_max=None, route_ratio=None, route_T=None):
+    """A second surrogate over the same networks and chemistry with other thresholds (and,
+    if given, other routing: cells go to the networks where dt > route_ratio * dt_chem, or
+    with no dt_chem yet where T > route_T; route_ratio <= 0 sends every cell above T_frozen)."""
     cfg = SurrogateCfg()
     C.memmove(C.byref(cfg), C.byref(sur.cfg), C.sizeof(SurrogateCfg))
     cfg.disagree_tol = disagree_tol
     if x_abs_max is not None:
         cfg.x_abs_max = x_abs_max
+    if route_ratio is not None:
+        cfg.route_ratio = route_ratio
+    if route_T is not None:
+        cfg.route_T = route_T
     return Surrogate(sur.chem, sur.net_a, sur.net_b, cfg)

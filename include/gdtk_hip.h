@@ -64,6 +64,7 @@ extern "C" {
 #define GDTK_FIELD_JLENGTH 10
 #define GDTK_FIELD_KLENGTH 11
 #define GDTK_FIELD_SHOCK 12 /* the cell's shock indicator fs.S (0 or 1; adaptive flux calculator) */
+#define GDTK_FIELD_DT_CHEM 13 /* the cell's dt_chem (fvcell.d:78; after a chemistry step) */
 #define GDTK_FIELD_U0 100   /* + conserved-quantity index: U[0][j] */
 #define GDTK_FIELD_DUDT0 200 /* + conserved-quantity index: dUdt[0][j] */
 #define GDTK_FIELD_MASSF 300 /* + species index: mass fraction (multi-species gas) */
@@ -136,11 +137,14 @@ int gdtk_hip_block_set_initial_massf(int sim, int block, const double* p, const 
  * (simcore.d:1619-1638) runs FVCell.thermochemical_increment (fvcell.d:858-972) on every
  * cell with dt_global and the cell's dt_chem (initially -1, simcore.d:335) through the
  * chemistry handle `chem` (gdtk_hip_chem_create, same species as the gas), or through the
- * learned surrogate `surrogate` (gdtk_hip_surrogate_create; -1: none).  A failed cell ends
- * the step with GDTK_ERR_STEP_FAILED. */
+ * learned surrogate `surrogate` (gdtk_hip_surrogate_create; -1: none).  A failed cell is the
+ * reference's fatal FlowSolverException (fvcell.d:923): gdtk_hip_sim_run and
+ * gdtk_hip_sim_step_fixed_dt return GDTK_ERR_STEP_FAILED.  The chemistry step does not wait
+ * for the device: its failure count is checked when the next step's flux attempt has waited
+ * for the stream, or before the call returns, so a failure can surface one flux step later. */
 int gdtk_hip_sim_set_chemistry(int sim, int chem, int surrogate);
-/* {chemistry steps, total ms, cells the surrogate sent to the integrator}; returns the
- * number of values written (<= 3). */
+/* {chemistry steps, total ms, network cells the surrogate sent back to the integrator in the
+ * last step}; returns the number of values written (<= 3). */
 int gdtk_hip_sim_chem_stats(int sim, double* stats, int nstats);
 
 /* Copies one field of all active cells to host memory (GDTK_FIELD_*). */
@@ -382,14 +386,19 @@ int gdtk_hip_chem_stats(int chem, double* stats, int nstats);
 int gdtk_hip_chem_substeps(int chem, int n, int* substeps);
 
 /* Launch shape of the integration kernel (set by gdtk_hip_chem_autotune):
- * GDTK_CHEM_TUNE_PERSISTENT  1 persistent lanes fed by a work queue, 0 one lane per cell
+ * GDTK_CHEM_TUNE_PERSISTENT  1 persistent lanes fed by a work queue, 0 one lane per cell,
+ *                            -1 (default) persistent above 2048 cells per CU
  * GDTK_CHEM_TUNE_CHUNK       cells taken from the queue per wave grab
  * GDTK_CHEM_TUNE_REFILL      substep trips between refills of finished lanes
- * GDTK_CHEM_TUNE_BLOCKS_PER_CU  persistent workgroups per CU (0 = occupancy limit) */
+ * GDTK_CHEM_TUNE_BLOCKS_PER_CU  persistent workgroups per CU (0 = occupancy limit)
+ * GDTK_CHEM_TUNE_HEAVY_RATIO cells whose first substep is shorter than dt / value are
+ *                            queued first in the persistent variant (0 = no such list;
+ *                            default 4), so the longest integrations start at once */
 #define GDTK_CHEM_TUNE_PERSISTENT 1
 #define GDTK_CHEM_TUNE_CHUNK 2
 #define GDTK_CHEM_TUNE_REFILL 3
 #define GDTK_CHEM_TUNE_BLOCKS_PER_CU 4
+#define GDTK_CHEM_TUNE_HEAVY_RATIO 5
 int gdtk_hip_chem_set_tuning(int chem, int key, int value);
 
 /* Times the integration variants on a sample batch of device arrays (left unchanged) and
@@ -453,6 +462,11 @@ typedef struct gdtk_surrogate_cfg {
     double disagree_tol;                   /* max |dY_a - dY_b| beyond this queues the cell */
     double elem[GDTK_SURR_MAX_ELEM][GDTK_MAX_SPECIES]; /* element mass per unit species mass */
     int n_elem;
+    /* Routing: a cell above T_frozen goes to the networks if its integration would be long,
+     * i.e. dt > route_ratio * its dt_chem (the last suggested substep), or, with dt_chem <= 0,
+     * T > route_T; otherwise to the integrator.  route_ratio <= 0 sends every cell above
+     * T_frozen to the networks. */
+    double route_ratio, route_T;
 } gdtk_surrogate_cfg;
 
 /* Two networks (nsp + 3 inputs, nsp outputs) and the chemistry handle that serves the
@@ -460,18 +474,21 @@ typedef struct gdtk_surrogate_cfg {
 int gdtk_hip_surrogate_create(int chem, int net_a, int net_b, const gdtk_surrogate_cfg* cfg, int* surrogate);
 int gdtk_hip_surrogate_destroy(int surrogate);
 /* Same arguments and per-cell results as gdtk_hip_chem_step_device, plus path [n] (may be
- * NULL): 0 network, 1 alpha-QSS integrator, 2 frozen, 3 failed.  Network cells: mean of the
- * two networks' changes, projected so that every element's mass is conserved and no mass
- * fraction is negative, then T by Newton from the unchanged u and p = rho R T; dt_chem is
- * passed through.  A network composition that admits no temperature is reported failed
+ * NULL): 0 network, 1 alpha-QSS integrator, 2 frozen, 3 failed.  Network cells (routed by
+ * the configuration's route_ratio and route_T): mean of the two networks' changes, projected
+ * so that every element's mass is conserved and no mass fraction is negative, then T by
+ * Newton from the unchanged u and p = rho R T; dt_chem is passed through.  A network cell
+ * outside the trained range, whose networks disagree or whose projection is infeasible goes
+ * to the integrator.  A network composition that admits no temperature is reported failed
  * (status -1, state unchanged, dt_chem -1), the reference's failure mode for such cells.
- * Returns the number of cells sent to the integrator. */
+ * Asynchronous on `stream`: nothing is read back to the host. */
 int gdtk_hip_surrogate_step_device(int surrogate, int n, const double* rho, const double* T, const double* u,
                                    const double* p, const double* massf, const double* dt_chem, double dt,
                                    double* T_out, double* u_out, double* p_out, double* massf_out,
                                    double* dt_chem_out, int* status, int* path, void* stream);
-/* {cells, cells sent to the integrator, ms features + networks, ms projection + T(u), ms
- * integrator, max network disagreement} of the last step. */
+/* {cells, network cells sent back to the integrator, ms routing + features + networks, ms
+ * projection + T(u), ms integrator, max network disagreement, network cells} of the last
+ * step (waits for it). */
 int gdtk_hip_surrogate_stats(int surrogate, double* stats, int nstats);
 /* The element-conservation projection of the surrogate step on its own, for device arrays
  * Y0, d0, d [nsp][n] (species-major) and ok [n], on `stream`: d is the change nearest d0 (least

@@ -307,16 +307,21 @@ def test_queue_variants_identical_and_autotune():
     n = 20000
     c = air_cells(n, seed=9)
     dtc = np.full(n, -1.0)
-    ref = None
+    ref = ref_sub = None
+    # heavy_ratio 1 and 4 put many of these cells (no substep history, so the first substep
+    # is the estimate) on the heavy list; 0 keeps the list empty
     for tu in (dict(persistent=0), dict(persistent=1, chunk=1, refill=1), dict(persistent=1, chunk=64, refill=8),
-               dict(persistent=1, chunk=256, refill=1, blocks_per_cu=1)):
-        ch.set_tuning(**{"blocks_per_cu": 0, **tu})
+               dict(persistent=1, chunk=256, refill=1, blocks_per_cu=1), dict(persistent=1, heavy_ratio=0),
+               dict(persistent=1, chunk=8, heavy_ratio=1), dict(persistent=-1)):
+        ch.set_tuning(**{"blocks_per_cu": 0, "heavy_ratio": 4, **tu})
         g = ch.step(c["rho"], c["T"], c["u"], c["p"], c["massf"], dtc, DT)
+        sub = ch.substeps(n)
         if ref is None:
-            ref = g
+            ref, ref_sub = g, sub
         else:
             for k in ("T", "u", "p", "massf", "dt_chem", "status"):
                 assert np.array_equal(g[k], ref[k]), (tu, k)
+            assert np.array_equal(sub, ref_sub), (tu, "substeps")
     dev = torch.device("cuda:0")
     t = {k: torch.tensor(v, dtype=torch.float64, device=dev) for k, v in c.items()}
     td = torch.tensor(dtc, device=dev)

@@ -161,15 +161,17 @@ def test_surrogate_step_conservation_and_fallback():
     dto = torch.empty_like(dtc)
     st = torch.empty(n, dtype=torch.int32, device=dev)
     path = torch.empty(n, dtype=torch.int32, device=dev)
-    m = sur.step_device(n, t_in["rho"].data_ptr(), t_in["T"].data_ptr(), t_in["u"].data_ptr(), t_in["p"].data_ptr(),
-                        t_in["massf"].data_ptr(), dtc.data_ptr(), 1.0e-7, out["T"].data_ptr(), out["u"].data_ptr(),
-                        out["p"].data_ptr(), out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), path.data_ptr(),
-                        torch.cuda.current_stream().cuda_stream)
+    sur.step_device(n, t_in["rho"].data_ptr(), t_in["T"].data_ptr(), t_in["u"].data_ptr(), t_in["p"].data_ptr(),
+                    t_in["massf"].data_ptr(), dtc.data_ptr(), 1.0e-7, out["T"].data_ptr(), out["u"].data_ptr(),
+                    out["p"].data_ptr(), out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), path.data_ptr(),
+                    torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     P = path.cpu().numpy()
     s = sur.stats()
-    print("surrogate", s, "queued", m, flush=True)
-    assert m == s["fallback"] == int(np.count_nonzero(P == 1))
+    print("surrogate", s, flush=True)
+    # default routing: every cell above T_frozen is offered to the networks
+    assert s["network_cells"] == n - 100
+    assert s["fallback"] == int(np.count_nonzero(P == 1))
     Y0 = c["massf"].reshape(5, n)
     Y = out["massf"].cpu().numpy().reshape(5, n)
     E = air_element_matrix(gd["species"], gd["M"])
@@ -251,3 +253,159 @@ def test_device_projection_matches_oracle():
     print(f"projection: {n} cells, {grew} with a species driven to zero, {int((ok == 0).sum())} infeasible, "
           f"max |d_gpu - d_oracle| {worst:.2e}", flush=True)
     assert grew > 100
+
+
+def test_surrogate_routing_by_integration_length():
+    """Routing: cells whose integration would be long go to the networks, the rest straight
+    to the integrator.  With no dt_chem yet (-1) the test is T > route_T; with a dt_chem from
+    a previous step it is dt > route_ratio * dt_chem.  Cells routed to the integrator equal
+    the integrator's own results bit for bit, and every network cell conserves elements."""
+    import torch
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, gas_record, mech_record
+    from gdtk_amd.chemcases import air_cells
+    from gdtk_amd.surrogate import integrator_samples, train_air_surrogate, with_tolerance
+    gd = gdata.load("tpg-air-5sp")
+    ch = Chemistry(gas_record(gd), mech_record(gdata.load("park-air-5sp-1T")))
+    sur0, _ = train_air_surrogate(ch, gd, n_train=1 << 15, epochs=(1,), lrs=(2e-3,), batch=2048,
+                                  x_abs_max=50.0, disagree_tol=1.0)
+    n, dt = 8192, 1.0e-7
+    c = air_cells(n, seed=7)
+    dev = torch.device("cuda:0")
+    t_in, ref, ref_st = integrator_samples(ch, c, dt, dev)
+    out = {k: torch.empty_like(t_in[k]) for k in ("T", "u", "p", "massf")}
+    dto = torch.empty(n, dtype=torch.float64, device=dev)
+    st = torch.empty(n, dtype=torch.int32, device=dev)
+    path = torch.empty(n, dtype=torch.int32, device=dev)
+    # the integrator's own dt_chem suggestions of one step, as the next step would see them
+    dtc_ref = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
+    o2 = {k: torch.empty_like(t_in[k]) for k in ("T", "u", "p", "massf")}
+    st2 = torch.empty_like(st)
+    ch.step_device(n, t_in["rho"].data_ptr(), t_in["T"].data_ptr(), t_in["u"].data_ptr(), t_in["p"].data_ptr(),
+                   t_in["massf"].data_ptr(), dtc_ref.data_ptr(), dt, o2["T"].data_ptr(), o2["u"].data_ptr(),
+                   o2["p"].data_ptr(), o2["massf"].data_ptr(), dto.data_ptr(), st2.data_ptr(),
+                   torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    dtc_prev = torch.where(st2 == 0, dto, torch.full_like(dto, -1.0))
+    T = c["T"]
+    for label, dtc_in, ratio, Tr, expect in (
+            ("by T", torch.full((n,), -1.0, dtype=torch.float64, device=dev), 5.0, 6000.0, T > 6000.0),
+            ("by dt_chem", dtc_prev, 5.0, 1.0e30, None)):
+        sur = with_tolerance(sur0, 1.0, route_ratio=ratio, route_T=Tr)
+        ref_i = {k: torch.empty_like(t_in[k]) for k in ("T", "u", "p", "massf")}
+        dto_i, st_i = torch.empty_like(dto), torch.empty_like(st)
+        ch.step_device(n, t_in["rho"].data_ptr(), t_in["T"].data_ptr(), t_in["u"].data_ptr(), t_in["p"].data_ptr(),
+                       t_in["massf"].data_ptr(), dtc_in.data_ptr(), dt, ref_i["T"].data_ptr(), ref_i["u"].data_ptr(),
+                       ref_i["p"].data_ptr(), ref_i["massf"].data_ptr(), dto_i.data_ptr(), st_i.data_ptr(),
+                       torch.cuda.current_stream().cuda_stream)
+        sur.step_device(n, t_in["rho"].data_ptr(), t_in["T"].data_ptr(), t_in["u"].data_ptr(), t_in["p"].data_ptr(),
+                        t_in["massf"].data_ptr(), dtc_in.data_ptr(), dt, out["T"].data_ptr(), out["u"].data_ptr(),
+                        out["p"].data_ptr(), out["massf"].data_ptr(), dto.data_ptr(), st.data_ptr(), path.data_ptr(),
+                        torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        P = path.cpu().numpy()
+        s = sur.stats()
+        if expect is None:
+            d = dtc_in.cpu().numpy()
+            expect = np.where(d > 0.0, dt > ratio * d, T > Tr)
+        offered = expect & (T > 300.0)
+        print(f"routing {label}: {int(offered.sum())} cells offered to the networks, stats {s}", flush=True)
+        assert s["network_cells"] == int(offered.sum()) and 0 < offered.sum() < n
+        assert np.all(P[~offered & (T > 300.0)] == 1)
+        assert np.all((P[offered] == 0) | (P[offered] == 1) | (P[offered] == 3))
+        fb = P == 1
+        for k in ("T", "u", "p"):
+            assert np.array_equal(out[k].cpu().numpy()[fb], ref_i[k].cpu().numpy()[fb]), (label, k)
+        assert np.array_equal(out["massf"].cpu().numpy().reshape(5, n)[:, fb],
+                              ref_i["massf"].cpu().numpy().reshape(5, n)[:, fb])
+        assert np.array_equal(dto.cpu().numpy()[fb], dto_i.cpu().numpy()[fb])
+        sur.close()
+
+
+def test_trained_network_gains_over_the_integrator():
+    """What the trained networks gain over the method without them, on a held-out batch of
+    2^18 air cells of the chemistry workload (T 1500-12000 K, dt = 1e-7 s, uneven
+    stiffness; networks trained on 2^20 other cells of the same distribution, 13 epochs):
+    - the cells the networks answer (most of them) take a fraction of the time the alpha-QSS
+      integrator takes over those same cells, and stay within the stated bounds of the
+      integrator's result;
+    - the whole step (networks, then the integrator for the cells they send back) is timed
+      beside the integrator's whole step.  It gains nothing here: the cells sent back are
+      among the stiffest, and one such cell's serial substeps set the integrator's time.
+    The full-size figures are the benchmark's."""
+    import torch
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import Chemistry, gas_record, mech_record
+    from gdtk_amd.chemcases import air_cells
+    from gdtk_amd.surrogate import integrator_samples, train_air_surrogate, with_tolerance
+    gd = gdata.load("tpg-air-5sp")
+    ch = Chemistry(gas_record(gd), mech_record(gdata.load("park-air-5sp-1T")))
+    dt = 1.0e-7
+    sur0, info = train_air_surrogate(ch, gd, n_train=1 << 20, dt=dt, seed=3, epochs=(6, 4, 3),
+                                     lrs=(2e-3, 5e-4, 1e-4), batch=4096)
+    sur = with_tolerance(sur0, 5.0e-3)
+    n = 1 << 18
+    c = air_cells(n, seed=424242)
+    dev = torch.device("cuda:0")
+    t_in, ref, ref_st = integrator_samples(ch, c, dt, dev)
+    stream = torch.cuda.current_stream()
+
+    def run(f, reps=3):
+        f()  # warm-up
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def bufs(m):
+        return ({k: torch.empty((5 * m,) if k == "massf" else (m,), dtype=torch.float64, device=dev)
+                 for k in ("T", "u", "p", "massf", "dtc")}, torch.empty(m, dtype=torch.int32, device=dev))
+
+    def args(t, m, o, st):
+        dtc = torch.full((m,), -1.0, dtype=torch.float64, device=dev)
+        bufs_keep.append(dtc)
+        return (m, t["rho"].data_ptr(), t["T"].data_ptr(), t["u"].data_ptr(), t["p"].data_ptr(), t["massf"].data_ptr(),
+                dtc.data_ptr(), dt, o["T"].data_ptr(), o["u"].data_ptr(), o["p"].data_ptr(), o["massf"].data_ptr(),
+                o["dtc"].data_ptr(), st.data_ptr())
+
+    bufs_keep = []
+    o, st = bufs(n)
+    path = torch.empty(n, dtype=torch.int32, device=dev)
+    a = args(t_in, n, o, st)
+    ms_int_all = run(lambda: ch.step_device(*a, stream.cuda_stream))
+    ms_sur_all = run(lambda: sur.step_device(*a, path.data_ptr(), stream.cuda_stream))
+    s = sur.stats()
+    P = path.cpu().numpy()
+    ok = (P == 0) & (ref_st.cpu().numpy() == 0)
+    # the network-answered cells alone, through each path
+    idx = torch.tensor(np.nonzero(P == 0)[0], dtype=torch.int64, device=dev)
+    m = int(idx.numel())
+    sub = {"rho": t_in["rho"][idx].contiguous(), "T": t_in["T"][idx].contiguous(), "u": t_in["u"][idx].contiguous(),
+           "p": t_in["p"][idx].contiguous(),
+           "massf": t_in["massf"].reshape(5, n)[:, idx].contiguous().reshape(-1)}
+    o2, st2 = bufs(m)
+    path2 = torch.empty(m, dtype=torch.int32, device=dev)
+    a2 = args(sub, m, o2, st2)
+    ms_int_sub = run(lambda: ch.step_device(*a2, stream.cuda_stream))
+    ms_net_sub = run(lambda: sur.step_device(*a2, path2.data_ptr(), stream.cuda_stream))
+    sent_back = int(sur.stats()["fallback"])
+    Y = o["massf"].cpu().numpy().reshape(5, n)
+    Yr = ref["massf"].cpu().numpy().reshape(5, n)
+    dY = np.abs(Y - Yr).max(axis=0)[ok]
+    Tr = ref["T"].cpu().numpy()
+    Terr = (np.abs(o["T"].cpu().numpy() - Tr) / Tr)[ok]
+    rep = dict(network_fraction=float(ok.mean()), answered_cells=m, integrator_ms_on_answered=ms_int_sub,
+               networks_ms_on_answered=ms_net_sub, gain_on_answered=ms_int_sub / ms_net_sub,
+               whole_step_integrator_ms=ms_int_all, whole_step_surrogate_ms=ms_sur_all,
+               whole_step_sent_back=s["fallback"], max_dY=float(dY.max()), p99_dY=float(np.quantile(dY, 0.99)),
+               p99_T=float(np.quantile(Terr, 0.99)), samples=info["samples"], train_s=info["train_s"])
+    print("surrogate gain:", rep, flush=True)
+    assert rep["network_fraction"] > 0.9
+    assert sent_back <= 0.001 * m       # the answered cells are answered again
+    assert rep["gain_on_answered"] > 3.0
+    assert rep["p99_dY"] < 1.0e-2 and rep["max_dY"] < 0.1 and rep["p99_T"] < 0.05
+    sur.close()
