@@ -28,7 +28,8 @@ class Reaction(C.Structure):
                 ("nprod", C.c_int), ("prod_sp", C.c_int * 3), ("prod_nu", C.c_int * 3),
                 ("neff", C.c_int), ("eff_sp", C.c_int * MAX_SPECIES), ("eff", C.c_double * MAX_SPECIES),
                 ("f_from_eq", C.c_int), ("fA", C.c_double), ("fn", C.c_double), ("fC", C.c_double),
-                ("b_from_eq", C.c_int), ("bA", C.c_double), ("bn", C.c_double), ("bC", C.c_double)]
+                ("b_from_eq", C.c_int), ("bA", C.c_double), ("bn", C.c_double), ("bC", C.c_double),
+                ("f_park", C.c_int), ("f_s", C.c_double), ("b_park", C.c_int), ("b_s", C.c_double)]
 
 
 class Mechanism(C.Structure):
@@ -87,10 +88,16 @@ def mech_record(d: dict, method=None, T_frozen=300.0, **overrides) -> Mechanism:
             if rc["model"] == "fromEqConst":
                 setattr(rr, pre + "_from_eq", 1)
             else:
+                if rc["model"] not in ("Arrhenius", "Park"):
+                    raise ValueError(f"rate constant model {rc['model']!r} is not supported")
                 setattr(rr, pre + "_from_eq", 0)
                 setattr(rr, pre + "A", float(rc["A"]))
                 setattr(rr, pre + "n", float(rc["n"]))
                 setattr(rr, pre + "C", float(rc["C"]))
+                if rc["model"] == "Park":
+                    # rate_constant.d:365-395: A Te^n exp(-C/Te), Te = T^s Tv^(1-s)
+                    setattr(rr, pre + "_park", 1)
+                    setattr(rr, pre + "_s", float(rc["s"]))
     cfg = d["config"]
     m.T_lower = float(cfg["tempLimits"]["lower"])
     m.T_upper = float(cfg["tempLimits"]["upper"])

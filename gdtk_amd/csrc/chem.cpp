@@ -69,7 +69,7 @@ void build_tpg_gas(const gdtk_tpg_gas& g, ChemDev& cd) {
 
 // Gas and mechanism records -> the device table, with the checks the reference's input
 // readers make (species indices, segment counts) and the options the kernels implement.
-static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDev& cd) {
+void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDev& cd, bool two_temperature) {
     build_tpg_gas(g, cd);
     if (m.nr < 0 || m.nr > CH_MAX_R) throw Error(GDTK_ERR_ARG, "nr must be 0..32");
     cd.nr = m.nr;
@@ -104,6 +104,12 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
         if (d.f_from_eq && d.b_from_eq) throw Error(GDTK_ERR_ARG, "both rate constants from K_c");
         d.fA = r.fA; d.fn = r.fn; d.fC = r.fC;
         d.bA = r.bA; d.bn = r.bn; d.bC = r.bC;
+        if ((r.f_park || r.b_park) && !two_temperature)
+            throw Error(GDTK_ERR_UNSUPPORTED, "Park two-temperature rate constants need the two-temperature reactor");
+        d.f_park = r.f_park ? 1 : 0;
+        d.b_park = r.b_park ? 1 : 0;
+        d.f_s = r.f_s;
+        d.b_s = r.b_s;
         // Dense form when every list is in strictly ascending species order.
         auto ascending = [](const int* sp, int k) {
             for (int q = 1; q < k; ++q)
@@ -151,7 +157,7 @@ static void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDe
     cd.T_lower = m.T_lower;
     cd.T_upper = m.T_upper;
     if (m.method != GDTK_ODE_ALPHA_QSS && m.method != GDTK_ODE_RKF) throw Error(GDTK_ERR_ARG, "bad ODE method");
-    if (m.tightTempCoupling && m.method != GDTK_ODE_ALPHA_QSS)
+    if (m.tightTempCoupling && m.method != GDTK_ODE_ALPHA_QSS && !two_temperature)
         throw Error(GDTK_ERR_UNSUPPORTED, "tightTempCoupling is implemented for the alpha-QSS step only");
     if (m.maxIters < 1 || m.maxSubcycles < 1 || m.maxAttempts < 1) throw Error(GDTK_ERR_ARG, "bad step control");
     cd.method = m.method;
@@ -189,7 +195,7 @@ int gdtk_hip_chem_create(const gdtk_tpg_gas* gas, const gdtk_mechanism* mech, in
         HIPCHK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw Error(GDTK_ERR_HIP, "no HIP device");
         auto c = std::make_unique<Chem>();
-        build_chemdev(*gas, *mech, c->host);
+        build_chemdev(*gas, *mech, c->host, false);
         c->nsp = c->host.nsp;
         c->nr = c->host.nr;
         if (!chem_supported(c->nsp, c->nr)) throw Error(GDTK_ERR_UNSUPPORTED, "mechanism size not supported");

@@ -45,6 +45,8 @@ struct ReacDev {
     double eff_d[CH_MAX_SP];
     int f_from_eq, b_from_eq;
     double fA, fn, fC, bA, bn, bC;
+    int f_park, b_park;  // Park2TRateConstant: the rate at T^s Tv^(1-s) (two-temperature reactor)
+    double f_s, b_s;
     int nu[CH_MAX_SP];
     int part[CH_MAX_SP];
 };
@@ -363,6 +365,31 @@ __device__ __forceinline__ void eval_rate_constants(const ChemDev& cd, double Ti
     }
 }
 
+// The forward and backward rates of one reaction row (dense form) at conc, with c2 = conc^2:
+// reaction.d:289-330 (eval_forward_rate / eval_backward_rate) for lists in ascending order.
+template <int NSP>
+__device__ __forceinline__ void dense_row_rates(const RateRow& rr, double kf, double kb, const double* conc,
+                                                const double* c2, double& wf_out, double& wb_out) {
+    int code[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) code[s] = rr.code[s];
+    double collider = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) collider += (code[s] & 256) ? rr.eff[s] * conc[s] : 0.0;
+    double wf = (rr.flags & 1) ? kf * collider : kf;
+    double wb = (rr.flags & 1) ? kb * collider : kb;
+    // c^e for e in 0..3 as (c or 1) * (c^2 or 1): c * (c * c) for e = 3, as ch_ipow
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) wf *= ((code[s] & 1) ? conc[s] : 1.0) * ((code[s] & 2) ? c2[s] : 1.0);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) wb *= ((code[s] & 4) ? conc[s] : 1.0) * ((code[s] & 8) ? c2[s] : 1.0);
+    wf_out = wf;
+    wb_out = wb;
+}
+
+// Net coefficient of species s in a dense row (4-bit two's complement in bits 4-7).
+__device__ __forceinline__ int dense_row_nu(const RateRow& rr, int s) { return (int)((unsigned)rr.code[s] << 24) >> 28; }
+
 // reaction_mechanism.d:83-96 (eval_split_rates) with reaction.d's forward/backward rates
 // (collider sum over the efficiency list, then the concentration products in list order)
 // and production/loss per participant in reaction order.
@@ -387,22 +414,11 @@ __device__ __forceinline__ void eval_split_rates(const ChemDev& cd, const double
         for (int ir = 0; ir < NRM; ++ir) {
             if (ir >= nr) break;
             const RateRow& rr = cd.rr[ir];
-            int code[NSP];
-#pragma unroll
-            for (int s = 0; s < NSP; ++s) code[s] = rr.code[s];
-            double collider = 0.0;
-#pragma unroll
-            for (int s = 0; s < NSP; ++s) collider += (code[s] & 256) ? rr.eff[s] * conc[s] : 0.0;
-            double wf = (rr.flags & 1) ? kf[ir] * collider : kf[ir];
-            double wb = (rr.flags & 1) ? kb[ir] * collider : kb[ir];
-            // c^e for e in 0..3 as (c or 1) * (c^2 or 1): c * (c * c) for e = 3, as ch_ipow
-#pragma unroll
-            for (int s = 0; s < NSP; ++s) wf *= ((code[s] & 1) ? conc[s] : 1.0) * ((code[s] & 2) ? c2[s] : 1.0);
-#pragma unroll
-            for (int s = 0; s < NSP; ++s) wb *= ((code[s] & 4) ? conc[s] : 1.0) * ((code[s] & 8) ? c2[s] : 1.0);
+            double wf, wb;
+            dense_row_rates<NSP>(rr, kf[ir], kb[ir], conc, c2, wf, wb);
 #pragma unroll
             for (int s = 0; s < NSP; ++s) {
-                const int nu = (int)((unsigned)code[s] << 24) >> 28;
+                const int nu = dense_row_nu(rr, s);
                 const double m = (double)(nu < 0 ? -nu : nu);
                 const bool pos = nu > 0;
                 q[s] += (nu != 0) ? m * (pos ? wf : wb) : 0.0;
@@ -566,7 +582,7 @@ __device__ __forceinline__ bool alpha_qss_step(const ChemDev& cd, const double* 
 // RKFStep.opCall, chemistry_update.d:385-484 (Cash-Karp tableau); yOut is always written.
 template <int NSP, int NRM, bool DENSE>
 __device__ __forceinline__ bool rkf_step(const ChemDev& cd, const double* kf, const double* kb, const double* y0, double h,
-                         double* yOut, double& hSuggest) {
+                         double* yOut, double& hSuggest, double* yLast = nullptr) {
     const double a21 = 1. / 5., a31 = 3. / 40., a32 = 9. / 40., a41 = 3. / 10., a42 = -9. / 10., a43 = 6. / 5.,
                  a51 = -11. / 54., a52 = 5. / 2., a53 = -70. / 27., a54 = 35. / 27., a61 = 1631. / 55296.,
                  a62 = 175. / 512., a63 = 575. / 13824., a64 = 44275. / 110592., a65 = 253. / 4096.;
@@ -595,6 +611,12 @@ __device__ __forceinline__ bool rkf_step(const ChemDev& cd, const double* kf, co
     for (int s = 0; s < NSP; ++s)
         yT[s] = y0[s] + h * (a61 * k1[s] + a62 * k2[s] + a63 * k3[s] + a64 * k4[s] + a65 * k5[s]);
     rates(yT, k6);
+    // the concentrations of the last rate evaluation (whose reaction rates the
+    // two-temperature reactor's energy source reads)
+    if (yLast) {
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) yLast[s] = yT[s];
+    }
     double err = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {

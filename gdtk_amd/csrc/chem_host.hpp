@@ -13,6 +13,9 @@ namespace gdtk {
 
 // Gas record -> the gas part of a device table (chem.cpp).
 void build_tpg_gas(const gdtk_tpg_gas& g, ChemDev& cd);
+// Gas and mechanism records -> a device table, with the reference readers' checks; Park
+// rate constants are accepted for the two-temperature reactor only (chem.cpp).
+void build_chemdev(const gdtk_tpg_gas& g, const gdtk_mechanism& m, ChemDev& cd, bool two_temperature);
 
 struct Chem {
     ChemDev host{};

@@ -14,6 +14,14 @@ Lua data files (run once where the reference tree is mounted; the JSON is commit
                         backward rates come from the equilibrium constant; the ODE
                         settings are prep-chem's defaults for alpha-QSS
                         (src/kinetics/prep_chem.lua:46-90).
+  gupta-air-5sp-2T.json the 5-species form of examples/eilmer/2D/fireII/legacy/
+                        air-11sp-reactions.inp (NO_SPECIES = 5 selects r1-r6): Park rates
+                        at T^s Tv^(1-s) forward, Arrhenius backward, converted to SI as
+                        prep-chem does (A written "%.12e", n and s "%f"); efficiencies
+                        default to 1, the file's values override, zeros are dropped, as
+                        prep-chem writes them (src/kinetics/reaction.lua:514-545, 583-596);
+                        the file's Config (tempLimits 300-30000 K, RKF errTol 1e-9, tight
+                        temperature coupling) with prep-chem's default step limits.
 """
 from __future__ import annotations
 
@@ -24,7 +32,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
-from gdtk_amd.data.lua_tables import parse_file  # noqa: E402
+from gdtk_amd.data.lua_tables import parse, parse_file  # noqa: E402
 
 REF = os.environ.get("GDTK_REFERENCE", "/root/reference")
 
@@ -132,6 +140,86 @@ def nitrogen_2sp_2r(species):
             "reactions": reactions}
 
 
+def _reaction_blocks(text):
+    """The top-level ``Reaction{...}`` statements of a mechanism input, by brace matching
+    (the rest of such a file -- variables, ``if`` blocks -- is not evaluated)."""
+    out, pos = [], 0
+    while True:
+        k = text.find("Reaction{", pos)
+        if k < 0:
+            return out
+        depth, j = 0, k + len("Reaction")
+        while True:
+            c = text[j]
+            if c == "{":
+                depth += 1
+            elif c == "}":
+                depth -= 1
+                if depth == 0:
+                    break
+            j += 1
+        out.append(text[k:j + 1])
+        pos = j + 1
+
+
+def _fmt_f(x):
+    return float("%f" % x)
+
+
+def gupta_2T(gas):
+    species = gas["species"]
+    path = os.path.join(REF, "examples/eilmer/2D/fireII/legacy/air-11sp-reactions.inp")
+    with open(path) as f:
+        text = f.read()
+    s_val = float(re.search(r"^s\s*=\s*([0-9.eE+-]+)", text, re.M).group(1))
+    reactions = []
+    for block in _reaction_blocks(text):
+        _, calls = parse(re.sub(r"\bs=s\b", f"s={s_val!r}", block))
+        (name, t), = calls
+        if t.get("label") not in ("r1", "r2", "r3", "r4", "r5", "r6"):
+            continue
+        eq = t[1]
+        lhs, rhs = eq.split("<=>")
+        reac, colL = parse_side(lhs, species)
+        prod, colR = parse_side(rhs, species)
+        assert colL == colR
+        nf = sum(reac.values()) + (1 if colL else 0)
+        nb = sum(prod.values()) + (1 if colR else 0)
+
+        def rate(rt, nc):
+            conv = 1.0e-6 ** (nc - 1)
+            d = {"model": rt[1], "A": float("%.12e" % (rt["A"] * conv)), "n": _fmt_f(rt["n"]),
+                 "C": float("%.12e" % rt["C"]), "A_cgs": rt["A"]}
+            if rt[1] == "Park":
+                d["s"] = _fmt_f(rt["s"])
+                d["mode"] = 0
+            else:
+                assert rt[1] == "Arrhenius"
+            return d
+
+        r = {"equation": eq, "label": t["label"], "type": "anonymous_collider" if colL else "elementary",
+             "reactants": [[i, c] for i, c in sorted(reac.items())],
+             "products": [[i, c] for i, c in sorted(prod.items())],
+             "frc": rate(t["fr"], nf), "brc": rate(t["br"], nb)}
+        if colL:
+            eff = {i: 1.0 for i in range(len(species))}
+            for sp, v in t.get("efficiencies", {}).items():
+                if sp in species:
+                    eff[species.index(sp)] = float(v)
+            r["efficiencies"] = [[i, float("%.6e" % v)] for i, v in sorted(eff.items()) if v != 0.0]
+        reactions.append(r)
+    assert [r["label"] for r in reactions] == ["r1", "r2", "r3", "r4", "r5", "r6"]
+    return {"species": species, "source": "examples/eilmer/2D/fireII/legacy/air-11sp-reactions.inp (NO_SPECIES = 5)",
+            "config": {"tempLimits": {"lower": 300.0, "upper": 30000.0},
+                       "odeStep": {"method": "rkf", "errTol": 1.0e-9},
+                       "tightTempCoupling": True, "maxSubcycles": 10000, "maxAttempts": 4},
+            "two_temperature": {"molecular": [s in ("N2", "O2", "NO") for s in species],
+                                "A_MW": [{"N2": 220.0, "O2": 129.0, "NO": 168.0}.get(s, 0.0) for s in species],
+                                "source": "src/kinetics/two_temperature_air_kinetics.d:714-720 (Gnoffo et al. "
+                                          "Table 1), src/gas/two_temperature_air.d:25"},
+            "reactions": reactions}
+
+
 def subset_gas(gas, species, source):
     idx = [gas["species"].index(s) for s in species]
     return {"model": gas["model"], "source": source, "species": list(species), "M": [gas["M"][i] for i in idx],
@@ -152,6 +240,8 @@ def main():
         json.dump(n2, f, indent=1)
     with open(os.path.join(HERE, "nitrogen-2sp-2r.json"), "w") as f:
         json.dump(nitrogen_2sp_2r(n2["species"]), f, indent=1)
+    with open(os.path.join(HERE, "gupta-air-5sp-2T.json"), "w") as f:
+        json.dump(gupta_2T(gas), f, indent=1)
 
 
 if __name__ == "__main__":

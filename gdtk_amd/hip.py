@@ -54,6 +54,7 @@ EXPORTS = [
     "gdtk_hip_mlp_infer", "gdtk_hip_mfma_probe", "gdtk_hip_surrogate_create", "gdtk_hip_surrogate_destroy",
     "gdtk_hip_surrogate_step_device", "gdtk_hip_surrogate_stats", "gdtk_hip_surrogate_project",
     "gdtk_hip_chem_step_device_noalloc", "gdtk_hip_chem_noalloc_max_cells",
+    "gdtk_hip_tt_create", "gdtk_hip_tt_destroy", "gdtk_hip_tt_step_device", "gdtk_hip_tt_stats",
 ]
 
 

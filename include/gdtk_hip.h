@@ -316,6 +316,10 @@ typedef struct gdtk_reaction {
     double eff[GDTK_MAX_SPECIES];
     int f_from_eq; double fA, fn, fC;           /* forward: Arrhenius or from K_c */
     int b_from_eq; double bA, bn, bC;           /* backward: Arrhenius or from K_c */
+    /* Park2TRateConstant (rate_constant.d:365-395): A Te^n exp(-C/Te) at Te = T^s Tv^(1-s);
+     * the two-temperature reactor only (gdtk_hip_chem_create refuses it) */
+    int f_park; double f_s;
+    int b_park; double b_s;
 } gdtk_reaction;
 
 typedef struct gdtk_mechanism {
@@ -500,6 +504,40 @@ int gdtk_hip_surrogate_project(int nsp, int n_elem, const double* elem, int n, c
 
 /* Message of the last failing call on this thread. */
 const char* gdtk_hip_last_error(void);
+
+/* ---- Two-temperature air reactor ------------------------------------------------------
+ * TwoTemperatureAirKinetics (src/kinetics/two_temperature_air_kinetics.d:44-706) on the
+ * 5-species neutral TwoTemperatureAir gas (src/gas/two_temperature_air.d): per substep the
+ * chemistry step (RKF; Park rates at T^s Tv^(1-s)) and a Cash-Karp step of the
+ * vibro-electronic energy u_v at fixed total energy (Landau-Teller exchange with
+ * Millikan-White + Park relaxation times and the chemical source of u_v), with tight
+ * temperature coupling as configured; per cell as FVCell.thermochemical_increment does
+ * (fvcell.d:858-972: retry with dt_chem = -1, then update_thermo_from_rhou). */
+typedef struct gdtk_tt_params {
+    int molecular[GDTK_MAX_SPECIES];   /* 1: species with a vibrational mode (N2, O2, NO;
+                                          two_temperature_air.d:25) */
+    double A_MW[GDTK_MAX_SPECIES];     /* Millikan-White A of those species (N2 220, O2 129,
+                                          NO 168: two_temperature_air_kinetics.d:714-720) */
+    double abs_tol, rel_tol;           /* energy step tolerances (1e-9: :375-376) */
+} gdtk_tt_params;
+
+/* Replaces init_thermochemical_reactor's TwoTemperatureAirKinetics branch
+ * (init_thermochemical_reactor.d:126-128).  The mechanism: RKF step, explicit backward
+ * rates (no equilibrium-constant rates), lists in ascending species order (as prep-chem
+ * writes them); 1 to 8 species, no free electrons. */
+int gdtk_hip_tt_create(const gdtk_tpg_gas* gas, const gdtk_mechanism* mech, const gdtk_tt_params* prm, int* tt);
+int gdtk_hip_tt_destroy(int tt);
+
+/* n cells on device arrays (species-major massf, outputs may alias inputs), asynchronous on
+ * stream: rho, T, Tv (T_modes[0]), u, uv (u_modes[0]), p, massf, dt_chem in; the same out;
+ * status 0 or -1 (both reactor attempts failed, or the Tv iteration did not converge). */
+int gdtk_hip_tt_step_device(int tt, int n, const double* rho, const double* T, const double* Tv, const double* u,
+                            const double* uv, const double* p, const double* massf, const double* dt_chem, double dt,
+                            double* T_out, double* Tv_out, double* u_out, double* uv_out, double* p_out,
+                            double* massf_out, double* dt_chem_out, int* status, void* stream);
+
+/* The last step (waits for it): {total substeps, max substeps, total step attempts, ms, cells}. */
+int gdtk_hip_tt_stats(int tt, double* stats, int nstats);
 
 #ifdef __cplusplus
 }

@@ -374,6 +374,39 @@ void ReactionMechanism::eval_rate_constants(const GasState& Qin) {
     }
 }
 
+void ReactionMechanism::eval_rate_constants_2T(double Tin, double Tv) {
+    double T = Tin;
+    if (T < T_lower_) T = T_lower_;
+    if (T > T_upper_) T = T_upper_;
+    for (int ir = 0; ir < nr; ++ir) {
+        const ReactionDef& d = defs_[ir];
+        if (d.frc.fromEqConst || d.brc.fromEqConst)
+            throw std::runtime_error("equilibrium-constant rates are not supported with two temperatures");
+        // rate_constant.d:72-75 (Arrhenius at T) and :391-394 (Park at T^s Tv^(1-s))
+        auto rc = [&](const RateConstantDef& k) {
+            double Te = T;
+            if (k.park) Te = std::pow(T, k.s) * std::pow(Tv, 1.0 - k.s);
+            return k.A * std::pow(Te, k.n) * std::exp(-k.C / Te);
+        };
+        k_f[ir] = rc(d.frc);
+        k_b[ir] = rc(d.brc);
+    }
+}
+
+double ReactionMechanism::rate(int ir, int isp) const {
+    // reaction.d:268-288 production(isp) - loss(isp)
+    const int nu = nu_[ir][isp];
+    const double prod = (nu > 0) ? nu * w_f_[ir] : (nu < 0) ? -nu * w_b_[ir] : 0.0;
+    const double loss = (nu > 0) ? nu * w_b_[ir] : (nu < 0) ? -nu * w_f_[ir] : 0.0;
+    return prod - loss;
+}
+
+bool ReactionMechanism::has_participant(int ir, int isp) const {
+    for (int p : participants_[ir])
+        if (p == isp) return true;
+    return false;
+}
+
 void ReactionMechanism::eval_split_rates(const double* conc, double* q, double* L) {
     for (int ir = 0; ir < nr; ++ir) {
         const ReactionDef& d = defs_[ir];

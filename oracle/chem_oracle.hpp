@@ -60,6 +60,9 @@ public:
 struct RateConstantDef {
     bool fromEqConst = false;
     double A = 0.0, n = 0.0, C = 0.0;
+    // Park2TRateConstant (rate_constant.d:365-395): A T^n exp(-C/T) at T = T^s Tv^(1-s)
+    bool park = false;
+    double s = 1.0;
 };
 
 // src/kinetics/reaction.d:58-457 (ElementaryReaction and AnonymousColliderReaction)
@@ -75,6 +78,13 @@ public:
     ReactionMechanism(const ThermallyPerfectGas* gm, const std::vector<ReactionDef>& defs, double T_lower,
                       double T_upper);
     void eval_rate_constants(const GasState& Q);                       // reaction_mechanism.d:50-63
+    // The same with a vibro-electronic temperature for Park rate constants (T is clamped to
+    // the temperature limits, Tv is not); equilibrium-constant rates are not supported.
+    void eval_rate_constants_2T(double T, double Tv);
+    // reaction_mechanism.d:118-121: production - loss of species isp by reaction ir at the
+    // rates of the last eval_split_rates call.
+    double rate(int ir, int isp) const;
+    bool has_participant(int ir, int isp) const;       // reaction_mechanism.d:133-142
     void eval_split_rates(const double* conc, double* q, double* L);   // reaction_mechanism.d:83-96
     void eval_rates(const double* conc, double* rates);                // reaction_mechanism.d:75-82
     double estimateStepSize(const double* conc);                       // reaction_mechanism.d:162-203

@@ -416,6 +416,90 @@ def chem_step_batch(gas_rec, mech_rec, rho, T, u, p, massf, dt_chem, dt, nthread
                 failed=failed)
 
 
+# Two-temperature air (tt_oracle.cpp)
+def _tt_lib():
+    L = _chem_lib()
+    if not getattr(L, "_tt_ready", False):
+        L.orc_tt_create.restype = vp
+        L.orc_tt_create.argtypes = [vp, vp, ip, dp]
+        L.orc_tt_destroy.argtypes = [vp]
+        L.orc_tt_update.argtypes = [vp, C.c_int, dp, dp, ip]
+        L.orc_tt_species_energy.argtypes = [vp, C.c_int, C.c_double, dp]
+        L.orc_tt_relaxation.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_tt_rate_constants.argtypes = [vp, C.c_double, C.c_double, dp, dp]
+        L.orc_tt_step_batch.argtypes = [vp, vp, ip, dp, C.c_int] + [dp] * 8 + [C.c_double, ip, ip, ip, C.c_int]
+        L._tt_ready = True
+    return L
+
+
+class TT:
+    """TwoTemperatureAir + TwoTemperatureAirKinetics of the oracle.  molecular[nsp], A_MW[nsp]
+    as in gdtk_amd.two_temp.tt_params."""
+
+    def __init__(self, gas_rec, mech_rec, molecular, A_MW):
+        self.gas_rec, self.mech_rec = gas_rec, mech_rec
+        self.nsp, self.nr = gas_rec.nsp, mech_rec.nr
+        self.mol = np.ascontiguousarray(molecular, dtype=np.int32)
+        self.A = np.ascontiguousarray(A_MW, dtype=np.float64)
+        self.h = _tt_lib().orc_tt_create(C.byref(gas_rec), C.byref(mech_rec), _i(self.mol)[1], _d(self.A)[1])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_tt_destroy(self.h)
+            self.h = None
+
+    def update(self, mode, massf, rho=0.0, p=0.0, T=0.0, Tv=0.0, u=0.0, uv=0.0):
+        """mode 'pT', 'rhou' or 'rhoT'; returns the state dict (and Newton iterations)."""
+        st, stp = _d([rho, p, T, Tv, u, uv])
+        mf, mfp = _d(massf)
+        ni = C.c_int(0)
+        rc = _tt_lib().orc_tt_update(self.h, {"pT": 0, "rhou": 1, "rhoT": 2}[mode], stp, mfp, C.byref(ni))
+        if rc != 0:
+            raise RuntimeError(err())
+        return dict(rho=st[0], p=st[1], T=st[2], Tv=st[3], u=st[4], uv=st[5], newton_iters=ni.value)
+
+    def species_energy(self, isp, T):
+        out = np.zeros(2)
+        _tt_lib().orc_tt_species_energy(self.h, isp, T, out.ctypes.data_as(dp))
+        return dict(e_ve=out[0], Cv_ve=out[1])
+
+    def relaxation(self, state, massf):
+        """(tau per species (0 for atoms), energy-exchange rate of u_v) at a state dict."""
+        st, stp = _d([state[k] for k in ("rho", "p", "T", "Tv", "u", "uv")])
+        mf, mfp = _d(massf)
+        tau = np.zeros(self.nsp)
+        rate = C.c_double(0.0)
+        _tt_lib().orc_tt_relaxation(self.h, stp, mfp, tau.ctypes.data_as(dp), C.byref(rate))
+        return tau, rate.value
+
+    def rate_constants(self, T, Tv):
+        kf, kb = np.zeros(self.nr), np.zeros(self.nr)
+        if _tt_lib().orc_tt_rate_constants(self.h, T, Tv, kf.ctypes.data_as(dp), kb.ctypes.data_as(dp)) != 0:
+            raise RuntimeError(err())
+        return kf, kb
+
+
+def tt_step_batch(gas_rec, mech_rec, molecular, A_MW, rho, T, Tv, u, uv, p, massf, dt_chem, dt, nthreads=1):
+    """Batched two-temperature thermochemical_increment; arrays are copied, results returned."""
+    n = len(rho)
+    T, Tv, u, uv, p, massf, dt_chem = (np.array(a, dtype=np.float64, copy=True)
+                                       for a in (T, Tv, u, uv, p, massf, dt_chem))
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    mol = np.ascontiguousarray(molecular, dtype=np.int32)
+    A = np.ascontiguousarray(A_MW, dtype=np.float64)
+    status = np.zeros(n, dtype=np.int32)
+    sub = np.zeros(n, dtype=np.int32)
+    att = np.zeros(n, dtype=np.int32)
+    failed = _tt_lib().orc_tt_step_batch(C.byref(gas_rec), C.byref(mech_rec), mol.ctypes.data_as(ip),
+                                         A.ctypes.data_as(dp), n, rho.ctypes.data_as(dp), T.ctypes.data_as(dp),
+                                         Tv.ctypes.data_as(dp), u.ctypes.data_as(dp), uv.ctypes.data_as(dp),
+                                         p.ctypes.data_as(dp), massf.ctypes.data_as(dp),
+                                         dt_chem.ctypes.data_as(dp), dt, status.ctypes.data_as(ip),
+                                         sub.ctypes.data_as(ip), att.ctypes.data_as(ip), nthreads)
+    return dict(T=T, Tv=Tv, u=u, uv=uv, p=p, massf=massf, dt_chem=dt_chem, status=status, substeps=sub,
+                attempts=att, failed=failed)
+
+
 def ffc_source(dims, face, other_face, orientation, a, b, n, nic, njc, nkc):
     """(i, j, k) of the full-face-copy source cell of ghost layer n at (a, b) of `face`."""
     out = (C.c_int * 3)()

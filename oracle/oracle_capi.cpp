@@ -406,6 +406,8 @@ struct OReaction {
     double eff[MS];
     int f_from_eq; double fA, fn, fC;
     int b_from_eq; double bA, bn, bC;
+    int f_park; double f_s;
+    int b_park; double b_s;
 };
 struct OMech {
     int nr;
@@ -482,10 +484,7 @@ int orc_tpg_update(void* gas, int mode, double* st, const double* massf, int* ne
     return 0;
 }
 
-void* orc_chem_create(const void* gasv, const void* mechv) {
-    const OGas* gd = static_cast<const OGas*>(gasv);
-    const OMech* md = static_cast<const OMech*>(mechv);
-    ThermallyPerfectGas* gas = make_tpg(gd);
+static std::vector<ReactionDef> make_defs(const OMech* md) {
     std::vector<ReactionDef> defs;
     for (int ir = 0; ir < md->nr; ++ir) {
         const OReaction& r = md->r[ir];
@@ -496,8 +495,28 @@ void* orc_chem_create(const void* gasv, const void* mechv) {
         for (int k = 0; k < r.neff; ++k) d.efficiencies.push_back({r.eff_sp[k], r.eff[k]});
         d.frc.fromEqConst = r.f_from_eq != 0; d.frc.A = r.fA; d.frc.n = r.fn; d.frc.C = r.fC;
         d.brc.fromEqConst = r.b_from_eq != 0; d.brc.A = r.bA; d.brc.n = r.bn; d.brc.C = r.bC;
+        d.frc.park = r.f_park != 0; d.frc.s = r.f_s;
+        d.brc.park = r.b_park != 0; d.brc.s = r.b_s;
         defs.push_back(d);
     }
+    return defs;
+}
+
+static ChemConfig make_cfg(const OMech* md) {
+    ChemConfig cfg;
+    cfg.method = md->method == 1 ? OdeMethod::rkf : OdeMethod::alpha_qss;
+    cfg.eps1 = md->eps1; cfg.eps2 = md->eps2; cfg.delta = md->delta; cfg.maxIters = md->maxIters;
+    cfg.errTol = md->errTol; cfg.tightTempCoupling = md->tightTempCoupling != 0;
+    cfg.maxSubcycles = md->maxSubcycles; cfg.maxAttempts = md->maxAttempts;
+    cfg.T_lower = md->T_lower; cfg.T_upper = md->T_upper;
+    return cfg;
+}
+
+void* orc_chem_create(const void* gasv, const void* mechv) {
+    const OGas* gd = static_cast<const OGas*>(gasv);
+    const OMech* md = static_cast<const OMech*>(mechv);
+    ThermallyPerfectGas* gas = make_tpg(gd);
+    std::vector<ReactionDef> defs = make_defs(md);
     ChemConfig cfg;
     cfg.method = md->method == 1 ? OdeMethod::rkf : OdeMethod::alpha_qss;
     cfg.eps1 = md->eps1; cfg.eps2 = md->eps2; cfg.delta = md->delta; cfg.maxIters = md->maxIters;
@@ -601,5 +620,132 @@ int orc_chem_step_batch(const void* gasv, const void* mechv, int n, const double
 }
 
 void orc_gas_destroy_tpg(void* g) { delete static_cast<ThermallyPerfectGas*>(g); }
+
+}  // extern "C"
+
+//---------------------------------------------------------------------------------
+// Two-temperature air (tt_oracle.hpp).  st = [rho, p, T, Tv, u, uv].
+#include "tt_oracle.hpp"
+
+namespace {
+struct OTT {
+    ThermallyPerfectGas* tpg;
+    TwoTemperatureAir* gas;
+    TwoTemperatureAirKinetics* kin;
+    double T_frozen;
+};
+void to_tt(const double* st, const double* massf, int nsp, TTState& Q) {
+    Q.rho = st[0]; Q.p = st[1]; Q.T = st[2]; Q.Tv = st[3]; Q.u = st[4]; Q.uv = st[5];
+    Q.massf.assign(massf, massf + nsp);
+}
+void from_tt(const TTState& Q, double* st, double* massf) {
+    st[0] = Q.rho; st[1] = Q.p; st[2] = Q.T; st[3] = Q.Tv; st[4] = Q.u; st[5] = Q.uv;
+    if (massf) for (size_t i = 0; i < Q.massf.size(); ++i) massf[i] = Q.massf[i];
+}
+}  // namespace
+
+extern "C" {
+
+// molecular[nsp]: 1 for species with a vibrational mode; A_MW[nsp]: Millikan-White A.
+void* orc_tt_create(const void* gasv, const void* mechv, const int* molecular, const double* A_MW) {
+    const OGas* gd = static_cast<const OGas*>(gasv);
+    const OMech* md = static_cast<const OMech*>(mechv);
+    ThermallyPerfectGas* tpg = make_tpg(gd);
+    std::vector<int> mol(molecular, molecular + gd->nsp);
+    auto* gas = new TwoTemperatureAir(tpg->mol_masses, tpg->curves, mol);
+    std::vector<double> A(A_MW, A_MW + gd->nsp);
+    auto* kin = new TwoTemperatureAirKinetics(gas, tpg, make_defs(md), make_cfg(md), A);
+    return new OTT{tpg, gas, kin, md->T_frozen};
+}
+
+void orc_tt_destroy(void* h) {
+    auto* t = static_cast<OTT*>(h);
+    delete t->kin;
+    delete t->gas;
+    delete t->tpg;
+    delete t;
+}
+
+// mode 0 pT (rho, u, uv out), 1 rhou (T, Tv, p out), 2 rhoT (p, u, uv out); returns 0 or -1.
+int orc_tt_update(void* h, int mode, double* st, const double* massf, int* newton_iters) {
+    auto* t = static_cast<OTT*>(h);
+    TTState Q;
+    to_tt(st, massf, t->gas->nsp, Q);
+    try {
+        switch (mode) {
+        case 0: t->gas->update_thermo_from_pT(Q); break;
+        case 1: t->gas->update_thermo_from_rhou(Q); break;
+        case 2: t->gas->update_thermo_from_rhoT(Q); break;
+        default: g_err = "bad mode"; return -2;
+        }
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    if (newton_iters) *newton_iters = t->gas->newton_iters;
+    from_tt(Q, st, nullptr);
+    return 0;
+}
+
+// out = {e_ve(T, isp), Cv_ve(T, isp)} of one species.
+void orc_tt_species_energy(void* h, int isp, double T, double* out) {
+    auto* t = static_cast<OTT*>(h);
+    out[0] = t->gas->vibElecEnergy(T, isp);
+    out[1] = t->gas->vibElecCv(T, isp);
+}
+
+// Relaxation time of each species (0 for atoms) and the energy-exchange rate at a state.
+void orc_tt_relaxation(void* h, const double* st, const double* massf, double* tau, double* rate) {
+    auto* t = static_cast<OTT*>(h);
+    TTState Q;
+    to_tt(st, massf, t->gas->nsp, Q);
+    *rate = t->kin->evalRate(Q);
+    for (int i = 0; i < t->gas->nsp; ++i) tau[i] = t->gas->molecular[i] ? t->kin->evalRelaxationTime(Q, i) : 0.0;
+}
+
+int orc_tt_rate_constants(void* h, double T, double Tv, double* kf, double* kb) {
+    auto* t = static_cast<OTT*>(h);
+    try {
+        t->kin->chem.rmech.eval_rate_constants_2T(T, Tv);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    for (int i = 0; i < t->kin->chem.rmech.nr; ++i) {
+        kf[i] = t->kin->chem.rmech.k_f[i];
+        kb[i] = t->kin->chem.rmech.k_b[i];
+    }
+    return 0;
+}
+
+// Batched two-temperature thermochemical_increment over n cells (species-major massf), with
+// nthreads OpenMP threads, one oracle object per thread.  status[n] 0 / -1; sub, att per cell.
+int orc_tt_step_batch(const void* gasv, const void* mechv, const int* molecular, const double* A_MW, int n,
+                      const double* rho, double* T, double* Tv, double* u, double* uv, double* p, double* massf,
+                      double* dt_chem, double dt, int* status, int* sub, int* att, int nthreads) {
+    int failed = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : failed)
+    {
+        OTT* t = static_cast<OTT*>(orc_tt_create(gasv, mechv, molecular, A_MW));
+        const int nsp = t->gas->nsp;
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 0; i < n; ++i) {
+            TTState Q;
+            Q.rho = rho[i]; Q.T = T[i]; Q.Tv = Tv[i]; Q.u = u[i]; Q.uv = uv[i]; Q.p = p[i];
+            Q.massf.resize(nsp);
+            for (int s = 0; s < nsp; ++s) Q.massf[s] = massf[(size_t)s * n + i];
+            ChemStats cs;
+            int rc = tt_thermochemical_increment(*t->kin, *t->gas, Q, dt, dt_chem[i], t->T_frozen, &cs);
+            status[i] = rc;
+            if (sub) sub[i] = cs.subcycles;
+            if (att) att[i] = cs.attempts;
+            if (rc != 0) { ++failed; continue; }
+            T[i] = Q.T; Tv[i] = Q.Tv; u[i] = Q.u; uv[i] = Q.uv; p[i] = Q.p;
+            for (int s = 0; s < nsp; ++s) massf[(size_t)s * n + i] = Q.massf[s];
+        }
+        orc_tt_destroy(t);
+    }
+    return failed;
+}
 
 }  // extern "C"
