@@ -734,6 +734,84 @@ def bench_coupled(steps, snapshot_every=250, tols=(5.0e-4, 2.0e-3), route_ratio=
     }
 
 
+def bench_two_temperature(ncells, dt=1.0e-7, reps=3, with_cpu=True, seed=77, log=None):
+    """The two-temperature air reactor (TwoTemperatureAirKinetics, Gupta 5-species r1-r6,
+    RKF, tight coupling) over ncells post-shock states (T 4000-14000 K, Tv 500-9000 K,
+    p 1-30 kPa, partly dissociated air; no substep history), timed with HIP events around
+    the kernel; the oracle on a bounded sample of the same states as CPU baseline."""
+    import os
+
+    import numpy as np
+    import torch
+
+    from gdtk_amd import data as gdata
+    from gdtk_amd.chem import gas_record, mech_record
+    from gdtk_amd.two_temp import TwoTemperatureReactor, tt_params
+    from oracle import oracle as O
+    gd, md = gdata.load("tpg-air-5sp"), gdata.load("gupta-air-5sp-2T")
+    gas, mech = gas_record(gd), mech_record(md)
+    prm = tt_params(gd["species"])
+    mol, A = [prm.molecular[i] for i in range(5)], [prm.A_MW[i] for i in range(5)]
+    ot = O.TT(gas, mech, mol, A)
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    Y = rng.dirichlet([7.0, 2.0, 0.3, 0.3, 0.3], size=ncells).T.copy()
+    T = rng.uniform(4000.0, 14000.0, ncells)
+    Tv = rng.uniform(500.0, 9000.0, ncells)
+    p = rng.uniform(1.0e3, 3.0e4, ncells)
+    cols = {k: np.zeros(ncells) for k in ("rho", "u", "uv")}
+    for i in range(ncells):
+        st = ot.update("pT", Y[:, i], p=p[i], T=T[i], Tv=Tv[i])
+        cols["rho"][i], cols["u"][i], cols["uv"][i] = st["rho"], st["u"], st["uv"]
+    cells = dict(rho=cols["rho"], T=T, Tv=Tv, u=cols["u"], uv=cols["uv"], p=p,
+                 massf=np.ascontiguousarray(Y).reshape(-1), dt_chem=np.full(ncells, -1.0))
+    if log:
+        log(f"two-temperature: {ncells} states in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    tin = {k: torch.tensor(v, dtype=torch.float64, device=dev) for k, v in cells.items()}
+    tout = {k: torch.empty_like(tin[k]) for k in ("T", "Tv", "u", "uv", "p", "massf", "dt_chem")}
+    status = torch.empty(ncells, dtype=torch.int32, device=dev)
+    r = TwoTemperatureReactor(gas, mech, prm)
+    stream = torch.cuda.current_stream()
+    ins = [tin[k].data_ptr() for k in ("rho", "T", "Tv", "u", "uv", "p", "massf", "dt_chem")]
+    outs = [tout[k].data_ptr() for k in ("T", "Tv", "u", "uv", "p", "massf", "dt_chem")]
+    ms = []
+    for rep in range(reps + 1):
+        r.step_device(ncells, *ins, dt, *outs, status.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        st = r.stats()
+        if rep:
+            ms.append(st["ms"])
+    failed = int((status != 0).sum().item())
+    ms_avg = sum(ms) / len(ms)
+    res = {"workload": "two-temperature air reactor, 5-species Gupta r1-r6 (Park rates, s = 0.5), RKF + tight coupling",
+           "cells": ncells, "dt": dt, "ms": ms_avg, "cells_per_s": ncells / (ms_avg * 1e-3),
+           "substeps_total": st["substeps_total"], "substeps_max": st["substeps_max"],
+           "attempts_total": st["attempts_total"], "failed": failed, "dtype": "f64"}
+    res["value"], res["unit"] = res["cells_per_s"], "cells/s"
+    if with_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        m = min(ncells, 2048)
+        sub = {k: (v[:m] if k != "massf" else np.ascontiguousarray(v.reshape(5, ncells)[:, :m]).reshape(-1))
+               for k, v in cells.items()}
+        done, el = 0, 0.0
+        while el < 8.0 and done < 64 * m:
+            t0 = time.perf_counter()
+            O.tt_step_batch(gas, mech, mol, A, sub["rho"], sub["T"], sub["Tv"], sub["u"], sub["uv"], sub["p"],
+                            sub["massf"], sub["dt_chem"], dt, nthreads=threads)
+            el += time.perf_counter() - t0
+            done += m
+        res["cpu_baseline"] = {"value": done / el, "unit": "cells/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle on the first {m} of the same states, {done} cell-steps in "
+                                         f"{el:.1f} s on {threads} OpenMP threads"}
+        res["speedup_vs_cpu"] = res["cells_per_s"] / res["cpu_baseline"]["value"]
+    r.close()
+    if log:
+        log(f"two-temperature: {res['cells_per_s']:.4g} cells/s ({ms_avg:.2f} ms for {ncells} cells), "
+            f"{st['substeps_total'] / ncells:.1f} substeps per cell, max {st['substeps_max']}")
+    return res
+
+
 def copy_probe(nbytes=1 << 30):
     """A plain device copy of known size for HBM counter calibration (FETCH_SIZE/WRITE_SIZE)."""
     import torch
@@ -1045,6 +1123,9 @@ def main():
     ap.add_argument("--no-n90", action="store_true", help="skip the reference's n90 reacting cylinder case")
     ap.add_argument("--n90-only", action="store_true", help="run only the n90 case")
     ap.add_argument("--flow-io-only", action="store_true", help="measure only the flow-file write/read")
+    ap.add_argument("--tt-cells", type=int, default=1 << 18,
+                    help="cells of the two-temperature reactor leg (0 skips it)")
+    ap.add_argument("--tt-only", action="store_true", help="measure only the two-temperature reactor")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -1085,6 +1166,8 @@ def main():
         return emit(bench_n90(300 if with_cpu else 0, log=log))
     if args.flow_io_only:
         return emit(bench_flow_io(nic, njc, nkc, log=log))
+    if args.tt_only:
+        return emit(bench_two_temperature(args.tt_cells, with_cpu=with_cpu, log=log))
     if args.lmr_only:
         return emit(bench_lmr(nic, njc, nkc, args.lmr_steps, log=log))
     if args.adaptive_only:
@@ -1118,6 +1201,8 @@ def main():
         legs["adaptive"] = bench_adaptive(nic, njc, nkc, args.adaptive_steps, log=log)
     if args.lmr_steps > 0:
         legs["lmr"] = bench_lmr(nic, njc, nkc, args.lmr_steps, log=log)
+    if args.tt_cells > 0:
+        legs["two_temperature"] = bench_two_temperature(args.tt_cells, with_cpu=with_cpu, log=log)
     if not args.no_flow_io:
         legs["flow_io"] = bench_flow_io(nic, njc, nkc, log=log)
     if not args.no_n90:
