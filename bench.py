@@ -99,7 +99,7 @@ def source_digest(names):
 # Sources of each kernel whose HBM traffic is taken from profiles/traffic.json.
 KERNEL_SOURCES = {
     "k_face_flux_batch": ("flux_kernels.hip", "gdtk_device.hpp", "kernel_common.hpp", "block.hpp"),
-    "k_face_flux_species": ("flux_species.hip", "flux_species.hpp", "chem_device.hpp", "gdtk_device.hpp",
+    "k_face_flux_species_pair": ("flux_species.hip", "flux_species.hpp", "chem_device.hpp", "gdtk_device.hpp",
                             "kernel_common.hpp", "block.hpp"),
 }
 
@@ -906,7 +906,7 @@ def bench_coupled_headline(steps, warmup, spinup, with_cpu=True, log=None):
     face_avg = face["ms"] / max(face["launches"], 1)
     bpf = species_face_bytes(2, 5)
     face_ach = bpf * face["items_per_launch"] / (face_avg * 1e-3) / 1e9
-    kname = "k_face_flux_species<2,0,5>"
+    kname = "k_face_flux_species_pair<2,0,5>"
     roof_face = {"bound": "hbm", "achieved": face_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": face_ach / HBM_PEAK_GBS, "traffic": measured_traffic(kname, face["items_per_launch"]),
                  "kernel": kname, "avg_launch_ms": face_avg, "bytes_per_face": bpf,
@@ -915,7 +915,7 @@ def bench_coupled_headline(steps, warmup, spinup, with_cpu=True, log=None):
     chem_flops = per_att * chs["attempts_total"] + per_it * chs["iters_total"]
     chem_ach = chem_flops / max(chs["ms_integrate"] * 1e-3, 1e-12) / 1e12
     roof_chem = {"bound": "fp64", "achieved": chem_ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                 "frac": chem_ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_chem_integrate<5,5,0>",
+                 "frac": chem_ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_chem_integrate<5,5,0,false,true>",
                  "avg_launch_ms": chs["ms_integrate"], "flops_per_launch": chem_flops}
     # the flux face launches (two per step) against the whole chemistry step
     dominant = "face_flux" if 2 * face_avg >= per_step["chemistry"] else "chemistry"
