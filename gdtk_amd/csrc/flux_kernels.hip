@@ -275,82 +275,113 @@ __global__ __launch_bounds__(256) void k_cell_update_batch(const BlockDev* __res
     cell_update_one<DIM, STAGE>(b, gas, *dtp, lim.low_T, lim.max_vel, lim.min_T, lim.max_T, t);
 }
 
-// Cell signal frequency (fvcell.d:975-1058) and per-workgroup reductions for the
-// block time-step estimate (fluidblock.d:987-1084).  part[3*wg] = {min dt_local,
-// max cfl_local, -}; the last interior cell writes its signal to last_signal.
+// Ordered integer keys of doubles: key(x) < key(y) exactly when x < y (NaN excluded), so an
+// integer atomic maximum is an exact floating-point maximum.  A slot holding 0 is empty.
+__device__ __forceinline__ unsigned long long dt_key(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dt_unkey(unsigned long long k) {
+    return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+}
+
+// The block time-step estimates of many blocks in one launch (fluidblock.d:987-1084 per
+// block).  Each lane takes one cell's signal frequency (fvcell.d:975-1058) and its dt_local
+// and cfl_local; the minimum and the maximum are reduced across the wavefront with lane
+// exchanges, across the workgroup's four wavefronts through LDS, and enter the block's slots
+// with one atomic maximum each (dt as the complement of its key), so the result is the
+// exact minimum/maximum whatever the order.  Like fmin/fmax a NaN never wins.  The block's
+// last cell stores its signal (fluidblock.d:1081).  The last workgroup to finish (a device
+// counter) writes out[3q..3q+2] = {min dt_local, max cfl_local, last signal} for every
+// block and empties the slots and the counter for the next launch.  A workgroup serves 256
+// cells of one block: start[q] is block q's first launch index (a multiple of 256), base
+// the launch index of workgroup 0.
 template <int DIM>
-__global__ void k_dt_partial(BlockDev b, double dt_current, double cfl_value, double* part, double* last_signal) {
-    __shared__ double sdt[256], scfl[256];
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t ncell = (int64_t)b.nic * b.njc * b.nkc;
+__global__ __launch_bounds__(256) void k_dt_batch(const BlockDev* __restrict__ tab, const int64_t* __restrict__ start,
+                                                  int nblocks, int64_t base, double dt_current, double cfl_value,
+                                                  unsigned long long* slots, unsigned* done, unsigned nwg,
+                                                  double* out) {
+    __shared__ double wdt[4], wcfl[4];
+    __shared__ int am_last;
+    const int64_t w0 = base + (int64_t)blockIdx.x * blockDim.x;
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= w0) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const BlockDev& b = tab[q];
+    const int64_t t = w0 - start[q] + threadIdx.x;
+    const int64_t ncell = (int64_t)b.nic * b.njc * b.nkc;
     double dtl = __builtin_huge_val(), cfl = -__builtin_huge_val();
-    bool have = t < ncell;
-    if (have) {
-        int i = (int)(t % b.nic);
-        int64_t rest = t / b.nic;
-        int j = (int)(rest % b.njc);
-        int k = (int)(rest / b.njc);
+    if (t < ncell) {
+        const int i = (int)(t % b.nic);
+        const int64_t rest = t / b.nic;
+        const int j = (int)(rest % b.njc);
+        const int k = (int)(rest / b.njc);
         const int64_t np = b.npad;
-        int64_t c = pidx(b, i, j, k);
+        const int64_t c = pidx(b, i, j, k);
         const double* fs = b.fs_in;
-        Vec3d v{fs[c + FS_VX * np], fs[c + FS_VY * np], fs[c + FS_VZ * np]};
-        double a = fs[c + FS_A * np];
-        FaceGeom gN = load_face(b.fgeom[1], b.nface[1], face_index(b, 1, i, j + 1, k));
-        FaceGeom gE = load_face(b.fgeom[0], b.nface[0], face_index(b, 0, i + 1, j, k));
-        double un_N = fabs(v.x * gN.n.x + v.y * gN.n.y + v.z * gN.n.z);
-        double un_E = fabs(v.x * gE.n.x + v.y * gE.n.y + v.z * gE.n.z);
+        const Vec3d v{fs[c + FS_VX * np], fs[c + FS_VY * np], fs[c + FS_VZ * np]};
+        const double a = fs[c + FS_A * np];
+        const FaceGeom gN = load_face(b.fgeom[1], b.nface[1], face_index(b, 1, i, j + 1, k));
+        const FaceGeom gE = load_face(b.fgeom[0], b.nface[0], face_index(b, 0, i + 1, j, k));
+        const double un_N = fabs(v.x * gN.n.x + v.y * gN.n.y + v.z * gN.n.z);
+        const double un_E = fabs(v.x * gE.n.x + v.y * gE.n.y + v.z * gE.n.z);
         double signal = 0.0;
-        double signalN = (un_N + a) / b.len[c + 1 * np];
+        const double signalN = (un_N + a) / b.len[c + 1 * np];
         signal = fmax(signal, signalN);
-        double signalE = (un_E + a) / b.len[c + 0 * np];
+        const double signalE = (un_E + a) / b.len[c + 0 * np];
         signal = fmax(signal, signalE);
         if (DIM == 3) {
-            FaceGeom gT = load_face(b.fgeom[2], b.nface[2], face_index(b, 2, i, j, k + 1));
-            double un_T = fabs(v.x * gT.n.x + v.y * gT.n.y + v.z * gT.n.z);
-            double signalT = (un_T + a) / b.len[c + 2 * np];
+            const FaceGeom gT = load_face(b.fgeom[2], b.nface[2], face_index(b, 2, i, j, k + 1));
+            const double un_T = fabs(v.x * gT.n.x + v.y * gT.n.y + v.z * gT.n.z);
+            const double signalT = (un_T + a) / b.len[c + 2 * np];
             signal = fmax(signal, signalT);
         }
         cfl = dt_current * signal;
         dtl = cfl_value / signal;
-        if (t == ncell - 1) *last_signal = signal;
-    }
-    sdt[threadIdx.x] = dtl;
-    scfl[threadIdx.x] = cfl;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            sdt[threadIdx.x] = fmin(sdt[threadIdx.x], sdt[threadIdx.x + s]);
-            scfl[threadIdx.x] = fmax(scfl[threadIdx.x], scfl[threadIdx.x + s]);
+        if (t == ncell - 1) {
+            atomicExch(&slots[3 * q + 2], (unsigned long long)__double_as_longlong(signal));
+            __threadfence();
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sdt[0];
-        part[2 * blockIdx.x + 1] = scfl[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        dtl = fmin(dtl, __shfl_xor(dtl, off));
+        cfl = fmax(cfl, __shfl_xor(cfl, off));
     }
-}
-
-__global__ void k_dt_final(const double* part, int nparts, double* out) {
-    __shared__ double sdt[256], scfl[256];
-    double dtl = __builtin_huge_val(), cfl = -__builtin_huge_val();
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        dtl = fmin(dtl, part[2 * p]);
-        cfl = fmax(cfl, part[2 * p + 1]);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wdt[wave] = dtl;
+        wcfl[wave] = cfl;
     }
-    sdt[threadIdx.x] = dtl;
-    scfl[threadIdx.x] = cfl;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            sdt[threadIdx.x] = fmin(sdt[threadIdx.x], sdt[threadIdx.x + s]);
-            scfl[threadIdx.x] = fmax(scfl[threadIdx.x], scfl[threadIdx.x + s]);
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        out[0] = sdt[0];
-        out[1] = scfl[0];
+        const int nw = (int)(blockDim.x >> 6);
+        double d = wdt[0], cm = wcfl[0];
+        for (int w = 1; w < nw; ++w) {
+            d = fmin(d, wdt[w]);
+            cm = fmax(cm, wcfl[w]);
+        }
+        if (d == d) atomicMax(&slots[3 * q], ~dt_key(d));
+        if (cm == cm) atomicMax(&slots[3 * q + 1], dt_key(cm));
+        __threadfence();
+        am_last = (atomicAdd(done, 1u) == nwg - 1) ? 1 : 0;
     }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    for (int r = threadIdx.x; r < nblocks; r += blockDim.x) {
+        const unsigned long long kd = atomicExch(&slots[3 * r], 0ull);
+        const unsigned long long kc = atomicExch(&slots[3 * r + 1], 0ull);
+        const unsigned long long ks = atomicExch(&slots[3 * r + 2], 0ull);
+        out[3 * r] = kd ? dt_unkey(~kd) : __builtin_huge_val();
+        out[3 * r + 1] = kc ? dt_unkey(kc) : -__builtin_huge_val();
+        out[3 * r + 2] = __longlong_as_double((long long)ks);
+    }
+    if (threadIdx.x == 0) atomicExch(done, 0u);
 }
 
 // Encode U from the flow state (fvcell.d:511-583), single species: used at start-up.
@@ -469,14 +500,17 @@ hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsi
     return hipGetLastError();
 }
 
-int dt_partial_count(const BlockDev& b) { return (int)nblk((int64_t)b.nic * b.njc * b.nkc, 256); }
-
-hipError_t launch_dt(const BlockDev& b, double dt_current, double cfl, double* part, double* out3, hipStream_t s) {
-    int64_t n = (int64_t)b.nic * b.njc * b.nkc;
-    int np = (int)nblk(n, 256);
-    if (b.dims == 3) hipLaunchKernelGGL((k_dt_partial<3>), dim3(np), dim3(256), 0, s, b, dt_current, cfl, part, out3 + 2);
-    else hipLaunchKernelGGL((k_dt_partial<2>), dim3(np), dim3(256), 0, s, b, dt_current, cfl, part, out3 + 2);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, part, np, out3);
+hipError_t launch_dt_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t base, int64_t count,
+                           int dims, double dt_current, double cfl, unsigned long long* slots, unsigned* done,
+                           double* out, hipStream_t s) {
+    if (count <= 0 || count % 256 != 0) return hipErrorInvalidValue;
+    const unsigned nwg = (unsigned)(count / 256);
+    if (dims == 3)
+        hipLaunchKernelGGL((k_dt_batch<3>), dim3(nwg), dim3(256), 0, s, tab, start, nblocks, base, dt_current, cfl,
+                           slots, done, nwg, out);
+    else
+        hipLaunchKernelGGL((k_dt_batch<2>), dim3(nwg), dim3(256), 0, s, tab, start, nblocks, base, dt_current, cfl,
+                           slots, done, nwg, out);
     return hipGetLastError();
 }
 

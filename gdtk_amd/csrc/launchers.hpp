@@ -16,8 +16,13 @@ struct CellLimits {
     double max_T = 50000.0;
 };
 
-int dt_partial_count(const BlockDev& b);
-hipError_t launch_dt(const BlockDev& b, double dt_current, double cfl, double* part, double* out3, hipStream_t s);
+// Block time-step estimates of blocks [0, nblocks) of tab in one launch of count lanes
+// (count a multiple of 256; start[q] = block q's first launch index, base = the launch
+// index of the first lane).  out[3q..3q+2] = {min dt_local, max cfl_local, last signal};
+// slots (3 per block) and done start at zero and are left at zero.
+hipError_t launch_dt_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t base, int64_t count,
+                           int dims, double dt_current, double cfl, unsigned long long* slots, unsigned* done,
+                           double* out, hipStream_t s);
 hipError_t launch_encode(const BlockDev& b, double* U, hipStream_t s);
 hipError_t launch_decode(const BlockDev& b, const IdealGasDev& gas, double* U, double* fs, double low_T,
                          hipStream_t s);

@@ -51,7 +51,7 @@ struct Block {
     gdtk_block_desc desc;
     int dims, nic, njc, nkc, ncq;
     BlockDev dev{};
-    DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], dtpart, dtout, fs_save;
+    DevBuf fs[2], U[2], dUdt0, len, vol, fgeom[3], flux[3], fs_save;
     DevArr<unsigned char> shock;  // padded cells (adaptive flux calculator)
     DevArr<unsigned char> fshock;  // faces of the three directions, one after another
     // Flow-file fields the update does not touch: cell centroids (3 planes of active cells)
@@ -124,7 +124,13 @@ struct Sim {
     // Face-per-thread launches over all blocks: first launch index of each block's faces
     // (per direction) and cells, aligned to the 256-thread workgroup.
     DevArr<int64_t> fstart[3], cstart;
+    std::vector<int64_t> cstart_h;
     int64_t ftotal[3] = {0, 0, 0}, ctotal = 0;
+    // Block time-step estimates: the batched reduction's slots (3 per block) and counter,
+    // both left empty by every launch, and its per-block results {dt, cfl, last signal}.
+    DevArr<unsigned long long> dt_slots;
+    DevArr<unsigned> dt_done;
+    DevArr<double> dt_out;
     // SimState (simcore.d)
     double time = 0.0, dt_global = 0.0, dt_allow = 0.0, cfl_max = 0.0;
     int step = 0, attempts_last = 0;
@@ -255,6 +261,10 @@ struct Sim {
         ctotal = st[nb];
         cstart.alloc(st.size());
         HIPCHK(hipMemcpy(cstart.p, st.data(), st.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        cstart_h = st;
+        dt_slots.alloc(3 * nb);
+        dt_done.alloc(1);
+        dt_out.alloc(3 * nb);
     }
 
     // Face-flux arrays of the face-per-thread kernels, allocated the first time they run.
@@ -657,8 +667,6 @@ struct Sim {
                 off += d.nface[q];
             }
         }
-        b->dtpart.alloc(2 * (size_t)dt_partial_count(d) + 2);
-        b->dtout.alloc(4);
         refresh_dev(*b);
         *id = b->id;
         blocks.push_back(std::move(b));
@@ -978,13 +986,17 @@ struct Sim {
         return GDTK_OK;
     }
 
-    void launch_block_dt(Block& b, double dt_current, double cfl) {
-        refresh_dev(b);
-        HIPCHK(launch_dt(b.dev, dt_current, cfl, b.dtpart.p, b.dtout.p, stream));
+    // Time-step estimates of blocks [q0, q0 + n) from the current flow state, one launch;
+    // results at dt_out[3 q ...].
+    void launch_dt(int q0, int n, double dt_current, double cfl) {
+        finalize();
+        const int64_t base = cstart_h[q0], count = cstart_h[q0 + n] - base;
+        HIPCHK(launch_dt_batch(tab(1, cur, u0) + q0, cstart.p + q0, n, base, count, cfg.dimensions, dt_current, cfl,
+                               dt_slots.p + 3 * q0, dt_done.p, dt_out.p + 3 * q0, stream));
     }
 
     // fluidblock.d:987-1084 from the reduced {dt_allow, cfl_max, last signal}.
-    static void finish_block_dt(const double h[4], bool check_cfl, double out[3]) {
+    static void finish_block_dt(const double h[3], bool check_cfl, double out[3]) {
         double dt_allow = h[0], cflm = h[1], last_signal = h[2];
         const double cfl_allow = 1.2, cfl_adjust = 0.5;  // fluidblock.d:1011, 1002
         if (check_cfl && (cflm < 0.0 || cflm > cfl_allow)) {
@@ -997,27 +1009,26 @@ struct Sim {
     }
 
     void block_dt(Block& b, double dt_current, double cfl, bool check_cfl, double out[3]) {
-        launch_block_dt(b, dt_current, cfl);
-        double h[4];
-        HIPCHK(hipMemcpyAsync(h, b.dtout.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        launch_dt(b.id, 1, dt_current, cfl);
+        double h[3];
+        HIPCHK(hipMemcpyAsync(h, dt_out.p + 3 * b.id, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         finish_block_dt(h, check_cfl, out);
     }
 
-    // simcore_gasdynamic_step.d:60-159: every block's estimate is launched, then read back
-    // with one synchronisation.
+    // simcore_gasdynamic_step.d:60-159: every block's estimate in one launch, read back
+    // with one copy and one synchronisation.
     void determine_time_step_size() {
         if ((step % cfg.cfl_count) != 0) return;
         const size_t nb = blocks.size();
-        std::vector<double> h(4 * nb);
-        for (auto& bp : blocks) launch_block_dt(*bp, dt_global, cfg.cfl_value);
-        for (size_t q = 0; q < nb; ++q)
-            HIPCHK(hipMemcpyAsync(&h[4 * q], blocks[q]->dtout.p, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        std::vector<double> h(3 * nb);
+        launch_dt(0, (int)nb, dt_global, cfg.cfl_value);
+        HIPCHK(hipMemcpyAsync(h.data(), dt_out.p, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         double da = std::numeric_limits<double>::max(), cm = 0.0;
         for (size_t q = 0; q < nb; ++q) {
             double r[3];
-            finish_block_dt(&h[4 * q], step > 0, r);
+            finish_block_dt(&h[3 * q], step > 0, r);
             da = std::min(da, r[0]);
             cm = std::max(cm, r[1]);
         }
