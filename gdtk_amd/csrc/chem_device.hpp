@@ -187,17 +187,22 @@ __host__ __device__ inline double cea_s(const CEADev& c, double T) {
 }
 
 // therm_perf_gas_mix_eos.d:61-67 with gas_model.d:410-419 (mass_average)
-template <int NSP>
+// SERIAL: a scheduling barrier after each species, so that the species' evaluations do
+// not interleave (fewer registers live at once, for kernels whose occupancy they limit).
+template <int NSP, bool SERIAL = false>
 __device__ __forceinline__ double tpg_energy(const ChemDev& cd, const double* massf, double T) {
     const double logT = log(T);
     double u = 0.0;
 #pragma unroll
-    for (int i = 0; i < NSP; ++i) u += massf[i] * (cea_h(cd.cea[i], T, logT) - cd.R[i] * T);
+    for (int i = 0; i < NSP; ++i) {
+        u += massf[i] * (cea_h(cd.cea[i], T, logT) - cd.R[i] * T);
+        if (SERIAL) __builtin_amdgcn_sched_barrier(0);
+    }
     return u;
 }
 // tpg_energy and tpg_Cv at one temperature (the Newton iteration of T(u) evaluates both at
 // each iterate), sharing each species' coefficient selection and log(T).
-template <int NSP>
+template <int NSP, bool SERIAL = false>
 __device__ __forceinline__ void tpg_energy_Cv(const ChemDev& cd, const double* massf, double T, double& u,
                                               double& Cv) {
     const double logT = log(T);
@@ -208,6 +213,7 @@ __device__ __forceinline__ void tpg_energy_Cv(const ChemDev& cd, const double* m
         cea_h_Cp(cd.cea[i], T, logT, h, Cp);
         uu += massf[i] * (h - cd.R[i] * T);
         v += massf[i] * (Cp - cd.R[i]);
+        if (SERIAL) __builtin_amdgcn_sched_barrier(0);
     }
     u = uu;
     Cv = v;
@@ -240,7 +246,21 @@ __device__ __forceinline__ double tpg_Rmix(const ChemDev& cd, const double* mass
 // and u, and a converged root then overwrites T, so u ends as the energy of the last
 // evaluated temperature.  Returns false where the reference throws; T is then restored
 // and u = energy(T), which is the state the reference leaves behind.
-template <int NSP>
+//
+// The reference's first pass evaluates the energy at both ends of the bracket
+// [T1, T1 + 1000] only to learn the signs of f = e_tgt - u(T) there, and in the usual case
+// (the root inside the bracket) those signs are f(T1) > 0 > f(T2), since u(T) increases
+// with T.  A speculative pass (pass -1) runs the same Newton iteration with that bracket
+// assignment and skips the two evaluations.  Its root is kept only when it proves the
+// assumption: a Newton-converged T* whose last evaluation lies within 1 K of the root
+// (|f| <= Cv) and T* at least 2 K inside both ends.  The root then lies >= 0.99 K inside,
+// where u(T1) and u(T2) differ from e_tgt by >= 0.99 K x Cv_min, with Cv >= 1.5 R_mix for
+// any gas, far beyond rounding, so the reference's pass 0 sees exactly the assumed signs
+// and takes the identical iteration from Tsave: same T, same u, bit for bit.  Any other
+// outcome falls through to the reference's passes 0 and 1 from the start.  SPEC = false
+// runs the reference's passes only (for callers whose guesses often lie far from the root:
+// one failing lane makes its whole wavefront run both).  SERIAL as for tpg_energy.
+template <int NSP, bool SERIAL = false, bool SPEC = true>
 __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const double* massf, double& T, double& u) {
     const double Tsave = T;
     const double TOL = 1.0e-6;
@@ -248,53 +268,68 @@ __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const 
     const double delT = 1000.0;
     const double T1 = fmax(T - 0.5 * delT, CH_T_MIN);
     const double T2 = T1 + delT;
-    for (int pass = 0; pass < 2; ++pass) {
-        const double xMin = (pass == 0) ? T1 : CH_T_MIN;
-        const double xMax = (pass == 0) ? T2 : CH_T_MAX;
-        double xL = xMin, xH = xMax;
-        T = xL;
-        u = tpg_energy<NSP>(cd, massf, T);
-        const double fL = e_tgt - u;
-        T = xH;
-        u = tpg_energy<NSP>(cd, massf, T);
-        const double fH = e_tgt - u;
-        if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) continue;
-        if (fL == 0.0) { T = xMin; return true; }
-        if (fH == 0.0) { T = xMax; return true; }
-        if (fL < 0.0) { xL = xMin; xH = xMax; }
-        else { xH = xMin; xL = xMax; }
+    constexpr double MARGIN = 2.0;
+    for (int pass = SPEC ? -1 : 0; pass < 2; ++pass) {
+        const double xMin = (pass <= 0) ? T1 : CH_T_MIN;
+        const double xMax = (pass <= 0) ? T2 : CH_T_MAX;
+        double xL, xH;
+        if (pass < 0) {
+            xH = xMin;
+            xL = xMax;
+        } else {
+            xL = xMin;
+            xH = xMax;
+            T = xL;
+            u = tpg_energy<NSP, SERIAL>(cd, massf, T);
+            const double fL = e_tgt - u;
+            T = xH;
+            u = tpg_energy<NSP, SERIAL>(cd, massf, T);
+            const double fH = e_tgt - u;
+            if ((fL > 0.0 && fH > 0.0) || (fL < 0.0 && fH < 0.0)) continue;
+            if (fL == 0.0) { T = xMin; return true; }
+            if (fH == 0.0) { T = xMax; return true; }
+            if (fL < 0.0) { xL = xMin; xH = xMax; }
+            else { xH = xMin; xL = xMax; }
+        }
         double rts = Tsave;
         double dxold = (xMax - xMin);
         double dx = dxold;
         T = rts;
         double Cv;
-        tpg_energy_Cv<NSP>(cd, massf, T, u, Cv);
+        tpg_energy_Cv<NSP, SERIAL>(cd, massf, T, u, Cv);
         double f0 = e_tgt - u;
         double df0 = -1.0 * Cv;
+        bool done = false;
         for (int j = 0; j < 30; ++j) {
             if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
                 dxold = dx;
                 dx = 0.5 * (xH - xL);
                 rts = xL + dx;
-                if (xL == rts) { T = rts; return true; }
+                if (xL == rts) { done = true; break; }
             } else {
                 dxold = dx;
                 dx = f0 / df0;
                 double tmp = rts;
                 rts -= dx;
-                if (tmp == rts) { T = rts; return true; }
+                if (tmp == rts) { done = true; break; }
             }
-            if (fabs(dx) < TOL) { T = rts; return true; }
+            if (fabs(dx) < TOL) { done = true; break; }
             T = rts;
-            tpg_energy_Cv<NSP>(cd, massf, T, u, Cv);
+            tpg_energy_Cv<NSP, SERIAL>(cd, massf, T, u, Cv);
             f0 = e_tgt - u;
             df0 = -1.0 * Cv;
             if (f0 < 0.0) xL = rts;
             else xH = rts;
         }
+        if (done) {
+            if (pass >= 0 || (Cv > 0.0 && fabs(f0) <= Cv && rts >= T1 + MARGIN && rts <= T2 - MARGIN)) {
+                T = rts;
+                return true;
+            }
+        }
     }
     T = Tsave;
-    u = tpg_energy<NSP>(cd, massf, T);
+    u = tpg_energy<NSP, SERIAL>(cd, massf, T);
     return false;
 }
 
