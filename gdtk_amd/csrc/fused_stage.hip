@@ -349,6 +349,37 @@ hipError_t launch_clear_u32(unsigned* p, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Copies of several device buffers in one launch (the step-start flow state of every
+// block), so that the copies are one node of a captured step.  Segment q covers chunks
+// [segs[q].start, segs[q + 1].start) of 1024 doubles; each lane moves 4 doubles of one
+// chunk (two 16-byte loads and stores where the chunk is whole).
+__global__ __launch_bounds__(256) void k_copy_segments(const CopySeg* __restrict__ segs, int nseg) {
+    const int64_t chunk = blockIdx.x;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].start <= chunk) lo = mid;
+        else hi = mid - 1;
+    }
+    const CopySeg sg = segs[__builtin_amdgcn_readfirstlane(lo)];
+    const int64_t i0 = (chunk - sg.start) * 1024 + 4 * (int64_t)threadIdx.x;
+    if (i0 + 4 <= sg.n) {
+        const double2* src = reinterpret_cast<const double2*>(sg.src + i0);
+        double2* dst = reinterpret_cast<double2*>(sg.dst + i0);
+        const double2 a = src[0], b = src[1];
+        dst[0] = a;
+        dst[1] = b;
+    } else {
+        for (int64_t i = i0; i < sg.n && i < i0 + 4; ++i) sg.dst[i] = sg.src[i];
+    }
+}
+
+hipError_t launch_copy_segments(const CopySeg* segs, int nseg, int64_t nchunks, hipStream_t s) {
+    if (nchunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_segments, dim3((unsigned)nchunks), dim3(256), 0, s, segs, nseg);
+    return hipGetLastError();
+}
+
 // Writes one double on the device (the step's dt), ordered on the stream.
 __global__ void k_set_f64(double* p, double v) {
     if (threadIdx.x == 0) *p = v;

@@ -79,6 +79,14 @@ hipError_t launch_fused_batch(const StageBatch& sb, int dims, int stage, int flu
                               double eps, const CellLimits& lim, const IdealGasDev& gas, int grid, int tx,
                               hipStream_t s);
 hipError_t launch_clear_u32(unsigned* p, int n, hipStream_t s);
+// One launch copying n doubles from src to dst for every segment: start = the segment's
+// first chunk of 1024 doubles (prefix sums of ceil(n / 1024)); src and dst 16-byte aligned.
+struct CopySeg {
+    const double* src;
+    double* dst;
+    int64_t n, start;
+};
+hipError_t launch_copy_segments(const CopySeg* segs, int nseg, int64_t nchunks, hipStream_t s);
 // Flow-solution records of one block (flow_io.hip): pack the flow state (fs_in) into nv
 // doubles per active cell, and unpack records into the flow-state planes fs.
 hipError_t launch_pack_flow(const BlockDev& b, const double* pos, const double* mu, const double* kc, const double* dtc,

@@ -128,6 +128,10 @@ struct Sim {
     int64_t ftotal[3] = {0, 0, 0}, ctotal = 0;
     // Block time-step estimates: the batched reduction's slots (3 per block) and counter,
     // both left empty by every launch, and its per-block results {dt, cfl, last signal}.
+    // Step-start copy of every block's flow state (multi-species gas): segments per
+    // flow-state copy (cur = 0, 1), one launch, part of the captured step.
+    DevArr<CopySeg> save_segs[2];
+    int64_t save_chunks = 0;
     DevArr<unsigned long long> dt_slots;
     DevArr<unsigned> dt_done;
     DevArr<double> dt_out;
@@ -522,6 +526,7 @@ struct Sim {
         profile = false;
         try {
             HIPCHK(launch_clear_u32(ctr.p, 2 * sc.region(), stream));
+            launch_save(0);
             launch_batched_stage(1, 0, u);
             launch_batched_stage(2, 1, u);
         } catch (...) {
@@ -1141,14 +1146,29 @@ struct Sim {
         }
     }
 
-    void save_step_start() {
-        if (nsp <= 1) return;
+    // The save buffers and the copy segments, made once (never during a capture).
+    void ensure_save() {
+        if (nsp <= 1 || save_chunks > 0) return;
+        const int nb = (int)blocks.size();
+        std::vector<CopySeg> h[2];
+        int64_t start = 0;
         for (auto& bp : blocks) {
             Block& b = *bp;
-            if (!b.fs_save.p) b.fs_save.alloc((size_t)b.dev.npad * b.dev.nfs);
-            HIPCHK(hipMemcpyAsync(b.fs_save.p, b.fs[cur].p, b.fs_save.n * sizeof(double), hipMemcpyDeviceToDevice,
-                                  stream));
+            b.fs_save.alloc((size_t)b.dev.npad * b.dev.nfs);
+            for (int c = 0; c < 2; ++c) h[c].push_back(CopySeg{b.fs[c].p, b.fs_save.p, (int64_t)b.fs_save.n, start});
+            start += ((int64_t)b.fs_save.n + 1023) / 1024;
         }
+        for (int c = 0; c < 2; ++c) {
+            save_segs[c].alloc(nb);
+            HIPCHK(hipMemcpy(save_segs[c].p, h[c].data(), nb * sizeof(CopySeg), hipMemcpyHostToDevice));
+        }
+        save_chunks = start;
+    }
+
+    // Keeps the flow state the step starts from (fs[cur]) for a retry (restore_from_U0).
+    void launch_save(int c) {
+        if (nsp <= 1) return;
+        HIPCHK(launch_copy_segments(save_segs[c].p, (int)blocks.size(), save_chunks, stream));
     }
 
     // One attempt of the pc step as the captured graph: both stages run, then the counters
@@ -1173,10 +1193,11 @@ struct Sim {
     // simcore_gasdynamic_step.d:906-1575 for the pc scheme.
     void gasdynamic_step() {
         double t0 = time;
-        save_step_start();
+        const bool graphed = use_graph && !profile && cur == 0;
+        ensure_save();
+        if (!graphed) launch_save(cur);  // the captured step saves as its first node
         int attempt = 0;
         bool failed = false;
-        const bool graphed = use_graph && !profile && cur == 0;
         do {
             ++attempt;
             failed = false;
