@@ -76,11 +76,14 @@ __device__ __forceinline__ bool scale_mass_fractions(double* massf) {
 // keeps the reference's bracket evaluations (no speculative pass): a face state near a
 // shock lies far from its cell's temperature, its lane's speculation fails, and the whole
 // wavefront then runs the speculative iteration and the full one after it (measured on the
-// Mach-10 cylinder: 10 % fewer instructions but 17 % longer face launches), while the
-// two bracket evaluations cost little beside the Newton iterations at 2 waves per SIMD.
+// Mach-10 cylinder: 10 % fewer instructions but 17 % longer face launches).  The species
+// evaluations run one after another (SERIAL): 162 instead of 202 VGPRs, 3 waves per SIMD
+// instead of 2, and face launches 26 % shorter on the same case.  (Holding the face frame
+// and velocity outside registers during the iteration reaches 4 waves per SIMD with a
+// small spill, 1 % faster: not kept.)
 template <int NSP>
 __device__ __forceinline__ bool tpg_update_rhou(const ChemDev& cd, FaceStateT<NSP>& s) {
-    if (!tpg_update_temperature<NSP, false, false>(cd, s.massf, s.T, s.u)) return false;
+    if (!tpg_update_temperature<NSP, true, false>(cd, s.massf, s.T, s.u)) return false;
     s.p = s.rho * tpg_Rmix<NSP>(cd, s.massf) * s.T;
     return true;
 }
