@@ -9,29 +9,6 @@
 
 namespace gdtk {
 
-// Two lanes per face (face_flux_species_pair): a workgroup of 256 lanes serves 128 faces of
-// one direction of one block (the FaceBatch ranges start at multiples of 256 faces).
-template <int DIM, int FC, int NSP>
-__global__ __launch_bounds__(256) void k_face_flux_species_pair(const BlockDev* __restrict__ tab, FaceBatch fb,
-                                                                int nblocks, double eps,
-                                                                const ChemDev* __restrict__ cdp) {
-    int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / 2);
-    const int d = __builtin_amdgcn_readfirstlane(face_batch_dir(fb, w0));
-    const int64_t* __restrict__ start = fb.start[d];
-    int lo = 0, hi = nblocks - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= w0) lo = mid;
-        else hi = mid - 1;
-    }
-    const int q = __builtin_amdgcn_readfirstlane(lo);
-    const BlockDev& b = tab[q];
-    const int64_t f = w0 - start[q] + (threadIdx.x >> 1);
-    if (f >= b.nface[d]) return;
-    if (!face_flux_species_pair<DIM, FC, NSP>(b, *cdp, d, eps, f, (int)(threadIdx.x & 1)))
-        atomicAdd(&b.counters[0], 1);
-}
-
 template <int DIM, int STAGE, int NSP>
 __global__ __launch_bounds__(256) void k_cell_update_species(const BlockDev* __restrict__ tab,
                                                              const int64_t* __restrict__ start, int nblocks,
@@ -107,27 +84,6 @@ static inline unsigned nblk256(int64_t n) { return (unsigned)((n + 255) / 256); 
 bool species_supported(int nsp) { return nsp == 2 || nsp == 5; }
 
 template <int NSP>
-static void face_species(const BlockDev* tab, const FaceBatch& fb, int nb, int64_t total, int dims, int fc,
-                         double eps, const ChemDev* cd, hipStream_t s) {
-    dim3 g(nblk256(2 * total)), blk(256);
-#define GDTK_FS(D, F) hipLaunchKernelGGL((k_face_flux_species_pair<D, F, NSP>), g, blk, 0, s, tab, fb, nb, eps, cd)
-    if (dims == 3) {
-        if (fc == 4) GDTK_FS(3, 4);
-        else if (fc == 3) GDTK_FS(3, 3);
-        else if (fc == 2) GDTK_FS(3, 2);
-        else if (fc == 1) GDTK_FS(3, 1);
-        else GDTK_FS(3, 0);
-    } else {
-        if (fc == 4) GDTK_FS(2, 4);
-        else if (fc == 3) GDTK_FS(2, 3);
-        else if (fc == 2) GDTK_FS(2, 2);
-        else if (fc == 1) GDTK_FS(2, 1);
-        else GDTK_FS(2, 0);
-    }
-#undef GDTK_FS
-}
-
-template <int NSP>
 static void cell_species(const BlockDev* tab, const int64_t* start, int nb, int64_t total, int dims, int stage,
                          const double* dtp, const CellLimits& lim, const ChemDev* cd, hipStream_t s) {
     dim3 g(nblk256(total)), blk(256);
@@ -141,11 +97,12 @@ static void cell_species(const BlockDev* tab, const int64_t* start, int nb, int6
 }
 
 hipError_t launch_face_flux_species(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
-                                   double eps, int nsp, const ChemDev* cd, hipStream_t s) {
+                                   int numerics, double eps, int nsp, const ChemDev* cd, hipStream_t s) {
     const int64_t total = fb.total[0] + fb.total[1] + fb.total[2];
     if (total <= 0) return hipSuccess;
-    if (nsp == 2) face_species<2>(tab, fb, nblocks, total, dims, flux_calc, eps, cd, s);
-    else if (nsp == 5) face_species<5>(tab, fb, nblocks, total, dims, flux_calc, eps, cd, s);
+    if (numerics == 1) return launch_face_flux_species_lmr(tab, fb, nblocks, dims, flux_calc, eps, nsp, cd, total, s);
+    if (nsp == 2) face_species<2, false>(tab, fb, nblocks, total, dims, flux_calc, eps, cd, s);
+    else if (nsp == 5) face_species<5, false>(tab, fb, nblocks, total, dims, flux_calc, eps, cd, s);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -102,7 +102,11 @@ __device__ __forceinline__ double pair_swap(double v) { return __shfl_xor(v, 1);
 // Newton iterations run side by side and a lane holds one side's state.  Both lanes of a
 // pair take the same branches up to the swap.  Returns false where the reference throws
 // (either side's mass fractions far from summing to one).
-template <int DIM, int FC, int NSP>
+//
+// LMR: Eilmer 5's reconstruction (lmr/onedinterp.d:137-169, 268-286, 314-315): the mass
+// fractions themselves are reconstructed and scaled, then the density, each with the
+// dimensionalised limiter epsilon; AUSMDV takes the smooth-maximum sound speed.
+template <int DIM, int FC, int NSP, bool LMR = false>
 __device__ __forceinline__ bool face_flux_species_pair(const BlockDev& b, const ChemDev& cd, int d, double eps,
                                                        int64_t f, int right) {
     using C = CQ<DIM, NSP>;
@@ -153,40 +157,41 @@ __device__ __forceinline__ bool face_flux_species_pair(const BlockDev& b, const 
     FaceStateT<NSP> st;
     st.T = fs[c1 + FS_T * np];
     st.a = fs[c1 + FS_A * np];
-    double rs[NSP];
     bool ok;
+    auto build = [&](const auto& c) {
+        st.vel.x = c.template value<LMR>(v0.x, v1.x, v2.x, eps);
+        st.vel.y = c.template value<LMR>(v0.y, v1.y, v2.y, eps);
+        st.vel.z = c.template value<LMR>(v0.z, v1.z, v2.z, eps);
+        if (LMR) {
+#pragma unroll
+            for (int i = 0; i < NSP; ++i) {
+                const int64_t pl = (FS_BASE_PLANES + i) * np;
+                st.massf[i] = c.template value<true>(fs[c0 + pl], fs[c1 + pl], fs[c2 + pl], eps);
+            }
+            ok = scale_mass_fractions<NSP>(st.massf);
+            st.rho = c.template value<true>(fs[c0 + FS_RHO * np], fs[c1 + FS_RHO * np], fs[c2 + FS_RHO * np], eps);
+        } else {
+            double rs[NSP];
+#pragma unroll
+            for (int i = 0; i < NSP; ++i) rs[i] = c.value(S0.rho_s[i], S1.rho_s[i], S2.rho_s[i], eps);
+            double rho = 0.0;
+#pragma unroll
+            for (int i = 0; i < NSP; ++i) rho += rs[i];
+            st.rho = rho;
+#pragma unroll
+            for (int i = 0; i < NSP; ++i) st.massf[i] = rs[i] / st.rho;
+            ok = scale_mass_fractions<NSP>(st.massf);
+        }
+        st.u = c.template value<LMR>(S0.u, S1.u, S2.u, eps);
+    };
     if (right) {
         L2R2Side<true> c;
         c.prepare(0.0, l0, l1, l2);
-        st.vel.x = c.value(v0.x, v1.x, v2.x, eps);
-        st.vel.y = c.value(v0.y, v1.y, v2.y, eps);
-        st.vel.z = c.value(v0.z, v1.z, v2.z, eps);
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) rs[i] = c.value(S0.rho_s[i], S1.rho_s[i], S2.rho_s[i], eps);
-        double rho = 0.0;
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) rho += rs[i];
-        st.rho = rho;
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) st.massf[i] = rs[i] / st.rho;
-        ok = scale_mass_fractions<NSP>(st.massf);
-        st.u = c.value(S0.u, S1.u, S2.u, eps);
+        build(c);
     } else {
         L2R2Side<false> c;
         c.prepare(l0, l1, l2, 0.0);
-        st.vel.x = c.value(v0.x, v1.x, v2.x, eps);
-        st.vel.y = c.value(v0.y, v1.y, v2.y, eps);
-        st.vel.z = c.value(v0.z, v1.z, v2.z, eps);
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) rs[i] = c.value(S0.rho_s[i], S1.rho_s[i], S2.rho_s[i], eps);
-        double rho = 0.0;
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) rho += rs[i];
-        st.rho = rho;
-#pragma unroll
-        for (int i = 0; i < NSP; ++i) st.massf[i] = rs[i] / st.rho;
-        ok = scale_mass_fractions<NSP>(st.massf);
-        st.u = c.value(S0.u, S1.u, S2.u, eps);
+        build(c);
     }
     if (!tpg_update_rhou<NSP>(cd, st)) copy_cell_state<NSP>(fs, np, c1, v1, st);
     st.vel = to_local(to_global(st.vel, g), g);
@@ -214,7 +219,7 @@ __device__ __forceinline__ bool face_flux_species_pair(const BlockDev& b, const 
     for (int j = 0; j < C::n; ++j) F[j] = 0.0;
     if (efm) efmflx<DIM, NSP>(Ls, Rs, right ? Cv_o : Cv, right ? Cv : Cv_o, F);
     else if (FC == 3 || (FC == 4 && shocked)) hanel<DIM, NSP>(Ls, Rs, F);
-    else ausmdv<DIM, NSP>(Ls, Rs, F);
+    else ausmdv<DIM, NSP, LMR>(Ls, Rs, F);
     const double fz = (DIM == 3) ? F[C::zMom] : 0.0;
     const Vec3d m = to_global(Vec3d{F[C::xMom], F[C::yMom], fz}, g);
     F[C::xMom] = m.x;
@@ -375,6 +380,52 @@ __device__ __forceinline__ void cell_update_species(const BlockDev& b, const Che
     for (int q = 0; q < NSP; ++q) f_sum += massf[q];
     if (f_sum < 0.99 || f_sum > 1.01 || !isfinite(f_sum)) ok = false;
     if (!ok) atomicAdd(&b.counters[1], 1);
+}
+
+// Two lanes per face (face_flux_species_pair): a workgroup of 256 lanes serves 128 faces of
+// one direction of one block (the FaceBatch ranges start at multiples of 256 faces).
+// LMR: Eilmer 5 reconstruction and AUSMDV (face_flux_species_pair).
+template <int DIM, int FC, int NSP, bool LMR>
+__global__ __launch_bounds__(256) void k_face_flux_species_pair(const BlockDev* __restrict__ tab, FaceBatch fb,
+                                                                int nblocks, double eps,
+                                                                const ChemDev* __restrict__ cdp) {
+    int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / 2);
+    const int d = __builtin_amdgcn_readfirstlane(face_batch_dir(fb, w0));
+    const int64_t* __restrict__ start = fb.start[d];
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= w0) lo = mid;
+        else hi = mid - 1;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const BlockDev& b = tab[q];
+    const int64_t f = w0 - start[q] + (threadIdx.x >> 1);
+    if (f >= b.nface[d]) return;
+    if (!face_flux_species_pair<DIM, FC, NSP, LMR>(b, *cdp, d, eps, f, (int)(threadIdx.x & 1)))
+        atomicAdd(&b.counters[0], 1);
+}
+
+// Launches the face kernel for the flux calculator code (total = FaceBatch launch size).
+template <int NSP, bool LMR>
+inline void face_species(const BlockDev* tab, const FaceBatch& fb, int nb, int64_t total, int dims, int fc,
+                         double eps, const ChemDev* cd, hipStream_t s) {
+    dim3 g((unsigned)((2 * total + 255) / 256)), blk(256);
+#define GDTK_FS(D, F) hipLaunchKernelGGL((k_face_flux_species_pair<D, F, NSP, LMR>), g, blk, 0, s, tab, fb, nb, eps, cd)
+    if (dims == 3) {
+        if (fc == 4) GDTK_FS(3, 4);
+        else if (fc == 3) GDTK_FS(3, 3);
+        else if (fc == 2) GDTK_FS(3, 2);
+        else if (fc == 1) GDTK_FS(3, 1);
+        else GDTK_FS(3, 0);
+    } else {
+        if (fc == 4) GDTK_FS(2, 4);
+        else if (fc == 3) GDTK_FS(2, 3);
+        else if (fc == 2) GDTK_FS(2, 2);
+        else if (fc == 1) GDTK_FS(2, 1);
+        else GDTK_FS(2, 0);
+    }
+#undef GDTK_FS
 }
 
 }  // namespace gdtk

@@ -127,9 +127,17 @@ struct L2R2Side {
             two = (2.0 * lR0 + lenR1);
         }
     }
-    // left: value(qL1, qL0, qR0); right: value(qL0, qR0, qR1)
-    __device__ __forceinline__ double value(double q0, double q1, double q2, double eps) const {
+    // left: value(qL1, qL0, qR0); right: value(qL0, qR0, qR1).  LMR: Eilmer 5's epsilon,
+    // eps_va scaled by max(1e-12, |qL0|, |qR0|) and by 2 / (lenR0 + lenL0), as L2R2::scalar.
+    template <bool LMR = false>
+    __device__ __forceinline__ double value(double q0, double q1, double q2, double eps_va) const {
         const double beta = 1.0;
+        double eps = eps_va;
+        if (LMR) {
+            const double qL0 = RIGHT ? q0 : q1, qR0 = RIGHT ? q1 : q2;
+            const double qq = fmax(fmax(1e-12, fabs(qL0)), fmax(1e-12, fabs(qR0)));
+            eps = qq * eps_va * tD;
+        }
         if (!RIGHT) {
             const double delLminus = (q1 - q0) * tO;
             const double del = (q2 - q1) * tD;

@@ -40,7 +40,9 @@ hipError_t launch_cell_update_batch(const BlockDev* tab, const int64_t* start, i
 struct ChemDev;
 bool species_supported(int nsp);
 hipError_t launch_face_flux_species(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
-                                   double eps, int nsp, const ChemDev* cd, hipStream_t s);
+                                   int numerics, double eps, int nsp, const ChemDev* cd, hipStream_t s);
+hipError_t launch_face_flux_species_lmr(const BlockDev* tab, const FaceBatch& fb, int nblocks, int dims, int flux_calc,
+                                       double eps, int nsp, const ChemDev* cd, int64_t total, hipStream_t s);
 hipError_t launch_cell_update_species(const BlockDev* tab, const int64_t* start, int nblocks, int64_t total, int dims,
                                      int stage, const double* dtp, const CellLimits& lim, int nsp, const ChemDev* cd,
                                      hipStream_t s);

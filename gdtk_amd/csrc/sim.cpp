@@ -455,8 +455,8 @@ struct Sim {
         if (!use_fused()) {
             timed(0, (double)nfaces, [&] {
                 if (nsp > 1)
-                    return launch_face_flux_species(t, fb, nb, cfg.dimensions, cfg.flux_calculator, eps, nsp,
-                                                    tpg_dev.p, stream);
+                    return launch_face_flux_species(t, fb, nb, cfg.dimensions, cfg.flux_calculator, numerics, eps,
+                                                    nsp, tpg_dev.p, stream);
                 return launch_face_flux_batch(t, fb, nb, cfg.dimensions, cfg.flux_calculator, numerics, eps, gas,
                                               stream);
             });
@@ -1364,8 +1364,6 @@ int gdtk_hip_sim_set_gas_tpg(int sim, const gdtk_tpg_gas* gas) {
     return guarded([&] {
         if (!gas) throw Error(GDTK_ERR_ARG, "null argument");
         Sim& s = get_sim(sim);
-        if (s.numerics == GDTK_NUMERICS_LMR && gas->nsp > 1)
-            throw Error(GDTK_ERR_UNSUPPORTED, "Eilmer 5 numerics are implemented for a single-species gas");
         s.set_gas_tpg(*gas);
         return GDTK_OK;
     });
@@ -1546,9 +1544,6 @@ int gdtk_hip_sim_set_numerics(int sim, int numerics) {
         Sim& s = get_sim(sim);
         if (numerics != GDTK_NUMERICS_EILMER4 && numerics != GDTK_NUMERICS_LMR)
             throw Error(GDTK_ERR_ARG, "unknown numerics");
-        // lmr reconstructs mass fractions rather than species densities (lmr/onedinterp.d:268-286)
-        if (numerics == GDTK_NUMERICS_LMR && s.nsp > 1)
-            throw Error(GDTK_ERR_UNSUPPORTED, "Eilmer 5 numerics are implemented for a single-species gas");
         s.numerics = numerics;
         s.drop_graphs();
         return GDTK_OK;

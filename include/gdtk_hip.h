@@ -220,9 +220,9 @@ int gdtk_hip_sim_set_shock_detector(int sim, const gdtk_shock_detector* opts);
  * max(1e-12, |qL0|, |qR0|) * 2 / (lenR0 + lenL0), and whose AUSMDV (lmr/fluxcalc.d, ausmdv)
  * takes the common sound speed as the smooth maximum
  * 0.5 (aL + aR) + 0.5 sqrt((aL - aR)^2 + eps^2), eps = 1e-6 * 0.5 (aL + aR) + 1e-12.
- * Implemented for a single-species gas (GDTK_ERR_UNSUPPORTED otherwise: lmr reconstructs
- * mass fractions, Eilmer 4 species densities).  The stage then runs the face-per-thread
- * kernels. */
+ * For a multi-species gas lmr reconstructs the mass fractions themselves (scaled after
+ * reconstruction) and the density (lmr/onedinterp.d:268-286, 314-315), where Eilmer 4
+ * reconstructs the species densities.  The stage then runs the face-per-thread kernels. */
 #define GDTK_NUMERICS_EILMER4 0
 #define GDTK_NUMERICS_LMR 1
 int gdtk_hip_sim_set_numerics(int sim, int numerics);

@@ -932,7 +932,24 @@ void SFluidBlock::interp_l2r2(FVInterface* f, FVCell* cL1, FVCell* cL0, FVCell* 
     interp_l2r2_scalar(pL1->y, pL0->y, pR0->y, pR1->y, L.vel.y, R.vel.y);
     interp_l2r2_scalar(pL1->z, pL0->z, pR0->z, pR1->z, L.vel.z, R.vel.z);
     const GasState &gL1 = cL1->fs.gas, &gL0 = cL0->fs.gas, &gR0 = cR0->fs.gas, &gR1 = cR1->fs.gas;
-    if (nsp > 1) {
+    if (nsp > 1 && cfg.lmr) {
+        // lmr/onedinterp.d:268-286, 314-315: Eilmer 5 reconstructs the mass fractions
+        // themselves, scales them, then the density (thermo interpolation rhou).  The face
+        // states' rho_s keep the copied cell values, which the flux does not read.
+        if (cfg.allow_reconstruction_for_species) {
+            for (int isp = 0; isp < nsp; ++isp)
+                interp_l2r2_scalar(gL1.massf[isp], gL0.massf[isp], gR0.massf[isp], gR1.massf[isp], L.gas.massf[isp],
+                                   R.gas.massf[isp]);
+            if (cfg.scale_species_after_reconstruction) {
+                scale_mass_fractions(L.gas.massf);
+                scale_mass_fractions(R.gas.massf);
+            }
+        } else {
+            L.gas.massf = gL0.massf;
+            R.gas.massf = gR0.massf;
+        }
+        interp_l2r2_scalar(gL1.rho, gL0.rho, gR0.rho, gR1.rho, L.gas.rho, R.gas.rho);
+    } else if (nsp > 1) {
         if (cfg.allow_reconstruction_for_species) {
             for (int isp = 0; isp < nsp; ++isp)
                 interp_l2r2_scalar(gL1.rho_s[isp], gL0.rho_s[isp], gR0.rho_s[isp], gR1.rho_s[isp], L.gas.rho_s[isp],

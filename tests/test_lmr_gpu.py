@@ -81,16 +81,45 @@ def test_lmr_sod_tube_reference_run():
             assert abs(g.get(0, k)[i] - v) / (abs(v) + 1.0) < 0.01, (x, k)
 
 
-def test_lmr_refuses_multi_species_gas():
+def test_lmr_unknown_numerics_refused():
     from gdtk_amd import Simulation, hip
-    case, rec = cases.species_box(16, 8, nsp=5)
-    with pytest.raises(hip.HipError, match="single-species"):
-        Simulation(2, flux="ausmdv", tpg=rec, numerics="lmr")
     g = Simulation(2, flux="ausmdv")
     with pytest.raises(ValueError, match="unknown numerics"):
         g.set_numerics("eilmer3")
     with pytest.raises(hip.HipError, match="unknown numerics"):
         hip._check(hip.lib().gdtk_hip_sim_set_numerics(g.h, 7))
+
+
+@pytest.mark.parametrize("nsp,flux", [(5, "ausmdv"), (5, "efm"), (2, "ausmdv"), (5, "adaptive_hanel_ausmdv")])
+def test_lmr_species_box_against_oracle(nsp, flux):
+    """Multi-species gas with lmr numerics (mass fractions and density reconstructed,
+    lmr/onedinterp.d:268-286): the species box with its hot dissociated blob, 12 steps,
+    against the oracle's lmr run at the 1e-12 bar (flow state, mass fractions, U), and
+    measurably apart from the oracle's Eilmer 4 run of the same case."""
+    from gdtk_amd import Simulation
+    from oracle import oracle as O
+    case, rec = cases.species_box(48, 24, nsp=nsp)
+    g = Simulation(2, flux=flux, tpg=rec, numerics="lmr")
+    g.add_block(case)
+    gas = O.TPG(rec)
+    o = O.Sim(gas, 2, flux, numerics="lmr")
+    o.add_block(case)
+    o4 = O.Sim(gas, 2, flux)
+    o4.add_block(case)
+    _, sg = g.run(12)
+    _, so = o.run(12)
+    o4.run(12)
+    assert sg["step"] == so["step"] == 12
+    assert sg["dt"] == pytest.approx(so["dt"], rel=1e-12)
+    names = ("rho", "vx", "vy", "p", "T")
+    errs = field_errors(lambda k: g.get(0, k), lambda k: o.get(0, k), names, g.get_U(0), o.get_U(0, g.ncq), 2)
+    for sp in range(nsp):
+        errs[f"Y{sp}"] = rel_err(g.get(0, 300 + sp), o.get(0, 300 + sp), np.ones(case.ncells))
+    apart = max(rel_err(o4.get(0, 300 + sp), o.get(0, 300 + sp), np.ones(case.ncells)) for sp in range(nsp))
+    print(f"lmr species nsp={nsp} {flux}", {k: f"{v:.2e}" for k, v in errs.items()}, f"E4 vs lmr {apart:.2e}",
+          flush=True)
+    assert max(errs.values()) <= TOL, errs
+    assert apart > 1e3 * TOL
 
 
 @pytest.mark.parametrize("flux", ["ausmdv", "adaptive"])

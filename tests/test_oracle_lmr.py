@@ -125,3 +125,67 @@ def test_lmr_numerics_change_a_smooth_field(oracle_lib):
         out[num] = s.get(0, "p")
     d = np.max(np.abs(out["lmr"] - out["eilmer4"]) / out["eilmer4"])
     assert d > 1.0e-11, d
+
+
+def _species_ramp(nic=12, njc=3):
+    """A 5-species box whose composition, T and p vary along i only (non-linear profiles,
+    different per species) in a uniform supersonic stream, so every interior i-face is
+    fully upwinded: AUSMDV's mass flux is rho_L u_L and its species fluxes are that mass
+    flux times the left state's mass fractions."""
+    case, rec = cases.species_box(nic, njc, nsp=5)
+    xc = cases.cell_centres(case)[:, 0] / 0.1
+    Y = np.stack([0.70 - 0.2 * xc ** 2, 0.20 + 0.05 * np.sin(5 * xc), 0.03 + 0.02 * xc,
+                  0.04 * np.exp(-((xc - 0.5) / 0.2) ** 2), 0.02 + 0.1 * xc ** 3], axis=1)
+    case.massf = Y / Y.sum(axis=1, keepdims=True)
+    case.T = 300.0 + 600.0 * xc ** 2
+    case.p = 2.0e3 * (1.0 + 0.5 * np.cos(3 * xc))
+    case.vel = np.zeros((case.ncells, 3))
+    case.vel[:, 0] = 3000.0
+    case.inflow[cases.WEST] = [case.p[0], case.T[0], 3000.0, 0.0, 0.0]
+    return case, rec
+
+
+def _scaled(mf):
+    mf = np.where(mf >= 0.0, mf, 0.0)
+    s = mf.sum()
+    return mf / s if abs(s - 1.0) > 0.0 else mf
+
+
+def test_lmr_species_faces_reconstruct_mass_fractions(oracle_lib):
+    """lmr/onedinterp.d:268-286: Eilmer 5 reconstructs the cells' mass fractions (each with
+    the scalar limiter pinned above) and scales them; Eilmer 4 reconstructs the species
+    densities and divides by their sum.  On upwinded faces the flux ratio F_species / F_mass
+    is the left state's mass fractions, so each numerics' face composition is read off the
+    oracle's fluxes and compared with the formula of that numerics.  The two formulas differ
+    on this field by far more than the tolerance, so a swapped branch fails."""
+    O = oracle_lib
+    case, rec = _species_ramp()
+    gas = O.TPG(rec)
+    nic, njc = case.nic, case.njc
+    dx = 0.1 / nic
+    Y = case.massf.reshape(njc, nic, 5)
+    got = {}
+    for numerics in ("lmr", "eilmer4"):
+        o = O.Sim(gas, 2, "ausmdv", numerics=numerics)
+        o.add_block(case)
+        # the cells' state before the stage (the stage's fluxes come from it)
+        massf = np.stack([o.get(0, 300 + s) for s in range(5)], axis=1).reshape(njc, nic, 5)
+        rho_s = np.stack([o.get(0, 104 + s) for s in range(5)], axis=1).reshape(njc, nic, 5)  # U[0] species
+        o.stage(1, 1.0e-9)
+        F = o.face_flux(0, 9)
+        worst, sep = 0.0, 0.0
+        for j in range(njc):
+            for fi in range(2, nic - 1):
+                k = fi + (nic + 1) * j
+                ratio = F[k, 4:9] / F[k, 0]
+                st = [fi - 2, fi - 1, fi, fi + 1]
+                y_lmr = _scaled(np.array([O.l2r2_scalar([dx] * 4, massf[j, st, s], lmr=True)[0] for s in range(5)]))
+                rs = np.array([O.l2r2_scalar([dx] * 4, rho_s[j, st, s])[0] for s in range(5)])
+                y_e4 = _scaled(rs / rs.sum())
+                want = y_lmr if numerics == "lmr" else y_e4
+                worst = max(worst, float(np.max(np.abs(ratio - want))))
+                sep = max(sep, float(np.max(np.abs(y_lmr - y_e4))))
+        got[numerics] = (worst, sep)
+        assert worst < 1e-14, (numerics, worst)
+        assert sep > 1e-6, sep
+    assert np.allclose(Y.sum(axis=2), 1.0)
