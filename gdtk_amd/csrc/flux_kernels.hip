@@ -291,16 +291,15 @@ __device__ __forceinline__ double dt_unkey(unsigned long long k) {
 // exchanges, across the workgroup's four wavefronts through LDS, and enter the block's slots
 // with one atomic maximum each (dt as the complement of its key), so the result is the
 // exact minimum/maximum whatever the order.  Like fmin/fmax a NaN never wins.  The block's
-// last cell stores its signal (fluidblock.d:1081).  The last workgroup to finish (a device
-// counter) writes out[3q..3q+2] = {min dt_local, max cfl_local, last signal} for every
-// block and empties the slots and the counter for the next launch.  A workgroup serves 256
-// cells of one block: start[q] is block q's first launch index (a multiple of 256), base
-// the launch index of workgroup 0.
+// last cell stores its signal (fluidblock.d:1081).  The last of a block's workgroups to
+// finish (a device counter per block) writes out[3q..3q+2] = {min dt_local, max cfl_local,
+// last signal} and empties the block's slots and counter for the next launch.  A workgroup
+// serves 256 cells of one block: start[q] is block q's first launch index (a multiple of
+// 256; start[nblocks] the end), base the launch index of workgroup 0.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_dt_batch(const BlockDev* __restrict__ tab, const int64_t* __restrict__ start,
                                                   int nblocks, int64_t base, double dt_current, double cfl_value,
-                                                  unsigned long long* slots, unsigned* done, unsigned nwg,
-                                                  double* out) {
+                                                  unsigned long long* slots, unsigned* done, double* out) {
     __shared__ double wdt[4], wcfl[4];
     __shared__ int am_last;
     const int64_t w0 = base + (int64_t)blockIdx.x * blockDim.x;
@@ -368,20 +367,19 @@ __global__ __launch_bounds__(256) void k_dt_batch(const BlockDev* __restrict__ t
         if (d == d) atomicMax(&slots[3 * q], ~dt_key(d));
         if (cm == cm) atomicMax(&slots[3 * q + 1], dt_key(cm));
         __threadfence();
-        am_last = (atomicAdd(done, 1u) == nwg - 1) ? 1 : 0;
+        const unsigned nwg = (unsigned)((start[q + 1] - start[q]) / 256);
+        am_last = (atomicAdd(&done[q], 1u) == nwg - 1) ? 1 : 0;
     }
     __syncthreads();
-    if (!am_last) return;
+    if (!am_last || threadIdx.x != 0) return;
     __threadfence();
-    for (int r = threadIdx.x; r < nblocks; r += blockDim.x) {
-        const unsigned long long kd = atomicExch(&slots[3 * r], 0ull);
-        const unsigned long long kc = atomicExch(&slots[3 * r + 1], 0ull);
-        const unsigned long long ks = atomicExch(&slots[3 * r + 2], 0ull);
-        out[3 * r] = kd ? dt_unkey(~kd) : __builtin_huge_val();
-        out[3 * r + 1] = kc ? dt_unkey(kc) : -__builtin_huge_val();
-        out[3 * r + 2] = __longlong_as_double((long long)ks);
-    }
-    if (threadIdx.x == 0) atomicExch(done, 0u);
+    const unsigned long long kd = atomicExch(&slots[3 * q], 0ull);
+    const unsigned long long kc = atomicExch(&slots[3 * q + 1], 0ull);
+    const unsigned long long ks = atomicExch(&slots[3 * q + 2], 0ull);
+    out[3 * q] = kd ? dt_unkey(~kd) : __builtin_huge_val();
+    out[3 * q + 1] = kc ? dt_unkey(kc) : -__builtin_huge_val();
+    out[3 * q + 2] = __longlong_as_double((long long)ks);
+    atomicExch(&done[q], 0u);
 }
 
 // Encode U from the flow state (fvcell.d:511-583), single species: used at start-up.
@@ -507,10 +505,10 @@ hipError_t launch_dt_batch(const BlockDev* tab, const int64_t* start, int nblock
     const unsigned nwg = (unsigned)(count / 256);
     if (dims == 3)
         hipLaunchKernelGGL((k_dt_batch<3>), dim3(nwg), dim3(256), 0, s, tab, start, nblocks, base, dt_current, cfl,
-                           slots, done, nwg, out);
+                           slots, done, out);
     else
         hipLaunchKernelGGL((k_dt_batch<2>), dim3(nwg), dim3(256), 0, s, tab, start, nblocks, base, dt_current, cfl,
-                           slots, done, nwg, out);
+                           slots, done, out);
     return hipGetLastError();
 }
 

@@ -17,9 +17,10 @@ struct CellLimits {
 };
 
 // Block time-step estimates of blocks [0, nblocks) of tab in one launch of count lanes
-// (count a multiple of 256; start[q] = block q's first launch index, base = the launch
-// index of the first lane).  out[3q..3q+2] = {min dt_local, max cfl_local, last signal};
-// slots (3 per block) and done start at zero and are left at zero.
+// (count a multiple of 256; start[q] = block q's first launch index, start[nblocks] the
+// end, base = the launch index of the first lane).  out[3q..3q+2] = {min dt_local, max
+// cfl_local, last signal}; slots (3 per block) and done (1 per block) start at zero and are
+// left at zero.
 hipError_t launch_dt_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t base, int64_t count,
                            int dims, double dt_current, double cfl, unsigned long long* slots, unsigned* done,
                            double* out, hipStream_t s);

@@ -126,8 +126,9 @@ struct Sim {
     DevArr<int64_t> fstart[3], cstart;
     std::vector<int64_t> cstart_h;
     int64_t ftotal[3] = {0, 0, 0}, ctotal = 0;
-    // Block time-step estimates: the batched reduction's slots (3 per block) and counter,
-    // both left empty by every launch, and its per-block results {dt, cfl, last signal}.
+    // Block time-step estimates: the batched reduction's slots (3 per block) and counters
+    // (1 per block), left empty by every launch, and its per-block results {dt, cfl, last
+    // signal}.
     // Step-start copy of every block's flow state (multi-species gas): segments per
     // flow-state copy (cur = 0, 1), one launch, part of the captured step.
     DevArr<CopySeg> save_segs[2];
@@ -267,7 +268,7 @@ struct Sim {
         HIPCHK(hipMemcpy(cstart.p, st.data(), st.size() * sizeof(int64_t), hipMemcpyHostToDevice));
         cstart_h = st;
         dt_slots.alloc(3 * nb);
-        dt_done.alloc(1);
+        dt_done.alloc(nb);
         dt_out.alloc(3 * nb);
     }
 
@@ -997,7 +998,7 @@ struct Sim {
         finalize();
         const int64_t base = cstart_h[q0], count = cstart_h[q0 + n] - base;
         HIPCHK(launch_dt_batch(tab(1, cur, u0) + q0, cstart.p + q0, n, base, count, cfg.dimensions, dt_current, cfl,
-                               dt_slots.p + 3 * q0, dt_done.p, dt_out.p + 3 * q0, stream));
+                               dt_slots.p + 3 * q0, dt_done.p + q0, dt_out.p + 3 * q0, stream));
     }
 
     // fluidblock.d:987-1084 from the reduced {dt_allow, cfl_max, last signal}.
