@@ -247,20 +247,20 @@ __device__ __forceinline__ double tpg_Rmix(const ChemDev& cd, const double* mass
 // evaluated temperature.  Returns false where the reference throws; T is then restored
 // and u = energy(T), which is the state the reference leaves behind.
 //
-// The reference's first pass evaluates the energy at both ends of the bracket
-// [T1, T1 + 1000] only to learn the signs of f = e_tgt - u(T) there, and in the usual case
-// (the root inside the bracket) those signs are f(T1) > 0 > f(T2), since u(T) increases
-// with T.  A speculative pass (pass -1) runs the same Newton iteration with that bracket
-// assignment and skips the two evaluations.  Its root is kept only when it proves the
-// assumption: a Newton-converged T* whose last evaluation lies within 1 K of the root
-// (|f| <= Cv) and T* at least 2 K inside both ends.  The root then lies >= 0.99 K inside,
-// where u(T1) and u(T2) differ from e_tgt by >= 0.99 K x Cv_min, with Cv >= 1.5 R_mix for
-// any gas, far beyond rounding, so the reference's pass 0 sees exactly the assumed signs
-// and takes the identical iteration from Tsave: same T, same u, bit for bit.  Any other
-// outcome falls through to the reference's passes 0 and 1 from the start.  SPEC = false
-// runs the reference's passes only (for callers whose guesses often lie far from the root:
-// one failing lane makes its whole wavefront run both).  SERIAL as for tpg_energy.
-template <int NSP, bool SERIAL = false, bool SPEC = true>
+// The reference evaluates u at both ends of the bracket [T1, T2] = [T - 500, T + 500] only
+// for the signs of f = e_tgt - u there (and exact zeros), then starts the Newton iteration
+// with an evaluation at T.  Here the evaluation at T comes first, and with it the signs are
+// often fixed without the two end evaluations: u(T) rises at Cv >= 1.5 R_mix for any gas
+// (translational energy alone; the CEA curves, their blends and their constant-Cp
+// extensions all keep Cp >= 2.5 R per species), so with g = e_tgt - u(T),
+//   f(T1) >= g + 1.4 R_mix (T - T1)   and   f(T2) <= g - 1.4 R_mix (T2 - T).
+// When the first bound is above and the second below a rounding margin (1e-6 of the
+// energies + 1 J/kg, against ~1e5 J/kg of slope over 500 K), f(T1) > 0 > f(T2) as the
+// reference would compute them: its pass 0 takes the bracket (xH, xL) = (T1, T2) and the
+// same iteration from T, so T and u come out bit for bit the same.  Otherwise (a root far
+// from T, a temperature near the 10 K floor, a negative mass fraction) the ends are
+// evaluated and the reference's passes run as written.  SERIAL as for tpg_energy.
+template <int NSP, bool SERIAL = false>
 __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const double* massf, double& T, double& u) {
     const double Tsave = T;
     const double TOL = 1.0e-6;
@@ -268,12 +268,28 @@ __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const 
     const double delT = 1000.0;
     const double T1 = fmax(T - 0.5 * delT, CH_T_MIN);
     const double T2 = T1 + delT;
-    constexpr double MARGIN = 2.0;
-    for (int pass = SPEC ? -1 : 0; pass < 2; ++pass) {
-        const double xMin = (pass <= 0) ? T1 : CH_T_MIN;
-        const double xMax = (pass <= 0) ? T2 : CH_T_MAX;
+    double u0, Cv0;
+    tpg_energy_Cv<NSP, SERIAL>(cd, massf, Tsave, u0, Cv0);
+    bool signs_known;
+    {
+        double Rm = 0.0;
+        bool nonneg = true;
+#pragma unroll
+        for (int i = 0; i < NSP; ++i) {
+            Rm += massf[i] * cd.R[i];
+            nonneg = nonneg && (massf[i] >= 0.0);
+        }
+        const double g = e_tgt - u0;
+        const double slope = 1.4 * Rm;
+        const double margin = 1.0e-6 * (fabs(e_tgt) + fabs(u0)) + 1.0;
+        signs_known = nonneg && (Tsave - T1 >= 1.0) && (T2 - Tsave >= 1.0) &&
+                      (g + slope * (Tsave - T1) > margin) && (g - slope * (T2 - Tsave) < -margin);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const double xMin = (pass == 0) ? T1 : CH_T_MIN;
+        const double xMax = (pass == 0) ? T2 : CH_T_MAX;
         double xL, xH;
-        if (pass < 0) {
+        if (pass == 0 && signs_known) {
             xH = xMin;
             xL = xMax;
         } else {
@@ -296,36 +312,35 @@ __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const 
         double dx = dxold;
         T = rts;
         double Cv;
-        tpg_energy_Cv<NSP, SERIAL>(cd, massf, T, u, Cv);
+        if (pass == 0 && signs_known) {
+            Cv = Cv0;
+            u = u0;
+        } else {
+            // the same evaluation again (fewer registers live across the end evaluations)
+            tpg_energy_Cv<NSP, SERIAL>(cd, massf, T, u, Cv);
+        }
         double f0 = e_tgt - u;
         double df0 = -1.0 * Cv;
-        bool done = false;
         for (int j = 0; j < 30; ++j) {
             if ((((rts - xH) * df0 - f0) * ((rts - xL) * df0 - f0) > 0.0) || (fabs(2.0 * f0) > fabs(dxold * df0))) {
                 dxold = dx;
                 dx = 0.5 * (xH - xL);
                 rts = xL + dx;
-                if (xL == rts) { done = true; break; }
+                if (xL == rts) { T = rts; return true; }
             } else {
                 dxold = dx;
                 dx = f0 / df0;
                 double tmp = rts;
                 rts -= dx;
-                if (tmp == rts) { done = true; break; }
+                if (tmp == rts) { T = rts; return true; }
             }
-            if (fabs(dx) < TOL) { done = true; break; }
+            if (fabs(dx) < TOL) { T = rts; return true; }
             T = rts;
             tpg_energy_Cv<NSP, SERIAL>(cd, massf, T, u, Cv);
             f0 = e_tgt - u;
             df0 = -1.0 * Cv;
             if (f0 < 0.0) xL = rts;
             else xH = rts;
-        }
-        if (done) {
-            if (pass >= 0 || (Cv > 0.0 && fabs(f0) <= Cv && rts >= T1 + MARGIN && rts <= T2 - MARGIN)) {
-                T = rts;
-                return true;
-            }
         }
     }
     T = Tsave;

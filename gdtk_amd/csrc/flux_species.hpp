@@ -72,18 +72,14 @@ __device__ __forceinline__ bool scale_mass_fractions(double* massf) {
 
 // update_thermo_from_rhou of the thermally perfect mixture (therm_perf_gas.d:235-239):
 // Newton T(u) from the state's T, then p = rho R_mix T (perf_gas_mix_eos.d:43-49).  The
-// sound speed is not updated, as in the reference's face reconstruction.  The face kernel
-// keeps the reference's bracket evaluations (no speculative pass): a face state near a
-// shock lies far from its cell's temperature, its lane's speculation fails, and the whole
-// wavefront then runs the speculative iteration and the full one after it (measured on the
-// Mach-10 cylinder: 10 % fewer instructions but 17 % longer face launches).  The species
+// sound speed is not updated, as in the reference's face reconstruction.  The species
 // evaluations run one after another (SERIAL): 162 instead of 202 VGPRs, 3 waves per SIMD
-// instead of 2, and face launches 26 % shorter on the same case.  (Holding the face frame
-// and velocity outside registers during the iteration reaches 4 waves per SIMD with a
-// small spill, 1 % faster: not kept.)
+// instead of 2, and face launches 26 % shorter on the Mach-10 cylinder.  (Holding the face
+// frame and velocity outside registers during the iteration reaches 4 waves per SIMD with
+// a small spill, 1 % faster: not kept.)
 template <int NSP>
 __device__ __forceinline__ bool tpg_update_rhou(const ChemDev& cd, FaceStateT<NSP>& s) {
-    if (!tpg_update_temperature<NSP, true, false>(cd, s.massf, s.T, s.u)) return false;
+    if (!tpg_update_temperature<NSP, true>(cd, s.massf, s.T, s.u)) return false;
     s.p = s.rho * tpg_Rmix<NSP>(cd, s.massf) * s.T;
     return true;
 }
