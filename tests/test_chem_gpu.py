@@ -130,6 +130,44 @@ def test_tpg_update_modes():
         assert max(errs.values()) <= TOL, (mode, errs)
 
 
+def test_tpg_rhou_guess_offsets():
+    """The rhou update's Newton iteration from guesses at set distances from the root:
+    inside the bracket by a wide margin (the bracket signs then come from the first
+    evaluation and a monotonicity bound), near its ends (499 and 501 K off, where the bound
+    cannot decide and the end points are evaluated), beyond it (2000 K: the reference's
+    second pass over [10, 1e5] K), and at 12 K (the bracket clamped to [10, 1010] K), against
+    the oracle's ThermallyPerfectGas, which always evaluates the ends."""
+    from oracle import oracle as O
+    gas, mech = _air()
+    ch = Chemistry(gas, mech)
+    og = O.TPG(gas)
+    base = air_cells(64, seed=23, T_lo=300.0, T_hi=12000.0)
+    offsets = np.array([0.0, 0.5, -0.5, 10.0, -10.0, 300.0, -300.0, 499.0, -499.0, 501.0, -501.0, 2000.0, -2000.0])
+    n0 = base["T"].size
+    Y0 = base["massf"].reshape(5, n0)
+    rows = []
+    for k in range(n0):
+        for d in offsets:
+            rows.append((k, max(base["T"][k] + d, 12.0)))
+    n = len(rows)
+    Y = np.zeros((5, n))
+    rho, u, Tg = np.zeros(n), np.zeros(n), np.zeros(n)
+    for i, (k, tg) in enumerate(rows):
+        Y[:, i], rho[i], u[i], Tg[i] = Y0[:, k], base["rho"][k], base["u"][k], tg
+    assert np.sum(Tg == 12.0) > 10  # guesses clamped near the floor: bracket [10, 1010] K
+    g = ch.tpg_update("rhou", Y.reshape(-1), rho=rho, T=Tg, u=u)
+    ref = {k: np.zeros(n) for k in ("p", "T", "u")}
+    for i in range(n):
+        st = og.update("rhou", Y[:, i], rho=rho[i], T=Tg[i], u=u[i])
+        for k in ref:
+            ref[k][i] = st[k]
+    assert g["failed"] == 0
+    errs = {k: rel_err(g[k], ref[k]) for k in ("p", "T")}
+    errs["u"] = rel_err(g["u"], ref["u"], np.abs(ref["u"]) + g["p"] / rho)
+    print("rhou from offset guesses", n, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) <= TOL, errs
+
+
 @pytest.mark.parametrize("persistent", [0, 1])
 def test_air_alpha_qss_cold_then_warm(persistent):
     """Two chemistry steps of 4096 uneven-stiffness air cells: the first from dt_chem = -1
