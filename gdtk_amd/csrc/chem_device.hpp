@@ -174,6 +174,70 @@ __host__ __device__ inline void cea_h_Cp(const CEADev& c, double T, double logT,
     h = c.R * h_on_RT;
 }
 
+// a / b correctly rounded from y = RN(1 / b): q = RN(a y) lies within an ulp of a / b, the
+// remainder a - b q is exact in one fma, and RN(q + r y) is the correctly rounded quotient
+// (Markstein's theorem; P. Markstein, IA-64 and Elementary Functions, 2000).  Bit for bit
+// a / b for the normal operands met here; a true division costs ~10 instructions, this 3,
+// and a divisor shared by several quotients needs its reciprocal once.
+__host__ __device__ inline double div_by(double a, double b, double y) {
+    const double q = a * y;
+    const double r = fma(-b, q, a);
+    return fma(r, y, q);
+}
+
+// The powers of T that every species' curve divides by, with their reciprocals, once per
+// temperature.
+struct CeaT {
+    double T, TT, rT, rTT, logT;
+    __host__ __device__ explicit CeaT(double t, double lg) : T(t), TT(t * t), rT(1.0 / t), rTT(1.0 / (t * t)), logT(lg) {}
+};
+
+// cea_h_Cp with the divisions of cea_thermo_curves.d:56-102 taken through div_by: the same
+// values bit for bit.
+__host__ __device__ inline void cea_h_Cp(const CEADev& c, const CeaT& p, double& h, double& Cp) {
+    const double T = p.T;
+    if (T < c.T_low) {
+        h = c.h_low - c.Cp_low * (c.T_low - T);
+        Cp = c.Cp_low;
+        return;
+    }
+    if (T > c.T_high) {
+        h = c.h_high + c.Cp_high * (T - c.T_high);
+        Cp = c.Cp_high;
+        return;
+    }
+    double a[9];
+    cea_coeffs(c, T, a);
+    double Cp_on_R = div_by(a[0], p.TT, p.rTT) + div_by(a[1], T, p.rT) + a[2] + a[3] * T;
+    Cp_on_R += a[4] * T * T + a[5] * T * T * T + a[6] * T * T * T * T;
+    Cp = c.R * Cp_on_R;
+    double h_on_RT = div_by(-a[0], T, p.rT) + a[1] * p.logT + a[2] * T + a[3] * T * T / 2.0;
+    h_on_RT += div_by(a[4] * T * T * T, 3.0, 1.0 / 3.0) + a[5] * T * T * T * T / 4.0 +
+               div_by(a[6] * T * T * T * T * T, 5.0, 1.0 / 5.0) + a[7];
+    h = c.R * h_on_RT;
+}
+__host__ __device__ inline double cea_h(const CEADev& c, const CeaT& p) {
+    const double T = p.T;
+    if (T < c.T_low) return c.h_low - c.Cp_low * (c.T_low - T);
+    if (T > c.T_high) return c.h_high + c.Cp_high * (T - c.T_high);
+    double a[9];
+    cea_coeffs(c, T, a);
+    double h_on_RT = div_by(-a[0], T, p.rT) + a[1] * p.logT + a[2] * T + a[3] * T * T / 2.0;
+    h_on_RT += div_by(a[4] * T * T * T, 3.0, 1.0 / 3.0) + a[5] * T * T * T * T / 4.0 +
+               div_by(a[6] * T * T * T * T * T, 5.0, 1.0 / 5.0) + a[7];
+    return c.R * h_on_RT;
+}
+__host__ __device__ inline double cea_Cp(const CEADev& c, const CeaT& p) {
+    const double T = p.T;
+    if (T < c.T_low) return c.Cp_low;
+    if (T > c.T_high) return c.Cp_high;
+    double a[9];
+    cea_coeffs(c, T, a);
+    double Cp_on_R = div_by(a[0], p.TT, p.rTT) + div_by(a[1], T, p.rT) + a[2] + a[3] * T;
+    Cp_on_R += a[4] * T * T + a[5] * T * T * T + a[6] * T * T * T * T;
+    return c.R * Cp_on_R;
+}
+
 // cea_thermo_curves.d:103-130
 __host__ __device__ inline double cea_s(const CEADev& c, double T) {
     if (T < c.T_low) return c.s_low - c.Cp_low * log(c.T_low / T);
@@ -191,11 +255,11 @@ __host__ __device__ inline double cea_s(const CEADev& c, double T) {
 // not interleave (fewer registers live at once, for kernels whose occupancy they limit).
 template <int NSP, bool SERIAL = false>
 __device__ __forceinline__ double tpg_energy(const ChemDev& cd, const double* massf, double T) {
-    const double logT = log(T);
+    const CeaT pw(T, log(T));
     double u = 0.0;
 #pragma unroll
     for (int i = 0; i < NSP; ++i) {
-        u += massf[i] * (cea_h(cd.cea[i], T, logT) - cd.R[i] * T);
+        u += massf[i] * (cea_h(cd.cea[i], pw) - cd.R[i] * T);
         if (SERIAL) __builtin_amdgcn_sched_barrier(0);
     }
     return u;
@@ -205,12 +269,12 @@ __device__ __forceinline__ double tpg_energy(const ChemDev& cd, const double* ma
 template <int NSP, bool SERIAL = false>
 __device__ __forceinline__ void tpg_energy_Cv(const ChemDev& cd, const double* massf, double T, double& u,
                                               double& Cv) {
-    const double logT = log(T);
+    const CeaT pw(T, log(T));
     double uu = 0.0, v = 0.0;
 #pragma unroll
     for (int i = 0; i < NSP; ++i) {
         double h, Cp;
-        cea_h_Cp(cd.cea[i], T, logT, h, Cp);
+        cea_h_Cp(cd.cea[i], pw, h, Cp);
         uu += massf[i] * (h - cd.R[i] * T);
         v += massf[i] * (Cp - cd.R[i]);
         if (SERIAL) __builtin_amdgcn_sched_barrier(0);
@@ -348,10 +412,19 @@ __device__ __forceinline__ bool tpg_update_temperature(const ChemDev& cd, const 
     return false;
 }
 
-// therm_perf_gas.d:394-404: frozen sound speed sqrt(gamma * (R T)), gamma = Cp/Cv.
+// therm_perf_gas.d:394-404: frozen sound speed sqrt(gamma * (R T)), gamma = Cp/Cv, with
+// tpg_Cp and tpg_Cv summed from one curve evaluation per species (the same sums).
 template <int NSP>
 __device__ __forceinline__ double tpg_sound_speed(const ChemDev& cd, const double* massf, double T) {
-    double gamma = tpg_Cp<NSP>(cd, massf, T) / tpg_Cv<NSP>(cd, massf, T);
+    const CeaT pw(T, 0.0);
+    double Cp = 0.0, Cv = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSP; ++i) {
+        const double c = cea_Cp(cd.cea[i], pw);
+        Cp += massf[i] * c;
+        Cv += massf[i] * (c - cd.R[i]);
+    }
+    double gamma = Cp / Cv;
     double R = tpg_Rmix<NSP>(cd, massf);
     return sqrt(gamma * (R * T));
 }

@@ -367,7 +367,7 @@ __global__ __launch_bounds__(CHEM_BLOCK) void k_chem_integrate(const ChemDev* __
 }
 
 template <int NSP>
-__global__ __launch_bounds__(CHEM_BLOCK) void k_chem_finish(const ChemDev* __restrict__ cdp, int n,
+__global__ __launch_bounds__(CHEM_BLOCK, 3) void k_chem_finish(const ChemDev* __restrict__ cdp, int n,
                                                             const double* __restrict__ rho, const double* T,
                                                             const double* u, const double* p, const double* massf,
                                                             const double* dtc_in, double* T_out, double* u_out,

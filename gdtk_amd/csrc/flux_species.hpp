@@ -381,8 +381,9 @@ __device__ __forceinline__ void cell_update_species(const BlockDev& b, const Che
 // Two lanes per face (face_flux_species_pair): a workgroup of 256 lanes serves 128 faces of
 // one direction of one block (the FaceBatch ranges start at multiples of 256 faces).
 // LMR: Eilmer 5 reconstruction and AUSMDV (face_flux_species_pair).
+// 3 waves per SIMD (168 VGPRs): the compiler fits the kernel there without spilling.
 template <int DIM, int FC, int NSP, bool LMR>
-__global__ __launch_bounds__(256) void k_face_flux_species_pair(const BlockDev* __restrict__ tab, FaceBatch fb,
+__global__ __launch_bounds__(256, 3) void k_face_flux_species_pair(const BlockDev* __restrict__ tab, FaceBatch fb,
                                                                 int nblocks, double eps,
                                                                 const ChemDev* __restrict__ cdp) {
     int64_t w0 = (int64_t)blockIdx.x * (blockDim.x / 2);
