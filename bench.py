@@ -919,7 +919,7 @@ def bench_coupled_headline(steps, warmup, spinup, with_cpu=True, log=None):
                  "kernel": kname, "avg_launch_ms": face_avg, "bytes_per_face": bpf,
                  "items_per_launch": face["items_per_launch"],
                  "note": "bound by instruction issue, not HBM: per face two Newton T(u) solves over the CEA "
-                         "curves of 5 species; VALU 60 % busy, 56 % of it FP64 "
+                         "curves of 5 species; VALU 58 % busy, 55 % of it FP64 "
                          "(profiles/coupled_cylinder_headline_counters.txt)"}
     per_att, per_it = alpha_qss_flops(mech, 5)
     chem_flops = per_att * chs["attempts_total"] + per_it * chs["iters_total"]
