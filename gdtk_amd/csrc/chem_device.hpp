@@ -238,6 +238,32 @@ __host__ __device__ inline double cea_Cp(const CEADev& c, const CeaT& p) {
     return c.R * Cp_on_R;
 }
 
+// cea_h and cea_s (cea_thermo_curves.d:71-130) at one temperature from one coefficient
+// selection, log(T) and the reciprocals of CeaT: the same values bit for bit.  The
+// divisor 2 T T of the entropy is exactly 2 TT, whose reciprocal is exactly rTT / 2.
+__host__ __device__ inline void cea_h_s(const CEADev& c, const CeaT& p, double& h, double& sv) {
+    const double T = p.T;
+    if (T < c.T_low) {
+        h = c.h_low - c.Cp_low * (c.T_low - T);
+        sv = c.s_low - c.Cp_low * log(c.T_low / T);
+        return;
+    }
+    if (T > c.T_high) {
+        h = c.h_high + c.Cp_high * (T - c.T_high);
+        sv = c.s_high + c.Cp_high * log(T / c.T_high);
+        return;
+    }
+    double a[9];
+    cea_coeffs(c, T, a);
+    double h_on_RT = div_by(-a[0], T, p.rT) + a[1] * p.logT + a[2] * T + a[3] * T * T / 2.0;
+    h_on_RT += div_by(a[4] * T * T * T, 3.0, 1.0 / 3.0) + a[5] * T * T * T * T / 4.0 +
+               div_by(a[6] * T * T * T * T * T, 5.0, 1.0 / 5.0) + a[7];
+    h = c.R * h_on_RT;
+    double s_on_R = div_by(-a[0], 2.0 * p.TT, 0.5 * p.rTT) - div_by(a[1], T, p.rT) + a[2] * p.logT + a[3] * T;
+    s_on_R += a[4] * T * T / 2.0 + div_by(a[5] * T * T * T, 3.0, 1.0 / 3.0) + a[6] * T * T * T * T / 4.0 + a[8];
+    sv = c.R * s_on_R;
+}
+
 // cea_thermo_curves.d:103-130
 __host__ __device__ inline double cea_s(const CEADev& c, double T) {
     if (T < c.T_low) return c.s_low - c.Cp_low * log(c.T_low / T);
@@ -449,10 +475,11 @@ __device__ __forceinline__ void eval_rate_constants(const ChemDev& cd, double Ti
     if (T < cd.T_lower) T = cd.T_lower;
     if (T > cd.T_upper) T = cd.T_upper;
     double g[NSP];
+    const CeaT pw(T, log(T));
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
-        double h = cea_h(cd.cea[s], T);
-        double sv = cea_s(cd.cea[s], T);
+        double h, sv;
+        cea_h_s(cd.cea[s], pw, h, sv);
         g[s] = h - T * sv;
     }
 #pragma unroll
@@ -477,10 +504,10 @@ __device__ __forceinline__ void eval_rate_constants(const ChemDev& cd, double Ti
         }
         double kfv, kbv;
         if (!r.f_from_eq) {
-            kfv = r.fA * pow(T, r.fn) * exp(-r.fC / T);
-            kbv = r.b_from_eq ? kfv / Keq : r.bA * pow(T, r.bn) * exp(-r.bC / T);
+            kfv = r.fA * pow(T, r.fn) * exp(div_by(-r.fC, T, pw.rT));
+            kbv = r.b_from_eq ? kfv / Keq : r.bA * pow(T, r.bn) * exp(div_by(-r.bC, T, pw.rT));
         } else {
-            kbv = r.bA * pow(T, r.bn) * exp(-r.bC / T);
+            kbv = r.bA * pow(T, r.bn) * exp(div_by(-r.bC, T, pw.rT));
             kfv = kbv * Keq;
         }
         kf[ir] = kfv;

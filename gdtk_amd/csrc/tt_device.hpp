@@ -45,13 +45,13 @@ __device__ __forceinline__ double tt_ev(const ChemDev& cd, const TTDev& td, int 
 template <int NSP>
 __device__ __forceinline__ void tt_ev_cv_mix(const ChemDev& cd, const TTDev& td, const double* massf, double Tve,
                                              double& e, double& cv) {
-    const double lg = log(Tve);
+    const CeaT pw(Tve, log(Tve));
     e = 0.0;
     cv = 0.0;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
         double h, Cp;
-        cea_h_Cp(cd.cea[s], Tve, lg, h, Cp);
+        cea_h_Cp(cd.cea[s], pw, h, Cp);
         e += massf[s] * (h - td.Cp_tr_rot[s] * (Tve - TT_T_REF) - td.del_hf[s]);
         cv += massf[s] * (Cp - td.Cp_tr_rot[s]);
     }
