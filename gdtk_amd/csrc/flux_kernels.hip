@@ -275,33 +275,48 @@ __global__ __launch_bounds__(256) void k_cell_update_batch(const BlockDev* __res
     cell_update_one<DIM, STAGE>(b, gas, *dtp, lim.low_T, lim.max_vel, lim.min_T, lim.max_T, t);
 }
 
-// Ordered integer keys of doubles: key(x) < key(y) exactly when x < y (NaN excluded), so an
-// integer atomic maximum is an exact floating-point maximum.  A slot holding 0 is empty.
-__device__ __forceinline__ unsigned long long dt_key(double x) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-__device__ __forceinline__ double dt_unkey(unsigned long long k) {
-    return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+// The block time-step estimates of many blocks in two launches (fluidblock.d:987-1084 per
+// block).  k_dt_batch: each lane takes one cell's signal frequency (fvcell.d:975-1058) and its
+// dt_local and cfl_local; the minimum and the maximum are reduced across the wavefront with
+// lane exchanges and across the workgroup's four wavefronts through LDS, and written as the
+// workgroup's partial (part[2 g], part[2 g + 1]); the block's last cell stores its signal
+// (fluidblock.d:1081).  k_dt_blocks: one workgroup per block reduces the block's partials the
+// same way.  Minimum and maximum are exact, so the result does not depend on the order; like
+// fmin/fmax a NaN never wins.  No atomics: a single large block has thousands of
+// workgroups, whose atomics on one address would serialise.  A workgroup serves 256 cells of
+// one block: start[q] is block q's first launch index (a multiple of 256; start[nblocks] the
+// end), base the launch index of workgroup 0.
+__device__ __forceinline__ void dt_wave_reduce(double& dtl, double& cfl) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        dtl = fmin(dtl, __shfl_xor(dtl, off));
+        cfl = fmax(cfl, __shfl_xor(cfl, off));
+    }
 }
 
-// The block time-step estimates of many blocks in one launch (fluidblock.d:987-1084 per
-// block).  Each lane takes one cell's signal frequency (fvcell.d:975-1058) and its dt_local
-// and cfl_local; the minimum and the maximum are reduced across the wavefront with lane
-// exchanges, across the workgroup's four wavefronts through LDS, and enter the block's slots
-// with one atomic maximum each (dt as the complement of its key), so the result is the
-// exact minimum/maximum whatever the order.  Like fmin/fmax a NaN never wins.  The block's
-// last cell stores its signal (fluidblock.d:1081).  The last of a block's workgroups to
-// finish (a device counter per block) writes out[3q..3q+2] = {min dt_local, max cfl_local,
-// last signal} and empties the block's slots and counter for the next launch.  A workgroup
-// serves 256 cells of one block: start[q] is block q's first launch index (a multiple of
-// 256; start[nblocks] the end), base the launch index of workgroup 0.
+// Across the workgroup's wavefronts; the result is valid in thread 0.
+__device__ __forceinline__ void dt_group_reduce(double& dtl, double& cfl) {
+    __shared__ double wdt[4], wcfl[4];
+    dt_wave_reduce(dtl, cfl);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wdt[wave] = dtl;
+        wcfl[wave] = cfl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        for (int w = 1; w < nw; ++w) {
+            dtl = fmin(dtl, wdt[w]);
+            cfl = fmax(cfl, wcfl[w]);
+        }
+    }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(256) void k_dt_batch(const BlockDev* __restrict__ tab, const int64_t* __restrict__ start,
                                                   int nblocks, int64_t base, double dt_current, double cfl_value,
-                                                  unsigned long long* slots, unsigned* done, double* out) {
-    __shared__ double wdt[4], wcfl[4];
-    __shared__ int am_last;
+                                                  double* part, double* last_signal) {
     const int64_t w0 = base + (int64_t)blockIdx.x * blockDim.x;
     int lo = 0, hi = nblocks - 1;
     while (lo < hi) {
@@ -341,45 +356,32 @@ __global__ __launch_bounds__(256) void k_dt_batch(const BlockDev* __restrict__ t
         }
         cfl = dt_current * signal;
         dtl = cfl_value / signal;
-        if (t == ncell - 1) {
-            atomicExch(&slots[3 * q + 2], (unsigned long long)__double_as_longlong(signal));
-            __threadfence();
-        }
+        if (t == ncell - 1) last_signal[q] = signal;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        dtl = fmin(dtl, __shfl_xor(dtl, off));
-        cfl = fmax(cfl, __shfl_xor(cfl, off));
-    }
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        wdt[wave] = dtl;
-        wcfl[wave] = cfl;
-    }
-    __syncthreads();
+    dt_group_reduce(dtl, cfl);
     if (threadIdx.x == 0) {
-        const int nw = (int)(blockDim.x >> 6);
-        double d = wdt[0], cm = wcfl[0];
-        for (int w = 1; w < nw; ++w) {
-            d = fmin(d, wdt[w]);
-            cm = fmax(cm, wcfl[w]);
-        }
-        if (d == d) atomicMax(&slots[3 * q], ~dt_key(d));
-        if (cm == cm) atomicMax(&slots[3 * q + 1], dt_key(cm));
-        __threadfence();
-        const unsigned nwg = (unsigned)((start[q + 1] - start[q]) / 256);
-        am_last = (atomicAdd(&done[q], 1u) == nwg - 1) ? 1 : 0;
+        part[2 * (int64_t)blockIdx.x] = dtl;
+        part[2 * (int64_t)blockIdx.x + 1] = cfl;
     }
-    __syncthreads();
-    if (!am_last || threadIdx.x != 0) return;
-    __threadfence();
-    const unsigned long long kd = atomicExch(&slots[3 * q], 0ull);
-    const unsigned long long kc = atomicExch(&slots[3 * q + 1], 0ull);
-    const unsigned long long ks = atomicExch(&slots[3 * q + 2], 0ull);
-    out[3 * q] = kd ? dt_unkey(~kd) : __builtin_huge_val();
-    out[3 * q + 1] = kc ? dt_unkey(kc) : -__builtin_huge_val();
-    out[3 * q + 2] = __longlong_as_double((long long)ks);
-    atomicExch(&done[q], 0u);
+}
+
+// One workgroup per block: out[3q..3q+2] = {min dt_local, max cfl_local, last signal}.
+__global__ __launch_bounds__(256) void k_dt_blocks(const int64_t* __restrict__ start, int64_t base,
+                                                   const double* __restrict__ part,
+                                                   const double* __restrict__ last_signal, double* out) {
+    const int q = blockIdx.x;
+    const int64_t g0 = (start[q] - base) / 256, g1 = (start[q + 1] - base) / 256;
+    double dtl = __builtin_huge_val(), cfl = -__builtin_huge_val();
+    for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+        dtl = fmin(dtl, part[2 * g]);
+        cfl = fmax(cfl, part[2 * g + 1]);
+    }
+    dt_group_reduce(dtl, cfl);
+    if (threadIdx.x == 0) {
+        out[3 * q] = dtl;
+        out[3 * q + 1] = cfl;
+        out[3 * q + 2] = last_signal[q];
+    }
 }
 
 // Encode U from the flow state (fvcell.d:511-583), single species: used at start-up.
@@ -499,16 +501,17 @@ hipError_t launch_ghost_batch(const BlockDev* tab, const SideDev* sides, int nsi
 }
 
 hipError_t launch_dt_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t base, int64_t count,
-                           int dims, double dt_current, double cfl, unsigned long long* slots, unsigned* done,
-                           double* out, hipStream_t s) {
-    if (count <= 0 || count % 256 != 0) return hipErrorInvalidValue;
+                           int dims, double dt_current, double cfl, double* part, double* last_signal, double* out,
+                           hipStream_t s) {
+    if (count <= 0 || count % 256 != 0 || nblocks <= 0) return hipErrorInvalidValue;
     const unsigned nwg = (unsigned)(count / 256);
     if (dims == 3)
         hipLaunchKernelGGL((k_dt_batch<3>), dim3(nwg), dim3(256), 0, s, tab, start, nblocks, base, dt_current, cfl,
-                           slots, done, out);
+                           part, last_signal);
     else
         hipLaunchKernelGGL((k_dt_batch<2>), dim3(nwg), dim3(256), 0, s, tab, start, nblocks, base, dt_current, cfl,
-                           slots, done, out);
+                           part, last_signal);
+    hipLaunchKernelGGL(k_dt_blocks, dim3((unsigned)nblocks), dim3(256), 0, s, start, base, part, last_signal, out);
     return hipGetLastError();
 }
 

@@ -16,14 +16,14 @@ struct CellLimits {
     double max_T = 50000.0;
 };
 
-// Block time-step estimates of blocks [0, nblocks) of tab in one launch of count lanes
-// (count a multiple of 256; start[q] = block q's first launch index, start[nblocks] the
-// end, base = the launch index of the first lane).  out[3q..3q+2] = {min dt_local, max
-// cfl_local, last signal}; slots (3 per block) and done (1 per block) start at zero and are
-// left at zero.
+// Block time-step estimates of blocks [0, nblocks) of tab: one launch of count lanes (count
+// a multiple of 256; start[q] = block q's first launch index, start[nblocks] the end, base =
+// the launch index of the first lane) writing one partial per workgroup into part (2 per
+// workgroup), then one workgroup per block: out[3q..3q+2] = {min dt_local, max cfl_local,
+// last signal}; last_signal holds one double per block.
 hipError_t launch_dt_batch(const BlockDev* tab, const int64_t* start, int nblocks, int64_t base, int64_t count,
-                           int dims, double dt_current, double cfl, unsigned long long* slots, unsigned* done,
-                           double* out, hipStream_t s);
+                           int dims, double dt_current, double cfl, double* part, double* last_signal, double* out,
+                           hipStream_t s);
 hipError_t launch_encode(const BlockDev& b, double* U, hipStream_t s);
 hipError_t launch_decode(const BlockDev& b, const IdealGasDev& gas, double* U, double* fs, double low_T,
                          hipStream_t s);

@@ -126,16 +126,13 @@ struct Sim {
     DevArr<int64_t> fstart[3], cstart;
     std::vector<int64_t> cstart_h;
     int64_t ftotal[3] = {0, 0, 0}, ctotal = 0;
-    // Block time-step estimates: the batched reduction's slots (3 per block) and counters
-    // (1 per block), left empty by every launch, and its per-block results {dt, cfl, last
-    // signal}.
+    // Block time-step estimates: one partial per workgroup of the cell launch, each block's
+    // last signal, and the per-block results {dt, cfl, last signal}.
+    DevArr<double> dt_part, dt_last, dt_out;
     // Step-start copy of every block's flow state (multi-species gas): segments per
     // flow-state copy (cur = 0, 1), one launch, part of the captured step.
     DevArr<CopySeg> save_segs[2];
     int64_t save_chunks = 0;
-    DevArr<unsigned long long> dt_slots;
-    DevArr<unsigned> dt_done;
-    DevArr<double> dt_out;
     // SimState (simcore.d)
     double time = 0.0, dt_global = 0.0, dt_allow = 0.0, cfl_max = 0.0;
     int step = 0, attempts_last = 0;
@@ -267,8 +264,8 @@ struct Sim {
         cstart.alloc(st.size());
         HIPCHK(hipMemcpy(cstart.p, st.data(), st.size() * sizeof(int64_t), hipMemcpyHostToDevice));
         cstart_h = st;
-        dt_slots.alloc(3 * nb);
-        dt_done.alloc(nb);
+        dt_part.alloc(2 * (size_t)(ctotal / 256));
+        dt_last.alloc(nb);
         dt_out.alloc(3 * nb);
     }
 
@@ -998,7 +995,7 @@ struct Sim {
         finalize();
         const int64_t base = cstart_h[q0], count = cstart_h[q0 + n] - base;
         HIPCHK(launch_dt_batch(tab(1, cur, u0) + q0, cstart.p + q0, n, base, count, cfg.dimensions, dt_current, cfl,
-                               dt_slots.p + 3 * q0, dt_done.p + q0, dt_out.p + 3 * q0, stream));
+                               dt_part.p + 2 * (base / 256), dt_last.p + q0, dt_out.p + 3 * q0, stream));
     }
 
     // fluidblock.d:987-1084 from the reduced {dt_allow, cfl_max, last signal}.
